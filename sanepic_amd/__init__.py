@@ -1,0 +1,286 @@
+"""sanepic_amd -- MI355X-native CG GLS map-maker (drop-in for SANEPIC's solve).
+
+The compute lives in ``lib/libsanepic_hip.so`` (hand-written HIP for gfx950)
+behind the C ABI of ``include/sanepic_hip.h``.  This module is a thin ctypes
+front end used by the tests and the benchmark; the drop-in command-line
+program is ``bin/sanepic``.  There is no CPU fallback: if the library is
+missing, importing ``Context`` raises.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libsanepic_hip.so")
+EXE_PATH = os.path.join(HERE, "bin", "sanepic")
+CSRC = os.path.join(HERE, "csrc")
+
+FP64, MIXED = 0, 1
+PRECOND_DIAG, PRECOND_BLOCK3 = 0, 1
+
+# The header's function list; tests check the library exports all of them.
+ABI_FUNCTIONS = [
+    "sanepic_create", "sanepic_destroy", "sanepic_last_error", "sanepic_abi_version",
+    "sanepic_set_detector", "sanepic_set_pointing", "sanepic_set_tod", "sanepic_pixelize",
+    "sanepic_get_pixels", "sanepic_get_indpix", "sanepic_setup", "sanepic_get_setup",
+    "sanepic_get_rhs", "sanepic_apply", "sanepic_chi2", "sanepic_solve_begin",
+    "sanepic_solve_step", "sanepic_get_map", "sanepic_get_nvar", "sanepic_get_sumcross",
+    "sanepic_true_residual", "sanepic_set_profiling", "sanepic_kernel_time",
+    "sanepic_device_bytes",
+]
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("nside", ctypes.c_int64), ("ndet", ctypes.c_int64), ("nseg", ctypes.c_int64),
+                ("ns", ctypes.c_int64), ("polar", ctypes.c_int32), ("hwp", ctypes.c_int32),
+                ("omega_hwp", ctypes.c_double), ("precision", ctypes.c_int32),
+                ("device", ctypes.c_int32)]
+
+
+class SolveOpts(ctypes.Structure):
+    _fields_ = [("precond", ctypes.c_int32), ("refresh", ctypes.c_int32)]
+
+
+class IterInfo(ctypes.Structure):
+    _fields_ = [("iter", ctypes.c_int32), ("crit", ctypes.c_double), ("crit2", ctypes.c_double),
+                ("chi2", ctypes.c_double), ("alpha", ctypes.c_double), ("beta", ctypes.c_double)]
+
+
+class SanepicError(RuntimeError):
+    pass
+
+
+def build(jobs=8):
+    """Compiles the HIP library and the drop-in executable (always rebuilds)."""
+    subprocess.run(["make", "-B", f"-j{jobs}", "-C", CSRC], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SanepicError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.c_void_p
+        d_p = ctypes.POINTER(ctypes.c_double)
+        i64p = ctypes.POINTER(ctypes.c_int64)
+        i64 = ctypes.c_int64
+        sig = {
+            "sanepic_create": [ctypes.POINTER(Config), ctypes.POINTER(P)],
+            "sanepic_destroy": [P],
+            "sanepic_abi_version": [],
+            "sanepic_set_detector": [P, i64, ctypes.c_double, ctypes.c_double, ctypes.c_double, d_p, i64],
+            "sanepic_set_pointing": [P, i64, d_p, d_p, d_p],
+            "sanepic_set_tod": [P, i64, i64, d_p],
+            "sanepic_pixelize": [P, i64p, i64p],
+            "sanepic_get_pixels": [P, i64, i64, i64p],
+            "sanepic_get_indpix": [P, i64p],
+            "sanepic_setup": [P],
+            "sanepic_get_setup": [P, i64p, d_p, d_p],
+            "sanepic_get_rhs": [P, d_p],
+            "sanepic_apply": [P, d_p, d_p],
+            "sanepic_chi2": [P, d_p, d_p],
+            "sanepic_solve_begin": [P, ctypes.POINTER(SolveOpts), d_p],
+            "sanepic_solve_step": [P, ctypes.POINTER(IterInfo)],
+            "sanepic_get_map": [P, d_p],
+            "sanepic_get_nvar": [P, d_p],
+            "sanepic_get_sumcross": [P, d_p],
+            "sanepic_true_residual": [P, d_p],
+            "sanepic_set_profiling": [P, ctypes.c_int32],
+            "sanepic_kernel_time": [P, ctypes.c_int32, d_p, i64p],
+            "sanepic_device_bytes": [P, i64p],
+        }
+        for name, args in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = None if name == "sanepic_destroy" else ctypes.c_int
+        L.sanepic_last_error.argtypes = [P]
+        L.sanepic_last_error.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Context:
+    """One map-making run on one GPU (wraps sanepic_ctx)."""
+
+    def __init__(self, nside, ndet, nseg, ns, hwp=0, omega=0.0, precision=FP64, device=0):
+        self.nside, self.ndet, self.nseg, self.ns = nside, ndet, nseg, ns
+        self.h = None
+        cfg = Config(nside, ndet, nseg, ns, 1, hwp, omega, precision, device)
+        h = ctypes.c_void_p()
+        rc = lib().sanepic_create(ctypes.byref(cfg), ctypes.byref(h))
+        if rc:
+            raise SanepicError(f"sanepic_create: {lib().sanepic_last_error(None).decode()}")
+        self.h = h
+        self.npix = None
+
+    def close(self):
+        if self.h:
+            lib().sanepic_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _chk(self, rc, what):
+        if rc:
+            raise SanepicError(f"{what} ({rc}): {lib().sanepic_last_error(self.h).decode()}")
+
+    def set_detector(self, idet, lel, xel, angle, ispf):
+        ispf = _f64(ispf)
+        self._chk(lib().sanepic_set_detector(self.h, idet, lel, xel, angle, _dp(ispf), ispf.size),
+                  "set_detector")
+
+    def set_pointing(self, iseg, ra, dec, psi):
+        ra, dec, psi = _f64(ra), _f64(dec), _f64(psi)
+        assert ra.size == dec.size == psi.size == self.ns
+        self._chk(lib().sanepic_set_pointing(self.h, iseg, _dp(ra), _dp(dec), _dp(psi)),
+                  "set_pointing")
+
+    def set_tod(self, idet, iseg, tod):
+        tod = _f64(tod)
+        assert tod.size == self.ns
+        self._chk(lib().sanepic_set_tod(self.h, idet, iseg, _dp(tod)), "set_tod")
+
+    def load(self, cfg, ds):
+        """Feeds a sanepic_amd.synth dataset (all detectors and segments)."""
+        ns = self.ns
+        for k in range(self.ndet):
+            lel, xel = cfg.offsets_deg[k]
+            self.set_detector(k, lel, xel, cfg.det_angles_deg[k], ds["ispf"][k])
+        for s in range(self.nseg):
+            sl = slice(s * ns, (s + 1) * ns)
+            self.set_pointing(s, ds["ra"][sl], ds["dec"][sl], ds["psi"][sl])
+        for k in range(self.ndet):
+            for s in range(self.nseg):
+                self.set_tod(k, s, ds["tod"][k, s * ns:(s + 1) * ns])
+
+    def pixelize(self):
+        npix = ctypes.c_int64()
+        ndef = ctypes.c_int64()
+        self._chk(lib().sanepic_pixelize(self.h, ctypes.byref(npix), ctypes.byref(ndef)), "pixelize")
+        self.npix = npix.value
+        return npix.value, ndef.value
+
+    def pixels(self, idet, iseg):
+        out = np.empty(self.ns, dtype=np.int64)
+        self._chk(lib().sanepic_get_pixels(self.h, idet, iseg, _ip(out)), "get_pixels")
+        return out
+
+    def all_pixels(self):
+        return np.concatenate([self.pixels(d, s) for d in range(self.ndet) for s in range(self.nseg)])
+
+    def indpix(self):
+        out = np.empty(12 * self.nside ** 2, dtype=np.int64)
+        self._chk(lib().sanepic_get_indpix(self.h, _ip(out)), "get_indpix")
+        return out
+
+    def setup(self):
+        self._chk(lib().sanepic_setup(self.h), "setup")
+
+    def get_setup(self):
+        n = self.npix
+        hits = np.empty(2 * n, dtype=np.int64)
+        mp = np.empty(3 * n)
+        sc = np.empty(3 * n)
+        self._chk(lib().sanepic_get_setup(self.h, _ip(hits), _dp(mp), _dp(sc)), "get_setup")
+        return hits, mp, sc
+
+    def rhs(self):
+        b = np.empty(3 * self.npix)
+        self._chk(lib().sanepic_get_rhs(self.h, _dp(b)), "get_rhs")
+        return b
+
+    def apply(self, x):
+        x = _f64(x)
+        y = np.empty(3 * self.npix)
+        self._chk(lib().sanepic_apply(self.h, _dp(x), _dp(y)), "apply")
+        return y
+
+    def chi2(self, x):
+        x = _f64(x)
+        out = ctypes.c_double()
+        self._chk(lib().sanepic_chi2(self.h, _dp(x), ctypes.byref(out)), "chi2")
+        return out.value
+
+    def solve_begin(self, precond=PRECOND_DIAG, refresh=10, x0=None):
+        opts = SolveOpts(precond, refresh)
+        xp = None if x0 is None else _dp(_f64(x0))
+        self._x0_keep = None if x0 is None else _f64(x0)
+        self._chk(lib().sanepic_solve_begin(self.h, ctypes.byref(opts), xp), "solve_begin")
+
+    def step(self):
+        info = IterInfo()
+        self._chk(lib().sanepic_solve_step(self.h, ctypes.byref(info)), "solve_step")
+        return dict(iter=info.iter, crit=info.crit, crit2=info.crit2, chi2=info.chi2,
+                    alpha=info.alpha, beta=info.beta)
+
+    def solve(self, precond=PRECOND_DIAG, stop=1e-15, maxiter=2000, refresh=10, x0=None):
+        """The reference's loop: while ((iter < maxiter && crit > stop) || iter == 0)."""
+        self.solve_begin(precond, refresh, x0)
+        hist = []
+        while True:
+            info = self.step()
+            hist.append(info)
+            it = info["iter"] + 1
+            if not ((it < maxiter and info["crit"] > stop) or it == 0):
+                break
+        return self.map(), hist
+
+    def map(self):
+        x = np.empty(3 * self.npix)
+        self._chk(lib().sanepic_get_map(self.h, _dp(x)), "get_map")
+        return x
+
+    def nvar(self):
+        x = np.empty(3 * self.npix)
+        self._chk(lib().sanepic_get_nvar(self.h, _dp(x)), "get_nvar")
+        return x
+
+    def sumcross(self):
+        x = np.empty(3 * self.npix)
+        self._chk(lib().sanepic_get_sumcross(self.h, _dp(x)), "get_sumcross")
+        return x
+
+    def true_residual(self):
+        out = ctypes.c_double()
+        self._chk(lib().sanepic_true_residual(self.h, ctypes.byref(out)), "true_residual")
+        return out.value
+
+    def set_profiling(self, on):
+        self._chk(lib().sanepic_set_profiling(self.h, 1 if on else 0), "set_profiling")
+
+    def kernel_time(self, which):
+        ms = ctypes.c_double()
+        n = ctypes.c_int64()
+        self._chk(lib().sanepic_kernel_time(self.h, which, ctypes.byref(ms), ctypes.byref(n)),
+                  "kernel_time")
+        return ms.value, n.value
+
+    def device_bytes(self):
+        b = ctypes.c_int64()
+        self._chk(lib().sanepic_device_bytes(self.h, ctypes.byref(b)), "device_bytes")
+        return b.value

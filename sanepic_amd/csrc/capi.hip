@@ -1,0 +1,1056 @@
+// capi.hip -- host orchestration behind include/sanepic_hip.h.
+//
+// One context = one map-making run on one GPU: boresight pointing and TODs
+// are uploaded once, every sample's pixel and cos/sin 2psi are computed once
+// and stay resident in HBM, together with the pixel-sorted sample order used
+// by P^T.  Each CG iteration is a fixed sequence of kernels on the context's
+// stream; the host only reads back the scalars of the stop test.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <hipcub/hipcub.hpp>
+#include <string>
+#include <tuple>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/sanepic_hip.h"
+#include "kernels.h"
+#include "pixel_host.h"
+
+namespace {
+
+thread_local std::string g_create_error;
+
+constexpr int kChunk = 4096;      // max sorted entries per P^T item
+constexpr int kProfClasses = 5;
+
+}  // namespace
+
+struct sanepic_ctx {
+  sanepic_config cfg{};
+  std::string err;
+  hipStream_t st = nullptr;
+  int64_t ns = 0, nseg = 0, ndet = 0, ntot = 0, nside = 0, npix = -1;
+  PairGeom g{}, gauto{};
+  bool single = false;
+  std::vector<DetConst> detc;
+  std::vector<double> offx, offy, spn;
+  std::vector<char> det_ok, pt_ok, tod_ok;
+  std::vector<int32_t> h_indpix, h_pixlist, h_cnt;
+  int64_t n_deferred = 0, n_bad = 0;
+  int64_t bytes = 0;
+  // device
+  double *ra = nullptr, *dec = nullptr, *psi = nullptr, *tod = nullptr;
+  DetConst *d_det = nullptr;
+  int32_t *pix = nullptr, *cnt = nullptr;
+  double2 *cs = nullptr, *csfp = nullptr, *cs_sorted = nullptr, *csfp_sorted = nullptr;
+  float2 *cs32 = nullptr, *cs_sorted32 = nullptr;
+  int64_t *off = nullptr;
+  uint32_t *perm = nullptr;
+  int32_t *it_pix = nullptr, *it_len = nullptr, *it_slot = nullptr, *sp_pix = nullptr,
+          *sp_first = nullptr, *sp_count = nullptr;
+  int64_t *it_beg = nullptr;
+  Items items{};
+  int64_t nslots = 0;
+  double *partial = nullptr;
+  double2 *tw = nullptr, *twlo = nullptr, *twhi = nullptr;
+  float2 *tw32 = nullptr, *twlo32 = nullptr, *twhi32 = nullptr;
+  double *wt = nullptr;
+  float *wt32 = nullptr;
+  double2 *buf = nullptr, *dspec = nullptr;
+  float2 *buf32 = nullptr;
+  double *z = nullptr;
+  float *z32 = nullptr, *map32 = nullptr;
+  double *chi2part = nullptr;
+  int64_t nchi2 = 0;
+  double *b = nullptr, *b_raw = nullptr, *nauto = nullptr, *pairs6 = nullptr, *cross = nullptr;
+  double *x = nullptr, *r = nullptr, *d = nullptr, *q = nullptr, *M = nullptr, *nvar = nullptr,
+         *part = nullptr, *tmpblk = nullptr;
+  CGScalars *sc = nullptr;
+  int *zero_flag = nullptr;
+  int iter = 0, refresh = 10, precond = 0;
+  bool begun = false, setup_done = false, pixelized = false;
+  bool prof = false;
+  std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> evs;
+  double kt_ms[kProfClasses] = {0};
+  int64_t kt_n[kProfClasses] = {0};
+  std::vector<void *> allocs;
+};
+
+namespace {
+
+int fail(sanepic_ctx *c, int code, const std::string &msg) {
+  c->err = msg;
+  return code;
+}
+
+#define HIPCHK(c, expr)                                                                  \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return fail(c, SANEPIC_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));  \
+  } while (0)
+
+#define LCHK(c, expr)                                                                    \
+  do {                                                                                   \
+    if ((expr) != 0) {                                                                   \
+      hipError_t e_ = hipGetLastError();                                                 \
+      return fail(c, SANEPIC_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));  \
+    }                                                                                    \
+  } while (0)
+
+template <typename T>
+int dalloc(sanepic_ctx *c, T *&p, int64_t n) {
+  if (p) return 0;
+  if (n <= 0) n = 1;
+  void *v = nullptr;
+  hipError_t e = hipMalloc(&v, (size_t)n * sizeof(T));
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(c, SANEPIC_ENOMEM,
+                "hipMalloc of " + std::to_string((size_t)n * sizeof(T)) + " bytes failed: " +
+                    hipGetErrorString(e));
+  }
+  p = (T *)v;
+  c->allocs.push_back(v);
+  c->bytes += (int64_t)((size_t)n * sizeof(T));
+  return 0;
+}
+
+template <typename T>
+void dfree(sanepic_ctx *c, T *&p, int64_t n) {
+  if (!p) return;
+  auto it = std::find(c->allocs.begin(), c->allocs.end(), (void *)p);
+  if (it != c->allocs.end()) c->allocs.erase(it);
+  (void)hipFree(p);
+  c->bytes -= (int64_t)((size_t)std::max<int64_t>(n, 1) * sizeof(T));
+  p = nullptr;
+}
+
+#define ACHK(c, expr)        \
+  do {                       \
+    int rc_ = (expr);        \
+    if (rc_ != 0) return rc_; \
+  } while (0)
+
+int ilog2(int64_t v) {
+  int l = 0;
+  while ((int64_t(1) << l) < v) l++;
+  return l;
+}
+
+// ---------------------------------------------------------------- profiling
+struct ProfScope {
+  sanepic_ctx *c;
+  int cls;
+  hipEvent_t a = nullptr, b = nullptr;
+  ProfScope(sanepic_ctx *c_, int cls_) : c(c_), cls(cls_) {
+    if (c->prof) {
+      (void)hipEventCreate(&a);
+      (void)hipEventCreate(&b);
+      (void)hipEventRecord(a, c->st);
+    }
+  }
+  ~ProfScope() {
+    if (c->prof && a) {
+      (void)hipEventRecord(b, c->st);
+      c->evs.emplace_back(cls, a, b);
+    }
+  }
+};
+
+int prof_collect(sanepic_ctx *c) {
+  if (c->evs.empty()) return 0;
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  for (auto &e : c->evs) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, std::get<1>(e), std::get<2>(e)) == hipSuccess) {
+      c->kt_ms[std::get<0>(e)] += ms;
+      c->kt_n[std::get<0>(e)] += 1;
+    }
+    (void)hipEventDestroy(std::get<1>(e));
+    (void)hipEventDestroy(std::get<2>(e));
+  }
+  c->evs.clear();
+  return 0;
+}
+
+// ------------------------------------------------------------------ tables
+int build_tables(sanepic_ctx *c) {
+  if (c->tw) return 0;
+  const int64_t N = c->ns;
+  std::vector<double2> t4(4096);
+  const long double PI2 = 6.283185307179586476925286766559005768L;
+  for (int j = 0; j < 4096; j++) {
+    long double a = PI2 * j / 4096.0L;
+    t4[j] = make_double2((double)cosl(a), -(double)sinl(a));
+  }
+  ACHK(c, dalloc(c, c->tw, 4096));
+  HIPCHK(c, hipMemcpy(c->tw, t4.data(), 4096 * sizeof(double2), hipMemcpyHostToDevice));
+  const int64_t nhi = std::max<int64_t>(1, N / 4096);
+  std::vector<double2> lo(4096), hi(nhi);
+  for (int64_t j = 0; j < 4096; j++) {
+    long double a = PI2 * (long double)(j % N) / (long double)N;
+    lo[j] = make_double2((double)cosl(a), -(double)sinl(a));
+  }
+  for (int64_t j = 0; j < nhi; j++) {
+    long double a = PI2 * (long double)((j * 4096) % N) / (long double)N;
+    hi[j] = make_double2((double)cosl(a), -(double)sinl(a));
+  }
+  ACHK(c, dalloc(c, c->twlo, 4096));
+  ACHK(c, dalloc(c, c->twhi, nhi));
+  HIPCHK(c, hipMemcpy(c->twlo, lo.data(), 4096 * sizeof(double2), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->twhi, hi.data(), nhi * sizeof(double2), hipMemcpyHostToDevice));
+  // weights in row layout: wt[det][k2*N1 + k1] = W[k2 + N2*k1], W[k] = SpN[min(k, N-k)]
+  const int64_t N1 = c->g.N1, N2 = c->g.N2;
+  std::vector<double> w((size_t)c->ndet * N);
+  for (int64_t det = 0; det < c->ndet; det++) {
+    const double *row = c->spn.data() + det * N;
+    for (int64_t k2 = 0; k2 < N2; k2++)
+      for (int64_t k1 = 0; k1 < N1; k1++) {
+        const int64_t k = k2 + N2 * k1;
+        w[det * N + k2 * N1 + k1] = row[std::min(k, N - k)];
+      }
+  }
+  ACHK(c, dalloc(c, c->wt, c->ndet * N));
+  HIPCHK(c, hipMemcpy(c->wt, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice));
+  if (c->cfg.precision == SANEPIC_MIXED) {
+    std::vector<float2> f4(4096), flo(4096), fhi(nhi);
+    for (int j = 0; j < 4096; j++) f4[j] = make_float2((float)t4[j].x, (float)t4[j].y);
+    for (int j = 0; j < 4096; j++) flo[j] = make_float2((float)lo[j].x, (float)lo[j].y);
+    for (int64_t j = 0; j < nhi; j++) fhi[j] = make_float2((float)hi[j].x, (float)hi[j].y);
+    std::vector<float> wf(w.size());
+    for (size_t i = 0; i < w.size(); i++) wf[i] = (float)w[i];
+    ACHK(c, dalloc(c, c->tw32, 4096));
+    ACHK(c, dalloc(c, c->twlo32, 4096));
+    ACHK(c, dalloc(c, c->twhi32, nhi));
+    ACHK(c, dalloc(c, c->wt32, c->ndet * N));
+    HIPCHK(c, hipMemcpy(c->tw32, f4.data(), 4096 * sizeof(float2), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->twlo32, flo.data(), 4096 * sizeof(float2), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->twhi32, fhi.data(), nhi * sizeof(float2), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->wt32, wf.data(), wf.size() * sizeof(float), hipMemcpyHostToDevice));
+  }
+  return 0;
+}
+
+template <typename T>
+OpArgs<T> op_args(sanepic_ctx *c);
+
+template <>
+OpArgs<double> op_args<double>(sanepic_ctx *c) {
+  OpArgs<double> a{};
+  a.g = c->g;
+  a.pix = c->pix;
+  a.cs = c->cs;
+  a.tod = c->tod;
+  a.buf = c->buf;
+  a.dspec = c->dspec;
+  a.wt = c->wt;
+  a.z = c->z;
+  a.tw4096 = c->tw;
+  a.twlo = c->twlo;
+  a.twhi = c->twhi;
+  a.chi2part = c->chi2part;
+  return a;
+}
+
+template <>
+OpArgs<float> op_args<float>(sanepic_ctx *c) {
+  OpArgs<float> a{};
+  a.g = c->g;
+  a.pix = c->pix;
+  a.cs = c->cs32;
+  a.tod = c->tod;
+  a.buf = c->buf32;
+  a.dspec = nullptr;
+  a.wt = c->wt32;
+  a.z = c->z32;
+  a.tw4096 = c->tw32;
+  a.twlo = c->twlo32;
+  a.twhi = c->twhi32;
+  a.chi2part = c->chi2part;
+  return a;
+}
+
+// N^-1 pass over all pairs: src -> z (time order).  Returns chi2 block count.
+template <typename T>
+int run_filter(sanepic_ctx *c, OpArgs<T> a, int src, int mode, int64_t *nchi) {
+  a.mode = mode;
+  int64_t nb = 0;
+  if (c->single) {
+    ProfScope ps(c, 1);
+    LCHK(c, launch_single<T>(a, src, c->st, &nb));
+  } else {
+    {
+      ProfScope ps(c, 0);
+      LCHK(c, launch_col_fwd<T>(a, src, c->st));
+    }
+    {
+      ProfScope ps(c, 1);
+      LCHK(c, launch_row<T>(a, c->st, &nb));
+    }
+    {
+      ProfScope ps(c, 2);
+      LCHK(c, launch_col_inv<T>(a, c->st));
+    }
+  }
+  if (nchi) *nchi = nb;
+  return 0;
+}
+
+template <typename T>
+int run_scatter(sanepic_ctx *c, const T *z, double *q, int mask) {
+  ScatterArgs<T> s{};
+  s.it = c->items;
+  s.perm = c->perm;
+  if constexpr (sizeof(T) == 8)
+    s.cs_sorted = c->cs_sorted;
+  else
+    s.cs_sorted = c->cs_sorted32;
+  s.z = z;
+  s.cnt = c->cnt;
+  s.q = q;
+  s.partial = c->partial;
+  s.mask = mask;
+  ProfScope ps(c, 3);
+  LCHK(c, launch_scatter<T>(s, c->st));
+  return 0;
+}
+
+// q = P^T N^-1 P x (x, q interleaved, device).  chi2 -> sc->chi2 when asked.
+int apply_A(sanepic_ctx *c, const double *x, double *q, int mask, bool chi2, bool fp64) {
+  ProfScope whole(c, 4);
+  const int mode = ROW_FWD | ROW_FILTER | ROW_INV | (chi2 ? ROW_CHI2 : 0);
+  int64_t nb = 0;
+  const bool use64 = fp64 || c->cfg.precision == SANEPIC_FP64;
+  if (chi2 && !use64) return fail(c, SANEPIC_EINVAL, "chi2 needs the FP64 operator");
+  if (use64) {
+    OpArgs<double> a = op_args<double>(c);
+    a.map = x;
+    ACHK(c, run_filter<double>(c, a, SRC_GATHER, mode, &nb));
+    ACHK(c, run_scatter<double>(c, c->z, q, mask));
+  } else {
+    LCHK(c, launch_d2f(x, c->map32, 3 * c->npix, c->st));
+    OpArgs<float> a = op_args<float>(c);
+    a.map = c->map32;
+    ACHK(c, run_filter<float>(c, a, SRC_GATHER, mode, &nb));
+    ACHK(c, run_scatter<float>(c, c->z32, q, mask));
+  }
+  if (chi2) LCHK(c, launch_reduce_chi2(c->chi2part, nb, c->sc, c->st));
+  return 0;
+}
+
+int check_ready(sanepic_ctx *c, bool need_setup) {
+  if (!c) return SANEPIC_EINVAL;
+  if (!c->pixelized) return fail(c, SANEPIC_EINVAL, "sanepic_pixelize has not run");
+  if (need_setup && !c->setup_done) return fail(c, SANEPIC_EINVAL, "sanepic_setup has not run");
+  return 0;
+}
+
+int to_inter(sanepic_ctx *c, const double *host_blk, double *dev_inter) {
+  ACHK(c, dalloc(c, c->tmpblk, 3 * c->npix));
+  HIPCHK(c, hipMemcpyAsync(c->tmpblk, host_blk, 3 * c->npix * sizeof(double), hipMemcpyHostToDevice,
+                           c->st));
+  LCHK(c, launch_block_to_inter(c->tmpblk, dev_inter, c->npix, c->st));
+  return 0;
+}
+
+int from_inter(sanepic_ctx *c, const double *dev_inter, double *host_blk) {
+  ACHK(c, dalloc(c, c->tmpblk, 3 * c->npix));
+  LCHK(c, launch_inter_to_block(dev_inter, c->tmpblk, c->npix, c->st));
+  HIPCHK(c, hipMemcpyAsync(host_blk, c->tmpblk, 3 * c->npix * sizeof(double), hipMemcpyDeviceToHost,
+                           c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return 0;
+}
+
+// Device -> host copy ordered after all work queued on the context's stream
+// (the stream is non-blocking, so a null-stream hipMemcpy would not wait).
+int d2h(sanepic_ctx *c, void *dst, const void *src, size_t bytes) {
+  HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return 0;
+}
+
+double pixel_eps() {
+  const char *e = getenv("SANEPIC_PIXEL_EPS");
+  if (e && *e) return atof(e);
+  return 1e-8;
+}
+
+}  // namespace
+
+// =====================================================================  ABI
+
+extern "C" {
+
+int sanepic_abi_version(void) { return SANEPIC_ABI_VERSION; }
+
+const char *sanepic_last_error(const sanepic_ctx *ctx) {
+  if (!ctx) return g_create_error.c_str();
+  return ctx->err.c_str();
+}
+
+int sanepic_create(const sanepic_config *cfg, sanepic_ctx **out) {
+  g_create_error.clear();
+  if (!cfg || !out) {
+    g_create_error = "null argument";
+    return SANEPIC_EINVAL;
+  }
+  *out = nullptr;
+  const int64_t ns = cfg->ns;
+  if (cfg->nside < 1 || cfg->nside > 8192) {
+    g_create_error = "nside out of range: " + std::to_string(cfg->nside);
+    return SANEPIC_EINVAL;
+  }
+  if (cfg->ndet < 1 || cfg->nseg < 1) {
+    g_create_error = "ndet and nseg must be >= 1";
+    return SANEPIC_EINVAL;
+  }
+  if (ns < 16 || ns > (int64_t(1) << 24) || (ns & (ns - 1)) != 0) {
+    g_create_error = "ns must be a power of two in [16, 2^24], got " + std::to_string(ns);
+    return SANEPIC_EINVAL;
+  }
+  if (cfg->polar != 1) {
+    g_create_error = "only -p 1 is supported (the reference reads psi unconditionally)";
+    return SANEPIC_EINVAL;
+  }
+  if (cfg->precision != SANEPIC_FP64 && cfg->precision != SANEPIC_MIXED) {
+    g_create_error = "unknown precision";
+    return SANEPIC_EINVAL;
+  }
+  const int64_t ntot = cfg->ndet * cfg->nseg * ns;
+  if (ntot >= (int64_t(1) << 31)) {
+    g_create_error = "more than 2^31 samples are not supported";
+    return SANEPIC_EINVAL;
+  }
+  if (hipSetDevice(cfg->device) != hipSuccess) {
+    g_create_error = "hipSetDevice failed";
+    return SANEPIC_EHIP;
+  }
+  sanepic_ctx *c = new sanepic_ctx();
+  c->cfg = *cfg;
+  c->ns = ns;
+  c->nseg = cfg->nseg;
+  c->ndet = cfg->ndet;
+  c->nside = cfg->nside;
+  c->ntot = ntot;
+  if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
+    g_create_error = "hipStreamCreate failed";
+    delete c;
+    return SANEPIC_EHIP;
+  }
+  const int l = ilog2(ns);
+  PairGeom &g = c->g;
+  g.ppd = (c->nseg + 1) / 2;
+  g.npairs = c->ndet * g.ppd;
+  g.nseg = c->nseg;
+  g.N = ns;
+  g.log2N = l;
+  if (l <= 12) {
+    c->single = true;
+    g.N1 = ns;
+    g.N2 = 1;
+    g.log2N1 = l;
+    g.log2N2 = 0;
+  } else {
+    g.log2N2 = std::max(4, l - 12);
+    g.log2N1 = l - g.log2N2;
+    g.N1 = int64_t(1) << g.log2N1;
+    g.N2 = int64_t(1) << g.log2N2;
+  }
+  c->gauto = g;
+  c->gauto.ppd = 1;
+  c->gauto.nseg = 1;
+  c->gauto.npairs = c->ndet;
+  c->detc.resize(c->ndet);
+  c->offx.resize(c->ndet);
+  c->offy.resize(c->ndet);
+  c->spn.assign((size_t)c->ndet * ns, 0.0);
+  c->det_ok.assign(c->ndet, 0);
+  c->pt_ok.assign(c->nseg, 0);
+  c->tod_ok.assign((size_t)c->ndet * c->nseg, 0);
+  int rc = dalloc(c, c->ra, c->nseg * ns);
+  if (!rc) rc = dalloc(c, c->dec, c->nseg * ns);
+  if (!rc) rc = dalloc(c, c->psi, c->nseg * ns);
+  if (!rc) rc = dalloc(c, c->sc, 1);
+  if (!rc) rc = dalloc(c, c->zero_flag, 1);
+  if (rc) {
+    g_create_error = c->err;
+    sanepic_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return SANEPIC_OK;
+}
+
+void sanepic_destroy(sanepic_ctx *c) {
+  if (!c) return;
+  if (c->st) (void)hipStreamSynchronize(c->st);
+  for (auto &e : c->evs) {
+    (void)hipEventDestroy(std::get<1>(e));
+    (void)hipEventDestroy(std::get<2>(e));
+  }
+  for (void *p : c->allocs) (void)hipFree(p);
+  if (c->st) (void)hipStreamDestroy(c->st);
+  delete c;
+}
+
+int sanepic_set_detector(sanepic_ctx *c, int64_t idet, double lel_deg, double xel_deg, double angle_deg,
+                         const double *ispf, int64_t nssp) {
+  if (!c) return SANEPIC_EINVAL;
+  if (idet < 0 || idet >= c->ndet) return fail(c, SANEPIC_EINVAL, "detector index out of range");
+  if (!ispf || nssp < 2) return fail(c, SANEPIC_EINVAL, "iSpf needs at least 2 values");
+  const int64_t ns = c->ns;
+  if ((int64_t)(double(ns - 1) / nssp * ns) >= nssp)
+    return fail(c, SANEPIC_EINVAL, "iSpf too short for ns: the reference would read past it (533-535)");
+  DetConst &d = c->detc[idet];
+  double offx, offy;
+  host_det_const(lel_deg, xel_deg, angle_deg, &offx, &offy, &d.psidet, &d.th_, &d.ph_, &d.sin_th,
+                 &d.cos_th, &d.drp2);
+  c->offx[idet] = offx;
+  c->offy[idet] = offy;
+  host_spn_row(ispf, nssp, ns, c->spn.data() + idet * ns);
+  c->det_ok[idet] = 1;
+  return SANEPIC_OK;
+}
+
+int sanepic_set_pointing(sanepic_ctx *c, int64_t iseg, const double *ra, const double *dec,
+                         const double *psi) {
+  if (!c) return SANEPIC_EINVAL;
+  if (iseg < 0 || iseg >= c->nseg) return fail(c, SANEPIC_EINVAL, "segment index out of range");
+  if (!c->ra) return fail(c, SANEPIC_EINVAL, "pointing already consumed by sanepic_pixelize");
+  const size_t nb = (size_t)c->ns * sizeof(double);
+  HIPCHK(c, hipMemcpy(c->ra + iseg * c->ns, ra, nb, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->dec + iseg * c->ns, dec, nb, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->psi + iseg * c->ns, psi, nb, hipMemcpyHostToDevice));
+  c->pt_ok[iseg] = 1;
+  return SANEPIC_OK;
+}
+
+int sanepic_set_tod(sanepic_ctx *c, int64_t idet, int64_t iseg, const double *tod) {
+  if (!c) return SANEPIC_EINVAL;
+  if (idet < 0 || idet >= c->ndet || iseg < 0 || iseg >= c->nseg)
+    return fail(c, SANEPIC_EINVAL, "TOD index out of range");
+  if (c->setup_done) return fail(c, SANEPIC_EINVAL, "TOD already consumed by sanepic_setup");
+  ACHK(c, dalloc(c, c->tod, c->ntot));
+  HIPCHK(c, hipMemcpy(c->tod + (idet * c->nseg + iseg) * c->ns, tod, (size_t)c->ns * sizeof(double),
+                      hipMemcpyHostToDevice));
+  c->tod_ok[idet * c->nseg + iseg] = 1;
+  return SANEPIC_OK;
+}
+
+int sanepic_pixelize(sanepic_ctx *c, int64_t *npix_out, int64_t *n_deferred) {
+  if (!c) return SANEPIC_EINVAL;
+  if (c->pixelized) return fail(c, SANEPIC_EINVAL, "sanepic_pixelize already ran");
+  for (int64_t i = 0; i < c->ndet; i++)
+    if (!c->det_ok[i]) return fail(c, SANEPIC_EINVAL, "detector " + std::to_string(i) + " not set");
+  for (int64_t i = 0; i < c->nseg; i++)
+    if (!c->pt_ok[i]) return fail(c, SANEPIC_EINVAL, "pointing of segment " + std::to_string(i) + " not set");
+  const int64_t ntot = c->ntot, nsky = 12 * c->nside * c->nside;
+  ACHK(c, dalloc(c, c->pix, ntot));
+  ACHK(c, dalloc(c, c->cs, ntot));
+  if (c->cfg.hwp) ACHK(c, dalloc(c, c->csfp, ntot));
+  ACHK(c, dalloc(c, c->d_det, c->ndet));
+  HIPCHK(c, hipMemcpy(c->d_det, c->detc.data(), c->ndet * sizeof(DetConst), hipMemcpyHostToDevice));
+  const int64_t cap = std::min<int64_t>(ntot, int64_t(1) << 20);
+  FlagRec *flags = nullptr;
+  unsigned long long *nflag = nullptr;
+  ACHK(c, dalloc(c, flags, cap));
+  ACHK(c, dalloc(c, nflag, 1));
+  HIPCHK(c, hipMemsetAsync(nflag, 0, sizeof(unsigned long long), c->st));
+  PixelArgs pa{};
+  pa.ntot = ntot;
+  pa.ns = c->ns;
+  pa.nseg = c->nseg;
+  pa.nside = (int)c->nside;
+  pa.hwp = c->cfg.hwp;
+  pa.omega = c->cfg.omega_hwp;
+  pa.eps = pixel_eps();
+  pa.ra = c->ra;
+  pa.dec = c->dec;
+  pa.psi = c->psi;
+  pa.det = c->d_det;
+  pa.pix = c->pix;
+  pa.cs = c->cs;
+  pa.csfp = c->csfp;
+  pa.flags = flags;
+  pa.nflag = nflag;
+  pa.flag_cap = cap;
+  LCHK(c, launch_pixelize(pa, c->st));
+  unsigned long long nf = 0;
+  HIPCHK(c, hipMemcpyAsync(&nf, nflag, sizeof(nf), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  if ((int64_t)nf > cap)
+    return fail(c, SANEPIC_EDATA, "too many samples need host resolution: " + std::to_string(nf));
+  // host resolution of ambiguous / invalid samples, in the reference's loop
+  // order so the previous-pixel rule (862-872) chains correctly
+  std::vector<FlagRec> fl(nf);
+  if (nf) ACHK(c, d2h(c, fl.data(), flags, nf * sizeof(FlagRec)));
+  std::sort(fl.begin(), fl.end(), [](const FlagRec &a, const FlagRec &b) { return a.g < b.g; });
+  std::vector<int64_t> fidx(nf);
+  std::vector<int32_t> fval(nf);
+  std::unordered_map<int64_t, int32_t> resolved;
+  int64_t nbad = 0;
+  for (size_t i = 0; i < nf; i++) {
+    const FlagRec &f = fl[i];
+    const int64_t det = f.g / (c->ns * c->nseg);
+    long v = host_sample_pixel(c->nside, c->offx[det], c->offy[det], f.ra, f.dec, f.psi);
+    if (v < 0) {
+      nbad++;
+      if (f.g == 0) {
+        v = 0;
+      } else {
+        auto it = resolved.find(f.g - 1);
+        if (it != resolved.end()) {
+          v = it->second;
+        } else {
+          int32_t pv = 0;
+          ACHK(c, d2h(c, &pv, c->pix + (f.g - 1), sizeof(int32_t)));
+          v = pv;
+        }
+      }
+    }
+    resolved[f.g] = (int32_t)v;
+    fidx[i] = f.g;
+    fval[i] = (int32_t)v;
+  }
+  c->n_bad = nbad;
+  c->n_deferred = (int64_t)nf;
+  if (nf) {
+    int64_t *didx = nullptr;
+    int32_t *dval = nullptr;
+    ACHK(c, dalloc(c, didx, nf));
+    ACHK(c, dalloc(c, dval, nf));
+    HIPCHK(c, hipMemcpy(didx, fidx.data(), nf * sizeof(int64_t), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(dval, fval.data(), nf * sizeof(int32_t), hipMemcpyHostToDevice));
+    LCHK(c, launch_set_values(c->pix, didx, dval, nf, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    dfree(c, didx, nf);
+    dfree(c, dval, nf);
+  }
+  dfree(c, flags, cap);
+  dfree(c, nflag, 1);
+  dfree(c, c->ra, c->nseg * c->ns);
+  dfree(c, c->dec, c->nseg * c->ns);
+  dfree(c, c->psi, c->nseg * c->ns);
+  // seen pixels (pixon, 873) and the compaction indpix (904-915)
+  int32_t *counts = nullptr;
+  ACHK(c, dalloc(c, counts, nsky));
+  HIPCHK(c, hipMemsetAsync(counts, 0, nsky * sizeof(int32_t), c->st));
+  LCHK(c, launch_histogram(c->pix, ntot, counts, c->st));
+  std::vector<int32_t> hc(nsky);
+  HIPCHK(c, hipMemcpyAsync(hc.data(), counts, nsky * sizeof(int32_t), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  c->h_indpix.assign(nsky, -1);
+  c->h_pixlist.clear();
+  c->h_cnt.clear();
+  for (int64_t i = 0; i < nsky; i++)
+    if (hc[i] != 0) {
+      c->h_indpix[i] = (int32_t)c->h_pixlist.size();
+      c->h_pixlist.push_back((int32_t)i);
+      c->h_cnt.push_back(hc[i]);
+    }
+  const int64_t npix = (int64_t)c->h_pixlist.size();
+  c->npix = npix;
+  HIPCHK(c, hipMemcpy(counts, c->h_indpix.data(), nsky * sizeof(int32_t), hipMemcpyHostToDevice));
+  LCHK(c, launch_compress_pix(c->pix, ntot, counts, c->st));
+  ACHK(c, dalloc(c, c->cnt, npix));
+  HIPCHK(c, hipMemcpy(c->cnt, c->h_cnt.data(), npix * sizeof(int32_t), hipMemcpyHostToDevice));
+  std::vector<int64_t> off(npix + 1, 0);
+  for (int64_t p = 0; p < npix; p++) off[p + 1] = off[p] + c->h_cnt[p];
+  ACHK(c, dalloc(c, c->off, npix + 1));
+  HIPCHK(c, hipMemcpy(c->off, off.data(), (npix + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  dfree(c, counts, nsky);
+  // stable sort of samples by compressed pixel -> perm
+  {
+    uint32_t *keys_out = nullptr;
+    uint32_t *vals_in = nullptr;
+    ACHK(c, dalloc(c, keys_out, ntot));
+    ACHK(c, dalloc(c, vals_in, ntot));
+    ACHK(c, dalloc(c, c->perm, ntot));
+    LCHK(c, launch_iota_u32(vals_in, ntot, c->st));
+    const int end_bit = std::max(1, ilog2(npix + 1));
+    size_t tbytes = 0;
+    HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(nullptr, tbytes, (const uint32_t *)c->pix, keys_out, vals_in, c->perm,
+                                                 (int)ntot, 0, end_bit, c->st));
+    char *tmp = nullptr;
+    ACHK(c, dalloc(c, tmp, (int64_t)tbytes));
+    HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(tmp, tbytes, (const uint32_t *)c->pix, keys_out, vals_in, c->perm, (int)ntot,
+                                                 0, end_bit, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    dfree(c, tmp, (int64_t)tbytes);
+    dfree(c, keys_out, ntot);
+    dfree(c, vals_in, ntot);
+  }
+  ACHK(c, dalloc(c, c->cs_sorted, ntot));
+  LCHK(c, launch_gather_cs(c->perm, c->cs, c->cs_sorted, ntot, c->st));
+  if (c->csfp) {
+    ACHK(c, dalloc(c, c->csfp_sorted, ntot));
+    LCHK(c, launch_gather_cs(c->perm, c->csfp, c->csfp_sorted, ntot, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    dfree(c, c->csfp, ntot);
+  }
+  if (c->cfg.precision == SANEPIC_MIXED) {
+    ACHK(c, dalloc(c, c->cs32, ntot));
+    ACHK(c, dalloc(c, c->cs_sorted32, ntot));
+    LCHK(c, launch_cs_to_float(c->cs, c->cs32, ntot, c->st));
+    LCHK(c, launch_gather_cs_f(c->perm, c->cs, c->cs_sorted32, ntot, c->st));
+  }
+  // P^T work items: pixels, split into <= kChunk-entry chunks
+  std::vector<int32_t> ipix, ilen, islot, spix, sfirst, scount;
+  std::vector<int64_t> ibeg;
+  int32_t nslots = 0;
+  for (int64_t p = 0; p < npix; p++) {
+    const int64_t n = c->h_cnt[p];
+    if (n <= kChunk) {
+      ipix.push_back((int32_t)p);
+      ibeg.push_back(off[p]);
+      ilen.push_back((int32_t)n);
+      islot.push_back(-1);
+    } else {
+      spix.push_back((int32_t)p);
+      sfirst.push_back(nslots);
+      int32_t k = 0;
+      for (int64_t e = 0; e < n; e += kChunk, k++) {
+        ipix.push_back((int32_t)p);
+        ibeg.push_back(off[p] + e);
+        ilen.push_back((int32_t)std::min<int64_t>(kChunk, n - e));
+        islot.push_back(nslots++);
+      }
+      scount.push_back(k);
+    }
+  }
+  const int64_t nit = (int64_t)ipix.size(), nsp = (int64_t)spix.size();
+  ACHK(c, dalloc(c, c->it_pix, nit));
+  ACHK(c, dalloc(c, c->it_beg, nit));
+  ACHK(c, dalloc(c, c->it_len, nit));
+  ACHK(c, dalloc(c, c->it_slot, nit));
+  ACHK(c, dalloc(c, c->sp_pix, nsp));
+  ACHK(c, dalloc(c, c->sp_first, nsp));
+  ACHK(c, dalloc(c, c->sp_count, nsp));
+  HIPCHK(c, hipMemcpy(c->it_pix, ipix.data(), nit * 4, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->it_beg, ibeg.data(), nit * 8, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->it_len, ilen.data(), nit * 4, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->it_slot, islot.data(), nit * 4, hipMemcpyHostToDevice));
+  if (nsp) {
+    HIPCHK(c, hipMemcpy(c->sp_pix, spix.data(), nsp * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->sp_first, sfirst.data(), nsp * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->sp_count, scount.data(), nsp * 4, hipMemcpyHostToDevice));
+  }
+  c->items = Items{c->it_pix, c->it_beg, c->it_len, c->it_slot, nit, c->sp_pix, c->sp_first, c->sp_count, nsp};
+  c->nslots = nslots;
+  ACHK(c, dalloc(c, c->partial, 6 * std::max<int64_t>(1, nslots)));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  c->pixelized = true;
+  if (npix_out) *npix_out = npix;
+  if (n_deferred) *n_deferred = c->n_deferred;
+  return SANEPIC_OK;
+}
+
+int sanepic_get_pixels(sanepic_ctx *c, int64_t idet, int64_t iseg, int64_t *out) {
+  ACHK(c, check_ready(c, false));
+  if (idet < 0 || idet >= c->ndet || iseg < 0 || iseg >= c->nseg)
+    return fail(c, SANEPIC_EINVAL, "index out of range");
+  std::vector<int32_t> h(c->ns);
+  ACHK(c, d2h(c, h.data(), c->pix + (idet * c->nseg + iseg) * c->ns, c->ns * 4));
+  for (int64_t i = 0; i < c->ns; i++) out[i] = c->h_pixlist[h[i]];
+  return SANEPIC_OK;
+}
+
+int sanepic_get_indpix(sanepic_ctx *c, int64_t *indpix) {
+  ACHK(c, check_ready(c, false));
+  for (size_t i = 0; i < c->h_indpix.size(); i++) indpix[i] = c->h_indpix[i];
+  return SANEPIC_OK;
+}
+
+int sanepic_setup(sanepic_ctx *c) {
+  ACHK(c, check_ready(c, false));
+  if (c->setup_done) return fail(c, SANEPIC_EINVAL, "sanepic_setup already ran");
+  for (size_t i = 0; i < c->tod_ok.size(); i++)
+    if (!c->tod_ok[i]) return fail(c, SANEPIC_EINVAL, "TOD block " + std::to_string(i) + " not set");
+  const int64_t npix = c->npix, ntot = c->ntot, N = c->ns;
+  ACHK(c, build_tables(c));
+  ACHK(c, dalloc(c, c->z, ntot));
+  ACHK(c, dalloc(c, c->dspec, c->g.npairs * N));
+  if (!c->single) ACHK(c, dalloc(c, c->buf, c->g.npairs * N));
+  const int64_t nchi = c->single ? c->g.npairs : c->g.npairs * (N / 4096);
+  ACHK(c, dalloc(c, c->chi2part, nchi + 1));
+  c->nchi2 = nchi;
+  ACHK(c, dalloc(c, c->b_raw, 3 * npix));
+  ACHK(c, dalloc(c, c->b, 3 * npix));
+  // b = P^T N^-1 d  (1069-1125); the data spectrum is kept for chi2
+  LCHK(c, launch_tod_nanfill(c->tod, ntot, c->ns, c->st));
+  {
+    OpArgs<double> a = op_args<double>(c);
+    ACHK(c, run_filter<double>(c, a, SRC_TOD, ROW_FWD | ROW_STORE_D | ROW_FILTER | ROW_INV, nullptr));
+    ACHK(c, run_scatter<double>(c, c->z, c->b_raw, 0));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  dfree(c, c->tod, ntot);
+  // noise autocorrelation N_ = c2r(|SpN|) per detector (2698-2706)
+  {
+    std::vector<double> wabs((size_t)c->ndet * N);
+    ACHK(c, d2h(c, wabs.data(), c->wt, wabs.size() * sizeof(double)));
+    for (auto &v : wabs) v = std::fabs(v);
+    double *dwabs = nullptr;
+    ACHK(c, dalloc(c, dwabs, c->ndet * N));
+    HIPCHK(c, hipMemcpy(dwabs, wabs.data(), wabs.size() * sizeof(double), hipMemcpyHostToDevice));
+    ACHK(c, dalloc(c, c->nauto, c->ndet * N));
+    OpArgs<double> a = op_args<double>(c);
+    a.g = c->gauto;
+    a.wt = dwabs;
+    a.z = c->nauto;
+    a.mode = ROW_SRC_WT | ROW_INV;
+    if (c->single) {
+      LCHK(c, launch_single<double>(a, SRC_GATHER, c->st, nullptr));
+    } else {
+      LCHK(c, launch_row<double>(a, c->st, nullptr));
+      LCHK(c, launch_col_inv<double>(a, c->st));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    dfree(c, dwabs, c->ndet * N);
+  }
+  // crosslink sums (first do_PtNd call of 1258, psipFP) and pair sums
+  ACHK(c, dalloc(c, c->cross, 2 * npix));
+  LCHK(c, launch_crosslink(c->items, c->csfp_sorted ? c->csfp_sorted : c->cs_sorted, c->cross, c->partial,
+                           c->st));
+  ACHK(c, dalloc(c, c->pairs6, 6 * npix));
+  PairSumArgs pa{};
+  pa.it = c->items;
+  pa.perm = c->perm;
+  pa.cs_sorted = c->cs_sorted;
+  pa.off = c->off;
+  pa.nauto = c->nauto;
+  pa.ns = c->ns;
+  pa.nseg = c->nseg;
+  pa.out = c->pairs6;
+  pa.partial = c->partial;
+  LCHK(c, launch_pairsum(pa, c->st));
+  // masked right-hand side
+  HIPCHK(c, hipMemcpyAsync(c->b, c->b_raw, 3 * npix * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+  LCHK(c, cg_launch_mask(c->b, c->cnt, npix, c->st));
+  // CG vectors
+  ACHK(c, dalloc(c, c->x, 3 * npix));
+  ACHK(c, dalloc(c, c->r, 3 * npix));
+  ACHK(c, dalloc(c, c->d, 3 * npix));
+  ACHK(c, dalloc(c, c->q, 3 * npix));
+  ACHK(c, dalloc(c, c->M, 6 * npix));
+  ACHK(c, dalloc(c, c->nvar, 3 * npix));
+  ACHK(c, dalloc(c, c->part, 4 * 512));
+  if (c->cfg.precision == SANEPIC_MIXED) {
+    if (!c->single) ACHK(c, dalloc(c, c->buf32, c->g.npairs * N));
+    ACHK(c, dalloc(c, c->z32, ntot));
+    ACHK(c, dalloc(c, c->map32, 3 * npix));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  c->setup_done = true;
+  return SANEPIC_OK;
+}
+
+int sanepic_get_setup(sanepic_ctx *c, int64_t *hits, double *mp, double *sumcross) {
+  ACHK(c, check_ready(c, true));
+  const int64_t npix = c->npix;
+  if (hits)
+    for (int64_t p = 0; p < npix; p++) {
+      hits[p] = 2 * (int64_t)c->h_cnt[p];
+      hits[npix + p] = 2 * (int64_t)c->h_cnt[p];
+    }
+  if (mp) {
+    std::vector<double> p6(6 * npix);
+    ACHK(c, d2h(c, p6.data(), c->pairs6, p6.size() * sizeof(double)));
+    for (int64_t p = 0; p < npix; p++) {
+      const double mpI = 2.0 * p6[6 * p];
+      mp[p] = mpI;
+      mp[npix + p] = mpI / 2.0;
+      mp[2 * npix + p] = mpI / 2.0;
+    }
+  }
+  if (sumcross) {
+    std::vector<double> cr(2 * npix);
+    ACHK(c, d2h(c, cr.data(), c->cross, cr.size() * sizeof(double)));
+    for (int64_t p = 0; p < npix; p++) {
+      sumcross[p] = cr[2 * p];
+      sumcross[npix + p] = cr[2 * p + 1];
+      sumcross[2 * npix + p] = 0.0;
+    }
+  }
+  return SANEPIC_OK;
+}
+
+int sanepic_get_rhs(sanepic_ctx *c, double *bout) {
+  ACHK(c, check_ready(c, true));
+  return from_inter(c, c->b_raw, bout);
+}
+
+int sanepic_apply(sanepic_ctx *c, const double *xin, double *yout) {
+  ACHK(c, check_ready(c, true));
+  double *xd = nullptr, *yd = nullptr;
+  ACHK(c, dalloc(c, xd, 3 * c->npix));
+  ACHK(c, dalloc(c, yd, 3 * c->npix));
+  int rc = to_inter(c, xin, xd);
+  if (!rc) rc = apply_A(c, xd, yd, 0, false, c->cfg.precision == SANEPIC_FP64);
+  if (!rc) rc = from_inter(c, yd, yout);
+  dfree(c, xd, 3 * c->npix);
+  dfree(c, yd, 3 * c->npix);
+  return rc;
+}
+
+int sanepic_chi2(sanepic_ctx *c, const double *xin, double *chi2) {
+  ACHK(c, check_ready(c, true));
+  double *xd = nullptr, *yd = nullptr;
+  ACHK(c, dalloc(c, xd, 3 * c->npix));
+  ACHK(c, dalloc(c, yd, 3 * c->npix));
+  int rc = to_inter(c, xin, xd);
+  if (!rc) rc = apply_A(c, xd, yd, 0, true, true);
+  CGScalars h{};
+  if (!rc) {
+    rc = d2h(c, &h, c->sc, sizeof(h));
+  }
+  dfree(c, xd, 3 * c->npix);
+  dfree(c, yd, 3 * c->npix);
+  if (!rc) *chi2 = h.chi2;
+  return rc;
+}
+
+static CGArgs cg_args(sanepic_ctx *c) {
+  CGArgs a{};
+  a.npix = c->npix;
+  a.x = c->x;
+  a.r = c->r;
+  a.d = c->d;
+  a.q = c->q;
+  a.b = c->b;
+  a.M = c->M;
+  a.cnt = c->cnt;
+  a.sc = c->sc;
+  a.part = c->part;
+  a.nblocks = 512;
+  return a;
+}
+
+int sanepic_solve_begin(sanepic_ctx *c, const sanepic_solve_opts *opts, const double *x0) {
+  ACHK(c, check_ready(c, true));
+  c->precond = opts ? opts->precond : SANEPIC_PRECOND_DIAG;
+  c->refresh = (opts && opts->refresh > 0) ? opts->refresh : 10;
+  if (c->precond != SANEPIC_PRECOND_DIAG && c->precond != SANEPIC_PRECOND_BLOCK3)
+    return fail(c, SANEPIC_EINVAL, "unknown preconditioner");
+  const int64_t npix = c->npix;
+  HIPCHK(c, hipMemsetAsync(c->zero_flag, 0, sizeof(int), c->st));
+  LCHK(c, cg_launch_precond(c->pairs6, c->cnt, npix, c->precond == SANEPIC_PRECOND_DIAG ? SANEPIC_PRECOND_DIAG_K
+                                                                                        : SANEPIC_PRECOND_BLOCK3_K,
+                            c->M, c->nvar, c->zero_flag, c->st));
+  int zf = 0;
+  HIPCHK(c, hipMemcpyAsync(&zf, c->zero_flag, sizeof(int), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  if (zf) return fail(c, SANEPIC_EDATA, "ERROR: Mp has elements = 0 (sanepic.cc:1344-1348)");
+  HIPCHK(c, hipMemsetAsync(c->q, 0, 3 * npix * sizeof(double), c->st));
+  if (x0) {
+    ACHK(c, to_inter(c, x0, c->x));
+    ACHK(c, apply_A(c, c->x, c->q, 1, false, true));
+    LCHK(c, cg_launch_mask(c->x, c->cnt, npix, c->st));
+  } else {
+    HIPCHK(c, hipMemsetAsync(c->x, 0, 3 * npix * sizeof(double), c->st));
+  }
+  CGArgs a = cg_args(c);
+  LCHK(c, cg_launch_init(a, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  c->iter = 0;
+  c->begun = true;
+  return SANEPIC_OK;
+}
+
+int sanepic_solve_step(sanepic_ctx *c, sanepic_iter_info *info) {
+  ACHK(c, check_ready(c, true));
+  if (!c->begun) return fail(c, SANEPIC_EINVAL, "sanepic_solve_begin has not run");
+  CGArgs a = cg_args(c);
+  ACHK(c, apply_A(c, c->d, c->q, 1, false, false));
+  LCHK(c, cg_launch_alpha(a, c->st));
+  const bool refresh = (c->iter % c->refresh) == 0;
+  LCHK(c, cg_launch_update(a, refresh ? 1 : 0, c->st));
+  if (refresh) {
+    ACHK(c, apply_A(c, c->x, c->q, 1, true, true));
+    LCHK(c, cg_launch_refresh_r(a, c->st));
+  }
+  LCHK(c, cg_launch_dir(a, c->st));
+  CGScalars h{};
+  HIPCHK(c, hipMemcpyAsync(&h, c->sc, sizeof(h), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  if (info) {
+    info->iter = c->iter;
+    info->crit = h.var_n / h.var00;
+    info->crit2 = h.delta_n / h.delta0;
+    info->chi2 = h.chi2;
+    info->alpha = h.alpha;
+    info->beta = h.beta;
+  }
+  c->iter++;
+  return SANEPIC_OK;
+}
+
+int sanepic_get_map(sanepic_ctx *c, double *xout) {
+  ACHK(c, check_ready(c, true));
+  if (!c->begun) return fail(c, SANEPIC_EINVAL, "sanepic_solve_begin has not run");
+  return from_inter(c, c->x, xout);
+}
+
+int sanepic_get_nvar(sanepic_ctx *c, double *nv) {
+  ACHK(c, check_ready(c, true));
+  if (!c->begun) return fail(c, SANEPIC_EINVAL, "sanepic_solve_begin has not run");
+  return from_inter(c, c->nvar, nv);
+}
+
+int sanepic_get_sumcross(sanepic_ctx *c, double *sumcross) {
+  ACHK(c, sanepic_get_setup(c, nullptr, nullptr, sumcross));
+  const int64_t npix = c->npix;
+  for (int64_t p = 0; p < npix; p++)
+    if (c->h_cnt[p] < 2) sumcross[npix + p] = 0.0;  // 1316-1330 masks [npix, 3 npix)
+  return SANEPIC_OK;
+}
+
+int sanepic_true_residual(sanepic_ctx *c, double *rel) {
+  ACHK(c, check_ready(c, true));
+  if (!c->begun) return fail(c, SANEPIC_EINVAL, "sanepic_solve_begin has not run");
+  CGArgs a = cg_args(c);
+  ACHK(c, apply_A(c, c->x, c->q, 1, false, true));
+  LCHK(c, cg_launch_resid(a, c->st));
+  CGScalars h{};
+  HIPCHK(c, hipMemcpyAsync(&h, c->sc, sizeof(h), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  *rel = h.tmp[1] > 0 ? std::sqrt(h.tmp[0] / h.tmp[1]) : 0.0;
+  return SANEPIC_OK;
+}
+
+int sanepic_set_profiling(sanepic_ctx *c, int32_t on) {
+  if (!c) return SANEPIC_EINVAL;
+  ACHK(c, prof_collect(c));
+  c->prof = on != 0;
+  for (int i = 0; i < kProfClasses; i++) {
+    c->kt_ms[i] = 0.0;
+    c->kt_n[i] = 0;
+  }
+  return SANEPIC_OK;
+}
+
+int sanepic_kernel_time(sanepic_ctx *c, int32_t which, double *ms, int64_t *count) {
+  if (!c) return SANEPIC_EINVAL;
+  if (which < 0 || which >= kProfClasses) return fail(c, SANEPIC_EINVAL, "kernel class out of range");
+  ACHK(c, prof_collect(c));
+  if (ms) *ms = c->kt_ms[which];
+  if (count) *count = c->kt_n[which];
+  return SANEPIC_OK;
+}
+
+int sanepic_device_bytes(sanepic_ctx *c, int64_t *bytes) {
+  if (!c) return SANEPIC_EINVAL;
+  *bytes = c->bytes;
+  return SANEPIC_OK;
+}
+
+}  // extern "C"

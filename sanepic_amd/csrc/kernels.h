@@ -1,0 +1,186 @@
+// kernels.h -- launch interfaces shared by the HIP translation units of
+// libsanepic_hip.so.  Host orchestration lives in capi.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+// Per-detector constants of pointingshift (sanepic.cc:3419, 3435), evaluated
+// on the host with the platform libm so they are bit-identical to the
+// reference's values.
+struct DetConst {
+  double th_, ph_, sin_th, cos_th, drp2, psidet;
+};
+
+// A sample whose pixel the device could not decide (or whose angles are
+// NaN): the host recomputes it with glibc.
+struct FlagRec {
+  int64_t g;
+  double ra, dec, psi;  // boresight values, psi already NaN-filled
+  int64_t bad;
+};
+
+struct PixelArgs {
+  int64_t ntot, ns, nseg;
+  int nside;
+  int hwp;
+  double omega;
+  double eps;
+  const double *ra, *dec, *psi;  // boresight [nseg][ns]
+  const DetConst *det;
+  int32_t *pix;                  // out: RING pixel, -1 if deferred as bad
+  double2 *cs;                   // out: cos 2psi, sin 2psi
+  double2 *csfp;                 // out (HWP only): the same for psipFP
+  FlagRec *flags;
+  unsigned long long *nflag;
+  int64_t flag_cap;
+};
+
+int launch_pixelize(const PixelArgs &a, hipStream_t s);
+
+// Geometry of the paired transform: segment pairs (2j, 2j+1) of one detector
+// share one complex FFT of length N = N1 * N2 (real part = even segment,
+// imaginary part = odd segment).  A missing odd segment reads as zero.
+struct PairGeom {
+  int64_t npairs;  // total pairs
+  int64_t ppd;     // pairs per detector
+  int64_t nseg;    // segments per detector
+  int64_t N;       // segment length
+  int64_t N1, N2;  // row length, column length (N2 == 1: single pass)
+  int log2N, log2N1, log2N2;
+};
+
+// Operator pass selection.
+enum : int {
+  SRC_GATHER = 0,  // Ps = I + cos2psi Q + sin2psi U from the map
+  SRC_TOD = 1,     // the TOD (right-hand side)
+};
+
+enum : int {
+  ROW_FWD = 1,      // forward transform of the row
+  ROW_STORE_D = 2,  // keep the spectrum (data FFT for chi2)
+  ROW_CHI2 = 4,     // accumulate chi2 against the stored spectrum
+  ROW_FILTER = 8,   // multiply by the noise weights
+  ROW_INV = 16,     // inverse transform of the row
+  ROW_SRC_WT = 32,  // input = weights themselves (noise autocorrelation)
+};
+
+template <typename T>
+struct OpArgs {
+  using V = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
+  PairGeom g;
+  const int32_t *pix;   // [ntot] compressed pixel
+  const V *cs;          // [ntot] cos 2psi, sin 2psi
+  const T *map;         // [npix][3] interleaved I, Q, U
+  const double *tod;    // [ntot] (SRC_TOD)
+  V *buf;               // [npairs][N] four-step work buffer
+  V *dspec;             // [npairs][N] stored data spectrum
+  const T *wt;          // [ndet][N] weights in row layout: wt[k2*N1 + k1] = W[k2 + N2 k1]
+  T *z;                 // [ntot] time-domain output
+  const V *tw4096;      // w_4096^j
+  const V *twlo, *twhi; // w_N^j = twlo[j & 4095] * twhi[j >> 12]
+  double *chi2part;     // per-block chi2 partials
+  int mode;
+};
+
+template <typename T> int launch_col_fwd(const OpArgs<T> &a, int src, hipStream_t s);
+template <typename T> int launch_row(const OpArgs<T> &a, hipStream_t s, int64_t *nblocks);
+template <typename T> int launch_col_inv(const OpArgs<T> &a, hipStream_t s);
+template <typename T> int launch_single(const OpArgs<T> &a, int src, hipStream_t s, int64_t *nblocks);
+
+// Pixel-sorted reduction (P^T): items are (pixel, entry range) chunks of the
+// pixel-sorted sample list; a pixel larger than one chunk is split and its
+// partial sums are combined by the fix-up kernel in item order.
+struct Items {
+  const int32_t *pix;     // [nitems] pixel of the item
+  const int64_t *beg;     // [nitems] first sorted entry
+  const int32_t *len;     // [nitems] entries
+  const int32_t *slot;    // [nitems] -1 = whole pixel, else partial slot
+  int64_t n;
+  const int32_t *split_pix;    // [nsplit] pixels that were split
+  const int32_t *split_first;  // [nsplit] first partial slot
+  const int32_t *split_count;  // [nsplit] number of partial slots
+  int64_t nsplit;
+};
+
+template <typename T>
+struct ScatterArgs {
+  using V = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
+  Items it;
+  const uint32_t *perm;   // sorted entry -> sample
+  const V *cs_sorted;     // sorted entry -> cos 2psi, sin 2psi
+  const T *z;             // [ntot] time-domain values
+  const int32_t *cnt;     // [npix] samples per pixel (mask: cnt < 2)
+  double *q;              // [npix][3] output
+  double *partial;        // [nslots][3]
+  int mask;               // apply the Q/U mask
+};
+
+template <typename T> int launch_scatter(const ScatterArgs<T> &a, hipStream_t s);
+
+// Sums over each pixel's entries of (cos 2psi, sin 2psi): the crosslink maps.
+int launch_crosslink(const Items &it, const double2 *cs_sorted, double *out /*[npix][2]*/,
+                     double *partial, hipStream_t s);
+
+// Pair sums of the noise autocorrelation inside each (pixel, segment) group:
+// out[p] = {sum N_, sum N_ c_k, sum N_ s_k, sum c_j N_ c_k, c_j N_ s_k, s_j N_ s_k}
+// over pairs (j, k) with |t_j - t_k| < ns/2 (compute_diagPtNPCorr, 2680-2762,
+// extended to the full 3x3 block).
+struct PairSumArgs {
+  Items it;
+  const uint32_t *perm;
+  const double2 *cs_sorted;
+  const int64_t *off;     // [npix+1] pixel offsets into the sorted list
+  const double *nauto;    // [ndet][ns] c2r(|SpN|)
+  int64_t ns, nseg;
+  double *out;            // [npix][6]
+  double *partial;        // [nslots][6]
+};
+int launch_pairsum(const PairSumArgs &a, hipStream_t s);
+
+// Gathers sorted copies: dst[j] = src[perm[j]].
+int launch_gather_cs(const uint32_t *perm, const double2 *src, double2 *dst, int64_t n, hipStream_t s);
+int launch_gather_cs_f(const uint32_t *perm, const double2 *src, float2 *dst, int64_t n, hipStream_t s);
+int launch_cs_to_float(const double2 *src, float2 *dst, int64_t n, hipStream_t s);
+
+// TOD NaN fill (784-790) on the device, per segment.
+int launch_tod_nanfill(double *tod, int64_t ntot, int64_t ns, hipStream_t s);
+
+// Pixel bookkeeping.
+int launch_histogram(const int32_t *pix, int64_t n, int32_t *counts, hipStream_t s);
+int launch_compress_pix(int32_t *pix, int64_t n, const int32_t *indpix, hipStream_t s);
+int launch_iota_u32(uint32_t *v, int64_t n, hipStream_t s);
+
+// CG on interleaved [npix][3] FP64 vectors with FP64 scalars on the device.
+struct CGScalars {
+  double delta_n, delta_o, var_n, rtq, alpha, beta, var00, delta0, chi2, var0;
+  double tmp[6];
+};
+
+struct CGArgs {
+  int64_t npix;
+  double *x, *r, *d, *q, *b;
+  const double *M;        // [npix][6] symmetric preconditioner blocks
+  const int32_t *cnt;     // mask source
+  CGScalars *sc;
+  double *part;           // [nblocks][4] partials
+  int64_t nblocks;
+};
+
+int cg_launch_mask(double *v, const int32_t *cnt, int64_t npix, hipStream_t s);
+int cg_launch_init(const CGArgs &a, hipStream_t s);      // r = b - q; d = M r; delta, var, var00
+int cg_launch_alpha(const CGArgs &a, hipStream_t s);     // rtq = q.d ; alpha = delta_n / rtq
+int cg_launch_update(const CGArgs &a, int refresh, hipStream_t s);  // x += alpha d; r -= alpha q (or nothing)
+int cg_launch_refresh_r(const CGArgs &a, hipStream_t s); // r = b - q
+int cg_launch_dir(const CGArgs &a, hipStream_t s);       // d = M r + beta d
+int cg_launch_resid(const CGArgs &a, hipStream_t s);     // tmp[0] = |b - q|^2 (q = A x masked), tmp[1] = |b|^2
+enum { SANEPIC_PRECOND_DIAG_K = 0, SANEPIC_PRECOND_BLOCK3_K = 1 };
+int cg_launch_precond(const double *p6, const int32_t *cnt, int64_t npix, int kind, double *M,
+                      double *nvar, int *zero, hipStream_t s);
+int launch_set_values(int32_t *dst, const int64_t *idx, const int32_t *val, int64_t n, hipStream_t s);
+int launch_reduce_chi2(const double *part, int64_t n, CGScalars *sc, hipStream_t s);
+
+// Layout conversion between the ABI's block layout and the device's
+// interleaved layout.
+int launch_block_to_inter(const double *blk, double *inter, int64_t npix, hipStream_t s);
+int launch_inter_to_block(const double *inter, double *blk, int64_t npix, hipStream_t s);
+int launch_d2f(const double *a, float *b, int64_t n, hipStream_t s);

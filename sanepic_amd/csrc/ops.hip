@@ -1,0 +1,346 @@
+// ops.hip -- the per-segment noise filter N^-1 fused with P (gather) on the
+// way in and producing time-ordered output for P^T on the way out.
+//
+// Reference semantics (src/sanepic.cc): write_tfAS (2390-2445) computes
+// r2c(P x) for every (detector, segment); do_PtNd (2489-2637) multiplies by
+// SpN_all (real, bins 0..ns/2) and applies c2r.  Both are unnormalised, so
+// the filtered segment is c2r(SpN . r2c(Ps)).  Because the weights are real
+// and even, two real segments of one detector are filtered together as the
+// real and imaginary parts of one complex sequence: IFFT(W . FFT(a + i b)) =
+// c2r(W r2c a) + i c2r(W r2c b) with W extended to all N bins as W[N-k] = W[k].
+//
+// Long segments use a four-step FFT, N = N1 * N2, x[n1 + N1 n2] viewed as an
+// N2 x N1 row-major matrix M[n2][n1]:
+//   column pass  : DFT over n2 of each column, times w_N^{n1 k2}   -> M[k2][n1]
+//   row pass     : DFT over n1 of row k2 gives X[k2 + N2 k1]; multiply by W;
+//                  inverse DFT over k1; times w_N^{-n1 k2}         -> M[k2][n1]
+//   column pass  : inverse DFT over k2 gives z[n1 + N1 n2]
+// The gather of P runs inside the first column pass and the output of the
+// last one is written in time order for the pixel-sorted P^T reduction.
+#include <type_traits>
+
+#include "fft_core.h"
+#include "kernels.h"
+
+using namespace sfft;
+
+namespace {
+
+template <typename T> using VT = typename Vec2<T>::type;
+
+__device__ __forceinline__ void pair_info(const PairGeom &g, int64_t pr, int &det, int64_t &ga,
+                                          int64_t &gb) {
+  det = (int)(pr / g.ppd);
+  const int64_t j = pr % g.ppd;
+  ga = ((int64_t)det * g.nseg + 2 * j) * g.N;
+  gb = (2 * j + 1 < g.nseg) ? ga + g.N : -1;
+}
+
+// P: Ps = S[m] + polard (S[npix+m] cos 2psi + S[2npix+m] sin 2psi), polard = 1
+// (deproject, 2865-2868); the map is interleaved [npix][3] on the device.
+template <typename T>
+__device__ __forceinline__ T gather_one(const OpArgs<T> &a, int64_t gi) {
+  const int p = a.pix[gi];
+  const VT<T> c = a.cs[gi];
+  const T *m = a.map + 3 * (int64_t)p;
+  return m[0] + (m[1] * c.x + m[2] * c.y);
+}
+
+template <typename T, int SRC>
+__device__ __forceinline__ T load_src(const OpArgs<T> &a, int64_t gi) {
+  if constexpr (SRC == SRC_GATHER) return gather_one<T>(a, gi);
+  else return (T)a.tod[gi];
+}
+
+template <typename V, int DIR>
+__device__ __forceinline__ V twN(const V *__restrict__ lo, const V *__restrict__ hi, int64_t j) {
+  V w = cmul(lo[j & 4095], hi[j >> 12]);
+  return DIR < 0 ? w : cconj(w);
+}
+
+__device__ __forceinline__ double block_sum256(double v, double *red) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0) s = ((red[0] + red[1]) + (red[2] + red[3]));
+  return s;
+}
+
+// ---------------------------------------------------------------- column fwd
+template <typename T, int LOG2N2, int SRC>
+__global__ __launch_bounds__(256) void k_col_fwd(OpArgs<T> a) {
+  using V = VT<T>;
+  constexpr int N2 = 1 << LOG2N2, TS = N2 / 16, B = 256 / TS;
+  __shared__ V lds[LdsSize<LOG2N2, B>::value];
+  const int tid = threadIdx.x, b = tid % B, t = tid / B;
+  const int64_t N1 = a.g.N1;
+  const int64_t ncb = N1 / B;
+  const int64_t pr = blockIdx.x / ncb, cb = blockIdx.x % ncb;
+  int det;
+  int64_t ga, gb;
+  pair_info(a.g, pr, det, ga, gb);
+  const int64_t n1 = cb * B + b;
+  V v[16];
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    const int64_t s = n1 + N1 * (int64_t)(t + TS * i);
+    const T xa = load_src<T, SRC>(a, ga + s);
+    const T xb = gb >= 0 ? load_src<T, SRC>(a, gb + s) : (T)0;
+    v[i] = mk<V>(xa, xb);
+  }
+  fft_reg<V, -1, LOG2N2, B>(v, t, lds + b, a.tw4096);
+  V *out = a.buf + pr * a.g.N;
+#pragma unroll
+  for (int s = 0; s < 16; s++) {
+    const int64_t k2 = t + TS * s;
+    out[k2 * N1 + n1] = cmul(v[s], twN<V, -1>(a.twlo, a.twhi, n1 * k2));
+  }
+}
+
+// ---------------------------------------------------------------------- row
+template <typename T, int LOG2N1>
+__global__ __launch_bounds__(256) void k_row(OpArgs<T> a) {
+  using V = VT<T>;
+  constexpr int N1 = 1 << LOG2N1, TS = N1 / 16, B = 256 / TS;
+  __shared__ V lds[LdsSize<LOG2N1, B>::value];
+  __shared__ double red[4];
+  const int tid = threadIdx.x, b = tid % B, t = tid / B;
+  const int64_t N = a.g.N, N2 = a.g.N2;
+  const int64_t nrb = N2 / B;
+  const int64_t pr = blockIdx.x / nrb, rb = blockIdx.x % nrb;
+  const int64_t k2 = rb * B + b;
+  int det;
+  int64_t ga, gb;
+  pair_info(a.g, pr, det, ga, gb);
+  const int64_t base = pr * N + k2 * N1;
+  const T *w = a.wt + (int64_t)det * N + k2 * N1;
+  const int mode = a.mode;
+  V v[16];
+  if (mode & ROW_SRC_WT) {
+#pragma unroll
+    for (int i = 0; i < 16; i++) v[i] = mk<V>(w[t + TS * i], (T)0);
+  } else {
+#pragma unroll
+    for (int i = 0; i < 16; i++) v[i] = a.buf[base + t + TS * i];
+  }
+  double chi = 0.0;
+  if (mode & ROW_FWD) {
+    fft_reg<V, -1, LOG2N1, B>(v, t, lds + b, a.tw4096);
+    if (mode & ROW_STORE_D) {
+#pragma unroll
+      for (int s = 0; s < 16; s++) a.dspec[base + t + TS * s] = v[s];
+    }
+    if (mode & ROW_CHI2) {
+#pragma unroll
+      for (int s = 0; s < 16; s++) {
+        const int k1 = t + TS * s;
+        const V D = a.dspec[base + k1];
+        const double rx = (double)D.x - (double)v[s].x, ry = (double)D.y - (double)v[s].y;
+        double e = (double)w[k1] * (rx * rx + ry * ry);
+        // bins 0 and N/2 are counted twice by the reference's per-segment sum
+        if (k2 == 0 && (k1 == 0 || k1 == N1 / 2)) e += e;
+        chi += e;
+      }
+    }
+    if (mode & ROW_FILTER) {
+#pragma unroll
+      for (int s = 0; s < 16; s++) {
+        const T ww = w[t + TS * s];
+        v[s] = mk<V>(v[s].x * ww, v[s].y * ww);
+      }
+    }
+  }
+  if (mode & ROW_INV) {
+    fft_reg<V, +1, LOG2N1, B>(v, t, lds + b, a.tw4096);
+#pragma unroll
+    for (int s = 0; s < 16; s++) {
+      const int64_t n1 = t + TS * s;
+      a.buf[base + n1] = cmul(v[s], twN<V, +1>(a.twlo, a.twhi, n1 * k2));
+    }
+  }
+  if (mode & ROW_CHI2) {
+    const double tot = block_sum256(chi, red);
+    if (tid == 0) a.chi2part[blockIdx.x] = tot;
+  }
+}
+
+// ---------------------------------------------------------------- column inv
+template <typename T, int LOG2N2>
+__global__ __launch_bounds__(256) void k_col_inv(OpArgs<T> a) {
+  using V = VT<T>;
+  constexpr int N2 = 1 << LOG2N2, TS = N2 / 16, B = 256 / TS;
+  __shared__ V lds[LdsSize<LOG2N2, B>::value];
+  const int tid = threadIdx.x, b = tid % B, t = tid / B;
+  const int64_t N1 = a.g.N1;
+  const int64_t ncb = N1 / B;
+  const int64_t pr = blockIdx.x / ncb, cb = blockIdx.x % ncb;
+  int det;
+  int64_t ga, gb;
+  pair_info(a.g, pr, det, ga, gb);
+  const int64_t n1 = cb * B + b;
+  const V *in = a.buf + pr * a.g.N;
+  V v[16];
+#pragma unroll
+  for (int i = 0; i < 16; i++) v[i] = in[(int64_t)(t + TS * i) * N1 + n1];
+  fft_reg<V, +1, LOG2N2, B>(v, t, lds + b, a.tw4096);
+#pragma unroll
+  for (int s = 0; s < 16; s++) {
+    const int64_t idx = n1 + N1 * (int64_t)(t + TS * s);
+    a.z[ga + idx] = v[s].x;
+    if (gb >= 0) a.z[gb + idx] = v[s].y;
+  }
+}
+
+// -------------------------------------------------------------- single pass
+// N <= 4096: the whole pair sequence lives in one block; B pairs per block.
+template <typename T, int LOG2N, int SRC>
+__global__ __launch_bounds__(256) void k_single(OpArgs<T> a) {
+  using V = VT<T>;
+  constexpr int N = 1 << LOG2N, TS = N / 16, B = 256 / TS;
+  __shared__ V lds[LdsSize<LOG2N, B>::value];
+  __shared__ double red[4];
+  const int tid = threadIdx.x, b = tid % B, t = tid / B;
+  const int64_t pr = (int64_t)blockIdx.x * B + b;
+  const bool valid = pr < a.g.npairs;
+  int det = 0;
+  int64_t ga = 0, gb = -1;
+  if (valid) pair_info(a.g, pr, det, ga, gb);
+  const T *w = a.wt + (int64_t)det * N;
+  const int mode = a.mode;
+  V v[16];
+  if (mode & ROW_SRC_WT) {
+#pragma unroll
+    for (int i = 0; i < 16; i++) v[i] = mk<V>(valid ? w[t + TS * i] : (T)0, (T)0);
+  } else {
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+      const int64_t s = t + TS * i;
+      T xa = (T)0, xb = (T)0;
+      if (valid) {
+        xa = load_src<T, SRC>(a, ga + s);
+        if (gb >= 0) xb = load_src<T, SRC>(a, gb + s);
+      }
+      v[i] = mk<V>(xa, xb);
+    }
+  }
+  double chi = 0.0;
+  const int64_t base = pr * N;
+  if (mode & ROW_FWD) {
+    fft_reg<V, -1, LOG2N, B>(v, t, lds + b, a.tw4096);
+    if (valid && (mode & ROW_STORE_D)) {
+#pragma unroll
+      for (int s = 0; s < 16; s++) a.dspec[base + t + TS * s] = v[s];
+    }
+    if (valid && (mode & ROW_CHI2)) {
+#pragma unroll
+      for (int s = 0; s < 16; s++) {
+        const int k = t + TS * s;
+        const V D = a.dspec[base + k];
+        const double rx = (double)D.x - (double)v[s].x, ry = (double)D.y - (double)v[s].y;
+        double e = (double)w[k] * (rx * rx + ry * ry);
+        if (k == 0 || k == N / 2) e += e;
+        chi += e;
+      }
+    }
+    if (mode & ROW_FILTER) {
+#pragma unroll
+      for (int s = 0; s < 16; s++) {
+        const T ww = valid ? w[t + TS * s] : (T)0;
+        v[s] = mk<V>(v[s].x * ww, v[s].y * ww);
+      }
+    }
+  }
+  if (mode & ROW_INV) {
+    fft_reg<V, +1, LOG2N, B>(v, t, lds + b, a.tw4096);
+    if (valid) {
+#pragma unroll
+      for (int s = 0; s < 16; s++) {
+        const int64_t idx = t + TS * s;
+        a.z[ga + idx] = v[s].x;
+        if (gb >= 0) a.z[gb + idx] = v[s].y;
+      }
+    }
+  }
+  if (mode & ROW_CHI2) {
+    const double tot = block_sum256(chi, red);
+    if (tid == 0) a.chi2part[blockIdx.x] = tot;
+  }
+}
+
+template <typename K, typename A>
+int launch1(K kern, int64_t nblocks, hipStream_t s, const A &a) {
+  if (nblocks <= 0) return 0;
+  if (nblocks > 0x7fffffffLL) return -1;
+  hipLaunchKernelGGL(kern, dim3((unsigned)nblocks), dim3(256), 0, s, a);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+}  // namespace
+
+template <typename T>
+int launch_col_fwd(const OpArgs<T> &a, int src, hipStream_t s) {
+  const int64_t nb = a.g.npairs * (a.g.N / 4096);
+#define CF(L)                                                                            \
+  case L:                                                                                \
+    return src == SRC_GATHER ? launch1(k_col_fwd<T, L, SRC_GATHER>, nb, s, a)            \
+                             : launch1(k_col_fwd<T, L, SRC_TOD>, nb, s, a);
+  switch (a.g.log2N2) {
+    CF(4) CF(5) CF(6) CF(7) CF(8) CF(9) CF(10) CF(11) CF(12)
+    default: return -2;
+  }
+#undef CF
+}
+
+template <typename T>
+int launch_row(const OpArgs<T> &a, hipStream_t s, int64_t *nblocks) {
+  const int64_t nb = a.g.npairs * (a.g.N / 4096);
+  if (nblocks) *nblocks = nb;
+  switch (a.g.log2N1) {
+    case 9: return launch1(k_row<T, 9>, nb, s, a);
+    case 10: return launch1(k_row<T, 10>, nb, s, a);
+    case 11: return launch1(k_row<T, 11>, nb, s, a);
+    case 12: return launch1(k_row<T, 12>, nb, s, a);
+    default: return -2;
+  }
+}
+
+template <typename T>
+int launch_col_inv(const OpArgs<T> &a, hipStream_t s) {
+  const int64_t nb = a.g.npairs * (a.g.N / 4096);
+#define CI(L) \
+  case L: return launch1(k_col_inv<T, L>, nb, s, a);
+  switch (a.g.log2N2) {
+    CI(4) CI(5) CI(6) CI(7) CI(8) CI(9) CI(10) CI(11) CI(12)
+    default: return -2;
+  }
+#undef CI
+}
+
+template <typename T>
+int launch_single(const OpArgs<T> &a, int src, hipStream_t s, int64_t *nblocks) {
+  const int log2n = a.g.log2N;
+  const int64_t B = 4096 >> log2n;  // pairs per block
+  const int64_t nb = (a.g.npairs + B - 1) / B;
+  if (nblocks) *nblocks = nb;
+#define SP(L)                                                                         \
+  case L:                                                                             \
+    return src == SRC_GATHER ? launch1(k_single<T, L, SRC_GATHER>, nb, s, a)          \
+                             : launch1(k_single<T, L, SRC_TOD>, nb, s, a);
+  switch (log2n) {
+    SP(4) SP(5) SP(6) SP(7) SP(8) SP(9) SP(10) SP(11) SP(12)
+    default: return -2;
+  }
+#undef SP
+}
+
+template int launch_col_fwd<double>(const OpArgs<double> &, int, hipStream_t);
+template int launch_row<double>(const OpArgs<double> &, hipStream_t, int64_t *);
+template int launch_col_inv<double>(const OpArgs<double> &, hipStream_t);
+template int launch_single<double>(const OpArgs<double> &, int, hipStream_t, int64_t *);
+template int launch_col_fwd<float>(const OpArgs<float> &, int, hipStream_t);
+template int launch_row<float>(const OpArgs<float> &, hipStream_t, int64_t *);
+template int launch_col_inv<float>(const OpArgs<float> &, hipStream_t);
+template int launch_single<float>(const OpArgs<float> &, int, hipStream_t, int64_t *);

@@ -1,0 +1,94 @@
+// pixel_host.cpp -- the reference's pixel chain evaluated with the platform
+// libm, for the few samples the GPU reports as ambiguous (a floor() argument
+// or branch test within the device libm's error bound of its boundary).
+//
+// Built with g++ -ffp-contract=off and no fast-math: the operation sequence
+// is that of src/sanepic.cc (pointingshift 3419-3428, pointing loop
+// 737-747, pixelisation 855-860, ang2pix_ring 2794-2842), so the result is
+// the reference's pixel whenever libm is glibc.
+#include <cmath>
+#include <cstdint>
+
+#include "pixel_host.h"
+
+namespace {
+constexpr double kD2PI = 6.2831853071795864769252867665590057683943387987502;
+
+long ring_pixel(long nside, double theta, double phi) {
+  const double piover2 = 0.5 * M_PI, twopi = 2.0 * M_PI, z0 = 2.0 / 3.0;
+  if (nside < 1 || nside > 8192) return -1;
+  if (theta < 0. || theta > M_PI) return -1;
+  const int nl2 = 2 * (int)nside, nl4 = 4 * (int)nside;
+  const int ncap = nl2 * ((int)nside - 1);
+  const int npix = 12 * (int)nside * (int)nside;
+  const double z = std::cos(theta);
+  const double za = std::fabs(z);
+  if (phi >= twopi) phi = phi - twopi;
+  if (phi < 0.) phi = phi + twopi;
+  const double tt = phi / piover2;
+  int jp, jm, ir, ip, ipix1;
+  if (za <= z0) {
+    jp = (int)std::floor(nside * (0.5 + tt - z * 0.75));
+    jm = (int)std::floor(nside * (0.5 + tt + z * 0.75));
+    ir = (int)nside + 1 + jp - jm;
+    const int kshift = ((ir & 1) == 0) ? 1 : 0;
+    ip = (int)std::floor((double)((jp + jm - (int)nside + kshift + 1) / 2)) + 1;
+    if (ip > nl4) ip = ip - nl4;
+    ipix1 = ncap + nl4 * (ir - 1) + ip;
+  } else {
+    const double tp = tt - std::floor(tt);
+    const double tmp = std::sqrt(3. * (1. - za));
+    jp = (int)std::floor(nside * tp * tmp);
+    jm = (int)std::floor(nside * (1. - tp) * tmp);
+    ir = jp + jm + 1;
+    ip = (int)std::floor(tt * ir) + 1;
+    if (ip > 4 * ir) ip = ip - 4 * ir;
+    ipix1 = 2 * ir * (ir - 1) + ip;
+    if (z <= 0.) ipix1 = npix - 2 * ir * (ir + 1) + ip;
+  }
+  return (long)ipix1 - 1;
+}
+}  // namespace
+
+long host_sample_pixel(long nside, double offx, double offy, double ra, double dec, double psi) {
+  const double x = offx, y = offy;
+  const double th_ = std::asin(std::sqrt(x * x + y * y));
+  const double ph_ = std::atan2(y, x);
+  const double thetap = kD2PI / 4.0 - dec;
+  const double phip = ra;
+  const double theta_o =
+      std::acos(std::sin(thetap) * std::sin(th_) * std::cos(ph_ + psi) + std::cos(thetap) * std::cos(th_));
+  const double phi_o = std::atan2(-std::sin(th_) * std::sin(ph_ + psi),
+                                  (-std::cos(thetap) * std::sin(th_) * std::cos(ph_ + psi) +
+                                   std::sin(thetap) * std::cos(th_))) +
+                       phip;
+  const double ra_d = phi_o;
+  const double dec_d = kD2PI / 4.0 - theta_o;
+  const double theta = kD2PI / 4.0 - dec_d;
+  const double phi = std::fmod(ra_d + kD2PI, kD2PI);
+  if (std::isnan(theta) || std::isnan(phi)) return -1;
+  const long v = ring_pixel(nside, theta, phi);
+  if (v < 0 || v > 12 * nside * nside - 1) return -1;
+  return v;
+}
+
+void host_det_const(double lel, double xel, double angl, double *offx, double *offy, double *psidet,
+                    double *th_, double *ph_, double *sin_th, double *cos_th, double *drp2) {
+  // read_bolo_offsets stores {xel, lel, angl}; 460-462 convert them.
+  const double o0 = xel, o1 = lel, o2 = angl;
+  *offx = o1 / 180.0 * kD2PI / 2.0;
+  *offy = o0 / 180.0 * kD2PI / 2.0;
+  *psidet = o2 / 180.0 * kD2PI / 2.0;
+  const double x = *offx, y = *offy;
+  *th_ = std::asin(std::sqrt(x * x + y * y));
+  *ph_ = std::atan2(y, x);
+  *sin_th = std::sin(*th_);
+  *cos_th = std::cos(*th_);
+  *drp2 = -std::sin(*th_) * std::cos(*ph_);
+}
+
+void host_spn_row(const double *ispf, long nssp, long ns, double *row) {
+  // 533-537, including the truncating index expression.
+  for (long ii = 0; ii < ns; ii++) row[ii] = ispf[long(double(ii) / nssp * ns)] / double(ns) / double(ns);
+  row[0] = ispf[1] * 1e-6 / double(ns) / double(ns);
+}

@@ -1,0 +1,333 @@
+// reduce.hip -- P^T as a pixel-sorted segmented reduction, plus the other
+// per-pixel sums of the set-up (crosslinks, noise pair sums) and small
+// bookkeeping kernels.
+//
+// The reference scatters sample by sample with read-modify-writes into the
+// map (do_PtNd, sanepic.cc:2584-2611).  Here all samples are sorted once by
+// (pixel, sample index); P^T then reads, for each pixel, its run of sorted
+// entries and sums z[perm[j]] * (1, cos 2psi_j, sin 2psi_j).  A 16-lane group
+// owns one item (a pixel, or a <= 4096-entry chunk of a large pixel) and
+// reduces it with a fixed shuffle tree; split pixels are combined by a
+// fix-up kernel in item order.  No atomics, bit-reproducible.
+#include <type_traits>
+
+#include "kernels.h"
+
+namespace {
+
+template <typename T> using VT = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
+
+template <typename T>
+__device__ __forceinline__ T red16(T v) {
+#pragma unroll
+  for (int o = 8; o > 0; o >>= 1) v += __shfl_xor(v, o, 16);
+  return v;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_scatter(ScatterArgs<T> a) {
+  using V = VT<T>;
+  const int grp = threadIdx.x >> 4, lane = threadIdx.x & 15;
+  const int64_t item = (int64_t)blockIdx.x * 16 + grp;
+  if (item >= a.it.n) return;
+  const int p = a.it.pix[item];
+  const int64_t e0 = a.it.beg[item];
+  const int n = a.it.len[item];
+  T s0 = 0, s1 = 0, s2 = 0;
+  for (int k = lane; k < n; k += 16) {
+    const int64_t j = e0 + k;
+    const T zv = a.z[a.perm[j]];
+    const V c = a.cs_sorted[j];
+    s0 += zv;
+    s1 += zv * c.x;
+    s2 += zv * c.y;
+  }
+  s0 = red16(s0);
+  s1 = red16(s1);
+  s2 = red16(s2);
+  if (lane == 0) {
+    const int slot = a.it.slot[item];
+    if (slot < 0) {
+      double q0 = s0, q1 = s1, q2 = s2;
+      if (a.mask && a.cnt[p] < 2) {  // hitstot_pol < 4 (doubled counts), 1455-1459
+        q1 = 0.0;
+        q2 = 0.0;
+      }
+      double *q = a.q + 3 * (int64_t)p;
+      q[0] = q0;
+      q[1] = q1;
+      q[2] = q2;
+    } else {
+      double *pp = a.partial + 3 * (int64_t)slot;
+      pp[0] = s0;
+      pp[1] = s1;
+      pp[2] = s2;
+    }
+  }
+}
+
+template <int NV>
+__global__ __launch_bounds__(256) void k_fix(Items it, const double *partial, double *out,
+                                             const int32_t *cnt, int mask) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= it.nsplit) return;
+  const int p = it.split_pix[i];
+  const int f = it.split_first[i], c = it.split_count[i];
+  double acc[NV];
+#pragma unroll
+  for (int v = 0; v < NV; v++) acc[v] = 0.0;
+  for (int k = 0; k < c; k++) {
+#pragma unroll
+    for (int v = 0; v < NV; v++) acc[v] += partial[(int64_t)(f + k) * NV + v];
+  }
+  if constexpr (NV == 3) {
+    if (mask && cnt[p] < 2) {
+      acc[1] = 0.0;
+      acc[2] = 0.0;
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < NV; v++) out[(int64_t)p * NV + v] = acc[v];
+}
+
+__global__ __launch_bounds__(256) void k_crosslink(Items it, const double2 *cs, double *out,
+                                                   double *partial) {
+  const int grp = threadIdx.x >> 4, lane = threadIdx.x & 15;
+  const int64_t item = (int64_t)blockIdx.x * 16 + grp;
+  if (item >= it.n) return;
+  const int p = it.pix[item];
+  const int64_t e0 = it.beg[item];
+  const int n = it.len[item];
+  double s0 = 0.0, s1 = 0.0;
+  for (int k = lane; k < n; k += 16) {
+    const double2 c = cs[e0 + k];
+    s0 += c.x;
+    s1 += c.y;
+  }
+  s0 = red16(s0);
+  s1 = red16(s1);
+  if (lane == 0) {
+    const int slot = it.slot[item];
+    double *o = slot < 0 ? out + 2 * (int64_t)p : partial + 2 * (int64_t)slot;
+    o[0] = s0;
+    o[1] = s1;
+  }
+}
+
+// Noise pair sums inside (pixel, detector-segment) groups.  Entries of one
+// pixel are sorted by sample index, so a group is a contiguous run and the
+// time offsets grow monotonically away from entry j.
+__global__ __launch_bounds__(256) void k_pairsum(PairSumArgs a) {
+  const int grp = threadIdx.x >> 4, lane = threadIdx.x & 15;
+  const int64_t item = (int64_t)blockIdx.x * 16 + grp;
+  if (item >= a.it.n) return;
+  const int p = a.it.pix[item];
+  const int64_t e0 = a.it.beg[item];
+  const int n = a.it.len[item];
+  const int64_t P0 = a.off[p], P1 = a.off[p + 1];
+  const int64_t ns = a.ns, half = ns / 2;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int k = lane; k < n; k += 16) {
+    const int64_t j = e0 + k;
+    const int64_t g = a.perm[j];
+    const int64_t blk = g / ns, t = g % ns;
+    const double *Na = a.nauto + (blk / a.nseg) * ns;
+    const double2 cj = a.cs_sorted[j];
+    double u0 = 0.0, u1 = 0.0, u2 = 0.0;
+    for (int64_t jj = j; jj >= P0; jj--) {
+      const int64_t gg = a.perm[jj];
+      if (gg / ns != blk) break;
+      const int64_t dt = t - gg % ns;
+      if (dt >= half) break;
+      const double w = Na[dt];
+      const double2 ck = a.cs_sorted[jj];
+      u0 += w;
+      u1 += w * ck.x;
+      u2 += w * ck.y;
+    }
+    for (int64_t jj = j + 1; jj < P1; jj++) {
+      const int64_t gg = a.perm[jj];
+      if (gg / ns != blk) break;
+      const int64_t dt = gg % ns - t;
+      if (dt >= half) break;
+      const double w = Na[dt];
+      const double2 ck = a.cs_sorted[jj];
+      u0 += w;
+      u1 += w * ck.x;
+      u2 += w * ck.y;
+    }
+    acc[0] += u0;
+    acc[1] += u1;
+    acc[2] += u2;
+    acc[3] += cj.x * u1;
+    acc[4] += cj.x * u2;
+    acc[5] += cj.y * u2;
+  }
+#pragma unroll
+  for (int v = 0; v < 6; v++) acc[v] = red16(acc[v]);
+  if (lane == 0) {
+    const int slot = a.it.slot[item];
+    double *o = slot < 0 ? a.out + 6 * (int64_t)p : a.partial + 6 * (int64_t)slot;
+#pragma unroll
+    for (int v = 0; v < 6; v++) o[v] = acc[v];
+  }
+}
+
+__global__ void k_gather_cs(const uint32_t *perm, const double2 *src, double2 *dst, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[perm[i]];
+}
+__global__ void k_gather_cs_f(const uint32_t *perm, const double2 *src, float2 *dst, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const double2 v = src[perm[i]];
+    dst[i] = make_float2((float)v.x, (float)v.y);
+  }
+}
+__global__ void k_cs_to_float(const double2 *src, float2 *dst, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = make_float2((float)src[i].x, (float)src[i].y);
+}
+
+__device__ __forceinline__ bool isnan_(double x) { return !(x > 0) && !(x <= 0.0); }
+
+// 784-790: a NaN sample takes the previous (already fixed) sample, i.e. the
+// last non-NaN value before it in its segment; a leading NaN stays NaN.
+// In place: a thread that scans back over a NaN run may see a slot another
+// thread already filled, and that slot holds the same last non-NaN value.
+__global__ void k_tod_nanfill(const double *tod, double *out, int64_t ntot, int64_t ns) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ntot) return;
+  double v = tod[g];
+  if (isnan_(v)) {
+    const int64_t seg0 = g - g % ns;
+    int64_t k = g - 1;
+    while (k >= seg0 && isnan_(tod[k])) k--;
+    if (k >= seg0) v = tod[k];
+  }
+  out[g] = v;
+}
+
+__global__ void k_histogram(const int32_t *pix, int64_t n, int32_t *counts) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) atomicAdd(&counts[pix[i]], 1);
+}
+__global__ void k_compress(int32_t *pix, int64_t n, const int32_t *indpix) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) pix[i] = indpix[pix[i]];
+}
+__global__ void k_iota(uint32_t *v, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = (uint32_t)i;
+}
+__global__ void k_b2i(const double *blk, double *inter, int64_t npix) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < npix) {
+    inter[3 * i] = blk[i];
+    inter[3 * i + 1] = blk[npix + i];
+    inter[3 * i + 2] = blk[2 * npix + i];
+  }
+}
+__global__ void k_i2b(const double *inter, double *blk, int64_t npix) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < npix) {
+    blk[i] = inter[3 * i];
+    blk[npix + i] = inter[3 * i + 1];
+    blk[2 * npix + i] = inter[3 * i + 2];
+  }
+}
+__global__ void k_d2f(const double *a, float *b, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
+}
+
+inline unsigned nblk(int64_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+inline int ok() { return hipGetLastError() == hipSuccess ? 0 : -1; }
+
+}  // namespace
+
+template <typename T>
+int launch_scatter(const ScatterArgs<T> &a, hipStream_t s) {
+  if (a.it.n > 0)
+    hipLaunchKernelGGL(k_scatter<T>, dim3(nblk(a.it.n, 16)), dim3(256), 0, s, a);
+  if (a.it.nsplit > 0)
+    hipLaunchKernelGGL(k_fix<3>, dim3(nblk(a.it.nsplit)), dim3(256), 0, s, a.it, a.partial, a.q,
+                       a.cnt, a.mask);
+  return ok();
+}
+template int launch_scatter<double>(const ScatterArgs<double> &, hipStream_t);
+template int launch_scatter<float>(const ScatterArgs<float> &, hipStream_t);
+
+int launch_crosslink(const Items &it, const double2 *cs_sorted, double *out, double *partial,
+                     hipStream_t s) {
+  if (it.n > 0)
+    hipLaunchKernelGGL(k_crosslink, dim3(nblk(it.n, 16)), dim3(256), 0, s, it, cs_sorted, out,
+                       partial);
+  if (it.nsplit > 0)
+    hipLaunchKernelGGL(k_fix<2>, dim3(nblk(it.nsplit)), dim3(256), 0, s, it, partial, out,
+                       (const int32_t *)nullptr, 0);
+  return ok();
+}
+
+int launch_pairsum(const PairSumArgs &a, hipStream_t s) {
+  if (a.it.n > 0) hipLaunchKernelGGL(k_pairsum, dim3(nblk(a.it.n, 16)), dim3(256), 0, s, a);
+  if (a.it.nsplit > 0)
+    hipLaunchKernelGGL(k_fix<6>, dim3(nblk(a.it.nsplit)), dim3(256), 0, s, a.it, a.partial, a.out,
+                       (const int32_t *)nullptr, 0);
+  return ok();
+}
+
+int launch_gather_cs(const uint32_t *perm, const double2 *src, double2 *dst, int64_t n,
+                     hipStream_t s) {
+  hipLaunchKernelGGL(k_gather_cs, dim3(nblk(n)), dim3(256), 0, s, perm, src, dst, n);
+  return ok();
+}
+int launch_gather_cs_f(const uint32_t *perm, const double2 *src, float2 *dst, int64_t n,
+                       hipStream_t s) {
+  hipLaunchKernelGGL(k_gather_cs_f, dim3(nblk(n)), dim3(256), 0, s, perm, src, dst, n);
+  return ok();
+}
+int launch_cs_to_float(const double2 *src, float2 *dst, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_cs_to_float, dim3(nblk(n)), dim3(256), 0, s, src, dst, n);
+  return ok();
+}
+int launch_tod_nanfill(double *tod, int64_t ntot, int64_t ns, hipStream_t s) {
+  hipLaunchKernelGGL(k_tod_nanfill, dim3(nblk(ntot)), dim3(256), 0, s, tod, tod, ntot, ns);
+  return ok();
+}
+int launch_histogram(const int32_t *pix, int64_t n, int32_t *counts, hipStream_t s) {
+  hipLaunchKernelGGL(k_histogram, dim3(nblk(n)), dim3(256), 0, s, pix, n, counts);
+  return ok();
+}
+int launch_compress_pix(int32_t *pix, int64_t n, const int32_t *indpix, hipStream_t s) {
+  hipLaunchKernelGGL(k_compress, dim3(nblk(n)), dim3(256), 0, s, pix, n, indpix);
+  return ok();
+}
+int launch_iota_u32(uint32_t *v, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_iota, dim3(nblk(n)), dim3(256), 0, s, v, n);
+  return ok();
+}
+int launch_block_to_inter(const double *blk, double *inter, int64_t npix, hipStream_t s) {
+  hipLaunchKernelGGL(k_b2i, dim3(nblk(npix)), dim3(256), 0, s, blk, inter, npix);
+  return ok();
+}
+int launch_inter_to_block(const double *inter, double *blk, int64_t npix, hipStream_t s) {
+  hipLaunchKernelGGL(k_i2b, dim3(nblk(npix)), dim3(256), 0, s, inter, blk, npix);
+  return ok();
+}
+int launch_d2f(const double *a, float *b, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_d2f, dim3(nblk(n)), dim3(256), 0, s, a, b, n);
+  return ok();
+}
+
+namespace {
+__global__ void k_set_values(int32_t *dst, const int64_t *idx, const int32_t *val, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[idx[i]] = val[i];
+}
+}  // namespace
+
+int launch_set_values(int32_t *dst, const int64_t *idx, const int32_t *val, int64_t n, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_set_values, dim3(nblk(n)), dim3(256), 0, s, dst, idx, val, n);
+  return ok();
+}

@@ -1,0 +1,386 @@
+// sanepic_main.cpp -- drop-in `sanepic` executable on top of libsanepic_hip.
+//
+// Same command line as patanch/SANEPIC's src/sanepic.cc (getopt string at
+// 315-317, run_sanepic.sh:55-73), same input files (segment list, pointing,
+// TODs, iSpf spectra, bolometer file), same stdout progress lines and the
+// same output map files (Appendix of the format: write_reduced_map 2364-2388,
+// names at 1618-1801).  The MPI master/worker split of the reference is
+// replaced by one process driving one GPU; when launched under mpirun, ranks
+// other than 0 exit 0 at once so `mpirun -n N sanepic ...` still works.
+#include <getopt.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <list>
+#include <string>
+#include <vector>
+
+#include "sanepic_hip.h"
+
+namespace {
+
+constexpr double kD2PI = 6.2831853071795864769252867665590057683943387987502;
+
+void usage(const char *name) {
+  std::cerr << "USAGE: " << name << ": Map-making (MI355X)\n\n"
+            << name << " [-F <data dir>] [-f <used samples (file name)>]\n"
+            << "-B <bolo ext>          bolo field extension\n"
+            << "-C <bolo name>         Add a single bolometer (can be repeated; see also -D)\n"
+            << "-d <Det number>        Number of detectors to include\n"
+            << "-D <detector list>     List of detectors to include (see also -C)\n"
+            << "-e <output file str>   id string in output file\n"
+            << "-F <data dir>          source data directory\n"
+            << "-Z <pointing dir>      source pointing directory + prefixe\n"
+            << "-f <sample list file>  file including list of 1st and last samples\n"
+            << "-k <noise prefixe>     noise power spectrum file prefixe\n"
+            << "-u Noise Nb samp       Number of samples of the noise power spectra\n"
+            << "-X <det location file> File providing pointing info for each detector wrt boresight\n"
+            << "-O <output dir>        output directory\n"
+            << "-P <pointing ext>      pointing field extension\n"
+            << "-p <polar>             set this keyword if polarization is included\n"
+            << "-l <data seg length>   Lenght of data segments. Must be the same for all segments\n"
+            << "-n <nb of segments>    Number of data segments\n"
+            << "-N <Nside>             Healpix nside\n"
+            << "-j <Delt iter>         Number of iterations after which the maps are recalculated\n"
+            << "-I <init map>          start from a compressed map (the _A_ output)\n"
+            << "-i <write iter maps>   write intermediate maps every -j iterations\n"
+            << "-h <HWP on>            Set if Half Wave Plate. Default is 0\n"
+            << "-w <Omega HWP>         HWP angular frequency per sample\n"
+            << "Accepted and ignored as in the reference: -H -J -A -m -G -L -g -z -M -o -R -r -c -b -K -E\n";
+  exit(1);
+}
+
+int mpi_rank() {
+  const char *vars[] = {"OMPI_COMM_WORLD_RANK", "PMI_RANK", "PMIX_RANK", "MV2_COMM_WORLD_RANK", "SLURM_PROCID"};
+  for (const char *v : vars) {
+    const char *s = getenv(v);
+    if (s && *s) return atoi(s);
+  }
+  return 0;
+}
+
+// ReadDoubleVECT / ReadLongVECT (3087-3132): fseek(imin * 8) then fread.
+template <typename T>
+bool read_vect(T *data, const std::string &fn, long imin, long imax) {
+  FILE *fp = fopen(fn.c_str(), "rb");
+  if (!fp) {
+    fprintf(stderr, "ERROR: cannot open %s\n", fn.c_str());
+    return false;
+  }
+  const long nn = imax - imin + 1;
+  if (fseek(fp, imin * (long)sizeof(T), SEEK_SET) != 0) {
+    fclose(fp);
+    fprintf(stderr, "ERROR: cannot seek in %s\n", fn.c_str());
+    return false;
+  }
+  const size_t got = fread(data, sizeof(T), nn, fp);
+  fclose(fp);
+  if ((long)got != nn) {
+    fprintf(stderr, "ERROR: %s holds %zu of the %ld values requested from %ld\n", fn.c_str(), got, nn, imin);
+    return false;
+  }
+  return true;
+}
+
+// read_bolofile (3134-3157): first word of every non-empty, non-# line.
+void read_bolofile(const std::string &fname, std::list<std::string> &bolos) {
+  std::ifstream in(fname.c_str());
+  if (!in.is_open()) {
+    std::cerr << "Error opening bolometer file '" << fname << "'. Exiting.\n";
+    exit(1);
+  }
+  std::string line;
+  while (std::getline(in, line)) {
+    line.erase(0, line.find_first_not_of(" \t"));
+    if (line.empty() || line[0] == '#') continue;
+    bolos.push_back(line.substr(0, line.find_first_of(" \t")));
+  }
+}
+
+// read_bolo_offsets (3159-3188): fields num pnum lel xel angl tel name; the
+// last line with the name wins.
+void read_bolo_offsets(const std::string &bolo, const std::string &file, double &lel, double &xel,
+                       double &angl) {
+  FILE *fp = fopen(file.c_str(), "r");
+  if (!fp) {
+    std::cerr << "ERROR: Can't find offset file. Exiting. \n";
+    exit(1);
+  }
+  long num, pnum;
+  double a, b2, c;
+  char tel[300], name[300];
+  bool found = false;
+  while (fscanf(fp, "%ld%ld%lf%lf%lf%299s%299s\n", &num, &pnum, &a, &b2, &c, tel, name) == 7) {
+    if (bolo == name) {
+      found = true;
+      lel = a;
+      xel = b2;
+      angl = c;
+    }
+  }
+  fclose(fp);
+  if (!found) {
+    std::cerr << "Bolometer name not found in offset list" << std::endl;
+    exit(1);
+  }
+}
+
+// write_reduced_map (2364-2388): full sky, 0 for unseen pixels.
+void write_reduced_map(const std::string &fn, long nside, const double *mapr, const std::vector<int64_t> &indpix) {
+  std::vector<double> full(12 * nside * nside, 0.0);
+  for (size_t i = 0; i < full.size(); i++)
+    if (indpix[i] != -1) full[i] = indpix[i] < 0 ? mapr[-indpix[i] - 2] : mapr[indpix[i]];
+  FILE *fp = fopen(fn.c_str(), "wb");
+  if (!fp || fwrite(full.data(), sizeof(double), full.size(), fp) != full.size()) {
+    printf("WARNING, PROBLEM WITH WRITTEN DATA\n");
+  }
+  if (fp) fclose(fp);
+}
+
+void write_raw(const std::string &fn, const double *v, long n) {
+  FILE *fp = fopen(fn.c_str(), "wb");
+  if (!fp || fwrite(v, sizeof(double), n, fp) != (size_t)n) printf("WARNING, PROBLEM WITH WRITTEN DATA\n");
+  if (fp) fclose(fp);
+}
+
+std::string fmt(const char *f, const std::string &a, const std::string &b) {
+  char buf[4096];
+  snprintf(buf, sizeof buf, f, a.c_str(), b.c_str());
+  return buf;
+}
+std::string fmt(const char *f, const std::string &a, const std::string &b, int k) {
+  char buf[4096];
+  snprintf(buf, sizeof buf, f, a.c_str(), b.c_str(), k);
+  return buf;
+}
+
+void die_lib(sanepic_ctx *ctx, int rc, const char *what) {
+  fprintf(stderr, "sanepic: %s failed (%d): %s\n", what, rc, sanepic_last_error(ctx));
+  exit(1);
+}
+
+double now() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+int main(int argc, char *argv[]) {
+  const int rank = mpi_rank();
+  int iterw = 10;
+  bool polar = false, hwp = false, initMapOn = false, writeIterOn = true, pfile = false;
+  long ndet = 0, nsegtot = 0, ns = 0, nssp = 0, nside = 0;
+  double omega = 0.0;
+  std::string outdir, bext, pext, termin_out, noisepre, filesegnum, dirfile, dirfilep, initMap, filedetinfo;
+  std::list<std::string> bolos;
+  int opt;
+  while ((opt = getopt(argc, argv,
+                       "F:Z:f:n:l:d:D:C:H:h:w:J:o:O:B:R:X:G:P:e:p:A:m:k:K:u:"
+                       "c:N:n:L:g:r:M:E:j:z:b:I:i:")) != -1) {
+    switch (opt) {
+      case 'F': dirfile = optarg; break;
+      case 'Z': dirfilep = optarg; pfile = true; break;
+      case 'f': filesegnum = optarg; break;
+      case 'p': polar = atoi(optarg); break;
+      case 'd': ndet = atoi(optarg); break;
+      case 'D': read_bolofile(optarg, bolos); std::cerr << "num bolos: " << bolos.size() << std::endl; break;
+      case 'C': bolos.push_back(optarg); break;
+      case 'O': outdir = optarg; break;
+      case 'B': bext = optarg; break;
+      case 'X': filedetinfo = optarg; break;
+      case 'P': pext = optarg; break;
+      case 'e': termin_out = optarg; break;
+      case 'k': noisepre = optarg; break;
+      case 'u': nssp = atoi(optarg); break;
+      case 'N': nside = atoi(optarg); break;
+      case 'n': nsegtot = atoi(optarg); break;
+      case 'l': ns = atoi(optarg); break;
+      case 'j': iterw = atoi(optarg); break;
+      case 'h': hwp = atoi(optarg); break;
+      case 'w': omega = atof(optarg); break;
+      case 'I': initMap = optarg; initMapOn = true; break;
+      case 'i': writeIterOn = atoi(optarg); break;
+      case 'E':
+        if (atoi(optarg)) {  // 513-515
+          printf("NOT IMPLEMENTED YET\n");
+          exit(0);
+        }
+        break;
+      case 'H': case 'J': case 'A': case 'm': case 'G': case 'L': case 'g': case 'z': case 'M':
+      case 'o': case 'R': case 'r': case 'c': case 'b': case 'K':
+        break;  // parsed and unused on the map-making path (see the reference's main)
+      default:
+        std::cerr << "Option '" << (char)opt << "' not valid. Exiting.\n\n";
+        usage(argv[0]);
+    }
+  }
+  if (rank != 0) return 0;  // one process drives the GPU
+  if (!pfile) dirfilep = dirfile;
+  std::vector<std::string> bolonames(bolos.begin(), bolos.end());
+  if ((long)bolonames.size() < ndet || ndet < 1) {
+    fprintf(stderr, "sanepic: -d %ld but %zu detectors given\n", ndet, bolonames.size());
+    exit(1);
+  }
+  bolonames.resize(ndet);
+  if (!polar) {
+    fprintf(stderr, "sanepic: only -p 1 is supported (the reference reads psi unconditionally)\n");
+    exit(1);
+  }
+  const double t0 = now();
+  // detector offsets (458-464)
+  std::vector<double> lel(ndet), xel(ndet), angl(ndet);
+  for (long i = 0; i < ndet; i++) {
+    read_bolo_offsets(bolonames[i], filedetinfo, lel[i], xel[i], angl[i]);
+    printf("Angle detector %ld = %10.15g\n", i, angl[i] / 180.0 * kD2PI / 2.0);
+  }
+  printf("nsegtot = %ld\n", nsegtot);
+  // segment list (494-502) and uniform length check (668-672)
+  std::vector<long> seg(2 * nsegtot);
+  if (!read_vect(seg.data(), filesegnum, 0, 2 * nsegtot - 1)) exit(1);
+  for (long i = 0; i < nsegtot; i++)
+    if (seg[2 * i + 1] - seg[2 * i] + 1 != ns) {
+      printf("Problem with segment size: %ld, %ld\n", seg[2 * i + 1] - seg[2 * i], ns);
+      exit(1);
+    }
+  sanepic_config cfg{};
+  cfg.nside = nside;
+  cfg.ndet = ndet;
+  cfg.nseg = nsegtot;
+  cfg.ns = ns;
+  cfg.polar = 1;
+  cfg.hwp = hwp ? 1 : 0;
+  cfg.omega_hwp = omega;
+  cfg.precision = SANEPIC_FP64;
+  if (const char *p = getenv("SANEPIC_PRECISION")) cfg.precision = strcmp(p, "mixed") == 0 ? SANEPIC_MIXED : SANEPIC_FP64;
+  cfg.device = 0;
+  sanepic_ctx *ctx = nullptr;
+  int rc = sanepic_create(&cfg, &ctx);
+  if (rc) die_lib(nullptr, rc, "sanepic_create");
+  // noise spectra (521-538)
+  std::vector<double> spec(nssp);
+  for (long i = 0; i < ndet; i++) {
+    const std::string fn = noisepre + bolonames[i];
+    printf("NoiseSpFiledet = %s\n", fn.c_str());
+    if (!read_vect(spec.data(), fn, 0, nssp - 1)) exit(1);
+    rc = sanepic_set_detector(ctx, i, lel[i], xel[i], angl[i], spec.data(), nssp);
+    if (rc) die_lib(ctx, rc, "sanepic_set_detector");
+  }
+  printf("NO CORRELATIONS BETWEEN DETECTORS INCLUDED\n");
+  printf("%ld\n", nside);
+  // pointing and TODs (667-817)
+  double tio = 0.0;
+  std::vector<double> ra(ns), dec(ns), psi(ns), tod(ns);
+  for (long s = 0; s < nsegtot; s++) {
+    const double ta = now();
+    const std::string base = dirfilep;
+    if (!read_vect(ra.data(), base + "ra" + pext + ".bi", seg[2 * s], seg[2 * s + 1]) ||
+        !read_vect(dec.data(), base + "dec" + pext + ".bi", seg[2 * s], seg[2 * s + 1]) ||
+        !read_vect(psi.data(), base + "psi" + pext + ".bi", seg[2 * s], seg[2 * s + 1]))
+      exit(1);
+    tio += now() - ta;
+    rc = sanepic_set_pointing(ctx, s, ra.data(), dec.data(), psi.data());
+    if (rc) die_lib(ctx, rc, "sanepic_set_pointing");
+    for (long i = 0; i < ndet; i++) {
+      const double tb = now();
+      const std::string fn = dirfile + "/data_" + bolonames[i] + bext + ".bi";
+      if (s == 0) printf("%s\n", fn.c_str());
+      if (!read_vect(tod.data(), fn, seg[2 * s], seg[2 * s + 1])) exit(1);
+      tio += now() - tb;
+      rc = sanepic_set_tod(ctx, i, s, tod.data());
+      if (rc) die_lib(ctx, rc, "sanepic_set_tod");
+    }
+  }
+  printf("DATA SUCCESSFULLY READ\n");
+  int64_t npix = 0, ndef = 0;
+  rc = sanepic_pixelize(ctx, &npix, &ndef);
+  if (rc) die_lib(ctx, rc, "sanepic_pixelize");
+  const long npixtot = 3 * npix;
+  printf("TOTAL NUMBER OF DETECTORS: %d\n", (int)ndet);
+  printf("TOTAL NUMBER OF DATA SEGMENTS: %d\n", (int)nsegtot);
+  printf("NUMBER OF SAMPLE PER DATA SEGMENT: %d\n", (int)ns);
+  printf("TOTAL NUMBER OF FILLED PIXELS IN ONE MAP: %ld, TOTAL NUMBER: %ld, FOR PROCESSOR %d: %ld, NB OF HEALPIX "
+         "PIXEL: %ld \n",
+         (long)npix, npixtot, 0, npixtot, 12 * nside * nside);
+  printf("Using input noise \n");
+  std::vector<int64_t> indpix(12 * nside * nside);
+  sanepic_get_indpix(ctx, indpix.data());
+  rc = sanepic_setup(ctx);
+  if (rc) die_lib(ctx, rc, "sanepic_setup");
+  printf("PtNd Computed...\n");
+  std::vector<double> x0;
+  if (initMapOn) {  // 1209-1227: compressed npixtot vector
+    x0.resize(npixtot);
+    if (!read_vect(x0.data(), initMap, 0, npixtot - 1)) exit(1);
+  }
+  sanepic_solve_opts so{};
+  so.precond = SANEPIC_PRECOND_DIAG;
+  if (const char *p = getenv("SANEPIC_PRECOND")) so.precond = strcmp(p, "block") == 0 ? SANEPIC_PRECOND_BLOCK3 : SANEPIC_PRECOND_DIAG;
+  so.refresh = 10;
+  rc = sanepic_solve_begin(ctx, &so, initMapOn ? x0.data() : nullptr);
+  if (rc == SANEPIC_EDATA) {
+    printf("%s\n", sanepic_last_error(ctx));
+    exit(0);  // the reference exits 0 here (1344-1348)
+  }
+  if (rc) die_lib(ctx, rc, "sanepic_solve_begin");
+  const double tsetup = now() - t0;
+  // 1415-1761
+  const double StopCriterium = 1e-15;
+  int iter = 0;
+  double crit = 1.0;
+  std::vector<double> S(npixtot), aux(npixtot);
+  const double tl0 = now();
+  while ((iter < 2000 && crit > StopCriterium) || iter == 0) {
+    sanepic_iter_info info{};
+    rc = sanepic_solve_step(ctx, &info);
+    if (rc) die_lib(ctx, rc, "sanepic_solve_step");
+    crit = info.crit;
+    printf("iter = %d, crit = %10.15g, crit2 = %10.15g, chi2 = %10.15g\n", iter, info.crit, info.crit2, info.chi2);
+    fflush(stdout);
+    if (iter == 1) {  // 1616-1707
+      std::vector<int64_t> hits(2 * npix);
+      std::vector<double> tmp(npix), sumc(npixtot);
+      sanepic_get_nvar(ctx, aux.data());
+      write_reduced_map(fmt("%s_NVAR%s", outdir, termin_out), nside, aux.data(), indpix);
+      sanepic_get_setup(ctx, hits.data(), nullptr, nullptr);
+      sanepic_get_sumcross(ctx, sumc.data());
+      for (long i = 0; i < npix; i++) tmp[i] = double(hits[i]);
+      write_reduced_map(fmt("%s_HITS%s", outdir, termin_out), nside, tmp.data(), indpix);
+      write_reduced_map(fmt("%s_COS2P_%s", outdir, termin_out), nside, sumc.data(), indpix);
+      write_reduced_map(fmt("%s_SIN2P_%s", outdir, termin_out), nside, sumc.data() + npix, indpix);
+      write_reduced_map(fmt("%s_Q_NVAR%s", outdir, termin_out), nside, aux.data() + npix, indpix);
+      write_reduced_map(fmt("%s_U_NVAR%s", outdir, termin_out), nside, aux.data() + 2 * npix, indpix);
+      for (long i = 0; i < npix; i++) tmp[i] = double(hits[i + npix]);
+      write_reduced_map(fmt("%s_POL_HITS%s", outdir, termin_out), nside, tmp.data(), indpix);
+    }
+    if (iterw && (iter % iterw) == 0 && writeIterOn) {  // 1711-1751
+      sanepic_get_map(ctx, S.data());
+      write_reduced_map(fmt("%s_I%s_%d", outdir, termin_out, iter), nside, S.data(), indpix);
+      write_reduced_map(fmt("%s_Q%s_%d", outdir, termin_out, iter), nside, S.data() + npix, indpix);
+      write_reduced_map(fmt("%s_U%s_%d", outdir, termin_out, iter), nside, S.data() + 2 * npix, indpix);
+    }
+    iter++;
+  }
+  printf("\n");
+  const double tloop = now() - tl0;
+  // 1773-1817
+  sanepic_get_map(ctx, S.data());
+  write_raw(fmt("%s_A_%s_%d", outdir, termin_out, 1), S.data(), npixtot);
+  write_reduced_map(fmt("%s_I_%s_%d", outdir, termin_out, 1), nside, S.data(), indpix);
+  write_reduced_map(fmt("%s_Q_%s_%d", outdir, termin_out, 1), nside, S.data() + npix, indpix);
+  write_reduced_map(fmt("%s_U_%s_%d", outdir, termin_out, 1), nside, S.data() + 2 * npix, indpix);
+  printf("Map-making Ends \n");
+  fprintf(stderr,
+          "sanepic: %d iterations, set-up %.3f s (file I/O %.3f s), CG loop %.3f s (%.3f ms/iter), "
+          "%ld deferred pixel samples, %.2f GB on device\n",
+          iter, tsetup, tio, tloop, 1e3 * tloop / std::max(iter, 1), (long)ndef,
+          [&] { int64_t b = 0; sanepic_device_bytes(ctx, &b); return b / 1e9; }());
+  sanepic_destroy(ctx);
+  return 0;
+}

@@ -1,0 +1,124 @@
+"""GPU path (C ABI) against the CPU oracle on the same seeded inputs."""
+import numpy as np
+import pytest
+
+import sanepic_amd
+from tests.helpers import make_case, oracle_problem, rel
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    # nside, ndet, nseg, ns  -- single-pass FFT sizes and four-step sizes
+    dict(nside=8, ndet=2, nseg=3, ns=1024),
+    dict(nside=16, ndet=1, nseg=4, ns=256),
+    dict(nside=16, ndet=2, nseg=2, ns=2 ** 13),
+    dict(nside=32, ndet=1, nseg=3, ns=2 ** 16),
+    dict(nside=32, ndet=2, nseg=2, ns=2 ** 17),
+]
+
+
+def _gpu(cfg, ds, precision=sanepic_amd.FP64):
+    ctx = sanepic_amd.Context(cfg.nside, cfg.ndet, cfg.nseg, cfg.ns, precision=precision)
+    ctx.load(cfg, ds)
+    npix, ndef = ctx.pixelize()
+    ctx.setup()
+    return ctx, npix, ndef
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: "n{nside}_d{ndet}_s{nseg}_l{ns}".format(**c))
+def test_pixels_hits_operator(case):
+    print("case", case, flush=True)
+    cfg, ds = make_case(**case)
+    P = oracle_problem(cfg, ds)
+    ctx, npix, ndef = _gpu(cfg, ds)
+    # pixel indices and the seen-pixel set: bit-exact
+    dp, _, _, ind = P.pixels()
+    assert npix == P.npix
+    assert np.array_equal(ctx.all_pixels(), dp)
+    assert np.array_equal(ctx.indpix(), ind)
+    # hits (doubled), crosslinks, diagonal preconditioner
+    ax, hits, mp, sc = P.setup_pass()
+    ghits, gmp, gsc = ctx.get_setup()
+    assert np.array_equal(ghits, hits)
+    assert rel(gsc, sc) < 1e-12
+    assert rel(gmp, mp) < 1e-12
+    # b = P^T N^-1 d and A x on a random map
+    assert rel(ctx.rhs(), P.rhs()) < 1e-11
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal(3 * npix)
+    assert rel(ctx.apply(x), P.apply_A(x)) < 1e-11
+    # chi2 of a map
+    c_or = P.chi2(x)
+    assert abs(ctx.chi2(x) - c_or) / c_or < 1e-10
+
+
+def _masked(v, hits, npix):
+    v = v.copy()
+    m = hits[npix:] < 4
+    v[npix:2 * npix][m] = 0.0
+    v[2 * npix:][m] = 0.0
+    return v
+
+
+def _dense_solution(P, b, hits, npix):
+    """x* of the masked system from the oracle's dense A (tiny cases only)."""
+    n3 = 3 * npix
+    keep = np.ones(n3, bool)
+    keep[npix:2 * npix][hits[npix:] < 4] = False
+    keep[2 * npix:][hits[npix:] < 4] = False
+    idx = np.flatnonzero(keep)
+    A = np.zeros((idx.size, idx.size))
+    for j, col in enumerate(idx):
+        e = np.zeros(n3)
+        e[col] = 1.0
+        A[:, j] = P.apply_A(e)[idx]
+    A = 0.5 * (A + A.T)
+    w, V = np.linalg.eigh(A)
+    x = np.zeros(n3)
+    x[idx] = V @ ((V.T @ b[idx]) / w)
+    return x, A, idx
+
+
+@pytest.mark.parametrize("case", CASES[:3], ids=lambda c: "n{nside}_d{ndet}_s{nseg}_l{ns}".format(**c))
+def test_solve_matches_oracle(case):
+    """The GPU PCG follows the reference's PCG.
+
+    The reference system is ill-conditioned by construction (the DC bin gets
+    1e-6 of the first bin's weight, sanepic.cc:537, so segment offsets and the
+    map monopole are nearly free), so two correct FP64 solvers that differ
+    only in summation order drift apart after a few iterations: the oracle
+    itself run with 1 and 4 threads is the yardstick.  Checks: (1) early
+    iterates agree within that spread, (2) after a fixed number of iterations
+    the GPU's A-norm error to the exact solution is as small as the
+    oracle's, (3) the masked outputs NVAR and crosslinks match."""
+    print("case", case, flush=True)
+    cfg, ds = make_case(**case)
+    P1 = oracle_problem(cfg, ds, nthreads=1)
+    P4 = oracle_problem(cfg, ds, nthreads=4)
+    ctx, npix, _ = _gpu(cfg, ds)
+    K = 8
+    r1 = P1.solve(stop=0.0, maxiter=K)
+    r4 = P4.solve(stop=0.0, maxiter=K)
+    SK, hK = ctx.solve(stop=0.0, maxiter=K)
+    spread = rel(r4["S"], r1["S"])
+    print(f"  after {K} its: oracle 1-vs-4-thread spread {spread:.2e}, gpu-oracle {rel(SK, r1['S']):.2e}")
+    assert rel(SK, r1["S"]) < max(100 * spread, 1e-9)
+    crit = np.array([h["crit"] for h in hK])
+    assert np.allclose(crit, r1["crit"], rtol=max(100 * spread, 1e-8), atol=0)
+    chi2 = np.array([h["chi2"] for h in hK])
+    assert np.allclose(chi2[:1], r1["chi2"][:1], rtol=1e-10, atol=0)
+    # convergence in the A-norm against the exact solution
+    b, hits = r1["b"], r1["hits"]
+    xs, A, idx = _dense_solution(P1, b, hits, npix)
+    it = 150
+    ro = P1.solve(stop=0.0, maxiter=it)
+    Sg, _ = ctx.solve(stop=0.0, maxiter=it)
+
+    def anorm(e):
+        return np.sqrt(abs(e[idx] @ A @ e[idx]))
+
+    eo, eg = anorm(ro["S"] - xs) / anorm(xs), anorm(Sg - xs) / anorm(xs)
+    print(f"  after {it} its: A-norm error oracle {eo:.2e} gpu {eg:.2e}")
+    assert eg <= 3.0 * eo + 1e-10
+    assert rel(ctx.nvar(), ro["Mptot"]) < 1e-12
+    assert rel(ctx.sumcross(), ro["sumCross"]) < 1e-12
