@@ -110,6 +110,11 @@ int sanepic_setup(sanepic_ctx *ctx);
  * (sum cos 2psi, sum sin 2psi, 0) -- all unmasked.  Any pointer may be NULL. */
 int sanepic_get_setup(sanepic_ctx *ctx, int64_t *hits, double *mp,
                       double *sumcross);
+/* Per-pixel 3x3 diagonal blocks of A, blocks[npix][6] = {B00, B01, B02,
+ * B11, B12, B22} (all lags of the circulant N^-1, unmasked): the block-Jacobi
+ * preconditioner before inversion.  No counterpart in the reference, whose
+ * preconditioner is the I diagonal only (compute_diagPtNPCorr, 2680-2762). */
+int sanepic_get_blocks(sanepic_ctx *ctx, double *blocks);
 /* b (unmasked) as accumulated by do_PtNd (1112). */
 int sanepic_get_rhs(sanepic_ctx *ctx, double *b);
 
@@ -122,11 +127,18 @@ int sanepic_chi2(sanepic_ctx *ctx, const double *x, double *chi2);
 typedef struct sanepic_solve_opts {
   int32_t precond;  /* SANEPIC_PRECOND_* */
   int32_t refresh;  /* explicit residual every `refresh` iterations (10) */
+  int32_t deflate;  /* 1: two-level preconditioner that solves the map's I
+                       monopole exactly (A-DEF2 with the single coarse vector
+                       Z = 1 on I).  The DC bin's 1e-6 weight (sanepic.cc:537)
+                       makes that mode almost null; deflating it does not
+                       change the solution, only the iteration count.
+                       0 reproduces the reference's PCG. */
 } sanepic_solve_opts;
 
 /* Start the PCG of sanepic.cc:1316-1410 from x0 (compressed 3 npix, NULL for
  * zero): masks, r0 = b - A x0, d0 = M r0, delta0, var0 = |r0|^2,
- * var00 = |b|^2. */
+ * var00 = |b|^2.  With deflate, x0 first gets the exact monopole correction
+ * and every preconditioner application adds the coarse step. */
 int sanepic_solve_begin(sanepic_ctx *ctx, const sanepic_solve_opts *opts,
                         const double *x0);
 

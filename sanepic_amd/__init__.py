@@ -25,7 +25,7 @@ ABI_FUNCTIONS = [
     "sanepic_create", "sanepic_destroy", "sanepic_last_error", "sanepic_abi_version",
     "sanepic_set_detector", "sanepic_set_pointing", "sanepic_set_tod", "sanepic_pixelize",
     "sanepic_get_pixels", "sanepic_get_indpix", "sanepic_setup", "sanepic_get_setup",
-    "sanepic_get_rhs", "sanepic_apply", "sanepic_chi2", "sanepic_solve_begin",
+    "sanepic_get_blocks", "sanepic_get_rhs", "sanepic_apply", "sanepic_chi2", "sanepic_solve_begin",
     "sanepic_solve_step", "sanepic_get_map", "sanepic_get_nvar", "sanepic_get_sumcross",
     "sanepic_true_residual", "sanepic_set_profiling", "sanepic_kernel_time",
     "sanepic_device_bytes",
@@ -40,7 +40,8 @@ class Config(ctypes.Structure):
 
 
 class SolveOpts(ctypes.Structure):
-    _fields_ = [("precond", ctypes.c_int32), ("refresh", ctypes.c_int32)]
+    _fields_ = [("precond", ctypes.c_int32), ("refresh", ctypes.c_int32),
+                ("deflate", ctypes.c_int32)]
 
 
 class IterInfo(ctypes.Structure):
@@ -82,6 +83,7 @@ def lib():
             "sanepic_get_indpix": [P, i64p],
             "sanepic_setup": [P],
             "sanepic_get_setup": [P, i64p, d_p, d_p],
+            "sanepic_get_blocks": [P, d_p],
             "sanepic_get_rhs": [P, d_p],
             "sanepic_apply": [P, d_p, d_p],
             "sanepic_chi2": [P, d_p, d_p],
@@ -209,6 +211,11 @@ class Context:
         self._chk(lib().sanepic_get_setup(self.h, _ip(hits), _dp(mp), _dp(sc)), "get_setup")
         return hits, mp, sc
 
+    def blocks(self):
+        out = np.empty(6 * self.npix)
+        self._chk(lib().sanepic_get_blocks(self.h, _dp(out)), "get_blocks")
+        return out.reshape(self.npix, 6)
+
     def rhs(self):
         b = np.empty(3 * self.npix)
         self._chk(lib().sanepic_get_rhs(self.h, _dp(b)), "get_rhs")
@@ -226,8 +233,8 @@ class Context:
         self._chk(lib().sanepic_chi2(self.h, _dp(x), ctypes.byref(out)), "chi2")
         return out.value
 
-    def solve_begin(self, precond=PRECOND_DIAG, refresh=10, x0=None):
-        opts = SolveOpts(precond, refresh)
+    def solve_begin(self, precond=PRECOND_DIAG, refresh=10, x0=None, deflate=False):
+        opts = SolveOpts(precond, refresh, 1 if deflate else 0)
         xp = None if x0 is None else _dp(_f64(x0))
         self._x0_keep = None if x0 is None else _f64(x0)
         self._chk(lib().sanepic_solve_begin(self.h, ctypes.byref(opts), xp), "solve_begin")
@@ -238,9 +245,10 @@ class Context:
         return dict(iter=info.iter, crit=info.crit, crit2=info.crit2, chi2=info.chi2,
                     alpha=info.alpha, beta=info.beta)
 
-    def solve(self, precond=PRECOND_DIAG, stop=1e-15, maxiter=2000, refresh=10, x0=None):
+    def solve(self, precond=PRECOND_DIAG, stop=1e-15, maxiter=2000, refresh=10, x0=None,
+              deflate=False):
         """The reference's loop: while ((iter < maxiter && crit > stop) || iter == 0)."""
-        self.solve_begin(precond, refresh, x0)
+        self.solve_begin(precond, refresh, x0, deflate)
         hist = []
         while True:
             info = self.step()

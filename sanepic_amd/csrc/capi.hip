@@ -74,6 +74,8 @@ struct sanepic_ctx {
   CGScalars *sc = nullptr;
   int *zero_flag = nullptr;
   int iter = 0, refresh = 10, precond = 0;
+  bool deflate = false;
+  double *az = nullptr;
   bool begun = false, setup_done = false, pixelized = false;
   bool prof = false;
   std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> evs;
@@ -746,7 +748,7 @@ int sanepic_pixelize(sanepic_ctx *c, int64_t *npix_out, int64_t *n_deferred) {
   }
   c->items = Items{c->it_pix, c->it_beg, c->it_len, c->it_slot, nit, c->sp_pix, c->sp_first, c->sp_count, nsp};
   c->nslots = nslots;
-  ACHK(c, dalloc(c, c->partial, 6 * std::max<int64_t>(1, nslots)));
+  ACHK(c, dalloc(c, c->partial, 8 * std::max<int64_t>(1, nslots)));
   HIPCHK(c, hipStreamSynchronize(c->st));
   c->pixelized = true;
   if (npix_out) *npix_out = npix;
@@ -821,7 +823,7 @@ int sanepic_setup(sanepic_ctx *c) {
   ACHK(c, dalloc(c, c->cross, 2 * npix));
   LCHK(c, launch_crosslink(c->items, c->csfp_sorted ? c->csfp_sorted : c->cs_sorted, c->cross, c->partial,
                            c->st));
-  ACHK(c, dalloc(c, c->pairs6, 6 * npix));
+  ACHK(c, dalloc(c, c->pairs6, 8 * npix));
   PairSumArgs pa{};
   pa.it = c->items;
   pa.perm = c->perm;
@@ -843,7 +845,7 @@ int sanepic_setup(sanepic_ctx *c) {
   ACHK(c, dalloc(c, c->q, 3 * npix));
   ACHK(c, dalloc(c, c->M, 6 * npix));
   ACHK(c, dalloc(c, c->nvar, 3 * npix));
-  ACHK(c, dalloc(c, c->part, 4 * 512));
+  ACHK(c, dalloc(c, c->part, 8 * 512));
   if (c->cfg.precision == SANEPIC_MIXED) {
     if (!c->single) ACHK(c, dalloc(c, c->buf32, c->g.npairs * N));
     ACHK(c, dalloc(c, c->z32, ntot));
@@ -863,10 +865,10 @@ int sanepic_get_setup(sanepic_ctx *c, int64_t *hits, double *mp, double *sumcros
       hits[npix + p] = 2 * (int64_t)c->h_cnt[p];
     }
   if (mp) {
-    std::vector<double> p6(6 * npix);
+    std::vector<double> p6(8 * npix);
     ACHK(c, d2h(c, p6.data(), c->pairs6, p6.size() * sizeof(double)));
     for (int64_t p = 0; p < npix; p++) {
-      const double mpI = 2.0 * p6[6 * p];
+      const double mpI = 2.0 * p6[8 * p + 6];
       mp[p] = mpI;
       mp[npix + p] = mpI / 2.0;
       mp[2 * npix + p] = mpI / 2.0;
@@ -881,6 +883,16 @@ int sanepic_get_setup(sanepic_ctx *c, int64_t *hits, double *mp, double *sumcros
       sumcross[2 * npix + p] = 0.0;
     }
   }
+  return SANEPIC_OK;
+}
+
+int sanepic_get_blocks(sanepic_ctx *c, double *blocks) {
+  ACHK(c, check_ready(c, true));
+  const int64_t npix = c->npix;
+  std::vector<double> p8(8 * npix);
+  ACHK(c, d2h(c, p8.data(), c->pairs6, p8.size() * sizeof(double)));
+  for (int64_t p = 0; p < npix; p++)
+    for (int k = 0; k < 6; k++) blocks[6 * p + k] = p8[8 * p + k];
   return SANEPIC_OK;
 }
 
@@ -931,7 +943,7 @@ static CGArgs cg_args(sanepic_ctx *c) {
   a.cnt = c->cnt;
   a.sc = c->sc;
   a.part = c->part;
-  a.nblocks = 512;
+  a.az = c->deflate ? c->az : nullptr;
   return a;
 }
 
@@ -939,6 +951,7 @@ int sanepic_solve_begin(sanepic_ctx *c, const sanepic_solve_opts *opts, const do
   ACHK(c, check_ready(c, true));
   c->precond = opts ? opts->precond : SANEPIC_PRECOND_DIAG;
   c->refresh = (opts && opts->refresh > 0) ? opts->refresh : 10;
+  c->deflate = opts ? opts->deflate != 0 : false;
   if (c->precond != SANEPIC_PRECOND_DIAG && c->precond != SANEPIC_PRECOND_BLOCK3)
     return fail(c, SANEPIC_EINVAL, "unknown preconditioner");
   const int64_t npix = c->npix;
@@ -947,8 +960,7 @@ int sanepic_solve_begin(sanepic_ctx *c, const sanepic_solve_opts *opts, const do
                                                                                         : SANEPIC_PRECOND_BLOCK3_K,
                             c->M, c->nvar, c->zero_flag, c->st));
   int zf = 0;
-  HIPCHK(c, hipMemcpyAsync(&zf, c->zero_flag, sizeof(int), hipMemcpyDeviceToHost, c->st));
-  HIPCHK(c, hipStreamSynchronize(c->st));
+  ACHK(c, d2h(c, &zf, c->zero_flag, sizeof(int)));
   if (zf) return fail(c, SANEPIC_EDATA, "ERROR: Mp has elements = 0 (sanepic.cc:1344-1348)");
   HIPCHK(c, hipMemsetAsync(c->q, 0, 3 * npix * sizeof(double), c->st));
   if (x0) {
@@ -957,6 +969,15 @@ int sanepic_solve_begin(sanepic_ctx *c, const sanepic_solve_opts *opts, const do
     LCHK(c, cg_launch_mask(c->x, c->cnt, npix, c->st));
   } else {
     HIPCHK(c, hipMemsetAsync(c->x, 0, 3 * npix * sizeof(double), c->st));
+  }
+  if (c->deflate) {
+    // AZ = A e_I (masked) once; r, d serve as scratch before cg_init
+    ACHK(c, dalloc(c, c->az, 3 * npix));
+    LCHK(c, cg_launch_monopole(c->d, npix, c->st));
+    ACHK(c, apply_A(c, c->d, c->az, 1, false, true));
+    CGArgs a = cg_args(c);
+    LCHK(c, cg_launch_coarse_setup(a, c->st));
+    LCHK(c, cg_launch_coarse_start(a, c->st));
   }
   CGArgs a = cg_args(c);
   LCHK(c, cg_launch_init(a, c->st));

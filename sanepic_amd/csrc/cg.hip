@@ -43,6 +43,15 @@ __device__ __forceinline__ void apply_m(const double *M, const double *r, double
   z[2] = M[2] * r[0] + M[4] * r[1] + M[5] * r[2];
 }
 
+constexpr int NP = 8;  // partial slots per block
+
+__device__ __forceinline__ void store_part(double *part, const double (&acc)[5]) {
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NP; k++) part[NP * blockIdx.x + k] = k < 5 ? acc[k] : 0.0;
+  }
+}
+
 __global__ __launch_bounds__(RBS) void k_mask(double *v, const int32_t *cnt, int64_t npix) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p < npix && cnt[p] < 2) {
@@ -51,153 +60,187 @@ __global__ __launch_bounds__(RBS) void k_mask(double *v, const int32_t *cnt, int
   }
 }
 
-// r = b - q, d = M r; partials: delta = r.d, var = r.r, var00 = b.b
-__global__ __launch_bounds__(RBS) void k_init(CGArgs a) {
-  __shared__ double sh[3][RBS / 64];
-  double acc[3] = {0.0, 0.0, 0.0};
-  for (int64_t p = (int64_t)blockIdx.x * RBS + threadIdx.x; p < a.npix; p += (int64_t)RGRID * RBS) {
-    double r[3], z[3];
+// Per-pixel residual terms shared by init / update / refresh:
+// acc[0] += r.Mr, acc[1] += r.r, acc[3] += r_I (Z^T r), acc[4] += AZ.Mr
+__device__ __forceinline__ void resid_terms(const CGArgs &a, int64_t p, const double (&r)[3],
+                                            double (&acc)[5]) {
+  double z[3];
+  apply_m(a.M + 6 * p, r, z);
 #pragma unroll
-    for (int c = 0; c < 3; c++) r[c] = a.b[3 * p + c] - a.q[3 * p + c];
-    apply_m(a.M + 6 * p, r, z);
+  for (int c = 0; c < 3; c++) {
+    acc[0] += r[c] * z[c];
+    acc[1] += r[c] * r[c];
+  }
+  if (a.az) {
+    acc[3] += r[0];
+#pragma unroll
+    for (int c = 0; c < 3; c++) acc[4] += a.az[3 * p + c] * z[c];
+  }
+}
+
+// r = b - q; partials of r.Mr, r.r, b.b, Z^T r, AZ.Mr
+__global__ __launch_bounds__(RBS) void k_init(CGArgs a) {
+  __shared__ double sh[5][RBS / 64];
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int64_t p = (int64_t)blockIdx.x * RBS + threadIdx.x; p < a.npix; p += (int64_t)RGRID * RBS) {
+    double r[3];
 #pragma unroll
     for (int c = 0; c < 3; c++) {
+      r[c] = a.b[3 * p + c] - a.q[3 * p + c];
       a.r[3 * p + c] = r[c];
-      a.d[3 * p + c] = z[c];
-      acc[0] += r[c] * z[c];
-      acc[1] += r[c] * r[c];
       acc[2] += a.b[3 * p + c] * a.b[3 * p + c];
     }
+    resid_terms(a, p, r, acc);
   }
-  block_sums<3>(acc, sh);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < 3; k++) a.part[4 * blockIdx.x + k] = acc[k];
-  }
+  block_sums<5>(acc, sh);
+  store_part(a.part, acc);
 }
 
-// Partials of q.d
-__global__ __launch_bounds__(RBS) void k_dot_qd(CGArgs a) {
-  __shared__ double sh[1][RBS / 64];
-  double acc[1] = {0.0};
-  for (int64_t i = (int64_t)blockIdx.x * RBS + threadIdx.x; i < 3 * a.npix; i += (int64_t)RGRID * RBS)
-    acc[0] += a.q[i] * a.d[i];
-  block_sums<1>(acc, sh);
-  if (threadIdx.x == 0) a.part[4 * blockIdx.x] = acc[0];
+// Partials of u.v (slot 0), or with isum of the I components of v - u (slot 3)
+__global__ __launch_bounds__(RBS) void k_dot(CGArgs a, const double *u, const double *v, int isum) {
+  __shared__ double sh[5][RBS / 64];
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  if (isum) {  // sum of the I components of v - u
+    for (int64_t p = (int64_t)blockIdx.x * RBS + threadIdx.x; p < a.npix; p += (int64_t)RGRID * RBS)
+      acc[3] += v[3 * p] - (u ? u[3 * p] : 0.0);
+  } else {
+    for (int64_t i = (int64_t)blockIdx.x * RBS + threadIdx.x; i < 3 * a.npix; i += (int64_t)RGRID * RBS)
+      acc[0] += u[i] * v[i];
+  }
+  block_sums<5>(acc, sh);
+  store_part(a.part, acc);
 }
 
-// Sum the partials in block order.  what: 0 init, 1 alpha, 2 beta, 3 resid.
+// Sum the partials in block order and update the scalars.
+//   what 0: start (r0 set): delta0, var0, var00, coarse coefficient
+//   what 1: alpha = delta_n / q.d
+//   what 2: new residual: delta, var, beta, coarse coefficient
+//   what 3: true residual |b - q|^2, |b|^2 into tmp[0..1]
+//   what 4: einv = 1 / (Z^T A Z) from the I-sum of AZ
+//   what 5: y0 = einv * Z^T (b - q): starting coarse correction into tmp[2]
 __global__ __launch_bounds__(RBS) void k_final(CGArgs a, int what) {
-  __shared__ double sh[4][RBS / 64];
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  __shared__ double sh[5][RBS / 64];
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   for (int i = threadIdx.x; i < RGRID; i += RBS) {
 #pragma unroll
-    for (int k = 0; k < 4; k++) acc[k] += a.part[4 * i + k];
+    for (int k = 0; k < 5; k++) acc[k] += a.part[NP * i + k];
   }
-  block_sums<4>(acc, sh);
+  block_sums<5>(acc, sh);
   if (threadIdx.x != 0) return;
   CGScalars *sc = a.sc;
-  if (what == 0) {
-    sc->delta_n = acc[0];
-    sc->delta0 = acc[0];
-    sc->var_n = acc[1];
-    sc->var0 = acc[1];
-    sc->var00 = acc[2];
+  if (what == 0 || what == 2) {
+    const double c = a.az ? sc->einv * (acc[3] - acc[4]) : 0.0;  // A-DEF2 coarse step
+    const double delta = acc[0] + c * acc[3];
+    sc->coarse = c;
+    if (what == 0) {
+      sc->delta_n = delta;
+      sc->delta0 = delta;
+      sc->var_n = acc[1];
+      sc->var0 = acc[1];
+      sc->var00 = acc[2];
+      sc->beta = 0.0;
+    } else {
+      sc->delta_o = sc->delta_n;
+      sc->delta_n = delta;
+      sc->var_n = acc[1];
+      sc->beta = sc->delta_n / sc->delta_o;
+    }
   } else if (what == 1) {
     sc->rtq = acc[0];
     sc->alpha = sc->delta_n / acc[0];
-  } else if (what == 2) {
-    sc->delta_o = sc->delta_n;
-    sc->delta_n = acc[0];
-    sc->var_n = acc[1];
-    sc->beta = sc->delta_n / sc->delta_o;
-  } else {
+  } else if (what == 3) {
     sc->tmp[0] = acc[0];
     sc->tmp[1] = acc[1];
+  } else if (what == 4) {
+    sc->einv = acc[3] != 0.0 ? 1.0 / acc[3] : 0.0;
+  } else if (what == 5) {
+    sc->tmp[2] = sc->einv * acc[3];
   }
 }
 
-// x += alpha d; unless refresh: r -= alpha q, then partials of r.Mr and r.r
+// x += alpha d; unless refresh: r -= alpha q and the residual partials
 __global__ __launch_bounds__(RBS) void k_update(CGArgs a, int refresh) {
-  __shared__ double sh[2][RBS / 64];
+  __shared__ double sh[5][RBS / 64];
   const double alpha = a.sc->alpha;
-  double acc[2] = {0.0, 0.0};
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   for (int64_t p = (int64_t)blockIdx.x * RBS + threadIdx.x; p < a.npix; p += (int64_t)RGRID * RBS) {
 #pragma unroll
     for (int c = 0; c < 3; c++) a.x[3 * p + c] += alpha * a.d[3 * p + c];
     if (!refresh) {
-      double r[3], z[3];
+      double r[3];
 #pragma unroll
       for (int c = 0; c < 3; c++) {
         r[c] = a.r[3 * p + c] - alpha * a.q[3 * p + c];
         a.r[3 * p + c] = r[c];
       }
-      apply_m(a.M + 6 * p, r, z);
-#pragma unroll
-      for (int c = 0; c < 3; c++) {
-        acc[0] += r[c] * z[c];
-        acc[1] += r[c] * r[c];
-      }
+      resid_terms(a, p, r, acc);
     }
   }
-  block_sums<2>(acc, sh);
-  if (threadIdx.x == 0) {
-    a.part[4 * blockIdx.x] = acc[0];
-    a.part[4 * blockIdx.x + 1] = acc[1];
-  }
+  block_sums<5>(acc, sh);
+  store_part(a.part, acc);
 }
 
-// r = b - q (q = A x, masked), partials of r.Mr and r.r
+// r = b - q (q = A x, masked) and the residual partials
 __global__ __launch_bounds__(RBS) void k_refresh(CGArgs a) {
-  __shared__ double sh[2][RBS / 64];
-  double acc[2] = {0.0, 0.0};
+  __shared__ double sh[5][RBS / 64];
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   for (int64_t p = (int64_t)blockIdx.x * RBS + threadIdx.x; p < a.npix; p += (int64_t)RGRID * RBS) {
-    double r[3], z[3];
+    double r[3];
 #pragma unroll
     for (int c = 0; c < 3; c++) {
       r[c] = a.b[3 * p + c] - a.q[3 * p + c];
       a.r[3 * p + c] = r[c];
     }
-    apply_m(a.M + 6 * p, r, z);
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-      acc[0] += r[c] * z[c];
-      acc[1] += r[c] * r[c];
-    }
+    resid_terms(a, p, r, acc);
   }
-  block_sums<2>(acc, sh);
-  if (threadIdx.x == 0) {
-    a.part[4 * blockIdx.x] = acc[0];
-    a.part[4 * blockIdx.x + 1] = acc[1];
-  }
+  block_sums<5>(acc, sh);
+  store_part(a.part, acc);
 }
 
-// d = M r + beta d
-__global__ __launch_bounds__(RBS) void k_dir(CGArgs a) {
-  const double beta = a.sc->beta;
+// d = M r + Z c + beta d  (first: d = M r + Z c)
+__global__ __launch_bounds__(RBS) void k_dir(CGArgs a, int first) {
+  const double beta = a.sc->beta, c0 = a.sc->coarse;
   for (int64_t p = (int64_t)blockIdx.x * RBS + threadIdx.x; p < a.npix; p += (int64_t)RGRID * RBS) {
     double r[3], z[3];
 #pragma unroll
     for (int c = 0; c < 3; c++) r[c] = a.r[3 * p + c];
     apply_m(a.M + 6 * p, r, z);
+    z[0] += c0;
 #pragma unroll
-    for (int c = 0; c < 3; c++) a.d[3 * p + c] = z[c] + beta * a.d[3 * p + c];
+    for (int c = 0; c < 3; c++) a.d[3 * p + c] = first ? z[c] : z[c] + beta * a.d[3 * p + c];
   }
 }
 
 // partials of |b - q|^2 and |b|^2
 __global__ __launch_bounds__(RBS) void k_resid(CGArgs a) {
-  __shared__ double sh[2][RBS / 64];
-  double acc[2] = {0.0, 0.0};
+  __shared__ double sh[5][RBS / 64];
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   for (int64_t i = (int64_t)blockIdx.x * RBS + threadIdx.x; i < 3 * a.npix; i += (int64_t)RGRID * RBS) {
     const double e = a.b[i] - a.q[i];
     acc[0] += e * e;
     acc[1] += a.b[i] * a.b[i];
   }
-  block_sums<2>(acc, sh);
-  if (threadIdx.x == 0) {
-    a.part[4 * blockIdx.x] = acc[0];
-    a.part[4 * blockIdx.x + 1] = acc[1];
+  block_sums<5>(acc, sh);
+  store_part(a.part, acc);
+}
+
+// x_I += tmp[2]; q += tmp[2] * AZ  (starting coarse correction)
+__global__ __launch_bounds__(RBS) void k_coarse_start(CGArgs a) {
+  const double y = a.sc->tmp[2];
+  for (int64_t p = (int64_t)blockIdx.x * RBS + threadIdx.x; p < a.npix; p += (int64_t)RGRID * RBS) {
+    a.x[3 * p] += y;
+#pragma unroll
+    for (int c = 0; c < 3; c++) a.q[3 * p + c] += y * a.az[3 * p + c];
+  }
+}
+
+// e_I: 1 on every I component, 0 on Q and U (the monopole coarse vector)
+__global__ __launch_bounds__(RBS) void k_monopole(double *v, int64_t npix) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < npix) {
+    v[3 * p] = 1.0;
+    v[3 * p + 1] = 0.0;
+    v[3 * p + 2] = 0.0;
   }
 }
 
@@ -211,12 +254,6 @@ __global__ __launch_bounds__(RBS) void k_reduce_chi2(const double *part, int64_t
 
 inline int ok() { return hipGetLastError() == hipSuccess ? 0 : -1; }
 
-// Partials beyond what a kernel writes must read as zero in k_final.
-__global__ void k_zero_part(double *part) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < 4 * RGRID) part[i] = 0.0;
-}
-
 }  // namespace
 
 int cg_launch_mask(double *v, const int32_t *cnt, int64_t npix, hipStream_t s) {
@@ -226,14 +263,14 @@ int cg_launch_mask(double *v, const int32_t *cnt, int64_t npix, hipStream_t s) {
 }
 
 int cg_launch_init(const CGArgs &a, hipStream_t s) {
-  hipLaunchKernelGGL(k_zero_part, dim3((4 * RGRID + 255) / 256), dim3(256), 0, s, a.part);
   hipLaunchKernelGGL(k_init, dim3(RGRID), dim3(RBS), 0, s, a);
   hipLaunchKernelGGL(k_final, dim3(1), dim3(RBS), 0, s, a, 0);
+  hipLaunchKernelGGL(k_dir, dim3(RGRID), dim3(RBS), 0, s, a, 1);
   return ok();
 }
 
 int cg_launch_alpha(const CGArgs &a, hipStream_t s) {
-  hipLaunchKernelGGL(k_dot_qd, dim3(RGRID), dim3(RBS), 0, s, a);
+  hipLaunchKernelGGL(k_dot, dim3(RGRID), dim3(RBS), 0, s, a, (const double *)a.q, (const double *)a.d, 0);
   hipLaunchKernelGGL(k_final, dim3(1), dim3(RBS), 0, s, a, 1);
   return ok();
 }
@@ -251,13 +288,34 @@ int cg_launch_refresh_r(const CGArgs &a, hipStream_t s) {
 }
 
 int cg_launch_dir(const CGArgs &a, hipStream_t s) {
-  hipLaunchKernelGGL(k_dir, dim3(RGRID), dim3(RBS), 0, s, a);
+  hipLaunchKernelGGL(k_dir, dim3(RGRID), dim3(RBS), 0, s, a, 0);
   return ok();
 }
 
 int cg_launch_resid(const CGArgs &a, hipStream_t s) {
   hipLaunchKernelGGL(k_resid, dim3(RGRID), dim3(RBS), 0, s, a);
   hipLaunchKernelGGL(k_final, dim3(1), dim3(RBS), 0, s, a, 3);
+  return ok();
+}
+
+int cg_launch_monopole(double *v, int64_t npix, hipStream_t s) {
+  if (npix > 0)
+    hipLaunchKernelGGL(k_monopole, dim3((unsigned)((npix + RBS - 1) / RBS)), dim3(RBS), 0, s, v, npix);
+  return ok();
+}
+
+int cg_launch_coarse_setup(const CGArgs &a, hipStream_t s) {
+  // einv from the I components of AZ
+  hipLaunchKernelGGL(k_dot, dim3(RGRID), dim3(RBS), 0, s, a, (const double *)nullptr, (const double *)a.az, 1);
+  hipLaunchKernelGGL(k_final, dim3(1), dim3(RBS), 0, s, a, 4);
+  return ok();
+}
+
+int cg_launch_coarse_start(const CGArgs &a, hipStream_t s) {
+  // y0 = einv * sum_I (b - q); x += y0 e_I; q += y0 AZ
+  hipLaunchKernelGGL(k_dot, dim3(RGRID), dim3(RBS), 0, s, a, (const double *)a.q, (const double *)a.b, 1);
+  hipLaunchKernelGGL(k_final, dim3(1), dim3(RBS), 0, s, a, 5);
+  hipLaunchKernelGGL(k_coarse_start, dim3(RGRID), dim3(RBS), 0, s, a);
   return ok();
 }
 
@@ -268,16 +326,17 @@ int launch_reduce_chi2(const double *part, int64_t n, CGScalars *sc, hipStream_t
 
 namespace {
 // Preconditioner blocks from the per-pixel pair sums p6 = {B00, B01, B02,
-// B11, B12, B22} (B00 = S, the reference's pair sum of one pass).
-// nvar gets the reference's inverted, masked diagonal (1/Mp with Mp = 2S for
-// I and S for Q, U; 0 where masked), 1351-1353.
+// B11, B12, B22, S, 0}: B is the exact diagonal block of A, S the
+// reference's truncated I pair sum of one pass.  nvar gets the reference's
+// inverted, masked diagonal (1/Mp with Mp = 2S for I and S for Q, U; 0 where
+// masked), 1351-1353; the DIAG preconditioner is exactly that diagonal.
 __global__ __launch_bounds__(RBS) void k_precond(const double *p6, const int32_t *cnt, int64_t npix,
                                                  int kind, double *M, double *nvar, int *zero) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= npix) return;
-  const double *b = p6 + 6 * p;
+  const double *b = p6 + 8 * p;
   const bool masked = cnt[p] < 2;
-  const double S = b[0];
+  const double S = b[6];
   if (S == 0.0) atomicOr(zero, 1);
   const double mpI = 2.0 * S, mpQ = mpI / 2.0;
   const double nI = mpI != 0.0 ? 1.0 / mpI : 0.0;

@@ -122,9 +122,9 @@ int launch_crosslink(const Items &it, const double2 *cs_sorted, double *out /*[n
                      double *partial, hipStream_t s);
 
 // Pair sums of the noise autocorrelation inside each (pixel, segment) group:
-// out[p] = {sum N_, sum N_ c_k, sum N_ s_k, sum c_j N_ c_k, c_j N_ s_k, s_j N_ s_k}
-// over pairs (j, k) with |t_j - t_k| < ns/2 (compute_diagPtNPCorr, 2680-2762,
-// extended to the full 3x3 block).
+// out[p][0..5] = the exact 3x3 diagonal block of A (all lags, circulant),
+// out[p][6] = the reference's I pair sum over |t_j - t_k| < ns/2
+// (compute_diagPtNPCorr, 2680-2762), out[p][7] = 0.
 struct PairSumArgs {
   Items it;
   const uint32_t *perm;
@@ -132,8 +132,8 @@ struct PairSumArgs {
   const int64_t *off;     // [npix+1] pixel offsets into the sorted list
   const double *nauto;    // [ndet][ns] c2r(|SpN|)
   int64_t ns, nseg;
-  double *out;            // [npix][6]
-  double *partial;        // [nslots][6]
+  double *out;            // [npix][8]
+  double *partial;        // [nslots][8]
 };
 int launch_pairsum(const PairSumArgs &a, hipStream_t s);
 
@@ -153,6 +153,7 @@ int launch_iota_u32(uint32_t *v, int64_t n, hipStream_t s);
 // CG on interleaved [npix][3] FP64 vectors with FP64 scalars on the device.
 struct CGScalars {
   double delta_n, delta_o, var_n, rtq, alpha, beta, var00, delta0, chi2, var0;
+  double einv, coarse;  // 1/(Z^T A Z) and the last coarse coefficient (A-DEF2)
   double tmp[6];
 };
 
@@ -162,8 +163,8 @@ struct CGArgs {
   const double *M;        // [npix][6] symmetric preconditioner blocks
   const int32_t *cnt;     // mask source
   CGScalars *sc;
-  double *part;           // [nblocks][4] partials
-  int64_t nblocks;
+  double *part;           // [512][8] partials
+  const double *az;       // A e_I for the monopole deflation, or null
 };
 
 int cg_launch_mask(double *v, const int32_t *cnt, int64_t npix, hipStream_t s);
@@ -173,6 +174,11 @@ int cg_launch_update(const CGArgs &a, int refresh, hipStream_t s);  // x += alph
 int cg_launch_refresh_r(const CGArgs &a, hipStream_t s); // r = b - q
 int cg_launch_dir(const CGArgs &a, hipStream_t s);       // d = M r + beta d
 int cg_launch_resid(const CGArgs &a, hipStream_t s);     // tmp[0] = |b - q|^2 (q = A x masked), tmp[1] = |b|^2
+// Monopole deflation (two-level A-DEF2 with Z = e_I): coarse vector, the
+// scalar 1/(Z^T A Z) from AZ, and the starting correction x += y0 Z.
+int cg_launch_monopole(double *v, int64_t npix, hipStream_t s);
+int cg_launch_coarse_setup(const CGArgs &a, hipStream_t s);
+int cg_launch_coarse_start(const CGArgs &a, hipStream_t s);
 enum { SANEPIC_PRECOND_DIAG_K = 0, SANEPIC_PRECOND_BLOCK3_K = 1 };
 int cg_launch_precond(const double *p6, const int32_t *cnt, int64_t npix, int kind, double *M,
                       double *nvar, int *zero, hipStream_t s);

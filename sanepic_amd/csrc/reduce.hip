@@ -115,8 +115,12 @@ __global__ __launch_bounds__(256) void k_crosslink(Items it, const double2 *cs, 
 }
 
 // Noise pair sums inside (pixel, detector-segment) groups.  Entries of one
-// pixel are sorted by sample index, so a group is a contiguous run and the
-// time offsets grow monotonically away from entry j.
+// pixel are sorted by sample index, so a group is a contiguous run.
+// out[p][0..5]: the exact 3x3 diagonal block of A for pixel p, i.e. the sum
+// over same-group pairs of N^-1[t_j, t_k] a_j a_k^T with the circulant
+// N^-1[t, t'] = nauto[(t - t') mod ns] and a = (1, cos 2psi, sin 2psi).
+// out[p][6]: the reference's I pair sum, which keeps only |t_j - t_k| < ns/2
+// (compute_diagPtNPCorr, sanepic.cc:2729-2752).
 __global__ __launch_bounds__(256) void k_pairsum(PairSumArgs a) {
   const int grp = threadIdx.x >> 4, lane = threadIdx.x & 15;
   const int64_t item = (int64_t)blockIdx.x * 16 + grp;
@@ -126,35 +130,35 @@ __global__ __launch_bounds__(256) void k_pairsum(PairSumArgs a) {
   const int n = a.it.len[item];
   const int64_t P0 = a.off[p], P1 = a.off[p + 1];
   const int64_t ns = a.ns, half = ns / 2;
-  double acc[6] = {0, 0, 0, 0, 0, 0};
+  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
   for (int k = lane; k < n; k += 16) {
     const int64_t j = e0 + k;
     const int64_t g = a.perm[j];
     const int64_t blk = g / ns, t = g % ns;
     const double *Na = a.nauto + (blk / a.nseg) * ns;
     const double2 cj = a.cs_sorted[j];
-    double u0 = 0.0, u1 = 0.0, u2 = 0.0;
+    double u0 = 0.0, u1 = 0.0, u2 = 0.0, ut = 0.0;
     for (int64_t jj = j; jj >= P0; jj--) {
       const int64_t gg = a.perm[jj];
       if (gg / ns != blk) break;
       const int64_t dt = t - gg % ns;
-      if (dt >= half) break;
       const double w = Na[dt];
       const double2 ck = a.cs_sorted[jj];
       u0 += w;
       u1 += w * ck.x;
       u2 += w * ck.y;
+      if (dt < half) ut += w;
     }
     for (int64_t jj = j + 1; jj < P1; jj++) {
       const int64_t gg = a.perm[jj];
       if (gg / ns != blk) break;
       const int64_t dt = gg % ns - t;
-      if (dt >= half) break;
       const double w = Na[dt];
       const double2 ck = a.cs_sorted[jj];
       u0 += w;
       u1 += w * ck.x;
       u2 += w * ck.y;
+      if (dt < half) ut += w;
     }
     acc[0] += u0;
     acc[1] += u1;
@@ -162,14 +166,16 @@ __global__ __launch_bounds__(256) void k_pairsum(PairSumArgs a) {
     acc[3] += cj.x * u1;
     acc[4] += cj.x * u2;
     acc[5] += cj.y * u2;
+    acc[6] += ut;
   }
 #pragma unroll
-  for (int v = 0; v < 6; v++) acc[v] = red16(acc[v]);
+  for (int v = 0; v < 7; v++) acc[v] = red16(acc[v]);
   if (lane == 0) {
     const int slot = a.it.slot[item];
-    double *o = slot < 0 ? a.out + 6 * (int64_t)p : a.partial + 6 * (int64_t)slot;
+    double *o = slot < 0 ? a.out + 8 * (int64_t)p : a.partial + 8 * (int64_t)slot;
 #pragma unroll
-    for (int v = 0; v < 6; v++) o[v] = acc[v];
+    for (int v = 0; v < 7; v++) o[v] = acc[v];
+    o[7] = 0.0;
   }
 }
 
@@ -272,7 +278,7 @@ int launch_crosslink(const Items &it, const double2 *cs_sorted, double *out, dou
 int launch_pairsum(const PairSumArgs &a, hipStream_t s) {
   if (a.it.n > 0) hipLaunchKernelGGL(k_pairsum, dim3(nblk(a.it.n, 16)), dim3(256), 0, s, a);
   if (a.it.nsplit > 0)
-    hipLaunchKernelGGL(k_fix<6>, dim3(nblk(a.it.nsplit)), dim3(256), 0, s, a.it, a.partial, a.out,
+    hipLaunchKernelGGL(k_fix<8>, dim3(nblk(a.it.nsplit)), dim3(256), 0, s, a.it, a.partial, a.out,
                        (const int32_t *)nullptr, 0);
   return ok();
 }
