@@ -50,6 +50,24 @@ def stream_bytes_per_sample(precision):
     return 4 + 2 * s
 
 
+def calibration_copy(nbytes=1 << 30):
+    """A plain device-to-device copy of known size (HIP's blit kernel), run
+    before the workload so hardware byte counters (FETCH_SIZE / WRITE_SIZE)
+    can be scaled by what they read for exactly nbytes in and nbytes out."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0  # device to device
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"calibration copy of {nbytes} bytes done")
+
+
 def cpu_baseline(wl, budget_s=20.0):
     """The oracle on a bounded sample: one detector, a few segments of the
     same length, same NSIDE; timed over full A-applications."""
@@ -90,6 +108,8 @@ def main():
     ap.add_argument("--no-deflate", action="store_true")
     ap.add_argument("--maxiter", type=int, default=600)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="run a 1 GiB device copy first (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -99,6 +119,8 @@ def main():
     import sanepic_amd
     from sanepic_amd import synth
 
+    if args.calib_copy:
+        calibration_copy()
     wl = WORKLOADS[args.workload]
     nseg = wl["nsamp"] // wl["ns"]
     cfg = synth.Config(nside=wl["nside"], ndet=wl["ndet"], nseg=nseg, ns=wl["ns"])
