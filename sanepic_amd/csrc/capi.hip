@@ -74,6 +74,7 @@ struct sanepic_ctx {
   CGScalars *sc = nullptr;
   int *zero_flag = nullptr;
   int iter = 0, refresh = 10, precond = 0;
+  int64_t chunk = 0;  // pairs per four-step chunk (0: all at once)
   bool deflate = false;
   double *az = nullptr;
   bool begun = false, setup_done = false, pixelized = false;
@@ -279,7 +280,15 @@ OpArgs<float> op_args<float>(sanepic_ctx *c) {
   return a;
 }
 
+int chunk_pairs() {
+  const char *e = getenv("SANEPIC_CHUNK_PAIRS");
+  return (e && *e) ? atoi(e) : 0;
+}
+
 // N^-1 pass over all pairs: src -> z (time order).  Returns chi2 block count.
+// With chunking, the three four-step passes run on groups of `chunk` pairs
+// that reuse one buffer region, so the intermediate stays in the Infinity
+// Cache instead of round-tripping through HBM.
 template <typename T>
 int run_filter(sanepic_ctx *c, OpArgs<T> a, int src, int mode, int64_t *nchi) {
   a.mode = mode;
@@ -288,17 +297,27 @@ int run_filter(sanepic_ctx *c, OpArgs<T> a, int src, int mode, int64_t *nchi) {
     ProfScope ps(c, 1);
     LCHK(c, launch_single<T>(a, src, c->st, &nb));
   } else {
-    {
-      ProfScope ps(c, 0);
-      LCHK(c, launch_col_fwd<T>(a, src, c->st));
-    }
-    {
-      ProfScope ps(c, 1);
-      LCHK(c, launch_row<T>(a, c->st, &nb));
-    }
-    {
-      ProfScope ps(c, 2);
-      LCHK(c, launch_col_inv<T>(a, c->st));
+    const int64_t np = a.g.npairs;
+    int64_t chunk = c->chunk > 0 ? std::min<int64_t>(c->chunk, np) : np;
+    for (int64_t p0 = 0; p0 < np; p0 += chunk) {
+      OpArgs<T> b = a;
+      b.g.pair0 = a.g.pair0 + p0;
+      b.g.npairs = std::min(chunk, np - p0);
+      b.chi2off = nb;
+      int64_t nbk = 0;
+      {
+        ProfScope ps(c, 0);
+        LCHK(c, launch_col_fwd<T>(b, src, c->st));
+      }
+      {
+        ProfScope ps(c, 1);
+        LCHK(c, launch_row<T>(b, c->st, &nbk));
+      }
+      {
+        ProfScope ps(c, 2);
+        LCHK(c, launch_col_inv<T>(b, c->st));
+      }
+      nb += nbk;
     }
   }
   if (nchi) *nchi = nb;
@@ -442,6 +461,7 @@ int sanepic_create(const sanepic_config *cfg, sanepic_ctx **out) {
   c->ndet = cfg->ndet;
   c->nside = cfg->nside;
   c->ntot = ntot;
+  c->chunk = chunk_pairs();
   if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
     g_create_error = "hipStreamCreate failed";
     delete c;

@@ -47,6 +47,7 @@ struct PairGeom {
   int64_t N;       // segment length
   int64_t N1, N2;  // row length, column length (N2 == 1: single pass)
   int log2N, log2N1, log2N2;
+  int64_t pair0;   // first pair of this launch (chunked four-step passes)
 };
 
 // Operator pass selection.
@@ -79,6 +80,7 @@ struct OpArgs {
   const V *tw4096;      // w_4096^j
   const V *twlo, *twhi; // w_N^j = twlo[j & 4095] * twhi[j >> 12]
   double *chi2part;     // per-block chi2 partials
+  int64_t chi2off;      // first partial slot of this launch
   int mode;
 };
 

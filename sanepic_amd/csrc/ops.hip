@@ -28,8 +28,11 @@ namespace {
 
 template <typename T> using VT = typename Vec2<T>::type;
 
+// pr is the launch-local pair; buffers of the four-step passes are indexed
+// locally (a chunk of pairs reuses one buffer), samples and spectra globally.
 __device__ __forceinline__ void pair_info(const PairGeom &g, int64_t pr, int &det, int64_t &ga,
                                           int64_t &gb) {
+  pr += g.pair0;
   det = (int)(pr / g.ppd);
   const int64_t j = pr % g.ppd;
   ga = ((int64_t)det * g.nseg + 2 * j) * g.N;
@@ -116,6 +119,7 @@ __global__ __launch_bounds__(256) void k_row(OpArgs<T> a) {
   int64_t ga, gb;
   pair_info(a.g, pr, det, ga, gb);
   const int64_t base = pr * N + k2 * N1;
+  const int64_t sbase = (pr + a.g.pair0) * N + k2 * N1;
   const T *w = a.wt + (int64_t)det * N + k2 * N1;
   const int mode = a.mode;
   V v[16];
@@ -131,13 +135,13 @@ __global__ __launch_bounds__(256) void k_row(OpArgs<T> a) {
     fft_reg<V, -1, LOG2N1, B>(v, t, lds + b, a.tw4096);
     if (mode & ROW_STORE_D) {
 #pragma unroll
-      for (int s = 0; s < 16; s++) a.dspec[base + t + TS * s] = v[s];
+      for (int s = 0; s < 16; s++) a.dspec[sbase + t + TS * s] = v[s];
     }
     if (mode & ROW_CHI2) {
 #pragma unroll
       for (int s = 0; s < 16; s++) {
         const int k1 = t + TS * s;
-        const V D = a.dspec[base + k1];
+        const V D = a.dspec[sbase + k1];
         const double rx = (double)D.x - (double)v[s].x, ry = (double)D.y - (double)v[s].y;
         double e = (double)w[k1] * (rx * rx + ry * ry);
         // bins 0 and N/2 are counted twice by the reference's per-segment sum
@@ -163,7 +167,7 @@ __global__ __launch_bounds__(256) void k_row(OpArgs<T> a) {
   }
   if (mode & ROW_CHI2) {
     const double tot = block_sum256(chi, red);
-    if (tid == 0) a.chi2part[blockIdx.x] = tot;
+    if (tid == 0) a.chi2part[a.chi2off + blockIdx.x] = tot;
   }
 }
 
