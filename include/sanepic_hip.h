@@ -133,6 +133,10 @@ typedef struct sanepic_solve_opts {
                        makes that mode almost null; deflating it does not
                        change the solution, only the iteration count.
                        0 reproduces the reference's PCG. */
+  double fallback_growth; /* SANEPIC_PRECOND_LEARNED only: a step along a
+                       network direction that lets |r| grow by more than this
+                       factor sends the next preconditioner application to
+                       block-Jacobi (0 selects the default, 2). */
 } sanepic_solve_opts;
 
 /* Start the PCG of sanepic.cc:1316-1410 from x0 (compressed 3 npix, NULL for
@@ -174,6 +178,104 @@ int sanepic_kernel_time(sanepic_ctx *ctx, int32_t which, double *ms,
                         int64_t *count);
 /* Device bytes held by the context. */
 int sanepic_device_bytes(sanepic_ctx *ctx, int64_t *bytes);
+
+/* ------------------------------------------------------------------------
+ * Learned preconditioner.  None of this has a counterpart in the reference:
+ * it replaces the diagonal preconditioner of sanepic.cc:1351-1356 by a
+ * flexible PCG whose preconditioner is z = M r + sigma * U(r), with M the
+ * 3x3 block-Jacobi, sigma the RMS of M r and U a BF16 U-Net on the 12 nested
+ * HEALPix base faces (nside must be a power of two >= 64).  The network's
+ * input channels per face pixel are M r / sigma (3), log hits (1) and the
+ * pixel's diagonal block of A (6); its output is 3 channels (I, Q, U).
+ * A step whose residual norm grows is undone and redone with block-Jacobi.
+ * ---------------------------------------------------------------------- */
+#define SANEPIC_PRECOND_LEARNED 2 /* flexible PCG with an attached network */
+
+typedef struct sanepic_unet sanepic_unet;
+
+/* A network for maps of the given nside on a device; parameters start at 0
+ * (sanepic_unet_init draws He-normal weights with a zero output layer). */
+int sanepic_unet_create(int64_t nside, int32_t device, sanepic_unet **out);
+void sanepic_unet_destroy(sanepic_unet *u);
+const char *sanepic_unet_last_error(const sanepic_unet *u);
+int sanepic_unet_nparams(sanepic_unet *u, int64_t *n);
+int sanepic_unet_init(sanepic_unet *u, uint64_t seed);
+/* FP32 master parameters: per convolution W[cout][9][cin] then bias[cout],
+ * convolutions in the order e0a e0b e1a e1b e2a e2b ba bb d2a d2b d1a d1b
+ * d0a d0b head. */
+int sanepic_unet_get_params(sanepic_unet *u, float *params);
+int sanepic_unet_set_params(sanepic_unet *u, const float *params);
+/* Network on face images: x[12][S][S][10] -> y[12][S][S][3] (S = nside). */
+int sanepic_unet_forward(sanepic_unet *u, const float *x, float *y);
+/* Gradient of sum(y * dy) with respect to every parameter (forward + hand-
+ * written backward), grad[nparams]; y optional (may be NULL). */
+int sanepic_unet_backward(sanepic_unet *u, const float *x, const float *dy,
+                          float *y, float *grad);
+
+/* Attach a network to a context after sanepic_setup (same device, nside).
+ * The context does not own it; detach with u = NULL before destroying it. */
+int sanepic_learn_attach(sanepic_ctx *ctx, sanepic_unet *u);
+/* Face position f*S*S + y*S + x (nested face f, row y, column x) of every
+ * seen pixel, pos[npix]. */
+int sanepic_learn_face_index(sanepic_ctx *ctx, int64_t *pos);
+/* The same for n arbitrary RING pixels (host only, no device needed). */
+int sanepic_ring_to_face(int64_t nside, int64_t n, const int64_t *ring,
+                         int64_t *pos);
+/* The network input of residual r (compressed 3 npix, block layout):
+ * feat[12][S][S][10], bf16-rounded; sigma optional. */
+int sanepic_learn_features(sanepic_ctx *ctx, const double *r, float *feat,
+                           double *sigma);
+
+typedef struct sanepic_harvest_opts {
+  uint64_t seed;      /* training right-hand side b' = A m' + P^T N^-1/2 w */
+  int32_t deflate;    /* solver used to generate the pairs */
+  int32_t maxiter;
+  double keep_crit;   /* keep iterates with |r|^2/|b|^2 >= keep_crit */
+  double stop_crit;   /* solve to |r|^2/|b|^2 <= stop_crit for x* */
+  int64_t capacity;   /* pair store size (allocated on first use) */
+} sanepic_harvest_opts;
+
+/* Generate training pairs (r_k, A^-1 r_k = x* - x_k) from a block-Jacobi
+ * PCG solve of a synthetic right-hand side: a Gaussian I/Q/U sky (100, 10,
+ * 10 uK) seen through A plus P^T N^-1 n for noise n drawn from the
+ * detectors' own spectra.  The context's data b is left untouched.
+ * Appends up to capacity - stored pairs; *added (may be NULL) gets the
+ * count, *iters the solver iterations. */
+int sanepic_learn_harvest(sanepic_ctx *ctx, const sanepic_harvest_opts *opts,
+                          int64_t *added, int32_t *iters);
+int sanepic_learn_npairs(sanepic_ctx *ctx, int64_t *n);
+int sanepic_learn_clear(sanepic_ctx *ctx);
+/* Pair k in block layout (r and e of 3 npix each). */
+int sanepic_learn_get_pair(sanepic_ctx *ctx, int64_t k, double *r, double *e);
+
+typedef struct sanepic_train_opts {
+  int64_t steps;      /* Adam steps, one pair each */
+  float lr, lr_final; /* cosine schedule from lr to lr_final */
+  float beta1, beta2, eps, weight_decay;
+  uint64_t seed;      /* pair order (a fresh permutation per epoch) */
+  int64_t first, count; /* train on pairs [first, first + count) */
+  int32_t loss_kind;  /* SANEPIC_LOSS_* */
+} sanepic_train_opts;
+
+/* Training losses of a pair (r, e = A^-1 r) with z = M r + sigma U(r):
+ * ANORM: |z - e|_A^2 / |e|_A^2, one operator application per step (its
+ * gradient w.r.t. z is 2 (A z - r) / e.r, no differentiation through A);
+ * BLOCK: the block-diagonal part of that A-norm, sum_p D_p^T B_p D_p with
+ * D = z - e, over sigma^2 and normalised by the mean b00 (no operator). */
+#define SANEPIC_LOSS_ANORM 0
+#define SANEPIC_LOSS_BLOCK 1
+
+/* Train the attached network on the stored pairs with Adam.  loss[steps]
+ * (may be NULL) receives each step's loss. */
+int sanepic_learn_train(sanepic_ctx *ctx, const sanepic_train_opts *opts,
+                        float *loss);
+/* Loss (SANEPIC_LOSS_*) of stored pair k under the current parameters. */
+int sanepic_learn_eval(sanepic_ctx *ctx, int64_t k, int32_t loss_kind,
+                       double *loss);
+/* Flexible-PCG statistics since solve_begin: steps taken with the network,
+ * steps redone with block-Jacobi, network outputs rejected as non-descent. */
+int sanepic_learn_stats(sanepic_ctx *ctx, int64_t *learned, int64_t *fallback,
+                        int64_t *rejected);
 
 #ifdef __cplusplus
 }

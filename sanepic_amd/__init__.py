@@ -18,7 +18,8 @@ EXE_PATH = os.path.join(HERE, "bin", "sanepic")
 CSRC = os.path.join(HERE, "csrc")
 
 FP64, MIXED = 0, 1
-PRECOND_DIAG, PRECOND_BLOCK3 = 0, 1
+PRECOND_DIAG, PRECOND_BLOCK3, PRECOND_LEARNED = 0, 1, 2
+LOSS_ANORM, LOSS_BLOCK = 0, 1
 
 # The header's function list; tests check the library exports all of them.
 ABI_FUNCTIONS = [
@@ -29,6 +30,13 @@ ABI_FUNCTIONS = [
     "sanepic_solve_step", "sanepic_get_map", "sanepic_get_nvar", "sanepic_get_sumcross",
     "sanepic_true_residual", "sanepic_set_profiling", "sanepic_kernel_time",
     "sanepic_device_bytes",
+    "sanepic_unet_create", "sanepic_unet_destroy", "sanepic_unet_last_error",
+    "sanepic_unet_nparams", "sanepic_unet_init", "sanepic_unet_get_params",
+    "sanepic_unet_set_params", "sanepic_unet_forward", "sanepic_unet_backward",
+    "sanepic_learn_attach", "sanepic_learn_face_index", "sanepic_ring_to_face",
+    "sanepic_learn_features", "sanepic_learn_harvest", "sanepic_learn_npairs",
+    "sanepic_learn_clear", "sanepic_learn_get_pair", "sanepic_learn_train", "sanepic_learn_eval",
+    "sanepic_learn_stats",
 ]
 
 
@@ -41,12 +49,25 @@ class Config(ctypes.Structure):
 
 class SolveOpts(ctypes.Structure):
     _fields_ = [("precond", ctypes.c_int32), ("refresh", ctypes.c_int32),
-                ("deflate", ctypes.c_int32)]
+                ("deflate", ctypes.c_int32), ("fallback_growth", ctypes.c_double)]
 
 
 class IterInfo(ctypes.Structure):
     _fields_ = [("iter", ctypes.c_int32), ("crit", ctypes.c_double), ("crit2", ctypes.c_double),
                 ("chi2", ctypes.c_double), ("alpha", ctypes.c_double), ("beta", ctypes.c_double)]
+
+
+class HarvestOpts(ctypes.Structure):
+    _fields_ = [("seed", ctypes.c_uint64), ("deflate", ctypes.c_int32), ("maxiter", ctypes.c_int32),
+                ("keep_crit", ctypes.c_double), ("stop_crit", ctypes.c_double),
+                ("capacity", ctypes.c_int64)]
+
+
+class TrainOpts(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_int64), ("lr", ctypes.c_float), ("lr_final", ctypes.c_float),
+                ("beta1", ctypes.c_float), ("beta2", ctypes.c_float), ("eps", ctypes.c_float),
+                ("weight_decay", ctypes.c_float), ("seed", ctypes.c_uint64),
+                ("first", ctypes.c_int64), ("count", ctypes.c_int64), ("loss_kind", ctypes.c_int32)]
 
 
 class SanepicError(RuntimeError):
@@ -71,6 +92,7 @@ def lib():
         d_p = ctypes.POINTER(ctypes.c_double)
         i64p = ctypes.POINTER(ctypes.c_int64)
         i64 = ctypes.c_int64
+        f_p = ctypes.POINTER(ctypes.c_float)
         sig = {
             "sanepic_create": [ctypes.POINTER(Config), ctypes.POINTER(P)],
             "sanepic_destroy": [P],
@@ -96,13 +118,35 @@ def lib():
             "sanepic_set_profiling": [P, ctypes.c_int32],
             "sanepic_kernel_time": [P, ctypes.c_int32, d_p, i64p],
             "sanepic_device_bytes": [P, i64p],
+            "sanepic_unet_create": [i64, ctypes.c_int32, ctypes.POINTER(P)],
+            "sanepic_unet_destroy": [P],
+            "sanepic_unet_nparams": [P, i64p],
+            "sanepic_unet_init": [P, ctypes.c_uint64],
+            "sanepic_unet_get_params": [P, f_p],
+            "sanepic_unet_set_params": [P, f_p],
+            "sanepic_unet_forward": [P, f_p, f_p],
+            "sanepic_unet_backward": [P, f_p, f_p, f_p, f_p],
+            "sanepic_learn_attach": [P, P],
+            "sanepic_learn_face_index": [P, i64p],
+            "sanepic_ring_to_face": [i64, i64, i64p, i64p],
+            "sanepic_learn_features": [P, d_p, f_p, d_p],
+            "sanepic_learn_harvest": [P, ctypes.POINTER(HarvestOpts), i64p,
+                                      ctypes.POINTER(ctypes.c_int32)],
+            "sanepic_learn_npairs": [P, i64p],
+            "sanepic_learn_clear": [P],
+            "sanepic_learn_get_pair": [P, i64, d_p, d_p],
+            "sanepic_learn_train": [P, ctypes.POINTER(TrainOpts), f_p],
+            "sanepic_learn_eval": [P, i64, ctypes.c_int32, d_p],
+            "sanepic_learn_stats": [P, i64p, i64p, i64p],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
             fn.argtypes = args
-            fn.restype = None if name == "sanepic_destroy" else ctypes.c_int
+            fn.restype = None if name in ("sanepic_destroy", "sanepic_unet_destroy") else ctypes.c_int
         L.sanepic_last_error.argtypes = [P]
         L.sanepic_last_error.restype = ctypes.c_char_p
+        L.sanepic_unet_last_error.argtypes = [P]
+        L.sanepic_unet_last_error.restype = ctypes.c_char_p
         _lib = L
     return _lib
 
@@ -117,6 +161,88 @@ def _ip(a):
 
 def _f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _fp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def ring_to_face(nside, ring):
+    """Face position f*S*S + y*S + x of RING pixels (nested faces, host only)."""
+    ring = np.ascontiguousarray(ring, dtype=np.int64)
+    pos = np.empty_like(ring)
+    rc = lib().sanepic_ring_to_face(nside, ring.size, _ip(ring), _ip(pos))
+    if rc:
+        raise SanepicError("sanepic_ring_to_face: bad nside or pixel")
+    return pos
+
+
+class UNet:
+    """The learned preconditioner's network (wraps sanepic_unet): BF16 U-Net
+    on the 12 nested base faces, widths 16-32-64-128, 3x3 convolutions."""
+
+    NFEAT, NOUT = 10, 3
+
+    def __init__(self, nside, device=0, seed=None):
+        self.nside = nside
+        self.h = None
+        h = ctypes.c_void_p()
+        rc = lib().sanepic_unet_create(nside, device, ctypes.byref(h))
+        if rc:
+            raise SanepicError(f"sanepic_unet_create: {lib().sanepic_unet_last_error(None).decode()}")
+        self.h = h
+        n = ctypes.c_int64()
+        lib().sanepic_unet_nparams(self.h, ctypes.byref(n))
+        self.nparams = n.value
+        if seed is not None:
+            self.init(seed)
+
+    def close(self):
+        if self.h:
+            lib().sanepic_unet_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, rc, what):
+        if rc:
+            raise SanepicError(f"{what} ({rc}): {lib().sanepic_unet_last_error(self.h).decode()}")
+
+    def init(self, seed):
+        self._chk(lib().sanepic_unet_init(self.h, seed), "unet_init")
+
+    def params(self):
+        out = np.empty(self.nparams, dtype=np.float32)
+        self._chk(lib().sanepic_unet_get_params(self.h, _fp(out)), "unet_get_params")
+        return out
+
+    def set_params(self, p):
+        p = _f32(p)
+        assert p.size == self.nparams
+        self._chk(lib().sanepic_unet_set_params(self.h, _fp(p)), "unet_set_params")
+
+    def forward(self, x):
+        """x[12][S][S][10] float32 -> y[12][S][S][3]."""
+        x = _f32(x)
+        S = self.nside
+        assert x.size == 12 * S * S * self.NFEAT
+        y = np.empty((12, S, S, self.NOUT), dtype=np.float32)
+        self._chk(lib().sanepic_unet_forward(self.h, _fp(x), _fp(y)), "unet_forward")
+        return y
+
+    def backward(self, x, dy):
+        """(y, gradient of sum(y * dy) w.r.t. every parameter)."""
+        x, dy = _f32(x), _f32(dy)
+        S = self.nside
+        y = np.empty((12, S, S, self.NOUT), dtype=np.float32)
+        g = np.empty(self.nparams, dtype=np.float32)
+        self._chk(lib().sanepic_unet_backward(self.h, _fp(x), _fp(dy), _fp(y), _fp(g)), "unet_backward")
+        return y, g
 
 
 class Context:
@@ -233,8 +359,8 @@ class Context:
         self._chk(lib().sanepic_chi2(self.h, _dp(x), ctypes.byref(out)), "chi2")
         return out.value
 
-    def solve_begin(self, precond=PRECOND_DIAG, refresh=10, x0=None, deflate=False):
-        opts = SolveOpts(precond, refresh, 1 if deflate else 0)
+    def solve_begin(self, precond=PRECOND_DIAG, refresh=10, x0=None, deflate=False, fallback_growth=0.0):
+        opts = SolveOpts(precond, refresh, 1 if deflate else 0, fallback_growth)
         xp = None if x0 is None else _dp(_f64(x0))
         self._x0_keep = None if x0 is None else _f64(x0)
         self._chk(lib().sanepic_solve_begin(self.h, ctypes.byref(opts), xp), "solve_begin")
@@ -246,9 +372,9 @@ class Context:
                     alpha=info.alpha, beta=info.beta)
 
     def solve(self, precond=PRECOND_DIAG, stop=1e-15, maxiter=2000, refresh=10, x0=None,
-              deflate=False):
+              deflate=False, fallback_growth=0.0):
         """The reference's loop: while ((iter < maxiter && crit > stop) || iter == 0)."""
-        self.solve_begin(precond, refresh, x0, deflate)
+        self.solve_begin(precond, refresh, x0, deflate, fallback_growth)
         hist = []
         while True:
             info = self.step()
@@ -292,3 +418,64 @@ class Context:
         b = ctypes.c_int64()
         self._chk(lib().sanepic_device_bytes(self.h, ctypes.byref(b)), "device_bytes")
         return b.value
+
+    # ------------------------------------------------ learned preconditioner
+    def learn_attach(self, unet):
+        self._chk(lib().sanepic_learn_attach(self.h, unet.h if unet is not None else None),
+                  "learn_attach")
+        self._unet = unet
+
+    def face_index(self):
+        out = np.empty(self.npix, dtype=np.int64)
+        self._chk(lib().sanepic_learn_face_index(self.h, _ip(out)), "learn_face_index")
+        return out
+
+    def features(self, r):
+        """Network input of residual r (block layout) and its sigma."""
+        r = _f64(r)
+        S = self.nside
+        feat = np.empty((12, S, S, UNet.NFEAT), dtype=np.float32)
+        sig = ctypes.c_double()
+        self._chk(lib().sanepic_learn_features(self.h, _dp(r), _fp(feat), ctypes.byref(sig)),
+                  "learn_features")
+        return feat, sig.value
+
+    def harvest(self, seed, capacity, deflate=True, maxiter=400, keep_crit=1e-12, stop_crit=1e-17):
+        o = HarvestOpts(seed, 1 if deflate else 0, maxiter, keep_crit, stop_crit, capacity)
+        added = ctypes.c_int64()
+        iters = ctypes.c_int32()
+        self._chk(lib().sanepic_learn_harvest(self.h, ctypes.byref(o), ctypes.byref(added),
+                                              ctypes.byref(iters)), "learn_harvest")
+        return added.value, iters.value
+
+    def npairs(self):
+        n = ctypes.c_int64()
+        self._chk(lib().sanepic_learn_npairs(self.h, ctypes.byref(n)), "learn_npairs")
+        return n.value
+
+    def clear_pairs(self):
+        self._chk(lib().sanepic_learn_clear(self.h), "learn_clear")
+
+    def pair(self, k):
+        r = np.empty(3 * self.npix)
+        e = np.empty(3 * self.npix)
+        self._chk(lib().sanepic_learn_get_pair(self.h, k, _dp(r), _dp(e)), "learn_get_pair")
+        return r, e
+
+    def train(self, steps, lr=1e-3, lr_final=1e-5, seed=0, first=0, count=0, beta1=0.9,
+              beta2=0.999, eps=1e-8, weight_decay=0.0, loss=LOSS_ANORM):
+        o = TrainOpts(steps, lr, lr_final, beta1, beta2, eps, weight_decay, seed, first, count, loss)
+        hist = np.empty(steps, dtype=np.float32)
+        self._chk(lib().sanepic_learn_train(self.h, ctypes.byref(o), _fp(hist)), "learn_train")
+        return hist
+
+    def eval_pair(self, k, loss=LOSS_ANORM):
+        out = ctypes.c_double()
+        self._chk(lib().sanepic_learn_eval(self.h, k, loss, ctypes.byref(out)), "learn_eval")
+        return out.value
+
+    def learn_stats(self):
+        a, b, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        self._chk(lib().sanepic_learn_stats(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)),
+                  "learn_stats")
+        return dict(learned=a.value, fallback=b.value, rejected=c.value)

@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "../../include/sanepic_hip.h"
+#include "ctx.h"
 #include "kernels.h"
 #include "pixel_host.h"
 
@@ -26,120 +27,11 @@ namespace {
 
 thread_local std::string g_create_error;
 
-constexpr int kChunk = 4096;      // max sorted entries per P^T item
-constexpr int kProfClasses = 5;
-
 }  // namespace
 
-struct sanepic_ctx {
-  sanepic_config cfg{};
-  std::string err;
-  hipStream_t st = nullptr;
-  int64_t ns = 0, nseg = 0, ndet = 0, ntot = 0, nside = 0, npix = -1;
-  PairGeom g{}, gauto{};
-  bool single = false;
-  std::vector<DetConst> detc;
-  std::vector<double> offx, offy, spn;
-  std::vector<char> det_ok, pt_ok, tod_ok;
-  std::vector<int32_t> h_indpix, h_pixlist, h_cnt;
-  int64_t n_deferred = 0, n_bad = 0;
-  int64_t bytes = 0;
-  // device
-  double *ra = nullptr, *dec = nullptr, *psi = nullptr, *tod = nullptr;
-  DetConst *d_det = nullptr;
-  int32_t *pix = nullptr, *cnt = nullptr;
-  double2 *cs = nullptr, *csfp = nullptr, *cs_sorted = nullptr, *csfp_sorted = nullptr;
-  float2 *cs32 = nullptr, *cs_sorted32 = nullptr;
-  int64_t *off = nullptr;
-  uint32_t *perm = nullptr;
-  int32_t *it_pix = nullptr, *it_len = nullptr, *it_slot = nullptr, *sp_pix = nullptr,
-          *sp_first = nullptr, *sp_count = nullptr;
-  int64_t *it_beg = nullptr;
-  Items items{};
-  int64_t nslots = 0;
-  double *partial = nullptr;
-  double2 *tw = nullptr, *twlo = nullptr, *twhi = nullptr;
-  float2 *tw32 = nullptr, *twlo32 = nullptr, *twhi32 = nullptr;
-  double *wt = nullptr;
-  float *wt32 = nullptr;
-  double2 *buf = nullptr, *dspec = nullptr;
-  float2 *buf32 = nullptr;
-  double *z = nullptr;
-  float *z32 = nullptr, *map32 = nullptr;
-  double *chi2part = nullptr;
-  int64_t nchi2 = 0;
-  double *b = nullptr, *b_raw = nullptr, *nauto = nullptr, *pairs6 = nullptr, *cross = nullptr;
-  double *x = nullptr, *r = nullptr, *d = nullptr, *q = nullptr, *M = nullptr, *nvar = nullptr,
-         *part = nullptr, *tmpblk = nullptr;
-  CGScalars *sc = nullptr;
-  int *zero_flag = nullptr;
-  int iter = 0, refresh = 10, precond = 0;
-  int64_t chunk = 0;  // pairs per four-step chunk (0: all at once)
-  bool deflate = false;
-  double *az = nullptr;
-  bool begun = false, setup_done = false, pixelized = false;
-  bool prof = false;
-  std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> evs;
-  double kt_ms[kProfClasses] = {0};
-  int64_t kt_n[kProfClasses] = {0};
-  std::vector<void *> allocs;
-};
+using namespace sctx;
 
 namespace {
-
-int fail(sanepic_ctx *c, int code, const std::string &msg) {
-  c->err = msg;
-  return code;
-}
-
-#define HIPCHK(c, expr)                                                                  \
-  do {                                                                                   \
-    hipError_t e_ = (expr);                                                              \
-    if (e_ != hipSuccess)                                                                \
-      return fail(c, SANEPIC_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));  \
-  } while (0)
-
-#define LCHK(c, expr)                                                                    \
-  do {                                                                                   \
-    if ((expr) != 0) {                                                                   \
-      hipError_t e_ = hipGetLastError();                                                 \
-      return fail(c, SANEPIC_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));  \
-    }                                                                                    \
-  } while (0)
-
-template <typename T>
-int dalloc(sanepic_ctx *c, T *&p, int64_t n) {
-  if (p) return 0;
-  if (n <= 0) n = 1;
-  void *v = nullptr;
-  hipError_t e = hipMalloc(&v, (size_t)n * sizeof(T));
-  if (e != hipSuccess) {
-    (void)hipGetLastError();
-    return fail(c, SANEPIC_ENOMEM,
-                "hipMalloc of " + std::to_string((size_t)n * sizeof(T)) + " bytes failed: " +
-                    hipGetErrorString(e));
-  }
-  p = (T *)v;
-  c->allocs.push_back(v);
-  c->bytes += (int64_t)((size_t)n * sizeof(T));
-  return 0;
-}
-
-template <typename T>
-void dfree(sanepic_ctx *c, T *&p, int64_t n) {
-  if (!p) return;
-  auto it = std::find(c->allocs.begin(), c->allocs.end(), (void *)p);
-  if (it != c->allocs.end()) c->allocs.erase(it);
-  (void)hipFree(p);
-  c->bytes -= (int64_t)((size_t)std::max<int64_t>(n, 1) * sizeof(T));
-  p = nullptr;
-}
-
-#define ACHK(c, expr)        \
-  do {                       \
-    int rc_ = (expr);        \
-    if (rc_ != 0) return rc_; \
-  } while (0)
 
 int ilog2(int64_t v) {
   int l = 0;
@@ -241,8 +133,9 @@ int build_tables(sanepic_ctx *c) {
   return 0;
 }
 
-template <typename T>
-OpArgs<T> op_args(sanepic_ctx *c);
+}  // namespace
+
+namespace sctx {
 
 template <>
 OpArgs<double> op_args<double>(sanepic_ctx *c) {
@@ -398,6 +291,31 @@ int d2h(sanepic_ctx *c, void *dst, const void *src, size_t bytes) {
   return 0;
 }
 
+CGArgs cg_args(sanepic_ctx *c) {
+  CGArgs a{};
+  a.npix = c->npix;
+  a.x = c->x;
+  a.r = c->r;
+  a.d = c->d;
+  a.q = c->q;
+  a.b = c->b;
+  a.M = c->M;
+  a.cnt = c->cnt;
+  a.sc = c->sc;
+  a.part = c->part;
+  a.az = c->deflate ? c->az : nullptr;
+  return a;
+}
+
+template int run_filter<double>(sanepic_ctx *, OpArgs<double>, int, int, int64_t *);
+template int run_filter<float>(sanepic_ctx *, OpArgs<float>, int, int, int64_t *);
+template int run_scatter<double>(sanepic_ctx *, const double *, double *, int);
+template int run_scatter<float>(sanepic_ctx *, const float *, double *, int);
+
+}  // namespace sctx
+
+namespace {
+
 double pixel_eps() {
   const char *e = getenv("SANEPIC_PIXEL_EPS");
   if (e && *e) return atof(e);
@@ -514,6 +432,11 @@ int sanepic_create(const sanepic_config *cfg, sanepic_ctx **out) {
 void sanepic_destroy(sanepic_ctx *c) {
   if (!c) return;
   if (c->st) (void)hipStreamSynchronize(c->st);
+  if (c->un) {  // the network outlives the context: give it back its stream
+    c->un->net.st = c->un->own;
+    c->un->attached = nullptr;
+    c->un = nullptr;
+  }
   for (auto &e : c->evs) {
     (void)hipEventDestroy(std::get<1>(e));
     (void)hipEventDestroy(std::get<2>(e));
@@ -951,29 +874,24 @@ int sanepic_chi2(sanepic_ctx *c, const double *xin, double *chi2) {
   return rc;
 }
 
-static CGArgs cg_args(sanepic_ctx *c) {
-  CGArgs a{};
-  a.npix = c->npix;
-  a.x = c->x;
-  a.r = c->r;
-  a.d = c->d;
-  a.q = c->q;
-  a.b = c->b;
-  a.M = c->M;
-  a.cnt = c->cnt;
-  a.sc = c->sc;
-  a.part = c->part;
-  a.az = c->deflate ? c->az : nullptr;
-  return a;
-}
 
 int sanepic_solve_begin(sanepic_ctx *c, const sanepic_solve_opts *opts, const double *x0) {
   ACHK(c, check_ready(c, true));
   c->precond = opts ? opts->precond : SANEPIC_PRECOND_DIAG;
   c->refresh = (opts && opts->refresh > 0) ? opts->refresh : 10;
   c->deflate = opts ? opts->deflate != 0 : false;
-  if (c->precond != SANEPIC_PRECOND_DIAG && c->precond != SANEPIC_PRECOND_BLOCK3)
+  {
+    const double g = opts ? opts->fallback_growth : 0.0;
+    if (g < 0.0 || !std::isfinite(g)) return fail(c, SANEPIC_EINVAL, "fallback_growth must be >= 0");
+    const double gg = g > 0.0 ? g : 2.0;
+    c->fgrowth = gg * gg;  // compared with |r|^2
+  }
+  if (c->precond != SANEPIC_PRECOND_DIAG && c->precond != SANEPIC_PRECOND_BLOCK3 &&
+      c->precond != SANEPIC_PRECOND_LEARNED)
     return fail(c, SANEPIC_EINVAL, "unknown preconditioner");
+  const bool learned = c->precond == SANEPIC_PRECOND_LEARNED;
+  if (learned && !c->un) return fail(c, SANEPIC_EINVAL, "no network attached (sanepic_learn_attach)");
+  c->begun = false;
   const int64_t npix = c->npix;
   HIPCHK(c, hipMemsetAsync(c->zero_flag, 0, sizeof(int), c->st));
   LCHK(c, cg_launch_precond(c->pairs6, c->cnt, npix, c->precond == SANEPIC_PRECOND_DIAG ? SANEPIC_PRECOND_DIAG_K
@@ -999,8 +917,12 @@ int sanepic_solve_begin(sanepic_ctx *c, const sanepic_solve_opts *opts, const do
     LCHK(c, cg_launch_coarse_setup(a, c->st));
     LCHK(c, cg_launch_coarse_start(a, c->st));
   }
-  CGArgs a = cg_args(c);
-  LCHK(c, cg_launch_init(a, c->st));
+  if (learned) {
+    ACHK(c, fpcg_begin(c));
+  } else {
+    CGArgs a = cg_args(c);
+    LCHK(c, cg_launch_init(a, c->st));
+  }
   HIPCHK(c, hipStreamSynchronize(c->st));
   c->iter = 0;
   c->begun = true;
@@ -1010,6 +932,10 @@ int sanepic_solve_begin(sanepic_ctx *c, const sanepic_solve_opts *opts, const do
 int sanepic_solve_step(sanepic_ctx *c, sanepic_iter_info *info) {
   ACHK(c, check_ready(c, true));
   if (!c->begun) return fail(c, SANEPIC_EINVAL, "sanepic_solve_begin has not run");
+  if (c->precond == SANEPIC_PRECOND_LEARNED) {
+    if (!c->un) return fail(c, SANEPIC_EINVAL, "the network was detached");
+    return fpcg_step(c, info);
+  }
   CGArgs a = cg_args(c);
   ACHK(c, apply_A(c, c->d, c->q, 1, false, false));
   LCHK(c, cg_launch_alpha(a, c->st));

@@ -1,0 +1,174 @@
+// ctx.h -- the context behind include/sanepic_hip.h and the helpers shared
+// by the translation units that implement its entry points.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/sanepic_hip.h"
+#include "kernels.h"
+#include "learn.h"
+#include "unet.h"
+
+constexpr int kChunk = 4096;  // max sorted entries per P^T item
+constexpr int kProfClasses = 5;
+
+struct sanepic_ctx {
+  sanepic_config cfg{};
+  std::string err;
+  hipStream_t st = nullptr;
+  int64_t ns = 0, nseg = 0, ndet = 0, ntot = 0, nside = 0, npix = -1;
+  PairGeom g{}, gauto{};
+  bool single = false;
+  std::vector<DetConst> detc;
+  std::vector<double> offx, offy, spn;
+  std::vector<char> det_ok, pt_ok, tod_ok;
+  std::vector<int32_t> h_indpix, h_pixlist, h_cnt;
+  int64_t n_deferred = 0, n_bad = 0;
+  int64_t bytes = 0;
+  // device
+  double *ra = nullptr, *dec = nullptr, *psi = nullptr, *tod = nullptr;
+  DetConst *d_det = nullptr;
+  int32_t *pix = nullptr, *cnt = nullptr;
+  double2 *cs = nullptr, *csfp = nullptr, *cs_sorted = nullptr, *csfp_sorted = nullptr;
+  float2 *cs32 = nullptr, *cs_sorted32 = nullptr;
+  int64_t *off = nullptr;
+  uint32_t *perm = nullptr;
+  int32_t *it_pix = nullptr, *it_len = nullptr, *it_slot = nullptr, *sp_pix = nullptr,
+          *sp_first = nullptr, *sp_count = nullptr;
+  int64_t *it_beg = nullptr;
+  Items items{};
+  int64_t nslots = 0;
+  double *partial = nullptr;
+  double2 *tw = nullptr, *twlo = nullptr, *twhi = nullptr;
+  float2 *tw32 = nullptr, *twlo32 = nullptr, *twhi32 = nullptr;
+  double *wt = nullptr;
+  float *wt32 = nullptr;
+  double2 *buf = nullptr, *dspec = nullptr;
+  float2 *buf32 = nullptr;
+  double *z = nullptr;
+  float *z32 = nullptr, *map32 = nullptr;
+  double *chi2part = nullptr;
+  int64_t nchi2 = 0;
+  double *b = nullptr, *b_raw = nullptr, *nauto = nullptr, *pairs6 = nullptr, *cross = nullptr;
+  double *x = nullptr, *r = nullptr, *d = nullptr, *q = nullptr, *M = nullptr, *nvar = nullptr,
+         *part = nullptr, *tmpblk = nullptr;
+  CGScalars *sc = nullptr;
+  int *zero_flag = nullptr;
+  int iter = 0, refresh = 10, precond = 0;
+  int64_t chunk = 0;  // pairs per four-step chunk (0: all at once)
+  bool deflate = false;
+  double *az = nullptr;
+  bool begun = false, setup_done = false, pixelized = false;
+  bool prof = false;
+  std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> evs;
+  double kt_ms[kProfClasses] = {0};
+  int64_t kt_n[kProfClasses] = {0};
+  std::vector<void *> allocs;
+  // learned preconditioner (flexible PCG)
+  sanepic_unet *un = nullptr;
+  int32_t *fidx = nullptr;          // [npix] face position of each seen pixel
+  double b00ref = 1.0, wnorm = 1.0; // feature / loss normalisation
+  FScalars *fs = nullptr;
+  double *zf = nullptr, *qx = nullptr;
+  double delta0 = 0.0;
+  double fgrowth = 4.0;             // |r|^2 growth that sends the next z to block-Jacobi
+  bool last_z_learned = false;
+  int64_t n_learned = 0, n_fallback = 0, n_rejected = 0;
+  // training pairs
+  double *wt_sqrt = nullptr, *b_train = nullptr, *m_train = nullptr, *tail = nullptr;
+  float *pairs = nullptr;           // [cap][2][3 npix]: r_k then e_k
+  int64_t pair_cap = 0, npairs = 0;
+};
+
+// A network handle of the ABI (the context that uses it holds a pointer).
+struct sanepic_unet {
+  unet::Net net;
+  hipStream_t own = nullptr;  // the network's stream when not attached
+  int device = 0;
+  std::string err;
+  sanepic_ctx *attached = nullptr;
+  float *stage = nullptr;     // [npx0][16] float staging for the ABI
+  bool x0_dirty = true, go_dirty = true;  // X0 / gO hold ABI data at unseen pixels
+};
+
+namespace sctx {
+
+inline 
+int fail(sanepic_ctx *c, int code, const std::string &msg) {
+  c->err = msg;
+  return code;
+}
+
+#define HIPCHK(c, expr)                                                                  \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return fail(c, SANEPIC_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));  \
+  } while (0)
+
+#define LCHK(c, expr)                                                                    \
+  do {                                                                                   \
+    if ((expr) != 0) {                                                                   \
+      hipError_t e_ = hipGetLastError();                                                 \
+      return fail(c, SANEPIC_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));  \
+    }                                                                                    \
+  } while (0)
+
+template <typename T>
+inline int dalloc(sanepic_ctx *c, T *&p, int64_t n) {
+  if (p) return 0;
+  if (n <= 0) n = 1;
+  void *v = nullptr;
+  hipError_t e = hipMalloc(&v, (size_t)n * sizeof(T));
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(c, SANEPIC_ENOMEM,
+                "hipMalloc of " + std::to_string((size_t)n * sizeof(T)) + " bytes failed: " +
+                    hipGetErrorString(e));
+  }
+  p = (T *)v;
+  c->allocs.push_back(v);
+  c->bytes += (int64_t)((size_t)n * sizeof(T));
+  return 0;
+}
+
+template <typename T>
+inline void dfree(sanepic_ctx *c, T *&p, int64_t n) {
+  if (!p) return;
+  auto it = std::find(c->allocs.begin(), c->allocs.end(), (void *)p);
+  if (it != c->allocs.end()) c->allocs.erase(it);
+  (void)hipFree(p);
+  c->bytes -= (int64_t)((size_t)std::max<int64_t>(n, 1) * sizeof(T));
+  p = nullptr;
+}
+
+#define ACHK(c, expr)        \
+  do {                       \
+    int rc_ = (expr);        \
+    if (rc_ != 0) return rc_; \
+  } while (0)
+
+
+// Filter / scatter passes and the operator (capi.hip).
+template <typename T> OpArgs<T> op_args(sanepic_ctx *c);
+template <> OpArgs<double> op_args<double>(sanepic_ctx *c);
+template <> OpArgs<float> op_args<float>(sanepic_ctx *c);
+template <typename T> int run_filter(sanepic_ctx *c, OpArgs<T> a, int src, int mode, int64_t *nchi);
+template <typename T> int run_scatter(sanepic_ctx *c, const T *z, double *q, int mask);
+// q = A x (interleaved device vectors); chi2 -> sc->chi2 when asked
+int apply_A(sanepic_ctx *c, const double *x, double *q, int mask, bool chi2, bool fp64);
+int check_ready(sanepic_ctx *c, bool need_setup);
+int to_inter(sanepic_ctx *c, const double *host_blk, double *dev_inter);
+int from_inter(sanepic_ctx *c, const double *dev_inter, double *host_blk);
+int d2h(sanepic_ctx *c, void *dst, const void *src, size_t bytes);
+CGArgs cg_args(sanepic_ctx *c);
+// Flexible PCG with the attached network (capi_learn.hip): start after x0 and
+// the deflation set-up (q = A x0), and one iteration.
+int fpcg_begin(sanepic_ctx *c);
+int fpcg_step(sanepic_ctx *c, sanepic_iter_info *info);
+
+}  // namespace sctx

@@ -54,7 +54,24 @@ struct PairGeom {
 enum : int {
   SRC_GATHER = 0,  // Ps = I + cos2psi Q + sin2psi U from the map
   SRC_TOD = 1,     // the TOD (right-hand side)
+  SRC_RNG = 2,     // white Gaussian samples from (seed, sample index) (FP64 only)
 };
+
+// Counter-based standard normal deviate: the SplitMix64 finalizer (Steele,
+// Lea and Flood 2014) hashes (seed, i) into two uniforms, Box-Muller turns
+// them into one N(0, 1) value.  Same value for the same (seed, i) anywhere.
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+__device__ __forceinline__ double normal_at(uint64_t seed, uint64_t i) {
+  const uint64_t h1 = mix64(seed * 0x9e3779b97f4a7c15ULL + 2 * i + 1);
+  const uint64_t h2 = mix64(h1 ^ (seed + 2 * i + 2));
+  const double u1 = (double)((h1 >> 11) + 1) * 0x1.0p-53;  // (0, 1]
+  const double u2 = (double)(h2 >> 11) * 0x1.0p-53;        // [0, 1)
+  return sqrt(-2.0 * log(u1)) * cospi(2.0 * u2);
+}
 
 enum : int {
   ROW_FWD = 1,      // forward transform of the row
@@ -82,6 +99,7 @@ struct OpArgs {
   double *chi2part;     // per-block chi2 partials
   int64_t chi2off;      // first partial slot of this launch
   int mode;
+  uint64_t seed;        // SRC_RNG stream
 };
 
 template <typename T> int launch_col_fwd(const OpArgs<T> &a, int src, hipStream_t s);

@@ -52,6 +52,7 @@ __device__ __forceinline__ T gather_one(const OpArgs<T> &a, int64_t gi) {
 template <typename T, int SRC>
 __device__ __forceinline__ T load_src(const OpArgs<T> &a, int64_t gi) {
   if constexpr (SRC == SRC_GATHER) return gather_one<T>(a, gi);
+  else if constexpr (SRC == SRC_RNG) return (T)normal_at(a.seed, (uint64_t)gi);
   else return (T)a.tod[gi];
 }
 
@@ -287,10 +288,12 @@ int launch1(K kern, int64_t nblocks, hipStream_t s, const A &a) {
 template <typename T>
 int launch_col_fwd(const OpArgs<T> &a, int src, hipStream_t s) {
   const int64_t nb = a.g.npairs * (a.g.N / 4096);
-#define CF(L)                                                                            \
-  case L:                                                                                \
-    return src == SRC_GATHER ? launch1(k_col_fwd<T, L, SRC_GATHER>, nb, s, a)            \
-                             : launch1(k_col_fwd<T, L, SRC_TOD>, nb, s, a);
+#define CF(L)                                                                      \
+  case L:                                                                          \
+    if (src == SRC_GATHER) return launch1(k_col_fwd<T, L, SRC_GATHER>, nb, s, a);  \
+    if (src == SRC_TOD) return launch1(k_col_fwd<T, L, SRC_TOD>, nb, s, a);        \
+    if constexpr (sizeof(T) == 8) return launch1(k_col_fwd<T, L, SRC_RNG>, nb, s, a); \
+    return -2;
   switch (a.g.log2N2) {
     CF(4) CF(5) CF(6) CF(7) CF(8) CF(9) CF(10) CF(11) CF(12)
     default: return -2;
@@ -329,10 +332,12 @@ int launch_single(const OpArgs<T> &a, int src, hipStream_t s, int64_t *nblocks) 
   const int64_t B = 4096 >> log2n;  // pairs per block
   const int64_t nb = (a.g.npairs + B - 1) / B;
   if (nblocks) *nblocks = nb;
-#define SP(L)                                                                         \
-  case L:                                                                             \
-    return src == SRC_GATHER ? launch1(k_single<T, L, SRC_GATHER>, nb, s, a)          \
-                             : launch1(k_single<T, L, SRC_TOD>, nb, s, a);
+#define SP(L)                                                                     \
+  case L:                                                                         \
+    if (src == SRC_GATHER) return launch1(k_single<T, L, SRC_GATHER>, nb, s, a);  \
+    if (src == SRC_TOD) return launch1(k_single<T, L, SRC_TOD>, nb, s, a);        \
+    if constexpr (sizeof(T) == 8) return launch1(k_single<T, L, SRC_RNG>, nb, s, a); \
+    return -2;
   switch (log2n) {
     SP(4) SP(5) SP(6) SP(7) SP(8) SP(9) SP(10) SP(11) SP(12)
     default: return -2;

@@ -6,6 +6,7 @@
 // is that of src/sanepic.cc (pointingshift 3419-3428, pointing loop
 // 737-747, pixelisation 855-860, ang2pix_ring 2794-2842), so the result is
 // the reference's pixel whenever libm is glibc.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 
@@ -91,4 +92,69 @@ void host_spn_row(const double *ispf, long nssp, long ns, double *row) {
   // 533-537, including the truncating index expression.
   for (long ii = 0; ii < ns; ii++) row[ii] = ispf[long(double(ii) / nssp * ns)] / double(ns) / double(ns);
   row[0] = ispf[1] * 1e-6 / double(ns) / double(ns);
+}
+
+namespace {
+int64_t isqrt64(int64_t v) {
+  int64_t r = (int64_t)std::sqrt((double)v);
+  while (r * r > v) r--;
+  while ((r + 1) * (r + 1) <= v) r++;
+  return r;
+}
+}  // namespace
+
+int host_ring2xyf(long nside, long pix, int *face, int *x, int *y) {
+  if (nside < 1 || (nside & (nside - 1))) return -1;
+  const int64_t ns = nside, npix = 12 * ns * ns, ncap = 2 * ns * (ns - 1);
+  if (pix < 0 || pix >= npix) return -1;
+  // centre (z, phi) of the ring pixel
+  double z, phi;
+  if (pix < ncap) {
+    const int64_t ir = (1 + isqrt64(1 + 2 * (int64_t)pix)) >> 1;
+    const int64_t iphi = pix + 1 - 2 * ir * (ir - 1);
+    z = 1.0 - (double)(ir * ir) / (3.0 * (double)(ns * ns));
+    phi = ((double)iphi - 0.5) * M_PI / (2.0 * (double)ir);
+  } else if (pix < npix - ncap) {
+    const int64_t ip = pix - ncap;
+    const int64_t ir = ip / (4 * ns) + ns;
+    const int64_t iphi = ip % (4 * ns) + 1;
+    const double fodd = ((ir + ns) & 1) ? 1.0 : 0.5;
+    z = (double)(2 * ns - ir) * 2.0 / (3.0 * (double)ns);
+    phi = ((double)iphi - fodd) * M_PI / (2.0 * (double)ns);
+  } else {
+    const int64_t ip = npix - pix;
+    const int64_t ir = (1 + isqrt64(2 * ip - 1)) >> 1;
+    const int64_t iphi = 4 * ir + 1 - (ip - 2 * ir * (ir - 1));
+    z = -1.0 + (double)(ir * ir) / (3.0 * (double)(ns * ns));
+    phi = ((double)iphi - 0.5) * M_PI / (2.0 * (double)ir);
+  }
+  // face and in-face coordinates of that direction (nested construction)
+  const double za = std::fabs(z);
+  double tt = std::fmod(phi * (2.0 / M_PI), 4.0);
+  if (tt < 0.0) tt += 4.0;
+  if (za <= 2.0 / 3.0) {
+    const double t1 = (double)ns * (0.5 + tt), t2 = (double)ns * (z * 0.75);
+    const int64_t jp = (int64_t)(t1 - t2), jm = (int64_t)(t1 + t2);
+    const int64_t ifp = jp / ns, ifm = jm / ns;
+    *face = (int)(ifp == ifm ? (ifp | 4) : (ifp < ifm ? ifp : ifm + 8));
+    *x = (int)(jm & (ns - 1));
+    *y = (int)(ns - (jp & (ns - 1)) - 1);
+  } else {
+    const int ntt = std::min(3, (int)tt);
+    const double tp = tt - ntt;
+    const double tmp = (double)ns * std::sqrt(3.0 * (1.0 - za));
+    int64_t jp = (int64_t)(tp * tmp), jm = (int64_t)((1.0 - tp) * tmp);
+    jp = std::min<int64_t>(jp, ns - 1);
+    jm = std::min<int64_t>(jm, ns - 1);
+    if (z >= 0) {
+      *face = ntt;
+      *x = (int)(ns - jm - 1);
+      *y = (int)(ns - jp - 1);
+    } else {
+      *face = ntt + 8;
+      *x = (int)jp;
+      *y = (int)jm;
+    }
+  }
+  return 0;
 }

@@ -13,3 +13,9 @@ void host_det_const(double lel, double xel, double angl, double *offx, double *o
 
 // SpN_all diagonal row of one detector (533-537).
 void host_spn_row(const double *ispf, long nssp, long ns, double *row);
+
+// Nested base-face coordinates of a RING pixel (nside a power of two): face
+// f in 0..11 and the pixel's column x and row y inside the face, from the
+// pixel centre (Gorski et al. 2005, eqs. 4-9 for the centre, and the
+// face/x/y construction of the NESTED scheme).  Returns 0, or -1 on bad input.
+int host_ring2xyf(long nside, long pix, int *face, int *x, int *y);
