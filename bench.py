@@ -2,20 +2,29 @@
 
 Workload (default): 16 detectors at angles k*pi/8, NSIDE 512, 2^26 samples per
 detector in segments of 2^20 (1,073,741,824 TOD samples), spin/precession
-scan, white + 1/f noise (fknee 0.1 Hz, index 1), FP64 operator, 3x3
-block-Jacobi PCG with the monopole deflated, residual refresh every 10
-iterations.  Synthetic inputs are generated in host RAM first (untimed).
+scan, white + 1/f noise (fknee 0.1 Hz, index 1).  Synthetic inputs are
+generated in host RAM first (untimed) and copied into pinned host buffers,
+the form a caller hands to the library.
 
+Two complete solves are timed, each from inputs in host RAM to a true
+relative residual |b - A x| / |b| <= 1e-6 (H2D, pointing, ang2pix, sort, b,
+preconditioner, CG; file I/O excluded):
+  baseline   FP64 operator, plain 3x3 block-Jacobi PCG
+  headline   FP32 operator with FP64 CG scalars and FP64 residual refresh,
+             flexible PCG with the learned U-Net preconditioner (trained
+             offline by sanepic_amd.train_precond on disjoint seeds) and the
+             monopole deflation; block-Jacobi mixed PCG if no trained
+             weights exist for the workload.
 Reported:
-  value            TOD samples per second per CG iteration (whole job = 1 GPU)
-  time_to_1e-6_s   wall time from inputs in host RAM to a true relative
-                   residual |b - A x| / |b| <= 1e-6 (H2D, pointing, ang2pix,
-                   sort, b, preconditioner, CG), file I/O excluded
-  roofline         the dominant kernel's algorithmic bytes per launch over its
-                   average launch duration (HIP events on the library stream)
+  value            TOD samples per second per CG iteration of the headline
+                   solve (W warm-up iterations, then exactly K timed)
+  time_to_1e-6_s   the headline solve's time to solution; the baseline's and
+                   the ratio are in "baseline"
+  roofline         the headline operator's dominant kernel: algorithmic bytes
+                   per launch over its average launch duration (HIP events on
+                   the library stream)
   cpu_baseline     the CPU oracle (an FP64 restatement of the reference's
                    path) on a bounded sample of the same workload
-One step = one CG iteration.  W warm-up iterations, then exactly K timed.
 """
 import argparse
 import json
@@ -29,12 +38,6 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12
-WORKLOADS = {
-    # name: nside, ndet, samples per detector, segment length
-    "c1": dict(nside=512, ndet=16, nsamp=2 ** 26, ns=2 ** 20),
-    "c0": dict(nside=64, ndet=1, nsamp=2 ** 22, ns=2 ** 19),
-    "small": dict(nside=64, ndet=4, nsamp=2 ** 21, ns=2 ** 18),
-}
 CLASS_NAMES = ["gather+column FFT", "row FFT + noise filter", "inverse column FFT",
                "pixel-sorted P^T reduction"]
 
@@ -97,15 +100,90 @@ def cpu_baseline(wl, budget_s=20.0):
                       f"{n} A-applications timed ({dt * 1e3:.1f} ms each)"}
 
 
+def run_solve(cfg, pin, precision, precond, deflate, weights, steps, warmup, maxiter, profile):
+    """One timed pipeline: create -> uploads -> pixelize -> setup -> solve to a
+    true residual <= 1e-6.  Returns timings, per-iteration time and kernel
+    times of `steps` iterations measured after `warmup`."""
+    import sanepic_amd
+    prec = sanepic_amd.FP64 if precision == "fp64" else sanepic_amd.MIXED
+    pc = {"block": sanepic_amd.PRECOND_BLOCK3, "diag": sanepic_amd.PRECOND_DIAG,
+          "learned": sanepic_amd.PRECOND_LEARNED}[precond]
+    un = None
+    if precond == "learned":
+        un = sanepic_amd.UNet(cfg.nside)
+        un.set_params(weights)  # loading the trained network is not part of the solve
+    t0 = time.perf_counter()
+    ctx = sanepic_amd.Context(cfg.nside, cfg.ndet, cfg.nseg, cfg.ns, precision=prec)
+    ctx.load_async(cfg, pin)
+    t_up = time.perf_counter()
+    npix, ndef = ctx.pixelize()
+    t_pix = time.perf_counter()
+    ctx.setup()
+    t_setup = time.perf_counter()
+    if un is not None:
+        ctx.learn_attach(un)
+    ctx.solve_begin(pc, 10, None, deflate)
+    t_begin = time.perf_counter()
+    # time to solution: iterate until crit <= 1e-12, then confirm the true
+    # residual with an explicit A x
+    hist = []
+    converged, true_res = False, None
+    while len(hist) < maxiter:
+        hist.append(ctx.step())
+        if hist[-1]["crit"] <= 1e-12:
+            true_res = ctx.true_residual()
+            if true_res <= 1e-6:
+                converged = True
+                break
+    t_end = time.perf_counter()
+    if true_res is None:
+        true_res = ctx.true_residual()
+    lstats = ctx.learn_stats() if un is not None else None
+    # per-iteration time: a fresh start, `warmup` iterations, then exactly
+    # `steps` iterations with the kernel timers on
+    kt, ms_per_step = None, None
+    if profile:
+        ctx.solve_begin(pc, 10, None, deflate)
+        for _ in range(warmup):
+            ctx.step()
+        ctx.set_profiling(True)
+        ts = time.perf_counter()
+        for _ in range(steps):
+            ctx.step()
+        ms_per_step = (time.perf_counter() - ts) * 1e3 / steps
+        kt = [ctx.kernel_time(k) for k in range(5)]
+        ctx.set_profiling(False)
+    its = len(hist)  # the first iteration whose true residual confirmed 1e-6
+    out = dict(precision=precision, precond=precond, deflate=deflate, npix=npix, deferred=ndef,
+               upload_s=t_up - t0, pixelize_s=t_pix - t_up, setup_s=t_setup - t_pix,
+               begin_s=t_begin - t_setup, iterations=its, converged=converged,
+               final_true_residual=true_res,
+               time_to_solution_s=(t_end - t0) if converged else None,
+               device_bytes=ctx.device_bytes(), ms_per_step=ms_per_step, kernel_times=kt)
+    if un is not None:
+        out["learned_stats"] = lstats
+    log(f"[{precision} {precond}{' +defl' if deflate else ''}] upload {out['upload_s']:.2f} s, "
+        f"pixelize {out['pixelize_s']:.2f} s, setup {out['setup_s']:.2f} s, begin {out['begin_s']:.2f} s; "
+        f"{its} iterations, true residual {true_res:.3e}, time to solution "
+        f"{out['time_to_solution_s'] if converged else float('nan'):.3f} s"
+        + (f", {ms_per_step:.2f} ms/iteration" if ms_per_step else "")
+        + (f", {out['learned_stats']}" if un is not None else ""))
+    ctx.close()
+    if un is not None:
+        un.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c1", choices=sorted(WORKLOADS))
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
-    ap.add_argument("--precond", default="block", choices=["block", "diag"])
+    ap.add_argument("--workload", default="c1")
+    ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
+    ap.add_argument("--precond", default="learned", choices=["learned", "block", "diag"])
     ap.add_argument("--no-deflate", action="store_true")
+    ap.add_argument("--no-baseline", action="store_true", help="skip the FP64 block-Jacobi solve")
     ap.add_argument("--maxiter", type=int, default=600)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calib-copy", action="store_true",
@@ -117,66 +195,40 @@ def main():
         sys.exit(2)
 
     import sanepic_amd
-    from sanepic_amd import synth
+    from sanepic_amd import synth, train_precond
 
     if args.calib_copy:
         calibration_copy()
-    wl = WORKLOADS[args.workload]
-    nseg = wl["nsamp"] // wl["ns"]
-    cfg = synth.Config(nside=wl["nside"], ndet=wl["ndet"], nseg=nseg, ns=wl["ns"])
+    wl = synth.WORKLOADS[args.workload]
+    cfg = synth.workload_config(args.workload)
     tg = time.perf_counter()
     ds = synth.make_dataset(cfg, workers=min(16, os.cpu_count() or 1))
+    pin = sanepic_amd.PinnedInputs(cfg, ds)
+    del ds
     log(f"inputs generated in {time.perf_counter() - tg:.1f} s "
-        f"({cfg.ndet} det x {cfg.nsamp} samples, NSIDE {cfg.nside})")
-    ntot = cfg.ndet * cfg.nsamp
-    prec = sanepic_amd.FP64 if args.precision == "fp64" else sanepic_amd.MIXED
-    pc = sanepic_amd.PRECOND_BLOCK3 if args.precond == "block" else sanepic_amd.PRECOND_DIAG
+        f"({cfg.ndet} det x {cfg.nseg * cfg.ns} samples, NSIDE {cfg.nside}), in pinned host memory")
+    ntot = cfg.ndet * cfg.nseg * cfg.ns
+
+    precond, weights, record = args.precond, None, None
+    if precond == "learned":
+        path = train_precond.weights_path(args.workload)
+        if os.path.exists(path):
+            weights, record = train_precond.load_weights(path)
+        else:
+            log(f"no trained network at {path}: block-Jacobi instead")
+            precond = "block"
     deflate = not args.no_deflate
 
-    # ---- time to solution: inputs in host RAM -> true residual <= 1e-6
-    t0 = time.perf_counter()
-    ctx = sanepic_amd.Context(cfg.nside, cfg.ndet, cfg.nseg, cfg.ns, precision=prec)
-    ctx.load(cfg, ds)
-    t_load = time.perf_counter()
-    npix, ndef = ctx.pixelize()
-    t_pix = time.perf_counter()
-    ctx.setup()
-    t_setup = time.perf_counter()
-    ctx.solve_begin(pc, 10, None, deflate)
-    t_begin = time.perf_counter()
-    log(f"load {t_load - t0:.2f} s, pixelize {t_pix - t_load:.2f} s (npix {npix}, "
-        f"{ndef} deferred), setup {t_setup - t_pix:.2f} s, solve_begin {t_begin - t_setup:.2f} s, "
-        f"device memory {ctx.device_bytes() / 1e9:.1f} GB")
+    base = None
+    if not args.no_baseline:
+        base = run_solve(cfg, pin, "fp64", "block", False, None, args.steps, args.warmup,
+                         args.maxiter, profile=False)
+    head = run_solve(cfg, pin, args.precision, precond, deflate, weights, args.steps, args.warmup,
+                     args.maxiter, profile=True)
+    pin.close()
 
-    hist = []
-    for _ in range(args.warmup):
-        hist.append(ctx.step())
-    ctx.set_profiling(True)
-    tk0 = time.perf_counter()
-    for _ in range(args.steps):
-        hist.append(ctx.step())
-    tk1 = time.perf_counter()
-    kt = [ctx.kernel_time(k) for k in range(5)]
-    ctx.set_profiling(False)
-    ms_per_step = (tk1 - tk0) * 1e3 / args.steps
-
-    converged, true_res = False, None
-    while len(hist) < args.maxiter:
-        if hist[-1]["crit"] <= 1e-12:
-            true_res = ctx.true_residual()
-            if true_res <= 1e-6:
-                converged = True
-                break
-        hist.append(ctx.step())
-    t_end = time.perf_counter()
-    if true_res is None or not converged:
-        true_res = ctx.true_residual()
-        converged = true_res <= 1e-6
-    tts = t_end - t0
-    log(f"{len(hist)} iterations, final crit {hist[-1]['crit']:.3e}, true residual {true_res:.3e}, "
-        f"time to solution {tts:.2f} s ({'reached' if converged else 'NOT reached'} 1e-6)")
-
-    # ---- roofline of the dominant kernel (live HIP events)
+    kt = head["kernel_times"]
+    ms_per_step = head["ms_per_step"]
     per = [(kt[k][0], kt[k][1], k) for k in range(4) if kt[k][1] > 0]
     tot_ms, cnt, dom = max(per)
     avg_ms = tot_ms / cnt
@@ -187,8 +239,9 @@ def main():
         if kt[k][1]:
             log(f"  {CLASS_NAMES[k]}: {kt[k][0] / kt[k][1]:.3f} ms x {kt[k][1]}")
     apply_ms = kt[4][0] / max(kt[4][1], 1)
-    b_alg = ntot * bps + cfg.ndet * (cfg.ns // 2 + 1) * (8 if args.precision == "fp64" else 4) \
-        + 12 * 3 * npix * 8
+    s = 8 if args.precision == "fp64" else 4
+    b_alg = ntot * bps + cfg.ndet * (cfg.ns // 2 + 1) * s + 12 * 3 * head["npix"] * 8
+    tts = head["time_to_solution_s"]
     out = {
         "metric": "time-to-1e-6 residual (s) + TOD samples/s per CG iter, NSIDE512 16det, 1 GPU",
         "value": ntot / (ms_per_step * 1e-3),
@@ -202,24 +255,36 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f32 operator, f64 CG scalars",
         "data": "synthetic (spin/precession scan, Gaussian I/Q/U sky, white + 1/f noise)",
-        "config": {"workload": f"{cfg.ndet} det x {cfg.nsamp} samples, NSIDE {cfg.nside}, "
-                               f"segments of {cfg.ns}, {args.precond} PCG"
+        "config": {"workload": f"{cfg.ndet} det x {cfg.nseg * cfg.ns} samples, NSIDE {cfg.nside}, "
+                               f"segments of {cfg.ns}, {precond} PCG"
                                f"{' + monopole deflation' if deflate else ''}",
-                   "nside": cfg.nside, "ndet": cfg.ndet, "samples_per_det": cfg.nsamp,
-                   "segment": cfg.ns, "npix": npix, "precond": args.precond,
+                   "nside": cfg.nside, "ndet": cfg.ndet, "samples_per_det": cfg.nseg * cfg.ns,
+                   "segment": cfg.ns, "npix": head["npix"], "precond": precond,
                    "deflate": deflate, "precision": args.precision},
-        "time_to_1e-6_s": tts if converged else None,
-        "iterations_to_1e-6": len(hist) if converged else None,
-        "final_true_residual": true_res,
-        "setup_s": t_begin - t0,
+        "time_to_1e-6_s": tts,
+        "iterations_to_1e-6": head["iterations"] if head["converged"] else None,
+        "final_true_residual": head["final_true_residual"],
+        "setup_s": head["upload_s"] + head["pixelize_s"] + head["setup_s"] + head["begin_s"],
+        "phases_s": {k: head[k] for k in ("upload_s", "pixelize_s", "setup_s", "begin_s")},
         "operator_ms": apply_ms,
         "operator_frac_hbm": b_alg / (apply_ms * 1e-3) / HBM_PEAK,
+        "device_bytes": head["device_bytes"],
         "roofline": {"bound": "hbm", "kernel": CLASS_NAMES[dom], "achieved": achieved / 1e9,
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
                      "traffic": None, "bytes_per_launch": alg, "avg_launch_ms": avg_ms},
     }
-    ctx.close()
-    del ds
+    if "learned_stats" in head:
+        out["learned"] = {"stats": head["learned_stats"],
+                          "training": {k: record[k] for k in (
+                              "pairs", "steps", "loss", "harvest_seeds", "harvest_seconds",
+                              "train_seconds", "holdout_block_jacobi", "holdout_trained")}}
+    if base is not None:
+        out["baseline"] = {"solver": "FP64 operator, plain block-Jacobi PCG",
+                           "time_to_1e-6_s": base["time_to_solution_s"],
+                           "iterations_to_1e-6": base["iterations"],
+                           "setup_s": base["upload_s"] + base["pixelize_s"] + base["setup_s"] + base["begin_s"],
+                           "speedup_time_to_solution": (base["time_to_solution_s"] / tts
+                                                        if tts and base["time_to_solution_s"] else None)}
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(wl)

@@ -86,6 +86,23 @@ int sanepic_set_pointing(sanepic_ctx *ctx, int64_t iseg, const double *ra,
 int sanepic_set_tod(sanepic_ctx *ctx, int64_t idet, int64_t iseg,
                     const double *tod);
 
+/* Pinned (page-locked) host memory for input buffers: copies from it run at
+ * the full host-link rate.  No counterpart in the reference, which reads its
+ * inputs with fread into ordinary buffers (sanepic.cc:680-692, 776-790). */
+int sanepic_host_alloc(uint64_t bytes, void **ptr);
+void sanepic_host_free(void *ptr);
+
+/* Queued forms of sanepic_set_pointing / sanepic_set_tod: the copy runs on an
+ * upload stream of the context and the call returns at once, so reading the
+ * next input overlaps the transfer and the TOD transfer overlaps
+ * sanepic_pixelize.  The buffers (ideally from sanepic_host_alloc) must stay
+ * unchanged until sanepic_pixelize (pointing) or sanepic_setup (TOD)
+ * returns; those calls wait for the copies. */
+int sanepic_set_pointing_async(sanepic_ctx *ctx, int64_t iseg, const double *ra,
+                               const double *dec, const double *psi);
+int sanepic_set_tod_async(sanepic_ctx *ctx, int64_t idet, int64_t iseg,
+                          const double *tod);
+
 /* Pointing transform, ang2pix and pixel compaction for all samples
  * (sanepic.cc:722-773 and 848-951).  Returns the number of seen pixels.
  * Pixel indices are bit-exact against a glibc evaluation of the reference's
@@ -99,6 +116,19 @@ int sanepic_get_pixels(sanepic_ctx *ctx, int64_t idet, int64_t iseg,
                        int64_t *pix);
 /* indpix[12 nside^2]: compressed index of each seen pixel, -1 if unseen. */
 int sanepic_get_indpix(sanepic_ctx *ctx, int64_t *indpix);
+
+/* Launch geometry (no counterpart in the reference).  Segments longer than
+ * 4096 samples are transformed as N = N1 x N2 four-step FFTs; the split
+ * (N1 = 2^log2_n1, 512 <= N1 <= 4096, 16 <= N2 <= 4096) changes the kernels'
+ * occupancy and memory strides, not the result beyond rounding.
+ * sanepic_autotune times one operator application for every admissible
+ * split on the context's own pointing and keeps the fastest; with a cache
+ * path the choice is looked up in (and appended to) that text file, keyed
+ * by device architecture, sizes and precision.  Both calls belong between
+ * sanepic_pixelize and sanepic_setup.  log2_n1 / op_ms may be NULL. */
+int sanepic_autotune(sanepic_ctx *ctx, const char *cache, int32_t *log2_n1,
+                     double *op_ms);
+int sanepic_set_fft_split(sanepic_ctx *ctx, int32_t log2_n1);
 
 /* Right-hand side b = P^T N^-1 d, hits, crosslink sums and preconditioner
  * ingredients (sanepic.cc:1069-1314).  Requires every TOD. */
@@ -137,6 +167,14 @@ typedef struct sanepic_solve_opts {
                        network direction that lets |r| grow by more than this
                        factor sends the next preconditioner application to
                        block-Jacobi (0 selects the default, 2). */
+  int32_t graph;    /* 1: each iteration's kernel sequence is captured once
+                       as a HIP graph and replayed (iterates are identical to
+                       the eager launches); ignored with
+                       SANEPIC_PRECOND_LEARNED and while profiling. */
+  int32_t fcg_window; /* SANEPIC_PRECOND_LEARNED only: each new direction is
+                       A-orthogonalised against the last fcg_window directions
+                       (truncated flexible CG, FCG(m); 1 is the Polak-Ribiere
+                       form; 0 selects the default 16; at most 32). */
 } sanepic_solve_opts;
 
 /* Start the PCG of sanepic.cc:1316-1410 from x0 (compressed 3 npix, NULL for
@@ -255,6 +293,10 @@ typedef struct sanepic_train_opts {
   uint64_t seed;      /* pair order (a fresh permutation per epoch) */
   int64_t first, count; /* train on pairs [first, first + count) */
   int32_t loss_kind;  /* SANEPIC_LOSS_* */
+  int64_t sched_offset, sched_total; /* step s of this call sits at
+                         (sched_offset + s) / sched_total of the cosine schedule
+                         (sched_total 0: this call's steps), so one schedule can
+                         span several calls */
 } sanepic_train_opts;
 
 /* Training losses of a pair (r, e = A^-1 r) with z = M r + sigma U(r):
@@ -264,6 +306,9 @@ typedef struct sanepic_train_opts {
  * D = z - e, over sigma^2 and normalised by the mean b00 (no operator). */
 #define SANEPIC_LOSS_ANORM 0
 #define SANEPIC_LOSS_BLOCK 1
+/* RESIDUAL: |A z - r|^2 / |r|^2, the residual a unit step along z would
+ * leave (two operator applications per step; gradient 2 A (A z - r) / |r|^2). */
+#define SANEPIC_LOSS_RESIDUAL 2
 
 /* Train the attached network on the stored pairs with Adam.  loss[steps]
  * (may be NULL) receives each step's loss. */

@@ -19,7 +19,7 @@ CSRC = os.path.join(HERE, "csrc")
 
 FP64, MIXED = 0, 1
 PRECOND_DIAG, PRECOND_BLOCK3, PRECOND_LEARNED = 0, 1, 2
-LOSS_ANORM, LOSS_BLOCK = 0, 1
+LOSS_ANORM, LOSS_BLOCK, LOSS_RESIDUAL = 0, 1, 2
 
 # The header's function list; tests check the library exports all of them.
 ABI_FUNCTIONS = [
@@ -36,7 +36,8 @@ ABI_FUNCTIONS = [
     "sanepic_learn_attach", "sanepic_learn_face_index", "sanepic_ring_to_face",
     "sanepic_learn_features", "sanepic_learn_harvest", "sanepic_learn_npairs",
     "sanepic_learn_clear", "sanepic_learn_get_pair", "sanepic_learn_train", "sanepic_learn_eval",
-    "sanepic_learn_stats",
+    "sanepic_learn_stats", "sanepic_host_alloc", "sanepic_host_free", "sanepic_set_pointing_async",
+    "sanepic_set_tod_async", "sanepic_autotune", "sanepic_set_fft_split",
 ]
 
 
@@ -49,7 +50,8 @@ class Config(ctypes.Structure):
 
 class SolveOpts(ctypes.Structure):
     _fields_ = [("precond", ctypes.c_int32), ("refresh", ctypes.c_int32),
-                ("deflate", ctypes.c_int32), ("fallback_growth", ctypes.c_double)]
+                ("deflate", ctypes.c_int32), ("fallback_growth", ctypes.c_double),
+                ("graph", ctypes.c_int32), ("fcg_window", ctypes.c_int32)]
 
 
 class IterInfo(ctypes.Structure):
@@ -67,7 +69,8 @@ class TrainOpts(ctypes.Structure):
     _fields_ = [("steps", ctypes.c_int64), ("lr", ctypes.c_float), ("lr_final", ctypes.c_float),
                 ("beta1", ctypes.c_float), ("beta2", ctypes.c_float), ("eps", ctypes.c_float),
                 ("weight_decay", ctypes.c_float), ("seed", ctypes.c_uint64),
-                ("first", ctypes.c_int64), ("count", ctypes.c_int64), ("loss_kind", ctypes.c_int32)]
+                ("first", ctypes.c_int64), ("count", ctypes.c_int64), ("loss_kind", ctypes.c_int32),
+                ("sched_offset", ctypes.c_int64), ("sched_total", ctypes.c_int64)]
 
 
 class SanepicError(RuntimeError):
@@ -138,11 +141,18 @@ def lib():
             "sanepic_learn_train": [P, ctypes.POINTER(TrainOpts), f_p],
             "sanepic_learn_eval": [P, i64, ctypes.c_int32, d_p],
             "sanepic_learn_stats": [P, i64p, i64p, i64p],
+            "sanepic_host_alloc": [ctypes.c_uint64, ctypes.POINTER(P)],
+            "sanepic_host_free": [P],
+            "sanepic_set_pointing_async": [P, i64, d_p, d_p, d_p],
+            "sanepic_set_tod_async": [P, i64, i64, d_p],
+            "sanepic_autotune": [P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int32), d_p],
+            "sanepic_set_fft_split": [P, ctypes.c_int32],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
             fn.argtypes = args
-            fn.restype = None if name in ("sanepic_destroy", "sanepic_unet_destroy") else ctypes.c_int
+            fn.restype = (None if name in ("sanepic_destroy", "sanepic_unet_destroy", "sanepic_host_free")
+                          else ctypes.c_int)
         L.sanepic_last_error.argtypes = [P]
         L.sanepic_last_error.restype = ctypes.c_char_p
         L.sanepic_unet_last_error.argtypes = [P]
@@ -179,6 +189,41 @@ def ring_to_face(nside, ring):
     if rc:
         raise SanepicError("sanepic_ring_to_face: bad nside or pixel")
     return pos
+
+
+class PinnedInputs:
+    """A synth dataset copied into pinned host memory (sanepic_host_alloc),
+    the form Context.load_async uploads at the full host-link rate.  Keep it
+    alive until Context.setup() has returned."""
+
+    def __init__(self, cfg, ds):
+        nsamp = cfg.nseg * cfg.ns
+        self._bufs = []
+        self.ra = self._alloc(nsamp)
+        self.dec = self._alloc(nsamp)
+        self.psi = self._alloc(nsamp)
+        self.tod = self._alloc(cfg.ndet * nsamp).reshape(cfg.ndet, nsamp)
+        self.ra[:] = ds["ra"]
+        self.dec[:] = ds["dec"]
+        self.psi[:] = ds["psi"]
+        self.tod[:] = ds["tod"]
+        self.ispf = ds["ispf"]
+
+    def _alloc(self, n):
+        p = ctypes.c_void_p()
+        if lib().sanepic_host_alloc(n * 8, ctypes.byref(p)):
+            raise SanepicError(f"sanepic_host_alloc: {lib().sanepic_last_error(None).decode()}")
+        self._bufs.append(p)
+        return np.ctypeslib.as_array((ctypes.c_double * n).from_address(p.value))
+
+    def close(self):
+        for p in self._bufs:
+            lib().sanepic_host_free(p)
+        self._bufs = []
+        self.ra = self.dec = self.psi = self.tod = None
+
+    def __del__(self):
+        self.close()
 
 
 class UNet:
@@ -306,6 +351,21 @@ class Context:
             for s in range(self.nseg):
                 self.set_tod(k, s, ds["tod"][k, s * ns:(s + 1) * ns])
 
+    def load_async(self, cfg, pin):
+        """Queues the uploads of a PinnedInputs (returns before they finish)."""
+        ns = self.ns
+        for k in range(self.ndet):
+            lel, xel = cfg.offsets_deg[k]
+            self.set_detector(k, lel, xel, cfg.det_angles_deg[k], pin.ispf[k])
+        for s in range(self.nseg):
+            sl = slice(s * ns, (s + 1) * ns)
+            self._chk(lib().sanepic_set_pointing_async(self.h, s, _dp(pin.ra[sl]), _dp(pin.dec[sl]),
+                                                       _dp(pin.psi[sl])), "set_pointing_async")
+        for k in range(self.ndet):
+            for s in range(self.nseg):
+                self._chk(lib().sanepic_set_tod_async(self.h, k, s, _dp(pin.tod[k, s * ns:(s + 1) * ns])),
+                          "set_tod_async")
+
     def pixelize(self):
         npix = ctypes.c_int64()
         ndef = ctypes.c_int64()
@@ -325,6 +385,17 @@ class Context:
         out = np.empty(12 * self.nside ** 2, dtype=np.int64)
         self._chk(lib().sanepic_get_indpix(self.h, _ip(out)), "get_indpix")
         return out
+
+    def autotune(self, cache=None):
+        """Times every four-step FFT split and keeps the fastest (before setup)."""
+        l1 = ctypes.c_int32()
+        ms = ctypes.c_double()
+        self._chk(lib().sanepic_autotune(self.h, cache.encode() if cache else None, ctypes.byref(l1),
+                                         ctypes.byref(ms)), "autotune")
+        return l1.value, ms.value
+
+    def set_fft_split(self, log2_n1):
+        self._chk(lib().sanepic_set_fft_split(self.h, log2_n1), "set_fft_split")
 
     def setup(self):
         self._chk(lib().sanepic_setup(self.h), "setup")
@@ -359,8 +430,10 @@ class Context:
         self._chk(lib().sanepic_chi2(self.h, _dp(x), ctypes.byref(out)), "chi2")
         return out.value
 
-    def solve_begin(self, precond=PRECOND_DIAG, refresh=10, x0=None, deflate=False, fallback_growth=0.0):
-        opts = SolveOpts(precond, refresh, 1 if deflate else 0, fallback_growth)
+    def solve_begin(self, precond=PRECOND_DIAG, refresh=10, x0=None, deflate=False, fallback_growth=0.0,
+                    graph=False, fcg_window=0):
+        opts = SolveOpts(precond, refresh, 1 if deflate else 0, fallback_growth, 1 if graph else 0,
+                         fcg_window)
         xp = None if x0 is None else _dp(_f64(x0))
         self._x0_keep = None if x0 is None else _f64(x0)
         self._chk(lib().sanepic_solve_begin(self.h, ctypes.byref(opts), xp), "solve_begin")
@@ -463,8 +536,9 @@ class Context:
         return r, e
 
     def train(self, steps, lr=1e-3, lr_final=1e-5, seed=0, first=0, count=0, beta1=0.9,
-              beta2=0.999, eps=1e-8, weight_decay=0.0, loss=LOSS_ANORM):
-        o = TrainOpts(steps, lr, lr_final, beta1, beta2, eps, weight_decay, seed, first, count, loss)
+              beta2=0.999, eps=1e-8, weight_decay=0.0, loss=LOSS_ANORM, sched_offset=0, sched_total=0):
+        o = TrainOpts(steps, lr, lr_final, beta1, beta2, eps, weight_decay, seed, first, count, loss,
+                      sched_offset, sched_total)
         hist = np.empty(steps, dtype=np.float32)
         self._chk(lib().sanepic_learn_train(self.h, ctypes.byref(o), _fp(hist)), "learn_train")
         return hist

@@ -76,8 +76,11 @@ int prof_collect(sanepic_ctx *c) {
 }
 
 // ------------------------------------------------------------------ tables
+int build_weights(sanepic_ctx *c);
+
+// Twiddles (once) and the noise weights in the current four-step layout.
 int build_tables(sanepic_ctx *c) {
-  if (c->tw) return 0;
+  if (c->tw) return build_weights(c);
   const int64_t N = c->ns;
   std::vector<double2> t4(4096);
   const long double PI2 = 6.283185307179586476925286766559005768L;
@@ -101,7 +104,26 @@ int build_tables(sanepic_ctx *c) {
   ACHK(c, dalloc(c, c->twhi, nhi));
   HIPCHK(c, hipMemcpy(c->twlo, lo.data(), 4096 * sizeof(double2), hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->twhi, hi.data(), nhi * sizeof(double2), hipMemcpyHostToDevice));
-  // weights in row layout: wt[det][k2*N1 + k1] = W[k2 + N2*k1], W[k] = SpN[min(k, N-k)]
+  if (c->cfg.precision == SANEPIC_MIXED) {
+    std::vector<float2> f4(4096), flo(4096), fhi(nhi);
+    for (int j = 0; j < 4096; j++) f4[j] = make_float2((float)t4[j].x, (float)t4[j].y);
+    for (int j = 0; j < 4096; j++) flo[j] = make_float2((float)lo[j].x, (float)lo[j].y);
+    for (int64_t j = 0; j < nhi; j++) fhi[j] = make_float2((float)hi[j].x, (float)hi[j].y);
+    ACHK(c, dalloc(c, c->tw32, 4096));
+    ACHK(c, dalloc(c, c->twlo32, 4096));
+    ACHK(c, dalloc(c, c->twhi32, nhi));
+    HIPCHK(c, hipMemcpy(c->tw32, f4.data(), 4096 * sizeof(float2), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->twlo32, flo.data(), 4096 * sizeof(float2), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->twhi32, fhi.data(), nhi * sizeof(float2), hipMemcpyHostToDevice));
+  }
+  return build_weights(c);
+}
+
+// Weights in row layout: wt[det][k2*N1 + k1] = W[k2 + N2*k1], W[k] = SpN[min(k, N-k)]
+// (depends on the four-step split; rebuilt when the autotuner changes it).
+int build_weights(sanepic_ctx *c) {
+  if (c->wt) return 0;
+  const int64_t N = c->ns;
   const int64_t N1 = c->g.N1, N2 = c->g.N2;
   std::vector<double> w((size_t)c->ndet * N);
   for (int64_t det = 0; det < c->ndet; det++) {
@@ -115,19 +137,9 @@ int build_tables(sanepic_ctx *c) {
   ACHK(c, dalloc(c, c->wt, c->ndet * N));
   HIPCHK(c, hipMemcpy(c->wt, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice));
   if (c->cfg.precision == SANEPIC_MIXED) {
-    std::vector<float2> f4(4096), flo(4096), fhi(nhi);
-    for (int j = 0; j < 4096; j++) f4[j] = make_float2((float)t4[j].x, (float)t4[j].y);
-    for (int j = 0; j < 4096; j++) flo[j] = make_float2((float)lo[j].x, (float)lo[j].y);
-    for (int64_t j = 0; j < nhi; j++) fhi[j] = make_float2((float)hi[j].x, (float)hi[j].y);
     std::vector<float> wf(w.size());
     for (size_t i = 0; i < w.size(); i++) wf[i] = (float)w[i];
-    ACHK(c, dalloc(c, c->tw32, 4096));
-    ACHK(c, dalloc(c, c->twlo32, 4096));
-    ACHK(c, dalloc(c, c->twhi32, nhi));
     ACHK(c, dalloc(c, c->wt32, c->ndet * N));
-    HIPCHK(c, hipMemcpy(c->tw32, f4.data(), 4096 * sizeof(float2), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->twlo32, flo.data(), 4096 * sizeof(float2), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->twhi32, fhi.data(), nhi * sizeof(float2), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->wt32, wf.data(), wf.size() * sizeof(float), hipMemcpyHostToDevice));
   }
   return 0;
@@ -316,6 +328,66 @@ template int run_scatter<float>(sanepic_ctx *, const float *, double *, int);
 
 namespace {
 
+// One plain PCG iteration (the loop body of sanepic.cc:1417-1605): q = A d,
+// alpha, x and r updates or the explicit residual with chi2, new direction.
+// Only kernel launches, so it can be captured into a graph.
+int enqueue_iteration(sanepic_ctx *c, bool refresh) {
+  CGArgs a = cg_args(c);
+  ACHK(c, apply_A(c, c->d, c->q, 1, false, false));
+  LCHK(c, cg_launch_alpha(a, c->st));
+  LCHK(c, cg_launch_update(a, refresh ? 1 : 0, c->st));
+  if (refresh) {
+    ACHK(c, apply_A(c, c->x, c->q, 1, true, true));
+    LCHK(c, cg_launch_refresh_r(a, c->st));
+  }
+  LCHK(c, cg_launch_dir(a, c->st));
+  return 0;
+}
+
+void drop_graphs(sanepic_ctx *c) {
+  for (auto &g : c->gexec)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+}
+
+// Admissible four-step splits: N1 = 2^lo .. 2^hi with the column length
+// 2^(log2N - log2N1) in the kernels' range (16 .. 4096, or 16 .. 1024 when
+// the columns carry the radix 3 or 5).
+void split_range(const PairGeom &g, int *lo, int *hi) {
+  const int l = g.log2N;
+  *lo = std::max(9, l - (g.M == 1 ? 12 : 10));
+  *hi = std::min(12, l - 4);
+}
+
+// Four-step split N1 = 2^log2n1 (the weight tables follow the split).
+int set_split(sanepic_ctx *c, int log2n1) {
+  PairGeom &g = c->g;
+  const int l = g.log2N;
+  if (c->single) {
+    if (log2n1 != l) return fail(c, SANEPIC_EINVAL, "segments of <= 4096 samples use one pass");
+    return 0;
+  }
+  int lo, hi;
+  split_range(g, &lo, &hi);
+  if (log2n1 < lo || log2n1 > hi) return fail(c, SANEPIC_EINVAL, "four-step split out of range");
+  if (log2n1 == g.log2N1) return 0;
+  g.log2N1 = log2n1;
+  g.log2N2 = l - log2n1;
+  g.N1 = int64_t(1) << g.log2N1;
+  g.N2 = (int64_t)g.M << g.log2N2;
+  for (PairGeom *h : {&c->gauto}) {
+    h->log2N1 = g.log2N1;
+    h->log2N2 = g.log2N2;
+    h->N1 = g.N1;
+    h->N2 = g.N2;
+  }
+  dfree(c, c->wt, c->ndet * c->ns);
+  dfree(c, c->wt32, c->ndet * c->ns);
+  return 0;
+}
+
 double pixel_eps() {
   const char *e = getenv("SANEPIC_PIXEL_EPS");
   if (e && *e) return atof(e);
@@ -351,8 +423,22 @@ int sanepic_create(const sanepic_config *cfg, sanepic_ctx **out) {
     g_create_error = "ndet and nseg must be >= 1";
     return SANEPIC_EINVAL;
   }
-  if (ns < 16 || ns > (int64_t(1) << 24) || (ns & (ns - 1)) != 0) {
-    g_create_error = "ns must be a power of two in [16, 2^24], got " + std::to_string(ns);
+  // ns = M 2^l with M in {1, 3, 5}: powers of two from 16, or 3 or 5 times a
+  // power of two from 2^13 (the odd factor goes into the four-step columns)
+  int radix = 1;
+  int64_t p2 = ns;
+  if (ns > 0 && ns % 5 == 0) {
+    radix = 5;
+    p2 = ns / 5;
+  } else if (ns > 0 && ns % 3 == 0) {
+    radix = 3;
+    p2 = ns / 3;
+  }
+  const bool pow2 = p2 > 0 && (p2 & (p2 - 1)) == 0;
+  if (!pow2 || (radix == 1 && (ns < 16 || ns > (int64_t(1) << 24))) ||
+      (radix > 1 && (p2 < (int64_t(1) << 13) || p2 > (int64_t(1) << 22)))) {
+    g_create_error = "ns must be 2^l (16 <= ns <= 2^24) or 3 or 5 times 2^l (2^13 <= 2^l <= 2^22), got " +
+                     std::to_string(ns);
     return SANEPIC_EINVAL;
   }
   if (cfg->polar != 1) {
@@ -385,24 +471,25 @@ int sanepic_create(const sanepic_config *cfg, sanepic_ctx **out) {
     delete c;
     return SANEPIC_EHIP;
   }
-  const int l = ilog2(ns);
+  const int l = ilog2(p2);
   PairGeom &g = c->g;
   g.ppd = (c->nseg + 1) / 2;
   g.npairs = c->ndet * g.ppd;
   g.nseg = c->nseg;
   g.N = ns;
   g.log2N = l;
-  if (l <= 12) {
+  g.M = radix;
+  if (radix == 1 && l <= 12) {
     c->single = true;
     g.N1 = ns;
     g.N2 = 1;
     g.log2N1 = l;
     g.log2N2 = 0;
   } else {
-    g.log2N2 = std::max(4, l - 12);
-    g.log2N1 = l - g.log2N2;
+    g.log2N1 = radix == 1 ? l - std::max(4, l - 12) : std::min(12, l - 4);
+    g.log2N2 = l - g.log2N1;
     g.N1 = int64_t(1) << g.log2N1;
-    g.N2 = int64_t(1) << g.log2N2;
+    g.N2 = (int64_t)radix << g.log2N2;
   }
   c->gauto = g;
   c->gauto.ppd = 1;
@@ -441,8 +528,13 @@ void sanepic_destroy(sanepic_ctx *c) {
     (void)hipEventDestroy(std::get<1>(e));
     (void)hipEventDestroy(std::get<2>(e));
   }
+  drop_graphs(c);
+  if (c->up_pt) (void)hipStreamSynchronize(c->up_pt);
+  if (c->up_tod) (void)hipStreamSynchronize(c->up_tod);
   for (void *p : c->allocs) (void)hipFree(p);
   if (c->st) (void)hipStreamDestroy(c->st);
+  if (c->up_pt) (void)hipStreamDestroy(c->up_pt);
+  if (c->up_tod) (void)hipStreamDestroy(c->up_tod);
   delete c;
 }
 
@@ -478,6 +570,56 @@ int sanepic_set_pointing(sanepic_ctx *c, int64_t iseg, const double *ra, const d
   return SANEPIC_OK;
 }
 
+int sanepic_host_alloc(uint64_t bytes, void **ptr) {
+  if (!ptr) return SANEPIC_EINVAL;
+  *ptr = nullptr;
+  if (hipHostMalloc(ptr, (size_t)std::max<uint64_t>(bytes, 1), hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    g_create_error = "hipHostMalloc of " + std::to_string(bytes) + " bytes failed";
+    *ptr = nullptr;
+    return SANEPIC_ENOMEM;
+  }
+  return SANEPIC_OK;
+}
+
+void sanepic_host_free(void *ptr) {
+  if (ptr) (void)hipHostFree(ptr);
+}
+
+static int upload_stream(sanepic_ctx *c, hipStream_t *s) {
+  if (!*s) HIPCHK(c, hipStreamCreateWithFlags(s, hipStreamNonBlocking));
+  return 0;
+}
+
+int sanepic_set_pointing_async(sanepic_ctx *c, int64_t iseg, const double *ra, const double *dec,
+                               const double *psi) {
+  if (!c) return SANEPIC_EINVAL;
+  if (iseg < 0 || iseg >= c->nseg) return fail(c, SANEPIC_EINVAL, "segment index out of range");
+  if (!c->ra) return fail(c, SANEPIC_EINVAL, "pointing already consumed by sanepic_pixelize");
+  if (!ra || !dec || !psi) return fail(c, SANEPIC_EINVAL, "null pointing buffer");
+  ACHK(c, upload_stream(c, &c->up_pt));
+  const size_t nb = (size_t)c->ns * sizeof(double);
+  HIPCHK(c, hipMemcpyAsync(c->ra + iseg * c->ns, ra, nb, hipMemcpyHostToDevice, c->up_pt));
+  HIPCHK(c, hipMemcpyAsync(c->dec + iseg * c->ns, dec, nb, hipMemcpyHostToDevice, c->up_pt));
+  HIPCHK(c, hipMemcpyAsync(c->psi + iseg * c->ns, psi, nb, hipMemcpyHostToDevice, c->up_pt));
+  c->pt_ok[iseg] = 1;
+  return SANEPIC_OK;
+}
+
+int sanepic_set_tod_async(sanepic_ctx *c, int64_t idet, int64_t iseg, const double *tod) {
+  if (!c) return SANEPIC_EINVAL;
+  if (idet < 0 || idet >= c->ndet || iseg < 0 || iseg >= c->nseg)
+    return fail(c, SANEPIC_EINVAL, "TOD index out of range");
+  if (c->setup_done) return fail(c, SANEPIC_EINVAL, "TOD already consumed by sanepic_setup");
+  if (!tod) return fail(c, SANEPIC_EINVAL, "null TOD buffer");
+  ACHK(c, dalloc(c, c->tod, c->ntot));
+  ACHK(c, upload_stream(c, &c->up_tod));
+  HIPCHK(c, hipMemcpyAsync(c->tod + (idet * c->nseg + iseg) * c->ns, tod, (size_t)c->ns * sizeof(double),
+                           hipMemcpyHostToDevice, c->up_tod));
+  c->tod_ok[idet * c->nseg + iseg] = 1;
+  return SANEPIC_OK;
+}
+
 int sanepic_set_tod(sanepic_ctx *c, int64_t idet, int64_t iseg, const double *tod) {
   if (!c) return SANEPIC_EINVAL;
   if (idet < 0 || idet >= c->ndet || iseg < 0 || iseg >= c->nseg)
@@ -492,6 +634,7 @@ int sanepic_set_tod(sanepic_ctx *c, int64_t idet, int64_t iseg, const double *to
 
 int sanepic_pixelize(sanepic_ctx *c, int64_t *npix_out, int64_t *n_deferred) {
   if (!c) return SANEPIC_EINVAL;
+  if (c->up_pt) HIPCHK(c, hipStreamSynchronize(c->up_pt));
   if (c->pixelized) return fail(c, SANEPIC_EINVAL, "sanepic_pixelize already ran");
   for (int64_t i = 0; i < c->ndet; i++)
     if (!c->det_ok[i]) return fail(c, SANEPIC_EINVAL, "detector " + std::to_string(i) + " not set");
@@ -715,8 +858,102 @@ int sanepic_get_indpix(sanepic_ctx *c, int64_t *indpix) {
   return SANEPIC_OK;
 }
 
+int sanepic_set_fft_split(sanepic_ctx *c, int32_t log2_n1) {
+  ACHK(c, check_ready(c, false));
+  if (c->setup_done) return fail(c, SANEPIC_EINVAL, "the split is fixed by sanepic_setup");
+  return set_split(c, log2_n1);
+}
+
+int sanepic_autotune(sanepic_ctx *c, const char *cache, int32_t *log2_n1, double *op_ms) {
+  ACHK(c, check_ready(c, false));
+  if (c->setup_done) return fail(c, SANEPIC_EINVAL, "call sanepic_autotune before sanepic_setup");
+  if (c->single) {
+    if (log2_n1) *log2_n1 = c->g.log2N1;
+    if (op_ms) *op_ms = 0.0;
+    return SANEPIC_OK;
+  }
+  hipDeviceProp_t prop{};
+  HIPCHK(c, hipGetDeviceProperties(&prop, c->cfg.device));
+  char key[512];
+  snprintf(key, sizeof(key), "%s ns=%ld ndet=%ld nseg=%ld npix=%ld precision=%d", prop.gcnArchName, (long)c->ns,
+           (long)c->ndet, (long)c->nseg, (long)c->npix, (int)c->cfg.precision);
+  if (cache) {
+    if (FILE *f = fopen(cache, "r")) {
+      char line[640];
+      int best = -1;
+      double ms = 0.0;
+      const size_t kl = strlen(key);
+      while (fgets(line, sizeof(line), f))
+        if (strncmp(line, key, kl) == 0 && line[kl] == ' ') sscanf(line + kl, " log2_n1=%d ms=%lf", &best, &ms);
+      fclose(f);
+      if (best > 0) {
+        ACHK(c, set_split(c, best));
+        if (log2_n1) *log2_n1 = best;
+        if (op_ms) *op_ms = ms;
+        return SANEPIC_OK;
+      }
+    }
+  }
+  // buffers the operator needs (sanepic_setup reuses them)
+  const int64_t N = c->ns;
+  ACHK(c, dalloc(c, c->z, c->ntot));
+  ACHK(c, dalloc(c, c->buf, c->g.npairs * N));
+  if (c->cfg.precision == SANEPIC_MIXED) {
+    ACHK(c, dalloc(c, c->buf32, c->g.npairs * N));
+    ACHK(c, dalloc(c, c->z32, c->ntot));
+    ACHK(c, dalloc(c, c->map32, 3 * c->npix));
+  }
+  double *xm = nullptr, *qm = nullptr;
+  ACHK(c, dalloc(c, xm, 3 * c->npix));
+  ACHK(c, dalloc(c, qm, 3 * c->npix));
+  LCHK(c, cg_launch_monopole(xm, c->npix, c->st));
+  hipEvent_t ea, eb;
+  HIPCHK(c, hipEventCreate(&ea));
+  HIPCHK(c, hipEventCreate(&eb));
+  const bool prof = c->prof;
+  c->prof = false;
+  int lo, hi;
+  split_range(c->g, &lo, &hi);
+  int best = c->g.log2N1;
+  double best_ms = 1e300;
+  int rc = 0;
+  for (int l1 = lo; l1 <= hi && !rc; l1++) {
+    rc = set_split(c, l1);
+    if (!rc) rc = build_tables(c);
+    if (!rc) rc = apply_A(c, xm, qm, 0, false, false);  // warm-up
+    if (rc) break;
+    if (hipEventRecord(ea, c->st) != hipSuccess) rc = fail(c, SANEPIC_EHIP, "hipEventRecord");
+    for (int k = 0; k < 3 && !rc; k++) rc = apply_A(c, xm, qm, 0, false, false);
+    if (!rc && hipEventRecord(eb, c->st) != hipSuccess) rc = fail(c, SANEPIC_EHIP, "hipEventRecord");
+    if (!rc && hipEventSynchronize(eb) != hipSuccess) rc = fail(c, SANEPIC_EHIP, "hipEventSynchronize");
+    float ms = 0.f;
+    if (!rc && hipEventElapsedTime(&ms, ea, eb) != hipSuccess) rc = fail(c, SANEPIC_EHIP, "hipEventElapsedTime");
+    if (!rc && ms / 3.0 < best_ms) {
+      best_ms = ms / 3.0;
+      best = l1;
+    }
+  }
+  c->prof = prof;
+  (void)hipEventDestroy(ea);
+  (void)hipEventDestroy(eb);
+  dfree(c, xm, 3 * c->npix);
+  dfree(c, qm, 3 * c->npix);
+  if (rc) return rc;
+  ACHK(c, set_split(c, best));
+  ACHK(c, build_tables(c));
+  if (cache)
+    if (FILE *f = fopen(cache, "a")) {
+      fprintf(f, "%s log2_n1=%d ms=%.4f\n", key, best, best_ms);
+      fclose(f);
+    }
+  if (log2_n1) *log2_n1 = best;
+  if (op_ms) *op_ms = best_ms;
+  return SANEPIC_OK;
+}
+
 int sanepic_setup(sanepic_ctx *c) {
   ACHK(c, check_ready(c, false));
+  if (c->up_tod) HIPCHK(c, hipStreamSynchronize(c->up_tod));
   if (c->setup_done) return fail(c, SANEPIC_EINVAL, "sanepic_setup already ran");
   for (size_t i = 0; i < c->tod_ok.size(); i++)
     if (!c->tod_ok[i]) return fail(c, SANEPIC_EINVAL, "TOD block " + std::to_string(i) + " not set");
@@ -885,11 +1122,16 @@ int sanepic_solve_begin(sanepic_ctx *c, const sanepic_solve_opts *opts, const do
     if (g < 0.0 || !std::isfinite(g)) return fail(c, SANEPIC_EINVAL, "fallback_growth must be >= 0");
     const double gg = g > 0.0 ? g : 2.0;
     c->fgrowth = gg * gg;  // compared with |r|^2
+    const int m = opts ? opts->fcg_window : 0;
+    if (m < 0 || m > kFcgMax) return fail(c, SANEPIC_EINVAL, "fcg_window must be in 0..32");
+    c->fcg_m = m > 0 ? m : 16;
   }
   if (c->precond != SANEPIC_PRECOND_DIAG && c->precond != SANEPIC_PRECOND_BLOCK3 &&
       c->precond != SANEPIC_PRECOND_LEARNED)
     return fail(c, SANEPIC_EINVAL, "unknown preconditioner");
   const bool learned = c->precond == SANEPIC_PRECOND_LEARNED;
+  drop_graphs(c);
+  c->use_graph = opts && opts->graph && !learned;
   if (learned && !c->un) return fail(c, SANEPIC_EINVAL, "no network attached (sanepic_learn_attach)");
   c->begun = false;
   const int64_t npix = c->npix;
@@ -936,16 +1178,27 @@ int sanepic_solve_step(sanepic_ctx *c, sanepic_iter_info *info) {
     if (!c->un) return fail(c, SANEPIC_EINVAL, "the network was detached");
     return fpcg_step(c, info);
   }
-  CGArgs a = cg_args(c);
-  ACHK(c, apply_A(c, c->d, c->q, 1, false, false));
-  LCHK(c, cg_launch_alpha(a, c->st));
   const bool refresh = (c->iter % c->refresh) == 0;
-  LCHK(c, cg_launch_update(a, refresh ? 1 : 0, c->st));
-  if (refresh) {
-    ACHK(c, apply_A(c, c->x, c->q, 1, true, true));
-    LCHK(c, cg_launch_refresh_r(a, c->st));
+  if (c->use_graph && !c->prof) {
+    hipGraphExec_t &gx = c->gexec[refresh ? 1 : 0];
+    if (!gx) {
+      HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+      const int rc = enqueue_iteration(c, refresh);
+      hipGraph_t graph = nullptr;
+      const hipError_t e = hipStreamEndCapture(c->st, &graph);
+      if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+      }
+      HIPCHK(c, e);
+      const hipError_t e2 = hipGraphInstantiate(&gx, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      HIPCHK(c, e2);
+    }
+    HIPCHK(c, hipGraphLaunch(gx, c->st));
+  } else {
+    ACHK(c, enqueue_iteration(c, refresh));
   }
-  LCHK(c, cg_launch_dir(a, c->st));
   CGScalars h{};
   HIPCHK(c, hipMemcpyAsync(&h, c->sc, sizeof(h), hipMemcpyDeviceToHost, c->st));
   HIPCHK(c, hipStreamSynchronize(c->st));

@@ -2,13 +2,14 @@
 // counterpart in the reference): the network handle, training-pair harvest,
 // training, and flexible PCG.
 //
-// Flexible PCG (Notay 2000; Polak-Ribiere beta) with z = M r + sigma U(r):
+// Truncated flexible CG, FCG(m) (Notay 2000), with z = M r + sigma U(r):
 //   q = A d;  alpha = (r.d)/(d.q);  x += alpha d;  r -= alpha q
 //   z = M r + sigma U(r), or z = M r (block-Jacobi) when the step just
 //     taken along a network direction let |r| grow by more than the factor
 //     `fallback_growth` (the step itself is kept: the exact line search
 //     lowered the A-norm error), and when r.z <= 0
-//   beta = -alpha (z.q) / delta_old;  d = z + beta d
+//   d = z - sum_{j = k-m+1..k} (z.A d_j / d_j.A d_j) d_j
+// (m = 1 is the Polak-Ribiere form), the directions and A d_j in a ring.
 // with the optional A-DEF2 monopole step folded into z as in the plain
 // solver, and the explicit residual b - A x every `refresh` iterations.
 #include <hip/hip_runtime.h>
@@ -77,6 +78,27 @@ FArgs f_args(sanepic_ctx *c) {
   a.sc = c->sc;
   a.fs = c->fs;
   a.part = c->part;
+  a.Dr = c->Dring;
+  a.Qr = c->Qring;
+  a.m = c->fcg_m;
+  a.slot = -1;
+  a.valid = 0;
+  a.fpart = c->fpart;
+  return a;
+}
+
+// FArgs whose d and q are the current FCG ring slot.
+FArgs f_args_ring(sanepic_ctx *c) {
+  FArgs a = f_args(c);
+  const int64_t n3 = 3 * c->npix;
+  a.slot = c->fcg_slot;
+  a.d = c->Dring + (int64_t)c->fcg_slot * n3;
+  a.q = c->Qring + (int64_t)c->fcg_slot * n3;
+  // the directions stored so far, at most the last m
+  const int64_t nv = std::min<int64_t>(c->fcg_stored, c->fcg_m);
+  uint32_t v = 0;
+  for (int64_t j = 0; j < nv; j++) v |= 1u << (int)((c->fcg_slot - j + c->fcg_m) % c->fcg_m);
+  a.valid = v;
   return a;
 }
 
@@ -125,21 +147,26 @@ int fpcg_precond(sanepic_ctx *c, bool learned) {
   return 0;
 }
 
-// New preconditioned residual, its dot products and the next direction.
-int fpcg_direction(sanepic_ctx *c, bool restart, bool use_net) {
-  FArgs fa = f_args(c);
+// New preconditioned residual z and the next direction, A-orthogonalised
+// against the stored ones (FCG(m)), written into the next ring slot.  A
+// network z with r.(z + c e_I) <= 0 is replaced by block-Jacobi.
+int fcg_direction(sanepic_ctx *c, bool use_net) {
+  FArgs fa = f_args_ring(c);
+  const int next = (c->fcg_stored == 0) ? c->fcg_slot : (c->fcg_slot + 1) % c->fcg_m;
   ACHK(c, fpcg_precond(c, use_net && c->un != nullptr));
-  LCHK(c, f_launch_dots_z(fa, restart ? 1 : 0, c->st));
+  LCHK(c, f_launch_fcg_dots(fa, c->st));
   if (c->last_z_learned) {
     FScalars h{};
     ACHK(c, d2h(c, &h, c->fs, sizeof(h)));
     if (!(h.delta > 0.0) || !std::isfinite(h.delta)) {
       c->n_rejected++;
       ACHK(c, fpcg_precond(c, false));
-      LCHK(c, f_launch_dots_z(fa, restart ? 1 : 2, c->st));
+      LCHK(c, f_launch_fcg_dots(fa, c->st));
     }
   }
-  LCHK(c, f_launch_dir(fa, restart ? 1 : 0, c->st));
+  LCHK(c, f_launch_fcg_dir(fa, next, c->st));
+  c->fcg_slot = next;
+  c->fcg_stored++;
   return 0;
 }
 
@@ -156,6 +183,15 @@ int pair_step(sanepic_ctx *c, int64_t k, int kind, bool train, float lr, const s
   if (unet::net_forward(n)) return fail(c, SANEPIC_EHIP, "network forward: " + n.err);
   if (kind == SANEPIC_LOSS_BLOCK) {
     LCHK(c, learn_loss(la, rk, ek, n.OUT, n.gO, train ? 1 : 0, c->st));
+  } else if (kind == SANEPIC_LOSS_RESIDUAL) {
+    LCHK(c, launch_f2d_vec(rk, c->r, n3, c->st));
+    LCHK(c, learn_apply_out(la, c->r, n.OUT, c->zf, c->st));
+    ACHK(c, apply_A(c, c->zf, c->qx, 1, false, false));
+    LCHK(c, learn_rloss(la, c->qx, c->r, c->zf, c->st));  // zf <- u = A z - r
+    if (train) {
+      ACHK(c, apply_A(c, c->zf, c->qx, 1, false, false));
+      LCHK(c, learn_rloss_grad(la, c->qx, n.gO, c->st));
+    }
   } else {
     LCHK(c, launch_f2d_vec(rk, c->r, n3, c->st));
     LCHK(c, learn_apply_out(la, c->r, n.OUT, c->zf, c->st));
@@ -176,14 +212,25 @@ namespace sctx {
 
 int fpcg_begin(sanepic_ctx *c) {
   ACHK(c, need_net(c));
+  const int64_t n3 = 3 * c->npix;
   ACHK(c, dalloc(c, c->fs, 1));
-  ACHK(c, dalloc(c, c->zf, 3 * c->npix));
-  ACHK(c, dalloc(c, c->qx, 3 * c->npix));
+  ACHK(c, dalloc(c, c->zf, n3));
+  ACHK(c, dalloc(c, c->qx, n3));
+  ACHK(c, dalloc(c, c->fpart, 512 * 40));
+  if (c->fcg_alloc_m < c->fcg_m) {
+    dfree(c, c->Dring, (int64_t)c->fcg_alloc_m * n3);
+    dfree(c, c->Qring, (int64_t)c->fcg_alloc_m * n3);
+    ACHK(c, dalloc(c, c->Dring, (int64_t)c->fcg_m * n3));
+    ACHK(c, dalloc(c, c->Qring, (int64_t)c->fcg_m * n3));
+    c->fcg_alloc_m = c->fcg_m;
+  }
   HIPCHK(c, hipMemsetAsync(c->fs, 0, sizeof(FScalars), c->st));
-  FArgs fa = f_args(c);
+  FArgs fa = f_args(c);  // r = b - q with the q of solve_begin
   LCHK(c, f_launch_init(fa, c->st));
   c->n_learned = c->n_fallback = c->n_rejected = 0;
-  ACHK(c, fpcg_direction(c, true, true));
+  c->fcg_slot = 0;
+  c->fcg_stored = 0;
+  ACHK(c, fcg_direction(c, true));
   FScalars h{};
   ACHK(c, d2h(c, &h, c->fs, sizeof(h)));
   c->delta0 = h.delta;
@@ -191,9 +238,9 @@ int fpcg_begin(sanepic_ctx *c) {
 }
 
 int fpcg_step(sanepic_ctx *c, sanepic_iter_info *info) {
-  FArgs fa = f_args(c);
+  FArgs fa = f_args_ring(c);
   FScalars h{};
-  ACHK(c, apply_A(c, c->d, c->q, 1, false, false));
+  ACHK(c, apply_A(c, fa.d, fa.q, 1, false, false));
   LCHK(c, f_launch_alpha(fa, c->st));
   LCHK(c, f_launch_update(fa, c->st));
   ACHK(c, d2h(c, &h, c->fs, sizeof(h)));
@@ -208,7 +255,7 @@ int fpcg_step(sanepic_ctx *c, sanepic_iter_info *info) {
     ACHK(c, apply_A(c, c->x, c->qx, 1, true, true));
     LCHK(c, f_launch_refresh(fa, c->st));
   }
-  ACHK(c, fpcg_direction(c, false, !fallback));
+  ACHK(c, fcg_direction(c, !fallback));
   CGScalars hs{};
   ACHK(c, d2h(c, &h, c->fs, sizeof(h)));
   ACHK(c, d2h(c, &hs, c->sc, sizeof(hs)));
@@ -218,7 +265,7 @@ int fpcg_step(sanepic_ctx *c, sanepic_iter_info *info) {
     info->crit2 = c->delta0 != 0 ? h.delta / c->delta0 : 0.0;
     info->chi2 = hs.chi2;
     info->alpha = h.alpha;
-    info->beta = h.beta;
+    info->beta = 0.0;
   }
   c->iter++;
   return 0;
@@ -438,12 +485,22 @@ int sanepic_learn_harvest(sanepic_ctx *c, const sanepic_harvest_opts *o, int64_t
     c->pair_cap = o->capacity;
     c->npairs = 0;
   }
-  // sqrt of the filter weights: the filter pass with W' = sqrt(W / N) applied
-  // twice equals the N^-1 pass (both are unnormalised FFT round trips)
+  // Weights that turn white noise w into N^-1 n with n ~ N(0, N_gen), as one
+  // unnormalised filter pass: V = W / sqrt(N W') where W' = W except at DC,
+  // where it is the first bin's.  The solver's DC weight (1e-6 of bin 1,
+  // sanepic.cc:537) regularises the segment offsets; as a noise model it
+  // would give every segment a huge random offset, which data (whose mean
+  // has the variance of the lowest bins) do not have.
   if (!c->wt_sqrt) {
     std::vector<double> w((size_t)(c->ndet * N));
     ACHK(c, d2h(c, w.data(), c->wt, w.size() * sizeof(double)));
-    for (auto &v : w) v = std::sqrt(std::fabs(v) / (double)N);
+    const int64_t i1 = c->g.N2 > 1 ? c->g.N1 : 1;  // row-layout position of bin 1
+    for (int64_t d = 0; d < c->ndet; d++) {
+      double *row = w.data() + d * N;
+      const double w0 = std::fabs(row[0]), w1 = std::fabs(row[i1]);
+      for (int64_t i = 0; i < N; i++) row[i] = std::sqrt(std::fabs(row[i]) / (double)N);
+      row[0] = w1 > 0.0 ? w0 / std::sqrt((double)N * w1) : 0.0;
+    }
     ACHK(c, dalloc(c, c->wt_sqrt, c->ndet * N));
     HIPCHK(c, hipMemcpyAsync(c->wt_sqrt, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
@@ -569,7 +626,8 @@ int sanepic_learn_train(sanepic_ctx *c, const sanepic_train_opts *o, float *loss
   ACHK(c, check_ready(c, true));
   ACHK(c, need_net(c));
   if (!o || o->steps <= 0 || !(o->lr > 0) ||
-      (o->loss_kind != SANEPIC_LOSS_ANORM && o->loss_kind != SANEPIC_LOSS_BLOCK))
+      (o->loss_kind != SANEPIC_LOSS_ANORM && o->loss_kind != SANEPIC_LOSS_BLOCK &&
+       o->loss_kind != SANEPIC_LOSS_RESIDUAL))
     return fail(c, SANEPIC_EINVAL, "bad training options");
   const int64_t first = std::max<int64_t>(0, o->first);
   const int64_t count = o->count > 0 ? std::min(o->count, c->npairs - first) : c->npairs - first;
@@ -586,7 +644,8 @@ int sanepic_learn_train(sanepic_ctx *c, const sanepic_train_opts *o, float *loss
       std::shuffle(order.begin(), order.end(), rng);
       pos = 0;
     }
-    const double frac = (double)s / (double)o->steps;
+    const int64_t total = o->sched_total > 0 ? o->sched_total : o->steps;
+    const double frac = std::min(1.0, (double)(o->sched_offset + s) / (double)total);
     const float lr = (float)(o->lr_final + 0.5 * (o->lr - o->lr_final) * (1.0 + std::cos(M_PI * frac)));
     ACHK(c, pair_step(c, order[pos++], o->loss_kind, true, lr, o));
     if (loss) {

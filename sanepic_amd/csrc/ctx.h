@@ -20,6 +20,7 @@ struct sanepic_ctx {
   sanepic_config cfg{};
   std::string err;
   hipStream_t st = nullptr;
+  hipStream_t up_pt = nullptr, up_tod = nullptr;  // upload streams (queued set_* calls)
   int64_t ns = 0, nseg = 0, ndet = 0, ntot = 0, nside = 0, npix = -1;
   PairGeom g{}, gauto{};
   bool single = false;
@@ -68,6 +69,9 @@ struct sanepic_ctx {
   double kt_ms[kProfClasses] = {0};
   int64_t kt_n[kProfClasses] = {0};
   std::vector<void *> allocs;
+  // HIP graphs of one plain PCG iteration: [0] normal, [1] residual refresh
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  bool use_graph = false;
   // learned preconditioner (flexible PCG)
   sanepic_unet *un = nullptr;
   int32_t *fidx = nullptr;          // [npix] face position of each seen pixel
@@ -76,6 +80,10 @@ struct sanepic_ctx {
   double *zf = nullptr, *qx = nullptr;
   double delta0 = 0.0;
   double fgrowth = 4.0;             // |r|^2 growth that sends the next z to block-Jacobi
+  // FCG(m) rings: directions and their images under A
+  double *Dring = nullptr, *Qring = nullptr, *fpart = nullptr;
+  int fcg_m = 16, fcg_alloc_m = 0, fcg_slot = 0;
+  int64_t fcg_stored = 0;           // directions stored since the start
   bool last_z_learned = false;
   int64_t n_learned = 0, n_fallback = 0, n_rejected = 0;
   // training pairs

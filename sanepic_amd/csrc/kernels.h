@@ -48,6 +48,7 @@ struct PairGeom {
   int64_t N1, N2;  // row length, column length (N2 == 1: single pass)
   int log2N, log2N1, log2N2;
   int64_t pair0;   // first pair of this launch (chunked four-step passes)
+  int M;           // odd radix of N: N = M 2^log2N, N2 = M 2^log2N2 (M in 1, 3, 5)
 };
 
 // Operator pass selection.

@@ -16,7 +16,11 @@ struct FScalars {
   double loss;                 // last training / validation loss
   double var00;                // |b|^2
   double pad[6];
+  // FCG(m): per stored direction d_j (ring slot j): d_j.A d_j, sum of the I
+  // components of A d_j, and the coefficient of d_j in the next direction
+  double ring_dq[32], ring_sq[32], ring_beta[32];
 };
+constexpr int kFcgMax = 32;
 
 struct LearnArgs {
   int64_t npix;
@@ -47,11 +51,19 @@ int learn_loss(const LearnArgs &a, const float *r, const float *e, const float *
 int learn_anorm_loss(const LearnArgs &a, const double *z, const double *q, const double *r, const float *e,
                      uint16_t *gO, int write_grad, hipStream_t s);
 int launch_f2d_vec(const float *src, double *dst, int64_t n, hipStream_t s);  // dst = (double) src
+// Residual training loss |A z - r|^2 / |r|^2 from q = A z (u = q - r is
+// written for the gradient), and its gradient from v = A u into gO.
+int learn_rloss(const LearnArgs &a, const double *q, const double *r, double *u, hipStream_t s);
+int learn_rloss_grad(const LearnArgs &a, const double *v, uint16_t *gO, hipStream_t s);
 
 // Flexible PCG on interleaved [npix][3] FP64 vectors.
 struct FArgs {
   int64_t npix;
   double *x, *r, *d, *q, *b, *z, *qx;
+  double *Dr, *Qr;      // FCG(m) rings of directions and their images, [m][3 npix]
+  int m, slot;          // ring size, slot of the current direction d (= Dr + slot)
+  uint32_t valid;       // slots whose directions the next one is A-orthogonalised against
+  double *fpart;        // [512][40] reduction partials of the multi-dot
   const double *M;
   const double *az;     // A e_I (monopole deflation) or null
   const CGScalars *sc;  // einv of the deflation
@@ -70,6 +82,14 @@ int f_launch_zbj(const FArgs &a, hipStream_t s);      // z = M r
 // rejected one in the same iteration (delta_old stays the previous one).
 int f_launch_dots_z(const FArgs &a, int mode, hipStream_t s);
 int f_launch_dir(const FArgs &a, int restart, hipStream_t s);  // d = z + c e_I + beta d
+// FCG(m) (Notay 2000): the next direction is z + c e_I A-orthogonalised
+// against the stored directions in `valid`:
+//   d_new = z + c e_I - sum_j (z + c e_I).A d_j / (d_j.A d_j) d_j,
+// written into ring slot `next`.  The multi-dot also gives r.z, the coarse
+// coefficient c and delta.  f_launch_alpha stores d.A d and sum (A d)_I for
+// the current slot.
+int f_launch_fcg_dots(const FArgs &a, hipStream_t s);            // multi-dot, c, delta, beta_j
+int f_launch_fcg_dir(const FArgs &a, int next, hipStream_t s);   // d_new into slot `next`
 
 // Training-pair harvest helpers.
 int launch_rng_map(double *m, const int32_t *cnt, int64_t npix, uint64_t seed, double sig_i, double sig_p,

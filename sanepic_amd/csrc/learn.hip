@@ -216,6 +216,41 @@ __global__ __launch_bounds__(RBS) void k_anorm_grad(LearnArgs a, const double *q
   gO[pos + 2] = masked ? (uint16_t)0 : unet::f2bf((float)(g * (q[3 * p + 2] - r[3 * p + 2])));
 }
 
+// Residual loss of one pair: L = |A z - r|^2 / |r|^2 (the residual the step
+// z would leave); u = A z - r is kept for the gradient 2 A u / |r|^2.
+__global__ __launch_bounds__(RBS) void k_rloss_part(LearnArgs a, const double *q, const double *r, double *u) {
+  double acc[2] = {0.0, 0.0};
+  PIX_LOOP(i, 3 * a.npix) {
+    const double v = q[i] - r[i];
+    u[i] = v;
+    acc[0] += v * v;
+    acc[1] += r[i] * r[i];
+  }
+  store_partials<2>(acc, a.part);
+}
+
+__global__ __launch_bounds__(RBS) void k_rloss_final(LearnArgs a) {
+  double acc[NP];
+  sum_partials(a.part, acc);
+  if (threadIdx.x == 0) {
+    const double rr = acc[1] > 0.0 ? acc[1] : 1.0;
+    a.fs->loss = acc[0] / rr;
+    a.fs->pad[0] = rr;
+  }
+}
+
+// gradient w.r.t. the network output: sigma * 2 (A u) / |r|^2
+__global__ __launch_bounds__(RBS) void k_rloss_grad(LearnArgs a, const double *v, uint16_t *gO) {
+  const int64_t p = (int64_t)blockIdx.x * RBS + threadIdx.x;
+  if (p >= a.npix) return;
+  const double g = 2.0 * a.fs->sigma / a.fs->pad[0];
+  const int64_t pos = (int64_t)a.fidx[p] * 16;
+  const bool masked = a.cnt[p] < 2;
+  gO[pos] = unet::f2bf((float)(g * v[3 * p]));
+  gO[pos + 1] = masked ? (uint16_t)0 : unet::f2bf((float)(g * v[3 * p + 1]));
+  gO[pos + 2] = masked ? (uint16_t)0 : unet::f2bf((float)(g * v[3 * p + 2]));
+}
+
 __global__ __launch_bounds__(RBS) void k_f2d_vec(const float *s, double *d, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * RBS + threadIdx.x;
   if (i < n) d[i] = (double)s[i];
@@ -234,12 +269,92 @@ __global__ __launch_bounds__(RBS) void k_f_init(FArgs a) {
 }
 
 __global__ __launch_bounds__(RBS) void k_f_dq(FArgs a) {
-  double acc[2] = {0.0, 0.0};
-  PIX_LOOP(i, 3 * a.npix) {
-    acc[0] += a.d[i] * a.q[i];
-    acc[1] += a.r[i] * a.d[i];
+  double acc[3] = {0.0, 0.0, 0.0};
+  PIX_LOOP(p, a.npix) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      acc[0] += a.d[3 * p + c] * a.q[3 * p + c];
+      acc[1] += a.r[3 * p + c] * a.d[3 * p + c];
+    }
+    acc[2] += a.q[3 * p];
   }
-  store_partials<2>(acc, a.part);
+  store_partials<3>(acc, a.part);
+}
+
+// FCG(m) multi-dot: slots 0: r.z, 1: sum r_I, 2: AZ.z, 3 + j: z.A d_j (j in valid)
+constexpr int FNP = 40;
+
+__global__ __launch_bounds__(RBS) void k_fcg_dots(FArgs a) {
+  __shared__ double sh[FNP][RBS / 64];
+  double acc[3 + kFcgMax];
+#pragma unroll
+  for (int k = 0; k < 3 + kFcgMax; k++) acc[k] = 0.0;
+  const int64_t n3 = 3 * a.npix;
+  PIX_LOOP(p, a.npix) {
+    double z[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      z[c] = a.z[3 * p + c];
+      acc[0] += a.r[3 * p + c] * z[c];
+      if (a.az) acc[2] += a.az[3 * p + c] * z[c];
+    }
+    acc[1] += a.r[3 * p];
+#pragma unroll
+    for (int j = 0; j < kFcgMax; j++) {
+      if (j < a.m && ((a.valid >> j) & 1u)) {
+        const double *q = a.Qr + (int64_t)j * n3 + 3 * p;
+        acc[3 + j] += z[0] * q[0] + z[1] * q[1] + z[2] * q[2];
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 3 + kFcgMax; k++) acc[k] = wsum(acc[k]);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int k = 0; k < 3 + kFcgMax; k++) sh[k][w] = acc[k];
+  __syncthreads();
+  for (int k = threadIdx.x; k < FNP; k += RBS)
+    a.fpart[FNP * blockIdx.x + k] = k < 3 + kFcgMax ? (sh[k][0] + sh[k][1]) + (sh[k][2] + sh[k][3]) : 0.0;
+}
+
+__global__ __launch_bounds__(RBS) void k_fcg_final(FArgs a) {
+  // thread k sums slot k over the blocks in block order
+  __shared__ double tot[FNP];
+  for (int k = threadIdx.x; k < FNP; k += RBS) {
+    double s = 0.0;
+    for (int b = 0; b < RGRID; b++) s += a.fpart[FNP * b + k];
+    tot[k] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  FScalars *f = a.fs;
+  const double c = a.az ? a.sc->einv * (tot[1] - tot[2]) : 0.0;
+  f->rz = tot[0];
+  f->sr = tot[1];
+  f->azz = tot[2];
+  f->coarse = c;
+  f->delta_prev = f->delta;
+  f->delta = tot[0] + c * tot[1];
+  for (int j = 0; j < kFcgMax; j++) {
+    double beta = 0.0;
+    if (j < a.m && ((a.valid >> j) & 1u) && f->ring_dq[j] != 0.0)
+      beta = -(tot[3 + j] + c * f->ring_sq[j]) / f->ring_dq[j];
+    f->ring_beta[j] = beta;
+  }
+}
+
+// d_new = z + c e_I + sum_j beta_j d_j into slot `next` (may be one of the
+// read slots: every element is read before it is written by the same thread)
+__global__ __launch_bounds__(RBS) void k_fcg_dir(FArgs a, int next) {
+  const int64_t n3 = 3 * a.npix;
+  const int64_t i = (int64_t)blockIdx.x * RBS + threadIdx.x;
+  if (i >= n3) return;
+  const FScalars *f = a.fs;
+  double v = a.z[i] + ((i % 3) == 0 ? f->coarse : 0.0);
+  for (int j = 0; j < a.m; j++)
+    if ((a.valid >> j) & 1u) v += f->ring_beta[j] * a.Dr[(int64_t)j * n3 + i];
+  a.Dr[(int64_t)next * n3 + i] = v;
 }
 
 __global__ __launch_bounds__(RBS) void k_f_update(FArgs a, double sign) {
@@ -309,6 +424,10 @@ __global__ __launch_bounds__(RBS) void k_f_final(FArgs a, int what) {
       f->dq = acc[0];
       f->rd = acc[1];
       f->alpha = acc[0] != 0.0 ? acc[1] / acc[0] : 0.0;
+      if (a.Dr && a.slot >= 0 && a.slot < kFcgMax) {
+        f->ring_dq[a.slot] = acc[0];
+        f->ring_sq[a.slot] = acc[2];
+      }
       break;
     case F_UPDATE:
       f->rr_prev = f->rr;
@@ -440,6 +559,17 @@ int learn_anorm_loss(const LearnArgs &a, const double *z, const double *q, const
   return ok();
 }
 
+int learn_rloss(const LearnArgs &a, const double *q, const double *r, double *u, hipStream_t s) {
+  hipLaunchKernelGGL(k_rloss_part, dim3(RGRID), dim3(RBS), 0, s, a, q, r, u);
+  hipLaunchKernelGGL(k_rloss_final, dim3(1), dim3(RBS), 0, s, a);
+  return ok();
+}
+
+int learn_rloss_grad(const LearnArgs &a, const double *v, uint16_t *gO, hipStream_t s) {
+  if (a.npix > 0) hipLaunchKernelGGL(k_rloss_grad, dim3(nb(a.npix)), dim3(RBS), 0, s, a, v, gO);
+  return ok();
+}
+
 int launch_f2d_vec(const float *src, double *dst, int64_t n, hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(k_f2d_vec, dim3(nb(n)), dim3(RBS), 0, s, src, dst, n);
   return ok();
@@ -488,6 +618,19 @@ int f_launch_dots_z(const FArgs &a, int mode, hipStream_t s) {
 
 int f_launch_dir(const FArgs &a, int restart, hipStream_t s) {
   if (a.npix > 0) hipLaunchKernelGGL(k_f_dir, dim3(nb(a.npix)), dim3(RBS), 0, s, a, restart);
+  return ok();
+}
+
+int f_launch_fcg_dots(const FArgs &a, hipStream_t s) {
+  if (a.m < 1 || a.m > kFcgMax) return -1;
+  hipLaunchKernelGGL(k_fcg_dots, dim3(RGRID), dim3(RBS), 0, s, a);
+  hipLaunchKernelGGL(k_fcg_final, dim3(1), dim3(RBS), 0, s, a);
+  return ok();
+}
+
+int f_launch_fcg_dir(const FArgs &a, int next, hipStream_t s) {
+  if (a.m < 1 || a.m > kFcgMax || next < 0 || next >= a.m) return -1;
+  if (a.npix > 0) hipLaunchKernelGGL(k_fcg_dir, dim3(nb(3 * a.npix)), dim3(RBS), 0, s, a, next);
   return ok();
 }
 

@@ -104,6 +104,134 @@ __global__ __launch_bounds__(256) void k_col_fwd(OpArgs<T> a) {
   }
 }
 
+// ------------------------------------------------- mixed-radix column passes
+// N2 = M P (M = 3 or 5, P = 2^LOG2P): with n2 = j + M u and k2 = v + P q,
+//   Y[v + P q] = sum_j w_M^{j q} w_N2^{j v} DFT_P(x[j + M u])[v],
+// so a column is M register FFTs of length P on the decimated sub-columns
+// (sharing one LDS region in turn), then per output v a radix-M DFT of the M
+// twiddled values, all in the same thread.  w_N2^{j v} = w_N^{j v N1}.
+
+// Y_q = sum_j y_j w_M^{DIR * j q}, w_M = exp(-2 pi i / M) for DIR = -1
+template <typename V, int M, int DIR>
+__device__ __forceinline__ void dft_m(const V (&y)[M], V (&Y)[M]) {
+  using T = decltype(y[0].x);
+  static_assert(M == 3 || M == 5, "radix 3 or 5");
+  constexpr double C3 = -0.5, S3 = 0.86602540378443864676;  // cos, sin of 2 pi / 3
+  constexpr double C51 = 0.30901699437494742410, S51 = 0.95105651629515357212;   // 2 pi / 5
+  constexpr double C52 = -0.80901699437494742410, S52 = 0.58778525229247312917;  // 4 pi / 5
+  T wr[M], wi[M];
+  wr[0] = (T)1;
+  wi[0] = (T)0;
+  if constexpr (M == 3) {
+    wr[1] = (T)C3; wi[1] = (T)(DIR * S3);
+    wr[2] = (T)C3; wi[2] = (T)(-DIR * S3);
+  } else {
+    wr[1] = (T)C51; wi[1] = (T)(DIR * S51);
+    wr[2] = (T)C52; wi[2] = (T)(DIR * S52);
+    wr[3] = (T)C52; wi[3] = (T)(-DIR * S52);
+    wr[4] = (T)C51; wi[4] = (T)(-DIR * S51);
+  }
+#pragma unroll
+  for (int q = 0; q < M; q++) {
+    V acc = y[0];
+#pragma unroll
+    for (int j = 1; j < M; j++) {
+      const int e = (j * q) % M;
+      acc.x += y[j].x * wr[e] - y[j].y * wi[e];
+      acc.y += y[j].x * wi[e] + y[j].y * wr[e];
+    }
+    Y[q] = acc;
+  }
+}
+
+template <typename T, int LOG2P, int M, int SRC>
+__global__ __launch_bounds__(256) void k_col_fwd_m(OpArgs<T> a) {
+  using V = VT<T>;
+  constexpr int P = 1 << LOG2P, TS = P / 16, B = 256 / TS;
+  __shared__ V lds[LdsSize<LOG2P, B>::value];
+  const int tid = threadIdx.x, b = tid % B, t = tid / B;
+  const int64_t N1 = a.g.N1;
+  const int64_t ncb = N1 / B;
+  const int64_t pr = blockIdx.x / ncb, cb = blockIdx.x % ncb;
+  int det;
+  int64_t ga, gb;
+  pair_info(a.g, pr, det, ga, gb);
+  const int64_t n1 = cb * B + b;
+  V zz[M][16];
+#pragma unroll
+  for (int j = 0; j < M; j++) {
+    V v[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+      const int64_t s = n1 + N1 * (int64_t)(j + M * (t + TS * i));
+      const T xa = load_src<T, SRC>(a, ga + s);
+      const T xb = gb >= 0 ? load_src<T, SRC>(a, gb + s) : (T)0;
+      v[i] = mk<V>(xa, xb);
+    }
+    if (j) __syncthreads();  // the previous sub-FFT's last LDS reads
+    fft_reg<V, -1, LOG2P, B>(v, t, lds + b, a.tw4096);
+#pragma unroll
+    for (int i = 0; i < 16; i++) zz[j][i] = v[i];
+  }
+  V *out = a.buf + pr * a.g.N;
+#pragma unroll
+  for (int s = 0; s < 16; s++) {
+    const int64_t vv = t + TS * s;
+    V y[M], Y[M];
+    y[0] = zz[0][s];
+#pragma unroll
+    for (int j = 1; j < M; j++) y[j] = cmul(zz[j][s], twN<V, -1>(a.twlo, a.twhi, (int64_t)j * vv * N1));
+    dft_m<V, M, -1>(y, Y);
+#pragma unroll
+    for (int q = 0; q < M; q++) {
+      const int64_t k2 = vv + (int64_t)P * q;
+      out[k2 * N1 + n1] = cmul(Y[q], twN<V, -1>(a.twlo, a.twhi, n1 * k2));
+    }
+  }
+}
+
+template <typename T, int LOG2P, int M>
+__global__ __launch_bounds__(256) void k_col_inv_m(OpArgs<T> a) {
+  using V = VT<T>;
+  constexpr int P = 1 << LOG2P, TS = P / 16, B = 256 / TS;
+  __shared__ V lds[LdsSize<LOG2P, B>::value];
+  const int tid = threadIdx.x, b = tid % B, t = tid / B;
+  const int64_t N1 = a.g.N1;
+  const int64_t ncb = N1 / B;
+  const int64_t pr = blockIdx.x / ncb, cb = blockIdx.x % ncb;
+  int det;
+  int64_t ga, gb;
+  pair_info(a.g, pr, det, ga, gb);
+  const int64_t n1 = cb * B + b;
+  const V *in = a.buf + pr * a.g.N;
+  V zz[M][16];
+#pragma unroll
+  for (int s = 0; s < 16; s++) {
+    const int64_t vv = t + TS * s;
+    V y[M], Z[M];
+#pragma unroll
+    for (int q = 0; q < M; q++) y[q] = in[(vv + (int64_t)P * q) * N1 + n1];
+    dft_m<V, M, +1>(y, Z);
+    zz[0][s] = Z[0];
+#pragma unroll
+    for (int j = 1; j < M; j++) zz[j][s] = cmul(Z[j], twN<V, +1>(a.twlo, a.twhi, (int64_t)j * vv * N1));
+  }
+#pragma unroll
+  for (int j = 0; j < M; j++) {
+    V v[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) v[i] = zz[j][i];
+    if (j) __syncthreads();
+    fft_reg<V, +1, LOG2P, B>(v, t, lds + b, a.tw4096);
+#pragma unroll
+    for (int s = 0; s < 16; s++) {
+      const int64_t idx = n1 + N1 * (int64_t)(j + M * (t + TS * s));
+      a.z[ga + idx] = v[s].x;
+      if (gb >= 0) a.z[gb + idx] = v[s].y;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------- row
 template <typename T, int LOG2N1>
 __global__ __launch_bounds__(256) void k_row(OpArgs<T> a) {
@@ -285,8 +413,41 @@ int launch1(K kern, int64_t nblocks, hipStream_t s, const A &a) {
 
 }  // namespace
 
+template <typename T, int M>
+int launch_col_fwd_m(const OpArgs<T> &a, int src, hipStream_t s) {
+  const int64_t B = 4096 >> a.g.log2N2;  // columns per block
+  const int64_t nb = a.g.npairs * (a.g.N1 / B);
+#define CFM(L)                                                                          \
+  case L:                                                                               \
+    if (src == SRC_GATHER) return launch1(k_col_fwd_m<T, L, M, SRC_GATHER>, nb, s, a);  \
+    if (src == SRC_TOD) return launch1(k_col_fwd_m<T, L, M, SRC_TOD>, nb, s, a);        \
+    if constexpr (sizeof(T) == 8) return launch1(k_col_fwd_m<T, L, M, SRC_RNG>, nb, s, a); \
+    return -2;
+  switch (a.g.log2N2) {
+    CFM(4) CFM(5) CFM(6) CFM(7) CFM(8) CFM(9) CFM(10)
+    default: return -2;
+  }
+#undef CFM
+}
+
+template <typename T, int M>
+int launch_col_inv_m(const OpArgs<T> &a, hipStream_t s) {
+  const int64_t B = 4096 >> a.g.log2N2;
+  const int64_t nb = a.g.npairs * (a.g.N1 / B);
+#define CIM(L) \
+  case L: return launch1(k_col_inv_m<T, L, M>, nb, s, a);
+  switch (a.g.log2N2) {
+    CIM(4) CIM(5) CIM(6) CIM(7) CIM(8) CIM(9) CIM(10)
+    default: return -2;
+  }
+#undef CIM
+}
+
 template <typename T>
 int launch_col_fwd(const OpArgs<T> &a, int src, hipStream_t s) {
+  if (a.g.M == 3) return launch_col_fwd_m<T, 3>(a, src, s);
+  if (a.g.M == 5) return launch_col_fwd_m<T, 5>(a, src, s);
+  if (a.g.M != 1) return -2;
   const int64_t nb = a.g.npairs * (a.g.N / 4096);
 #define CF(L)                                                                      \
   case L:                                                                          \
@@ -316,6 +477,9 @@ int launch_row(const OpArgs<T> &a, hipStream_t s, int64_t *nblocks) {
 
 template <typename T>
 int launch_col_inv(const OpArgs<T> &a, hipStream_t s) {
+  if (a.g.M == 3) return launch_col_inv_m<T, 3>(a, s);
+  if (a.g.M == 5) return launch_col_inv_m<T, 5>(a, s);
+  if (a.g.M != 1) return -2;
   const int64_t nb = a.g.npairs * (a.g.N / 4096);
 #define CI(L) \
   case L: return launch1(k_col_inv<T, L>, nb, s, a);

@@ -321,7 +321,39 @@ int main(int argc, char *argv[]) {
   }
   sanepic_solve_opts so{};
   so.precond = SANEPIC_PRECOND_DIAG;
-  if (const char *p = getenv("SANEPIC_PRECOND")) so.precond = strcmp(p, "block") == 0 ? SANEPIC_PRECOND_BLOCK3 : SANEPIC_PRECOND_DIAG;
+  sanepic_unet *unet = nullptr;
+  if (const char *p = getenv("SANEPIC_PRECOND")) {
+    if (strcmp(p, "block") == 0) so.precond = SANEPIC_PRECOND_BLOCK3;
+    if (strcmp(p, "learned") == 0) {
+      // trained parameters (float32, the library's layout), e.g. written by
+      // python -m sanepic_amd.train_precond for this scan
+      const char *w = getenv("SANEPIC_UNET_WEIGHTS");
+      if (!w) {
+        fprintf(stderr, "SANEPIC_PRECOND=learned needs SANEPIC_UNET_WEIGHTS=<parameter file>\n");
+        exit(1);
+      }
+      rc = sanepic_unet_create(nside, cfg.device, &unet);
+      if (rc) {
+        fprintf(stderr, "sanepic_unet_create: %s\n", sanepic_unet_last_error(nullptr));
+        exit(1);
+      }
+      int64_t np = 0;
+      sanepic_unet_nparams(unet, &np);
+      std::vector<float> par(np);
+      FILE *fw = fopen(w, "rb");
+      if (!fw || fread(par.data(), sizeof(float), np, fw) != (size_t)np) {
+        fprintf(stderr, "cannot read %ld parameters from %s\n", (long)np, w);
+        exit(1);
+      }
+      fclose(fw);
+      if (sanepic_unet_set_params(unet, par.data()) || sanepic_learn_attach(ctx, unet)) {
+        fprintf(stderr, "learned preconditioner: %s\n", sanepic_last_error(ctx));
+        exit(1);
+      }
+      so.precond = SANEPIC_PRECOND_LEARNED;
+      so.deflate = 1;
+    }
+  }
   so.refresh = 10;
   rc = sanepic_solve_begin(ctx, &so, initMapOn ? x0.data() : nullptr);
   if (rc == SANEPIC_EDATA) {
@@ -382,5 +414,6 @@ int main(int argc, char *argv[]) {
           iter, tsetup, tio, tloop, 1e3 * tloop / std::max(iter, 1), (long)ndef,
           [&] { int64_t b = 0; sanepic_device_bytes(ctx, &b); return b / 1e9; }());
   sanepic_destroy(ctx);
+  sanepic_unet_destroy(unet);
   return 0;
 }

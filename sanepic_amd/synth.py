@@ -137,6 +137,22 @@ def noise_segment(ns, fsamp, sigma, fknee, alpha, seed):
     return np.fft.irfft(np.fft.rfft(w) * amp, ns)
 
 
+# Named workloads: nside, detectors, samples per detector, segment length.
+WORKLOADS = {
+    # 16 detectors at k pi/8, NSIDE 512, 2^26 samples each in segments of 2^20
+    "c1": dict(nside=512, ndet=16, nsamp=2 ** 26, ns=2 ** 20),
+    # 1 detector, NSIDE 64, 2^22 samples in 8 segments of 2^19
+    "c0": dict(nside=64, ndet=1, nsamp=2 ** 22, ns=2 ** 19),
+    "small": dict(nside=64, ndet=4, nsamp=2 ** 21, ns=2 ** 18),
+}
+
+
+def workload_config(name, **kw):
+    """Config of a named workload (same scan, sky and noise recipe)."""
+    w = WORKLOADS[name]
+    return Config(nside=w["nside"], ndet=w["ndet"], nseg=w["nsamp"] // w["ns"], ns=w["ns"], **kw)
+
+
 class Config:
     """One synthetic run.  Defaults reproduce the 1-detector NSIDE-64 case."""
 

@@ -173,6 +173,9 @@ def test_loss_matches_numpy(problem):
     ref = (z @ Az - 2 * z @ r + e @ r) / (e @ r)
     got = ctx.eval_pair(1, sanepic_amd.LOSS_ANORM)
     assert abs(got - ref) <= 1e-6 * abs(ref)
+    ref_r = np.sum((Az - r) ** 2) / np.sum(r ** 2)
+    got_r = ctx.eval_pair(1, sanepic_amd.LOSS_RESIDUAL)
+    assert abs(got_r - ref_r) <= 1e-6 * abs(ref_r)
 
 
 def test_training_gain_over_block_jacobi(problem):
@@ -186,10 +189,11 @@ def test_training_gain_over_block_jacobi(problem):
         ctx.harvest(sd, capacity=3000)
     n = ctx.npairs()
     ntrain = int(0.9 * n)
-    base = np.mean([ctx.eval_pair(k) for k in range(ntrain, n)])
-    loss = ctx.train(10000, lr=1e-3, lr_final=1e-5, seed=0, first=0, count=ntrain)
-    held = np.mean([ctx.eval_pair(k) for k in range(ntrain, n)])
-    print(f"  {n} pairs; held-out A-norm error of one application: block-Jacobi {base:.3f}, "
+    R = sanepic_amd.LOSS_RESIDUAL
+    base = np.mean([ctx.eval_pair(k, R) for k in range(ntrain, n)])
+    loss = ctx.train(6000, lr=1e-3, lr_final=1e-5, seed=0, first=0, count=ntrain, loss=R)
+    held = np.mean([ctx.eval_pair(k, R) for k in range(ntrain, n)])
+    print(f"  {n} pairs; held-out |A z - r|^2/|r|^2 of one application: block-Jacobi {base:.3f}, "
           f"trained {held:.3f}; training loss {loss[:100].mean():.3f} -> {loss[-100:].mean():.3f}",
           flush=True)
     assert held < 0.8 * base

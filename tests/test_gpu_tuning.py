@@ -1,0 +1,102 @@
+"""Launch-geometry choices must not change results: every four-step split
+gives the oracle's operator and right-hand side, the autotuner's cache is
+honoured, and a HIP-graph replay of the PCG iteration reproduces the eager
+launches bit for bit."""
+import os
+
+import numpy as np
+import pytest
+
+import sanepic_amd
+from tests.helpers import make_case, oracle_problem, rel
+
+pytestmark = pytest.mark.gpu
+
+CASE = dict(nside=32, ndet=2, nseg=3, ns=2 ** 16)
+
+
+@pytest.fixture(scope="module")
+def data():
+    cfg, ds = make_case(**CASE)
+    P = oracle_problem(cfg, ds)
+    return cfg, ds, P
+
+
+def _ctx(cfg, ds, split=None, precision=sanepic_amd.FP64):
+    ctx = sanepic_amd.Context(cfg.nside, cfg.ndet, cfg.nseg, cfg.ns, precision=precision)
+    ctx.load(cfg, ds)
+    ctx.pixelize()
+    if split is not None:
+        ctx.set_fft_split(split)
+    ctx.setup()
+    return ctx
+
+
+@pytest.mark.parametrize("split", [9, 10, 11, 12])
+def test_every_split_matches_oracle(data, split):
+    cfg, ds, P = data
+    print("case", CASE, "log2 N1", split, flush=True)
+    ctx = _ctx(cfg, ds, split)
+    x = np.random.default_rng(7).standard_normal(3 * ctx.npix)
+    assert rel(ctx.apply(x), P.apply_A(x)) < 1e-11
+    assert rel(ctx.rhs(), P.rhs()) < 1e-11
+    c_or = P.chi2(x)
+    assert abs(ctx.chi2(x) - c_or) / c_or < 1e-10
+    ctx.close()
+
+
+def test_split_out_of_range_is_rejected(data):
+    cfg, ds, _ = data
+    ctx = sanepic_amd.Context(cfg.nside, cfg.ndet, cfg.nseg, cfg.ns)
+    ctx.load(cfg, ds)
+    ctx.pixelize()
+    with pytest.raises(sanepic_amd.SanepicError):
+        ctx.set_fft_split(13)
+    with pytest.raises(sanepic_amd.SanepicError):
+        ctx.set_fft_split(8)
+    ctx.close()
+
+
+def test_autotune_and_cache(data, tmp_path):
+    cfg, ds, P = data
+    cache = str(tmp_path / "tune.txt")
+    ctx = sanepic_amd.Context(cfg.nside, cfg.ndet, cfg.nseg, cfg.ns)
+    ctx.load(cfg, ds)
+    ctx.pixelize()
+    l1, ms = ctx.autotune(cache)
+    print(f"  autotune: log2 N1 = {l1}, {ms:.3f} ms per operator", flush=True)
+    assert 9 <= l1 <= 12 and ms > 0
+    ctx.setup()
+    x = np.random.default_rng(8).standard_normal(3 * ctx.npix)
+    assert rel(ctx.apply(x), P.apply_A(x)) < 1e-11
+    ctx.close()
+    text = open(cache).read()
+    assert f"log2_n1={l1}" in text and "ns=65536" in text
+    # a second run takes the cached choice
+    ctx = sanepic_amd.Context(cfg.nside, cfg.ndet, cfg.nseg, cfg.ns)
+    ctx.load(cfg, ds)
+    ctx.pixelize()
+    assert ctx.autotune(cache)[0] == l1
+    ctx.close()
+    assert open(cache).read() == text
+
+
+@pytest.mark.parametrize("precision", [sanepic_amd.FP64, sanepic_amd.MIXED])
+def test_graph_replay_matches_eager(data, precision):
+    cfg, ds, _ = data
+    print("graph replay, precision", precision, flush=True)
+    ctx = _ctx(cfg, ds, precision=precision)
+    y = np.random.default_rng(9).standard_normal(3 * ctx.npix)
+    runs = []
+    for graph in (False, True):
+        ctx.solve_begin(sanepic_amd.PRECOND_BLOCK3, 10, None, True, graph=graph)
+        h = []
+        for k in range(25):  # 3 refresh iterations, many replays of each graph
+            h.append(ctx.step())
+            if graph and k % 4 == 1:
+                ctx.apply(y)  # eager work on the same stream between replays
+        runs.append((np.array([v["crit"] for v in h]), np.array([v["alpha"] for v in h]), ctx.map()))
+    (c0, a0, x0), (c1, a1, x1) = runs
+    assert np.array_equal(c0, c1) and np.array_equal(a0, a1)
+    assert np.array_equal(x0, x1)
+    ctx.close()
