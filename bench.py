@@ -77,8 +77,9 @@ def cpu_baseline(wl, budget_s=20.0):
     import oracle
     from sanepic_amd import synth
     ncores = os.cpu_count() or 1
-    nseg = 4 if wl["ns"] >= 2 ** 20 else 8
-    cfg = synth.Config(nside=wl["nside"], ndet=1, nseg=nseg, ns=wl["ns"])
+    ns = wl.get("ns", 2 ** 20)
+    nseg = 4 if ns >= 2 ** 20 else 8
+    cfg = synth.Config(nside=wl["nside"], ndet=1, nseg=nseg, ns=ns)
     ds = synth.make_dataset(cfg, workers=min(8, ncores))
     P = oracle.Problem(cfg.nside, 1, nseg, cfg.ns, nthreads=ncores)
     P.set_detector(0, 0.0, 0.0, 0.0, ds["ispf"][0])
@@ -113,8 +114,12 @@ def run_solve(cfg, pin, precision, precond, deflate, weights, steps, warmup, max
         un = sanepic_amd.UNet(cfg.nside)
         un.set_params(weights)  # loading the trained network is not part of the solve
     t0 = time.perf_counter()
-    ctx = sanepic_amd.Context(cfg.nside, cfg.ndet, cfg.nseg, cfg.ns, precision=prec)
-    ctx.load_async(cfg, pin)
+    if hasattr(cfg, "seglen"):  # per-detector segment lengths
+        ctx = sanepic_amd.Context.uneven(cfg.nside, cfg.seglen, cfg.nsamp, precision=prec)
+        ctx.load_uneven(cfg, pin, queued=True)
+    else:
+        ctx = sanepic_amd.Context(cfg.nside, cfg.ndet, cfg.nseg, cfg.ns, precision=prec)
+        ctx.load_async(cfg, pin)
     t_up = time.perf_counter()
     npix, ndef = ctx.pixelize()
     t_pix = time.perf_counter()
@@ -199,15 +204,17 @@ def main():
 
     if args.calib_copy:
         calibration_copy()
-    wl = synth.WORKLOADS[args.workload]
+    uneven = args.workload in synth.UNEVEN_WORKLOADS
+    wl = synth.UNEVEN_WORKLOADS[args.workload] if uneven else synth.WORKLOADS[args.workload]
     cfg = synth.workload_config(args.workload)
     tg = time.perf_counter()
-    ds = synth.make_dataset(cfg, workers=min(16, os.cpu_count() or 1))
+    workers = min(16, os.cpu_count() or 1)
+    ds = synth.make_uneven_dataset(cfg, workers) if uneven else synth.make_dataset(cfg, workers)
     pin = sanepic_amd.PinnedInputs(cfg, ds)
     del ds
+    ntot = sum(cfg.used(d) for d in range(cfg.ndet)) if uneven else cfg.ndet * cfg.nseg * cfg.ns
     log(f"inputs generated in {time.perf_counter() - tg:.1f} s "
-        f"({cfg.ndet} det x {cfg.nseg * cfg.ns} samples, NSIDE {cfg.nside}), in pinned host memory")
-    ntot = cfg.ndet * cfg.nseg * cfg.ns
+        f"({cfg.ndet} det, {ntot} samples, NSIDE {cfg.nside}), in pinned host memory")
 
     precond, weights, record = args.precond, None, None
     if precond == "learned":
@@ -240,7 +247,8 @@ def main():
             log(f"  {CLASS_NAMES[k]}: {kt[k][0] / kt[k][1]:.3f} ms x {kt[k][1]}")
     apply_ms = kt[4][0] / max(kt[4][1], 1)
     s = 8 if args.precision == "fp64" else 4
-    b_alg = ntot * bps + cfg.ndet * (cfg.ns // 2 + 1) * s + 12 * 3 * head["npix"] * 8
+    lens = cfg.seglen if uneven else [cfg.ns] * cfg.ndet
+    b_alg = ntot * bps + sum(L // 2 + 1 for L in lens) * s + 12 * 3 * head["npix"] * 8
     tts = head["time_to_solution_s"]
     out = {
         "metric": "time-to-1e-6 residual (s) + TOD samples/s per CG iter, NSIDE512 16det, 1 GPU",
@@ -255,11 +263,13 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f32 operator, f64 CG scalars",
         "data": "synthetic (spin/precession scan, Gaussian I/Q/U sky, white + 1/f noise)",
-        "config": {"workload": f"{cfg.ndet} det x {cfg.nseg * cfg.ns} samples, NSIDE {cfg.nside}, "
-                               f"segments of {cfg.ns}, {precond} PCG"
-                               f"{' + monopole deflation' if deflate else ''}",
-                   "nside": cfg.nside, "ndet": cfg.ndet, "samples_per_det": cfg.nseg * cfg.ns,
-                   "segment": cfg.ns, "npix": head["npix"], "precond": precond,
+        "config": {"workload": (f"{cfg.ndet} det, {ntot} samples, NSIDE {cfg.nside}, per-detector "
+                                f"segment lengths m 2^l (m in 1,3,5), {precond} PCG" if uneven else
+                                f"{cfg.ndet} det x {cfg.nseg * cfg.ns} samples, NSIDE {cfg.nside}, "
+                                f"segments of {cfg.ns}, {precond} PCG")
+                               + (" + monopole deflation" if deflate else ""),
+                   "nside": cfg.nside, "ndet": cfg.ndet, "samples": ntot,
+                   "segment": sorted(set(lens)) if uneven else cfg.ns, "npix": head["npix"], "precond": precond,
                    "deflate": deflate, "precision": args.precision},
         "time_to_1e-6_s": tts,
         "iterations_to_1e-6": head["iterations"] if head["converged"] else None,

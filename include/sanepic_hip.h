@@ -86,6 +86,38 @@ int sanepic_set_pointing(sanepic_ctx *ctx, int64_t iseg, const double *ra,
 int sanepic_set_tod(sanepic_ctx *ctx, int64_t idet, int64_t iseg,
                     const double *tod);
 
+/* Detectors with different segment lengths.  No counterpart in the
+ * reference, which requires one length for every detector (sanepic.cc:
+ * 668-672).  Detector d splits the common boresight timeline of nsamp
+ * samples into floor(nsamp / seglen[d]) segments of seglen[d] samples
+ * (2^l with 16 <= 2^l <= 2^24, or 3 or 5 times 2^l with 2^13 <= 2^l <=
+ * 2^22); samples past the last whole segment are not used.  Detectors of
+ * equal length are processed together.  Pointing and TOD are uploaded with
+ * the range calls below; every other entry point works as for
+ * sanepic_create contexts (sanepic_get_pixels takes the detector's own
+ * segment index). */
+typedef struct sanepic_uneven_config {
+  int64_t nside;
+  int64_t ndet;
+  int64_t nsamp;          /* boresight timeline samples */
+  const int64_t *seglen;  /* [ndet] segment length of each detector */
+  int32_t polar, hwp;
+  double omega_hwp;
+  int32_t precision, device;
+} sanepic_uneven_config;
+int sanepic_create_uneven(const sanepic_uneven_config *cfg, sanepic_ctx **out);
+/* Segment length and count of detector idet (any context). */
+int sanepic_get_seglen(sanepic_ctx *ctx, int64_t idet, int64_t *seglen,
+                       int64_t *nseg);
+/* Boresight ra, dec, psi for timeline samples [t0, t0 + n); TOD of detector
+ * idet for its samples [t0, t0 + n) (within its whole segments).  queued: the
+ * copy runs on an upload stream as with sanepic_set_*_async. */
+int sanepic_set_pointing_range(sanepic_ctx *ctx, int64_t t0, int64_t n,
+                               const double *ra, const double *dec,
+                               const double *psi, int32_t queued);
+int sanepic_set_tod_range(sanepic_ctx *ctx, int64_t idet, int64_t t0,
+                          int64_t n, const double *tod, int32_t queued);
+
 /* Pinned (page-locked) host memory for input buffers: copies from it run at
  * the full host-link rate.  No counterpart in the reference, which reads its
  * inputs with fread into ordinary buffers (sanepic.cc:680-692, 776-790). */

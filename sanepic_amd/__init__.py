@@ -37,7 +37,8 @@ ABI_FUNCTIONS = [
     "sanepic_learn_features", "sanepic_learn_harvest", "sanepic_learn_npairs",
     "sanepic_learn_clear", "sanepic_learn_get_pair", "sanepic_learn_train", "sanepic_learn_eval",
     "sanepic_learn_stats", "sanepic_host_alloc", "sanepic_host_free", "sanepic_set_pointing_async",
-    "sanepic_set_tod_async", "sanepic_autotune", "sanepic_set_fft_split",
+    "sanepic_set_tod_async", "sanepic_autotune", "sanepic_set_fft_split", "sanepic_create_uneven",
+    "sanepic_get_seglen", "sanepic_set_pointing_range", "sanepic_set_tod_range",
 ]
 
 
@@ -46,6 +47,13 @@ class Config(ctypes.Structure):
                 ("ns", ctypes.c_int64), ("polar", ctypes.c_int32), ("hwp", ctypes.c_int32),
                 ("omega_hwp", ctypes.c_double), ("precision", ctypes.c_int32),
                 ("device", ctypes.c_int32)]
+
+
+class UnevenConfig(ctypes.Structure):
+    _fields_ = [("nside", ctypes.c_int64), ("ndet", ctypes.c_int64), ("nsamp", ctypes.c_int64),
+                ("seglen", ctypes.POINTER(ctypes.c_int64)), ("polar", ctypes.c_int32),
+                ("hwp", ctypes.c_int32), ("omega_hwp", ctypes.c_double),
+                ("precision", ctypes.c_int32), ("device", ctypes.c_int32)]
 
 
 class SolveOpts(ctypes.Structure):
@@ -147,6 +155,10 @@ def lib():
             "sanepic_set_tod_async": [P, i64, i64, d_p],
             "sanepic_autotune": [P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int32), d_p],
             "sanepic_set_fft_split": [P, ctypes.c_int32],
+            "sanepic_create_uneven": [ctypes.POINTER(UnevenConfig), ctypes.POINTER(P)],
+            "sanepic_get_seglen": [P, i64, i64p, i64p],
+            "sanepic_set_pointing_range": [P, i64, i64, d_p, d_p, d_p, ctypes.c_int32],
+            "sanepic_set_tod_range": [P, i64, i64, i64, d_p, ctypes.c_int32],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -197,16 +209,23 @@ class PinnedInputs:
     alive until Context.setup() has returned."""
 
     def __init__(self, cfg, ds):
-        nsamp = cfg.nseg * cfg.ns
         self._bufs = []
+        nsamp = cfg.nsamp
         self.ra = self._alloc(nsamp)
         self.dec = self._alloc(nsamp)
         self.psi = self._alloc(nsamp)
-        self.tod = self._alloc(cfg.ndet * nsamp).reshape(cfg.ndet, nsamp)
         self.ra[:] = ds["ra"]
         self.dec[:] = ds["dec"]
         self.psi[:] = ds["psi"]
-        self.tod[:] = ds["tod"]
+        if hasattr(cfg, "seglen"):  # per-detector lengths: one flat buffer, a view per detector
+            off = np.concatenate([[0], np.cumsum([len(t) for t in ds["tod"]])])
+            flat = self._alloc(int(off[-1]))
+            self.tod = [flat[off[d]:off[d + 1]] for d in range(cfg.ndet)]
+            for d in range(cfg.ndet):
+                self.tod[d][:] = ds["tod"][d]
+        else:
+            self.tod = self._alloc(cfg.ndet * nsamp).reshape(cfg.ndet, nsamp)
+            self.tod[:] = ds["tod"]
         self.ispf = ds["ispf"]
 
     def _alloc(self, n):
@@ -304,6 +323,50 @@ class Context:
         self.h = h
         self.npix = None
 
+    @classmethod
+    def uneven(cls, nside, seglen, nsamp, hwp=0, omega=0.0, precision=FP64, device=0):
+        """A context whose detectors have their own segment lengths (seglen[d])
+        on one boresight timeline of nsamp samples."""
+        obj = cls.__new__(cls)
+        obj.h = None
+        sl = np.ascontiguousarray(seglen, dtype=np.int64)
+        cfg = UnevenConfig(nside, sl.size, nsamp, _ip(sl), 1, hwp, omega, precision, device)
+        h = ctypes.c_void_p()
+        rc = lib().sanepic_create_uneven(ctypes.byref(cfg), ctypes.byref(h))
+        if rc:
+            raise SanepicError(f"sanepic_create_uneven: {lib().sanepic_last_error(None).decode()}")
+        obj.h = h
+        obj.nside, obj.ndet, obj.nseg, obj.ns, obj.npix = nside, sl.size, None, None, None
+        obj.nsamp = nsamp
+        return obj
+
+    def seglen(self, idet):
+        """(segment length, number of segments) of detector idet."""
+        L, n = ctypes.c_int64(), ctypes.c_int64()
+        self._chk(lib().sanepic_get_seglen(self.h, idet, ctypes.byref(L), ctypes.byref(n)), "get_seglen")
+        return L.value, n.value
+
+    def set_pointing_range(self, t0, ra, dec, psi, queued=False):
+        ra, dec, psi = _f64(ra), _f64(dec), _f64(psi)
+        self._keep = getattr(self, "_keep", []) + ([ra, dec, psi] if queued else [])
+        self._chk(lib().sanepic_set_pointing_range(self.h, t0, ra.size, _dp(ra), _dp(dec), _dp(psi),
+                                                   1 if queued else 0), "set_pointing_range")
+
+    def set_tod_range(self, idet, t0, tod, queued=False):
+        tod = _f64(tod)
+        self._keep = getattr(self, "_keep", []) + ([tod] if queued else [])
+        self._chk(lib().sanepic_set_tod_range(self.h, idet, t0, tod.size, _dp(tod), 1 if queued else 0),
+                  "set_tod_range")
+
+    def load_uneven(self, cfg, ds, queued=False):
+        """Feeds a synth.make_uneven_dataset (or PinnedInputs of one) dataset."""
+        for k in range(self.ndet):
+            lel, xel = cfg.offsets_deg[k]
+            self.set_detector(k, lel, xel, cfg.det_angles_deg[k], ds["ispf"][k])
+        self.set_pointing_range(0, ds["ra"], ds["dec"], ds["psi"], queued)
+        for k in range(self.ndet):
+            self.set_tod_range(k, 0, ds["tod"][k], queued)
+
     def close(self):
         if self.h:
             lib().sanepic_destroy(self.h)
@@ -335,7 +398,7 @@ class Context:
 
     def set_tod(self, idet, iseg, tod):
         tod = _f64(tod)
-        assert tod.size == self.ns
+        assert self.ns is None or tod.size == self.ns
         self._chk(lib().sanepic_set_tod(self.h, idet, iseg, _dp(tod)), "set_tod")
 
     def load(self, cfg, ds):
@@ -374,11 +437,15 @@ class Context:
         return npix.value, ndef.value
 
     def pixels(self, idet, iseg):
-        out = np.empty(self.ns, dtype=np.int64)
+        n = self.ns if self.ns else self.seglen(idet)[0]
+        out = np.empty(n, dtype=np.int64)
         self._chk(lib().sanepic_get_pixels(self.h, idet, iseg, _ip(out)), "get_pixels")
         return out
 
     def all_pixels(self):
+        if self.nseg is None:  # per-detector lengths
+            return np.concatenate([self.pixels(d, s) for d in range(self.ndet)
+                                   for s in range(self.seglen(d)[1])])
         return np.concatenate([self.pixels(d, s) for d in range(self.ndet) for s in range(self.nseg)])
 
     def indpix(self):

@@ -250,6 +250,7 @@ int run_scatter(sanepic_ctx *c, const T *z, double *q, int mask) {
 
 // q = P^T N^-1 P x (x, q interleaved, device).  chi2 -> sc->chi2 when asked.
 int apply_A(sanepic_ctx *c, const double *x, double *q, int mask, bool chi2, bool fp64) {
+  if (!c->sub.empty()) return group_apply_A(c, x, q, mask, chi2, fp64);
   ProfScope whole(c, 4);
   const int mode = ROW_FWD | ROW_FILTER | ROW_INV | (chi2 ? ROW_CHI2 : 0);
   int64_t nb = 0;
@@ -396,6 +397,10 @@ double pixel_eps() {
 
 }  // namespace
 
+namespace sctx {
+void set_create_error(const std::string &m) { g_create_error = m; }
+}  // namespace sctx
+
 // =====================================================================  ABI
 
 extern "C" {
@@ -519,6 +524,8 @@ int sanepic_create(const sanepic_config *cfg, sanepic_ctx **out) {
 void sanepic_destroy(sanepic_ctx *c) {
   if (!c) return;
   if (c->st) (void)hipStreamSynchronize(c->st);
+  for (sanepic_ctx *s : c->sub) sanepic_destroy(s);  // groups borrow this stream
+  c->sub.clear();
   if (c->un) {  // the network outlives the context: give it back its stream
     c->un->net.st = c->un->own;
     c->un->attached = nullptr;
@@ -532,7 +539,7 @@ void sanepic_destroy(sanepic_ctx *c) {
   if (c->up_pt) (void)hipStreamSynchronize(c->up_pt);
   if (c->up_tod) (void)hipStreamSynchronize(c->up_tod);
   for (void *p : c->allocs) (void)hipFree(p);
-  if (c->st) (void)hipStreamDestroy(c->st);
+  if (c->st && !c->borrowed) (void)hipStreamDestroy(c->st);
   if (c->up_pt) (void)hipStreamDestroy(c->up_pt);
   if (c->up_tod) (void)hipStreamDestroy(c->up_tod);
   delete c;
@@ -543,6 +550,13 @@ int sanepic_set_detector(sanepic_ctx *c, int64_t idet, double lel_deg, double xe
   if (!c) return SANEPIC_EINVAL;
   if (idet < 0 || idet >= c->ndet) return fail(c, SANEPIC_EINVAL, "detector index out of range");
   if (!ispf || nssp < 2) return fail(c, SANEPIC_EINVAL, "iSpf needs at least 2 values");
+  if (!c->sub.empty()) {  // the detector's group computes its constants and SpN row
+    sanepic_ctx *s = c->sub[(size_t)c->det_group[(size_t)idet]];
+    const int rc = sanepic_set_detector(s, c->det_local[(size_t)idet], lel_deg, xel_deg, angle_deg, ispf, nssp);
+    if (rc) return fail(c, rc, s->err);
+    c->det_ok[(size_t)idet] = 1;
+    return SANEPIC_OK;
+  }
   const int64_t ns = c->ns;
   if ((int64_t)(double(ns - 1) / nssp * ns) >= nssp)
     return fail(c, SANEPIC_EINVAL, "iSpf too short for ns: the reference would read past it (533-535)");
@@ -560,6 +574,7 @@ int sanepic_set_detector(sanepic_ctx *c, int64_t idet, double lel_deg, double xe
 int sanepic_set_pointing(sanepic_ctx *c, int64_t iseg, const double *ra, const double *dec,
                          const double *psi) {
   if (!c) return SANEPIC_EINVAL;
+  if (!c->sub.empty()) return fail(c, SANEPIC_EINVAL, "use the range uploads for per-detector lengths");
   if (iseg < 0 || iseg >= c->nseg) return fail(c, SANEPIC_EINVAL, "segment index out of range");
   if (!c->ra) return fail(c, SANEPIC_EINVAL, "pointing already consumed by sanepic_pixelize");
   const size_t nb = (size_t)c->ns * sizeof(double);
@@ -594,6 +609,7 @@ static int upload_stream(sanepic_ctx *c, hipStream_t *s) {
 int sanepic_set_pointing_async(sanepic_ctx *c, int64_t iseg, const double *ra, const double *dec,
                                const double *psi) {
   if (!c) return SANEPIC_EINVAL;
+  if (!c->sub.empty()) return fail(c, SANEPIC_EINVAL, "use the range uploads for per-detector lengths");
   if (iseg < 0 || iseg >= c->nseg) return fail(c, SANEPIC_EINVAL, "segment index out of range");
   if (!c->ra) return fail(c, SANEPIC_EINVAL, "pointing already consumed by sanepic_pixelize");
   if (!ra || !dec || !psi) return fail(c, SANEPIC_EINVAL, "null pointing buffer");
@@ -608,6 +624,7 @@ int sanepic_set_pointing_async(sanepic_ctx *c, int64_t iseg, const double *ra, c
 
 int sanepic_set_tod_async(sanepic_ctx *c, int64_t idet, int64_t iseg, const double *tod) {
   if (!c) return SANEPIC_EINVAL;
+  if (!c->sub.empty()) return fail(c, SANEPIC_EINVAL, "use the range uploads for per-detector lengths");
   if (idet < 0 || idet >= c->ndet || iseg < 0 || iseg >= c->nseg)
     return fail(c, SANEPIC_EINVAL, "TOD index out of range");
   if (c->setup_done) return fail(c, SANEPIC_EINVAL, "TOD already consumed by sanepic_setup");
@@ -622,6 +639,7 @@ int sanepic_set_tod_async(sanepic_ctx *c, int64_t idet, int64_t iseg, const doub
 
 int sanepic_set_tod(sanepic_ctx *c, int64_t idet, int64_t iseg, const double *tod) {
   if (!c) return SANEPIC_EINVAL;
+  if (!c->sub.empty()) return fail(c, SANEPIC_EINVAL, "use the range uploads for per-detector lengths");
   if (idet < 0 || idet >= c->ndet || iseg < 0 || iseg >= c->nseg)
     return fail(c, SANEPIC_EINVAL, "TOD index out of range");
   if (c->setup_done) return fail(c, SANEPIC_EINVAL, "TOD already consumed by sanepic_setup");
@@ -634,6 +652,12 @@ int sanepic_set_tod(sanepic_ctx *c, int64_t idet, int64_t iseg, const double *to
 
 int sanepic_pixelize(sanepic_ctx *c, int64_t *npix_out, int64_t *n_deferred) {
   if (!c) return SANEPIC_EINVAL;
+  if (c->pixelized) return fail(c, SANEPIC_EINVAL, "sanepic_pixelize already ran");
+  if (!c->sub.empty()) {
+    for (int64_t i = 0; i < c->ndet; i++)
+      if (!c->det_ok[(size_t)i]) return fail(c, SANEPIC_EINVAL, "detector " + std::to_string(i) + " not set");
+    return group_pixelize(c, npix_out, n_deferred);
+  }
   if (c->up_pt) HIPCHK(c, hipStreamSynchronize(c->up_pt));
   if (c->pixelized) return fail(c, SANEPIC_EINVAL, "sanepic_pixelize already ran");
   for (int64_t i = 0; i < c->ndet; i++)
@@ -844,6 +868,12 @@ int sanepic_pixelize(sanepic_ctx *c, int64_t *npix_out, int64_t *n_deferred) {
 
 int sanepic_get_pixels(sanepic_ctx *c, int64_t idet, int64_t iseg, int64_t *out) {
   ACHK(c, check_ready(c, false));
+  if (!c->sub.empty()) {
+    if (idet < 0 || idet >= c->ndet) return fail(c, SANEPIC_EINVAL, "index out of range");
+    sanepic_ctx *s = c->sub[(size_t)c->det_group[(size_t)idet]];
+    const int rc = sanepic_get_pixels(s, c->det_local[(size_t)idet], iseg, out);
+    return rc ? fail(c, rc, s->err) : SANEPIC_OK;
+  }
   if (idet < 0 || idet >= c->ndet || iseg < 0 || iseg >= c->nseg)
     return fail(c, SANEPIC_EINVAL, "index out of range");
   std::vector<int32_t> h(c->ns);
@@ -867,6 +897,19 @@ int sanepic_set_fft_split(sanepic_ctx *c, int32_t log2_n1) {
 int sanepic_autotune(sanepic_ctx *c, const char *cache, int32_t *log2_n1, double *op_ms) {
   ACHK(c, check_ready(c, false));
   if (c->setup_done) return fail(c, SANEPIC_EINVAL, "call sanepic_autotune before sanepic_setup");
+  if (!c->sub.empty()) {  // every group tunes its own split; report the sum of their times
+    double tot = 0.0;
+    for (sanepic_ctx *s : c->sub) {
+      int32_t l1 = 0;
+      double ms = 0.0;
+      const int rc = sanepic_autotune(s, cache, &l1, &ms);
+      if (rc) return fail(c, rc, s->err);
+      tot += ms;
+    }
+    if (log2_n1) *log2_n1 = 0;
+    if (op_ms) *op_ms = tot;
+    return SANEPIC_OK;
+  }
   if (c->single) {
     if (log2_n1) *log2_n1 = c->g.log2N1;
     if (op_ms) *op_ms = 0.0;
@@ -953,8 +996,9 @@ int sanepic_autotune(sanepic_ctx *c, const char *cache, int32_t *log2_n1, double
 
 int sanepic_setup(sanepic_ctx *c) {
   ACHK(c, check_ready(c, false));
-  if (c->up_tod) HIPCHK(c, hipStreamSynchronize(c->up_tod));
   if (c->setup_done) return fail(c, SANEPIC_EINVAL, "sanepic_setup already ran");
+  if (!c->sub.empty()) return group_setup(c);
+  if (c->up_tod) HIPCHK(c, hipStreamSynchronize(c->up_tod));
   for (size_t i = 0; i < c->tod_ok.size(); i++)
     if (!c->tod_ok[i]) return fail(c, SANEPIC_EINVAL, "TOD block " + std::to_string(i) + " not set");
   const int64_t npix = c->npix, ntot = c->ntot, N = c->ns;
@@ -1249,6 +1293,10 @@ int sanepic_true_residual(sanepic_ctx *c, double *rel) {
 
 int sanepic_set_profiling(sanepic_ctx *c, int32_t on) {
   if (!c) return SANEPIC_EINVAL;
+  for (sanepic_ctx *s : c->sub) {
+    const int rc = sanepic_set_profiling(s, on);
+    if (rc) return fail(c, rc, s->err);
+  }
   ACHK(c, prof_collect(c));
   c->prof = on != 0;
   for (int i = 0; i < kProfClasses; i++) {
@@ -1262,14 +1310,26 @@ int sanepic_kernel_time(sanepic_ctx *c, int32_t which, double *ms, int64_t *coun
   if (!c) return SANEPIC_EINVAL;
   if (which < 0 || which >= kProfClasses) return fail(c, SANEPIC_EINVAL, "kernel class out of range");
   ACHK(c, prof_collect(c));
-  if (ms) *ms = c->kt_ms[which];
-  if (count) *count = c->kt_n[which];
+  double tot = c->kt_ms[which];
+  int64_t n = c->kt_n[which];
+  for (sanepic_ctx *s : c->sub) {  // groups: launches of every group
+    double m = 0.0;
+    int64_t k = 0;
+    const int rc = sanepic_kernel_time(s, which, &m, &k);
+    if (rc) return fail(c, rc, s->err);
+    tot += m;
+    n += k;
+  }
+  if (ms) *ms = tot;
+  if (count) *count = n;
   return SANEPIC_OK;
 }
 
 int sanepic_device_bytes(sanepic_ctx *c, int64_t *bytes) {
   if (!c) return SANEPIC_EINVAL;
-  *bytes = c->bytes;
+  int64_t b = c->bytes;
+  for (sanepic_ctx *s : c->sub) b += s->bytes;
+  *bytes = b;
   return SANEPIC_OK;
 }
 

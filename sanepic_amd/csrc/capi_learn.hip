@@ -210,6 +210,52 @@ int pair_step(sanepic_ctx *c, int64_t k, int kind, bool train, float lr, const s
 
 namespace sctx {
 
+// q = P^T N^-1 n (masked) for noise n drawn from seed with the detectors'
+// own spectra, as one unnormalised filter pass over white noise.
+// Scale: iSpf is the inverse noise power per sample, so n has covariance
+// eigenvalues 1/iSpf = 1/(N^2 W) (W = SpN = iSpf / N^2, sanepic.cc:533-537)
+// while the operator's N^-1 = c2r(W r2c .) has eigenvalues N W, i.e. it is
+// the true inverse covariance over N (a scale the solution does not see).
+// N^-1 n then has amplitudes sqrt(W) per unit white noise: weights
+// V = sqrt(W) / N.  At DC the solver's weight (1e-6 of bin 1, 537)
+// regularises the segment offsets and is no noise model: n's mean gets the
+// first bin's variance, V_0 = W_0 / (N sqrt(W_1)).
+int noise_rhs(sanepic_ctx *c, uint64_t seed, double *q) {
+  if (!c->sub.empty()) {
+    LCHK(c, launch_fill(q, 0.0, 3 * c->npix, c->st));
+    for (size_t k = 0; k < c->sub.size(); k++) {
+      sanepic_ctx *s = c->sub[k];
+      ACHK(c, dalloc(s, s->q, 3 * s->npix));
+      const int rc = noise_rhs(s, seed + 0x9e3779b97f4a7c15ULL * (k + 1), s->q);
+      if (rc) return fail(c, rc, s->err);
+      LCHK(c, launch_group_scatter_add(q, c->gmap[k], s->q, s->npix, 3, c->st));
+    }
+    LCHK(c, cg_launch_mask(q, c->cnt, c->npix, c->st));
+    return 0;
+  }
+  const int64_t N = c->ns;
+  if (!c->wt_sqrt) {
+    std::vector<double> w((size_t)(c->ndet * N));
+    ACHK(c, d2h(c, w.data(), c->wt, w.size() * sizeof(double)));
+    const int64_t i1 = c->g.N2 > 1 ? c->g.N1 : 1;  // row-layout position of bin 1
+    for (int64_t d = 0; d < c->ndet; d++) {
+      double *row = w.data() + d * N;
+      const double w0 = std::fabs(row[0]), w1 = std::fabs(row[i1]);
+      for (int64_t i = 0; i < N; i++) row[i] = std::sqrt(std::fabs(row[i])) / (double)N;
+      row[0] = w1 > 0.0 ? w0 / ((double)N * std::sqrt(w1)) : 0.0;
+    }
+    ACHK(c, dalloc(c, c->wt_sqrt, c->ndet * N));
+    HIPCHK(c, hipMemcpyAsync(c->wt_sqrt, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+  }
+  OpArgs<double> a = op_args<double>(c);
+  a.wt = c->wt_sqrt;
+  a.seed = seed;
+  ACHK(c, run_filter<double>(c, a, SRC_RNG, ROW_FWD | ROW_FILTER | ROW_INV, nullptr));
+  ACHK(c, run_scatter<double>(c, c->z, q, 1));
+  return 0;
+}
+
 int fpcg_begin(sanepic_ctx *c) {
   ACHK(c, need_net(c));
   const int64_t n3 = 3 * c->npix;
@@ -479,47 +525,21 @@ int sanepic_learn_features(sanepic_ctx *c, const double *r, float *feat, double 
 int sanepic_learn_harvest(sanepic_ctx *c, const sanepic_harvest_opts *o, int64_t *added, int32_t *iters) {
   ACHK(c, check_ready(c, true));
   if (!o || o->maxiter <= 0 || o->capacity <= 0) return fail(c, SANEPIC_EINVAL, "bad harvest options");
-  const int64_t npix = c->npix, n3 = 3 * npix, N = c->ns;
+  const int64_t npix = c->npix, n3 = 3 * npix;
   if (!c->pairs) {
     ACHK(c, dalloc(c, c->pairs, 2 * o->capacity * n3));
     c->pair_cap = o->capacity;
     c->npairs = 0;
   }
-  // Weights that turn white noise w into N^-1 n with n ~ N(0, N_gen), as one
-  // unnormalised filter pass: V = W / sqrt(N W') where W' = W except at DC,
-  // where it is the first bin's.  The solver's DC weight (1e-6 of bin 1,
-  // sanepic.cc:537) regularises the segment offsets; as a noise model it
-  // would give every segment a huge random offset, which data (whose mean
-  // has the variance of the lowest bins) do not have.
-  if (!c->wt_sqrt) {
-    std::vector<double> w((size_t)(c->ndet * N));
-    ACHK(c, d2h(c, w.data(), c->wt, w.size() * sizeof(double)));
-    const int64_t i1 = c->g.N2 > 1 ? c->g.N1 : 1;  // row-layout position of bin 1
-    for (int64_t d = 0; d < c->ndet; d++) {
-      double *row = w.data() + d * N;
-      const double w0 = std::fabs(row[0]), w1 = std::fabs(row[i1]);
-      for (int64_t i = 0; i < N; i++) row[i] = std::sqrt(std::fabs(row[i]) / (double)N);
-      row[0] = w1 > 0.0 ? w0 / std::sqrt((double)N * w1) : 0.0;
-    }
-    ACHK(c, dalloc(c, c->wt_sqrt, c->ndet * N));
-    HIPCHK(c, hipMemcpyAsync(c->wt_sqrt, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice, c->st));
-    HIPCHK(c, hipStreamSynchronize(c->st));
-  }
   ACHK(c, dalloc(c, c->b_train, n3));
   ACHK(c, dalloc(c, c->m_train, n3));
   ACHK(c, dalloc(c, c->tail, n3));
   ACHK(c, dalloc(c, c->qx, n3));
-  // b' = A m' + P^T N^-1/2 w, w white: P^T N^-1 n with n ~ N(0, N)
+  // b' = A m' + P^T N^-1 n'
   LCHK(c, launch_rng_map(c->m_train, c->cnt, npix, o->seed, 1e-4, 1e-5, c->st));
   ACHK(c, apply_A(c, c->m_train, c->b_train, 1, false, true));
-  {
-    OpArgs<double> a = op_args<double>(c);
-    a.wt = c->wt_sqrt;
-    a.seed = o->seed * 2654435761ULL + 0x5bd1e995ULL;
-    ACHK(c, run_filter<double>(c, a, SRC_RNG, ROW_FWD | ROW_FILTER | ROW_INV, nullptr));
-    ACHK(c, run_scatter<double>(c, c->z, c->qx, 1));
-    LCHK(c, launch_axpy(c->b_train, c->qx, 1.0, n3, c->st));
-  }
+  ACHK(c, noise_rhs(c, o->seed * 2654435761ULL + 0x5bd1e995ULL, c->qx));
+  LCHK(c, launch_axpy(c->b_train, c->qx, 1.0, n3, c->st));
   // plain block-Jacobi PCG (optionally deflated) on b'
   double *b_data = c->b;
   const bool defl_save = c->deflate;

@@ -1,0 +1,313 @@
+// capi_uneven.hip -- detectors with different segment lengths.
+//
+// The reference requires one segment length for every detector
+// (sanepic.cc:668-672).  Here each detector d has its own length L_d
+// (2^l, or 3 or 5 times 2^l) and splits the common boresight timeline into
+// floor(T / L_d) segments.  Detectors of equal length form a group that runs
+// as an ordinary uniform sub-context: it reads the parent's timeline
+// pointing in place (a uniform context's [nseg][ns] pointing layout is the
+// timeline's first nseg ns samples), shares the parent's stream, and keeps
+// its own pixel indices, pixel-sorted P^T, noise weights and four-step
+// geometry.  The parent holds the union of seen pixels and the solver; its
+// operator is
+//   A x = sum_g S_g A_g S_g^T x,
+// with S_g^T the gather of a group's pixels out of the parent's vector and
+// S_g the scatter-add back (gmap: group pixel -> parent pixel), followed by
+// the parent's Q/U mask.  Every step is a kernel launch on one stream, so
+// the whole iteration can be captured in a HIP graph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <map>
+#include <numeric>
+#include <vector>
+
+#include "ctx.h"
+
+using namespace sctx;
+
+namespace sctx {
+
+int group_apply_A(sanepic_ctx *c, const double *x, double *q, int mask, bool chi2, bool fp64) {
+  LCHK(c, launch_fill(q, 0.0, 3 * c->npix, c->st));
+  if (chi2) LCHK(c, launch_fill(&c->sc->chi2, 0.0, 1, c->st));
+  for (size_t k = 0; k < c->sub.size(); k++) {
+    sanepic_ctx *s = c->sub[k];
+    LCHK(c, launch_group_gather(x, c->gmap[k], s->x, s->npix, 3, c->st));
+    const int rc = apply_A(s, s->x, s->q, 0, chi2, fp64);
+    if (rc) return fail(c, rc, s->err);
+    LCHK(c, launch_group_scatter_add(q, c->gmap[k], s->q, s->npix, 3, c->st));
+    if (chi2) LCHK(c, launch_add_scalar(&c->sc->chi2, &s->sc->chi2, c->st));
+  }
+  if (mask) LCHK(c, cg_launch_mask(q, c->cnt, c->npix, c->st));
+  return 0;
+}
+
+int group_pixelize(sanepic_ctx *c, int64_t *npix_out, int64_t *ndef_out) {
+  // the timeline pointing every group reads must be complete
+  int64_t need = 0;
+  for (sanepic_ctx *s : c->sub) need = std::max(need, s->nseg * s->ns);
+  std::vector<std::pair<int64_t, int64_t>> iv = c->tl_set;
+  std::sort(iv.begin(), iv.end());
+  int64_t covered = 0;
+  for (auto &r : iv)
+    if (r.first <= covered) covered = std::max(covered, r.second);
+  if (covered < need)
+    return fail(c, SANEPIC_EINVAL, "boresight pointing not set for samples from " + std::to_string(covered));
+  if (c->up_pt) HIPCHK(c, hipStreamSynchronize(c->up_pt));
+  int64_t ndef = 0;
+  for (sanepic_ctx *s : c->sub) {
+    std::fill(s->pt_ok.begin(), s->pt_ok.end(), 1);
+    int64_t np = 0, nd = 0;
+    const int rc = sanepic_pixelize(s, &np, &nd);
+    if (rc) return fail(c, rc, s->err);
+    ndef += nd;
+  }
+  // union of the groups' seen pixels, ascending RING order (904-915)
+  const int64_t nsky = 12 * c->nside * c->nside;
+  std::vector<int64_t> hc((size_t)nsky, 0);
+  for (sanepic_ctx *s : c->sub)
+    for (int64_t p = 0; p < s->npix; p++) hc[(size_t)s->h_pixlist[p]] += s->h_cnt[p];
+  c->h_indpix.assign((size_t)nsky, -1);
+  c->h_pixlist.clear();
+  c->h_cnt.clear();
+  for (int64_t i = 0; i < nsky; i++)
+    if (hc[(size_t)i]) {
+      c->h_indpix[(size_t)i] = (int32_t)c->h_pixlist.size();
+      c->h_pixlist.push_back((int32_t)i);
+      c->h_cnt.push_back((int32_t)hc[(size_t)i]);
+    }
+  c->npix = (int64_t)c->h_pixlist.size();
+  ACHK(c, dalloc(c, c->cnt, c->npix));
+  HIPCHK(c, hipMemcpy(c->cnt, c->h_cnt.data(), c->npix * sizeof(int32_t), hipMemcpyHostToDevice));
+  c->gmap.assign(c->sub.size(), nullptr);
+  for (size_t k = 0; k < c->sub.size(); k++) {
+    sanepic_ctx *s = c->sub[k];
+    std::vector<int32_t> gm((size_t)s->npix);
+    for (int64_t p = 0; p < s->npix; p++) gm[(size_t)p] = c->h_indpix[(size_t)s->h_pixlist[p]];
+    ACHK(c, dalloc(c, c->gmap[k], s->npix));
+    HIPCHK(c, hipMemcpy(c->gmap[k], gm.data(), s->npix * sizeof(int32_t), hipMemcpyHostToDevice));
+  }
+  dfree(c, c->ra, c->tl_len);
+  dfree(c, c->dec, c->tl_len);
+  dfree(c, c->psi, c->tl_len);
+  c->n_deferred = ndef;
+  c->pixelized = true;
+  if (npix_out) *npix_out = c->npix;
+  if (ndef_out) *ndef_out = ndef;
+  return SANEPIC_OK;
+}
+
+int group_setup(sanepic_ctx *c) {
+  if (c->up_tod) HIPCHK(c, hipStreamSynchronize(c->up_tod));
+  for (int64_t d = 0; d < c->ndet; d++) {
+    const sanepic_ctx *s = c->sub[(size_t)c->det_group[(size_t)d]];
+    if (c->tod_set[(size_t)d] < s->nseg * s->ns)
+      return fail(c, SANEPIC_EINVAL, "TOD of detector " + std::to_string(d) + " not fully set");
+  }
+  for (sanepic_ctx *s : c->sub) {
+    std::fill(s->tod_ok.begin(), s->tod_ok.end(), 1);
+    const int rc = sanepic_setup(s);
+    if (rc) return fail(c, rc, s->err);
+  }
+  const int64_t npix = c->npix;
+  ACHK(c, dalloc(c, c->b_raw, 3 * npix));
+  ACHK(c, dalloc(c, c->b, 3 * npix));
+  ACHK(c, dalloc(c, c->pairs6, 8 * npix));
+  ACHK(c, dalloc(c, c->cross, 2 * npix));
+  LCHK(c, launch_fill(c->b_raw, 0.0, 3 * npix, c->st));
+  LCHK(c, launch_fill(c->pairs6, 0.0, 8 * npix, c->st));
+  LCHK(c, launch_fill(c->cross, 0.0, 2 * npix, c->st));
+  for (size_t k = 0; k < c->sub.size(); k++) {
+    sanepic_ctx *s = c->sub[k];
+    LCHK(c, launch_group_scatter_add(c->b_raw, c->gmap[k], s->b_raw, s->npix, 3, c->st));
+    LCHK(c, launch_group_scatter_add(c->pairs6, c->gmap[k], s->pairs6, s->npix, 8, c->st));
+    LCHK(c, launch_group_scatter_add(c->cross, c->gmap[k], s->cross, s->npix, 2, c->st));
+  }
+  HIPCHK(c, hipMemcpyAsync(c->b, c->b_raw, 3 * npix * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+  LCHK(c, cg_launch_mask(c->b, c->cnt, npix, c->st));
+  ACHK(c, dalloc(c, c->x, 3 * npix));
+  ACHK(c, dalloc(c, c->r, 3 * npix));
+  ACHK(c, dalloc(c, c->d, 3 * npix));
+  ACHK(c, dalloc(c, c->q, 3 * npix));
+  ACHK(c, dalloc(c, c->M, 6 * npix));
+  ACHK(c, dalloc(c, c->nvar, 3 * npix));
+  ACHK(c, dalloc(c, c->part, 8 * 512));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  c->setup_done = true;
+  return SANEPIC_OK;
+}
+
+}  // namespace sctx
+
+namespace {
+
+int need_groups(sanepic_ctx *c) {
+  if (c->sub.empty())
+    return fail(c, SANEPIC_EINVAL, "range uploads are for contexts made by sanepic_create_uneven");
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sanepic_create_uneven(const sanepic_uneven_config *cfg, sanepic_ctx **out) {
+  if (!cfg || !out || !cfg->seglen) return SANEPIC_EINVAL;
+  *out = nullptr;
+  if (cfg->ndet < 1 || cfg->nsamp < 1) {
+    set_create_error("ndet and nsamp must be >= 1");
+    return SANEPIC_EINVAL;
+  }
+  // groups of equal segment length, in order of first appearance
+  std::map<int64_t, int> gid;
+  std::vector<int64_t> glen;
+  std::vector<std::vector<int32_t>> gdet;
+  for (int64_t d = 0; d < cfg->ndet; d++) {
+    const int64_t L = cfg->seglen[d];
+    if (L < 16 || L > cfg->nsamp) {
+      set_create_error("segment length of detector " + std::to_string(d) + " out of range");
+      return SANEPIC_EINVAL;
+    }
+    auto it = gid.find(L);
+    if (it == gid.end()) {
+      it = gid.emplace(L, (int)glen.size()).first;
+      glen.push_back(L);
+      gdet.emplace_back();
+    }
+    gdet[(size_t)it->second].push_back((int32_t)d);
+  }
+  sanepic_config base{};
+  base.nside = cfg->nside;
+  base.polar = cfg->polar;
+  base.hwp = cfg->hwp;
+  base.omega_hwp = cfg->omega_hwp;
+  base.precision = cfg->precision;
+  base.device = cfg->device;
+  std::vector<sanepic_ctx *> subs;
+  auto cleanup = [&]() {
+    for (sanepic_ctx *s : subs) sanepic_destroy(s);
+  };
+  for (size_t g = 0; g < glen.size(); g++) {
+    sanepic_config sc = base;
+    sc.ndet = (int64_t)gdet[g].size();
+    sc.ns = glen[g];
+    sc.nseg = cfg->nsamp / glen[g];
+    sanepic_ctx *s = nullptr;
+    const int rc = sanepic_create(&sc, &s);
+    if (rc) {
+      cleanup();
+      return rc;
+    }
+    subs.push_back(s);
+  }
+  // the parent: validated like a uniform context of one segment per detector
+  sanepic_config pc = base;
+  pc.ndet = cfg->ndet;
+  pc.nseg = 1;
+  pc.ns = 16;
+  sanepic_ctx *c = nullptr;
+  int rc = sanepic_create(&pc, &c);
+  if (rc) {
+    cleanup();
+    return rc;
+  }
+  // the parent keeps only what a group context does not: the timeline
+  // pointing (read by every group in place), the solver and the union map
+  dfree(c, c->ra, 16);
+  dfree(c, c->dec, 16);
+  dfree(c, c->psi, 16);
+  c->tl_len = cfg->nsamp;
+  c->ns = 0;
+  c->nseg = 0;
+  rc = dalloc(c, c->ra, cfg->nsamp);
+  if (!rc) rc = dalloc(c, c->dec, cfg->nsamp);
+  if (!rc) rc = dalloc(c, c->psi, cfg->nsamp);
+  if (rc) {
+    set_create_error(c->err);
+    cleanup();
+    sanepic_destroy(c);
+    return SANEPIC_ENOMEM;
+  }
+  c->det_group.assign((size_t)cfg->ndet, 0);
+  c->det_local.assign((size_t)cfg->ndet, 0);
+  c->tod_set.assign((size_t)cfg->ndet, 0);
+  c->ntot = 0;
+  for (size_t g = 0; g < subs.size(); g++) {
+    sanepic_ctx *s = subs[g];
+    dfree(s, s->ra, s->nseg * s->ns);
+    dfree(s, s->dec, s->nseg * s->ns);
+    dfree(s, s->psi, s->nseg * s->ns);
+    s->ra = c->ra;
+    s->dec = c->dec;
+    s->psi = c->psi;
+    (void)hipStreamDestroy(s->st);
+    s->st = c->st;
+    s->borrowed = true;
+    for (size_t j = 0; j < gdet[g].size(); j++) {
+      c->det_group[(size_t)gdet[g][j]] = (int32_t)g;
+      c->det_local[(size_t)gdet[g][j]] = (int32_t)j;
+    }
+    c->ntot += s->ntot;
+  }
+  c->sub = subs;
+  *out = c;
+  return SANEPIC_OK;
+}
+
+int sanepic_get_seglen(sanepic_ctx *c, int64_t idet, int64_t *seglen, int64_t *nseg) {
+  if (!c) return SANEPIC_EINVAL;
+  if (idet < 0 || idet >= c->ndet) return fail(c, SANEPIC_EINVAL, "detector index out of range");
+  const sanepic_ctx *s = c->sub.empty() ? c : c->sub[(size_t)c->det_group[(size_t)idet]];
+  if (seglen) *seglen = s->ns;
+  if (nseg) *nseg = s->nseg;
+  return SANEPIC_OK;
+}
+
+int sanepic_set_pointing_range(sanepic_ctx *c, int64_t t0, int64_t n, const double *ra, const double *dec,
+                               const double *psi, int32_t queued) {
+  if (!c) return SANEPIC_EINVAL;
+  ACHK(c, need_groups(c));
+  if (!c->ra) return fail(c, SANEPIC_EINVAL, "pointing already consumed by sanepic_pixelize");
+  if (t0 < 0 || n < 0 || t0 + n > c->tl_len) return fail(c, SANEPIC_EINVAL, "timeline range out of bounds");
+  if (n == 0) return SANEPIC_OK;
+  if (!ra || !dec || !psi) return fail(c, SANEPIC_EINVAL, "null pointing buffer");
+  const size_t nb = (size_t)n * sizeof(double);
+  if (queued) {
+    if (!c->up_pt) HIPCHK(c, hipStreamCreateWithFlags(&c->up_pt, hipStreamNonBlocking));
+    HIPCHK(c, hipMemcpyAsync(c->ra + t0, ra, nb, hipMemcpyHostToDevice, c->up_pt));
+    HIPCHK(c, hipMemcpyAsync(c->dec + t0, dec, nb, hipMemcpyHostToDevice, c->up_pt));
+    HIPCHK(c, hipMemcpyAsync(c->psi + t0, psi, nb, hipMemcpyHostToDevice, c->up_pt));
+  } else {
+    HIPCHK(c, hipMemcpy(c->ra + t0, ra, nb, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->dec + t0, dec, nb, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->psi + t0, psi, nb, hipMemcpyHostToDevice));
+  }
+  c->tl_set.emplace_back(t0, t0 + n);
+  return SANEPIC_OK;
+}
+
+int sanepic_set_tod_range(sanepic_ctx *c, int64_t idet, int64_t t0, int64_t n, const double *tod, int32_t queued) {
+  if (!c) return SANEPIC_EINVAL;
+  ACHK(c, need_groups(c));
+  if (idet < 0 || idet >= c->ndet) return fail(c, SANEPIC_EINVAL, "detector index out of range");
+  if (c->setup_done) return fail(c, SANEPIC_EINVAL, "TOD already consumed by sanepic_setup");
+  sanepic_ctx *s = c->sub[(size_t)c->det_group[(size_t)idet]];
+  const int64_t len = s->nseg * s->ns;
+  if (t0 < 0 || n < 0 || t0 + n > len) return fail(c, SANEPIC_EINVAL, "TOD range beyond the detector's segments");
+  if (n == 0) return SANEPIC_OK;
+  if (!tod) return fail(c, SANEPIC_EINVAL, "null TOD buffer");
+  const int rc = dalloc(s, s->tod, s->ntot);
+  if (rc) return fail(c, rc, s->err);
+  double *dst = s->tod + (int64_t)c->det_local[(size_t)idet] * len + t0;
+  if (queued) {
+    if (!c->up_tod) HIPCHK(c, hipStreamCreateWithFlags(&c->up_tod, hipStreamNonBlocking));
+    HIPCHK(c, hipMemcpyAsync(dst, tod, (size_t)n * sizeof(double), hipMemcpyHostToDevice, c->up_tod));
+  } else {
+    HIPCHK(c, hipMemcpy(dst, tod, (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+  }
+  c->tod_set[(size_t)idet] += n;
+  return SANEPIC_OK;
+}
+
+}  // extern "C"

@@ -69,6 +69,16 @@ struct sanepic_ctx {
   double kt_ms[kProfClasses] = {0};
   int64_t kt_n[kProfClasses] = {0};
   std::vector<void *> allocs;
+  // Per-detector segment lengths: each group of detectors with one length is
+  // a uniform sub-context that borrows this context's boresight timeline and
+  // stream; the parent holds the union of seen pixels and the solver.
+  std::vector<sanepic_ctx *> sub;
+  std::vector<int32_t> det_group, det_local;   // per detector
+  std::vector<int32_t *> gmap;                 // per group: group pixel -> pixel
+  int64_t tl_len = 0;                          // boresight timeline samples
+  std::vector<std::pair<int64_t, int64_t>> tl_set;  // pointing ranges uploaded
+  std::vector<int64_t> tod_set;                // samples uploaded per detector
+  bool borrowed = false;                       // stream and pointing belong to a parent
   // HIP graphs of one plain PCG iteration: [0] normal, [1] residual refresh
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
   bool use_graph = false;
@@ -144,13 +154,16 @@ inline int dalloc(sanepic_ctx *c, T *&p, int64_t n) {
   return 0;
 }
 
+// Frees a buffer this context allocated; a borrowed pointer is only dropped.
 template <typename T>
 inline void dfree(sanepic_ctx *c, T *&p, int64_t n) {
   if (!p) return;
   auto it = std::find(c->allocs.begin(), c->allocs.end(), (void *)p);
-  if (it != c->allocs.end()) c->allocs.erase(it);
-  (void)hipFree(p);
-  c->bytes -= (int64_t)((size_t)std::max<int64_t>(n, 1) * sizeof(T));
+  if (it != c->allocs.end()) {
+    c->allocs.erase(it);
+    (void)hipFree(p);
+    c->bytes -= (int64_t)((size_t)std::max<int64_t>(n, 1) * sizeof(T));
+  }
   p = nullptr;
 }
 
@@ -174,6 +187,14 @@ int to_inter(sanepic_ctx *c, const double *host_blk, double *dev_inter);
 int from_inter(sanepic_ctx *c, const double *dev_inter, double *host_blk);
 int d2h(sanepic_ctx *c, void *dst, const void *src, size_t bytes);
 CGArgs cg_args(sanepic_ctx *c);
+// Detector groups (capi_uneven.hip): the operator, pixelisation, set-up and
+// training-noise right-hand side of a context with per-detector lengths.
+int group_apply_A(sanepic_ctx *c, const double *x, double *q, int mask, bool chi2, bool fp64);
+int group_pixelize(sanepic_ctx *c, int64_t *npix, int64_t *n_deferred);
+int group_setup(sanepic_ctx *c);
+// q = P^T N^-1 n (masked) for noise n with the detectors' spectra drawn from seed
+int noise_rhs(sanepic_ctx *c, uint64_t seed, double *q);
+void set_create_error(const std::string &m);  // sanepic_last_error(NULL) of this thread
 // Flexible PCG with the attached network (capi_learn.hip): start after x0 and
 // the deflation set-up (q = A x0), and one iteration.
 int fpcg_begin(sanepic_ctx *c);

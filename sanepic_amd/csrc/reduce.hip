@@ -337,3 +337,57 @@ int launch_set_values(int32_t *dst, const int64_t *idx, const int32_t *val, int6
   if (n > 0) hipLaunchKernelGGL(k_set_values, dim3(nblk(n)), dim3(256), 0, s, dst, idx, val, n);
   return ok();
 }
+
+// ------------------------------------------------------- detector groups
+// A context whose detectors have different segment lengths runs each group
+// of equal length as a uniform sub-problem over its own seen pixels; gmap[p]
+// is the parent's pixel of the group's pixel p (injective, so the scatter-add
+// has no conflicts).
+namespace {
+__global__ void k_group_gather(const double *x, const int32_t *gmap, double *xg, int64_t npg, int width) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npg * width) return;
+  const int64_t p = i / width;
+  const int c = (int)(i % width);
+  xg[i] = x[(int64_t)gmap[p] * width + c];
+}
+
+__global__ void k_group_scatter_add(double *x, const int32_t *gmap, const double *xg, int64_t npg, int width) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npg * width) return;
+  const int64_t p = i / width;
+  const int c = (int)(i % width);
+  x[(int64_t)gmap[p] * width + c] += xg[i];
+}
+
+__global__ void k_fill(double *v, double val, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = val;
+}
+
+__global__ void k_add_scalar(double *dst, const double *src) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *dst += *src;
+}
+}  // namespace
+
+int launch_group_gather(const double *x, const int32_t *gmap, double *xg, int64_t npg, int width, hipStream_t s) {
+  if (npg > 0) hipLaunchKernelGGL(k_group_gather, dim3(nblk(npg * width)), dim3(256), 0, s, x, gmap, xg, npg, width);
+  return ok();
+}
+
+int launch_group_scatter_add(double *x, const int32_t *gmap, const double *xg, int64_t npg, int width,
+                             hipStream_t s) {
+  if (npg > 0)
+    hipLaunchKernelGGL(k_group_scatter_add, dim3(nblk(npg * width)), dim3(256), 0, s, x, gmap, xg, npg, width);
+  return ok();
+}
+
+int launch_fill(double *v, double val, int64_t n, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_fill, dim3(nblk(n)), dim3(256), 0, s, v, val, n);
+  return ok();
+}
+
+int launch_add_scalar(double *dst, const double *src, hipStream_t s) {
+  hipLaunchKernelGGL(k_add_scalar, dim3(1), dim3(64), 0, s, dst, src);
+  return ok();
+}

@@ -147,8 +147,21 @@ WORKLOADS = {
 }
 
 
+# Batched uneven workload: 64 detectors at NSIDE 1024, each with its own
+# segment length m 2^l (m in 1, 3, 5; l in 16..21), knee frequency
+# (0.02-0.5 Hz) and spectral index (0.8-2.0), on a timeline of 2^24 samples
+# (1.07e9 samples in all: 2^26 per detector would need ~390 GB of resident
+# per-sample state, more than one GPU's 288 GB).
+UNEVEN_WORKLOADS = {
+    "c3": dict(nside=1024, ndet=64, nsamp=2 ** 24, seed=7),
+}
+
+
 def workload_config(name, **kw):
     """Config of a named workload (same scan, sky and noise recipe)."""
+    if name in UNEVEN_WORKLOADS:
+        w = UNEVEN_WORKLOADS[name]
+        return draw_uneven(w["nside"], w["ndet"], w["nsamp"], seed=w["seed"], **kw)
     w = WORKLOADS[name]
     return Config(nside=w["nside"], ndet=w["ndet"], nseg=w["nsamp"] // w["ns"], ns=w["ns"], **kw)
 
@@ -230,6 +243,105 @@ def make_dataset(cfg, workers=None, log=None):
                      for _ in range(cfg.ndet)])
     _SHARED.clear()
     return dict(ra=ra, dec=dec, psi=psi, tod=tod, ispf=ispf, sky=sky, pix=pix)
+
+
+class UnevenConfig:
+    """Detectors with their own segment length, knee frequency and spectral
+    index on one boresight timeline of nsamp samples; detector d uses
+    floor(nsamp / seglen[d]) whole segments."""
+
+    def __init__(self, nside, seglen, nsamp, fknee=None, alpha=None, fsamp=19.0, sigma=50e-6,
+                 det_angles_deg=None, sky_seed=1, noise_seed0=1000, start=0, spin_min=20.0,
+                 prec_min=192.0):
+        self.nside, self.nsamp = nside, nsamp
+        self.seglen = [int(v) for v in seglen]
+        self.ndet = ndet = len(self.seglen)
+        self.fknee = list(fknee) if fknee is not None else [0.1] * ndet
+        self.alpha = list(alpha) if alpha is not None else [1.0] * ndet
+        self.fsamp, self.sigma = fsamp, sigma
+        self.det_angles_deg = list(det_angles_deg) if det_angles_deg is not None else \
+            [22.5 * (k % 8) + 2.8125 * (k // 8) for k in range(ndet)]
+        self.offsets_deg = [(0.0, 0.0)] * ndet
+        self.sky_seed, self.noise_seed0, self.start = sky_seed, noise_seed0, start
+        self.spin_min, self.prec_min = spin_min, prec_min
+
+    def nseg(self, d):
+        return self.nsamp // self.seglen[d]
+
+    def used(self, d):
+        return self.nseg(d) * self.seglen[d]
+
+
+def draw_uneven(nside, ndet, nsamp, seed=7, log2_range=(16, 21), radices=(1, 3, 5),
+                fknee_range=(0.02, 0.5), alpha_range=(0.8, 2.0), **kw):
+    """A batched uneven case: per detector a segment length m 2^l (m from
+    radices, l uniform in log2_range, at most nsamp), fknee and alpha uniform
+    in their ranges, drawn from seed."""
+    rng = np.random.default_rng(seed)
+    seglen, fk, al = [], [], []
+    for _ in range(ndet):
+        while True:
+            L = int(rng.choice(radices)) * 2 ** int(rng.integers(log2_range[0], log2_range[1] + 1))
+            if L <= nsamp:
+                break
+        seglen.append(L)
+        fk.append(float(rng.uniform(*fknee_range)))
+        al.append(float(rng.uniform(*alpha_range)))
+    return UnevenConfig(nside, seglen, nsamp, fknee=fk, alpha=al, **kw)
+
+
+def _uneven_worker(args):
+    idet, seg_lo, seg_hi = args
+    cfg = _SHARED["cfg"]
+    off = _SHARED["off"]
+    tod = np.frombuffer(_SHARED["tod_buf"], dtype=np.float64)
+    pix, psi, sky = _SHARED["pix"], _SHARED["psi"], _SHARED["sky"]
+    ang = math.radians(cfg.det_angles_deg[idet])
+    L = cfg.seglen[idet]
+    for iseg in range(seg_lo, seg_hi):
+        sl = slice(iseg * L, (iseg + 1) * L)
+        p = pix[sl]
+        a2 = 2.0 * (psi[sl] + ang)
+        sig = sky[0][p] + sky[1][p] * np.cos(a2) + sky[2][p] * np.sin(a2)
+        n = noise_segment(L, cfg.fsamp, cfg.sigma, cfg.fknee[idet], cfg.alpha[idet],
+                          (cfg.noise_seed0 + idet, iseg))
+        tod[off[idet] + iseg * L:off[idet] + (iseg + 1) * L] = sig + n
+    return idet
+
+
+def make_uneven_dataset(cfg, workers=None):
+    """Returns dict(ra, dec, psi [nsamp], tod: list of per-detector arrays
+    [used(d)], ispf: list [seglen(d)], sky, pix, off)."""
+    ra, dec, psi = scan_pointing(cfg.nsamp, cfg.fsamp, spin_min=cfg.spin_min,
+                                 prec_min=cfg.prec_min, start=cfg.start)
+    sky = make_sky(cfg.nside, cfg.sky_seed)
+    pix = ang2pix_ring(cfg.nside, 0.5 * np.pi - dec, ra)
+    off = np.zeros(cfg.ndet + 1, dtype=np.int64)
+    for d in range(cfg.ndet):
+        off[d + 1] = off[d] + cfg.used(d)
+    buf = _alloc_shared(int(off[-1]) * 8)
+    _SHARED.update(cfg=cfg, tod_buf=buf, pix=pix, psi=psi, sky=sky, off=off)
+    jobs = []
+    for d in range(cfg.ndet):
+        ns_d = cfg.nseg(d)
+        chunk = max(1, min(ns_d, 8))
+        jobs += [(d, s, min(s + chunk, ns_d)) for s in range(0, ns_d, chunk)]
+    if workers is None:
+        workers = min(16, os.cpu_count() or 1)
+    if workers > 1 and len(jobs) > 1:
+        ctx = mp.get_context("fork")
+        with ProcessPoolExecutor(max_workers=workers, mp_context=ctx) as ex:
+            for _ in ex.map(_uneven_worker, jobs):
+                pass
+    else:
+        for j in jobs:
+            _uneven_worker(j)
+    flat = np.frombuffer(buf, dtype=np.float64)
+    tod = [flat[off[d]:off[d + 1]] for d in range(cfg.ndet)]
+    ispf = [ispf_spectrum(cfg.seglen[d], cfg.fsamp, cfg.sigma, cfg.fknee[d], cfg.alpha[d])
+            for d in range(cfg.ndet)]
+    _SHARED.clear()
+    return dict(ra=ra, dec=dec, psi=psi, tod=tod, ispf=ispf, sky=sky, pix=pix, off=off)
 
 
 def segment_list(nseg, ns, start=0):
