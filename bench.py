@@ -101,7 +101,7 @@ def cpu_baseline(wl, budget_s=20.0):
                       f"{n} A-applications timed ({dt * 1e3:.1f} ms each)"}
 
 
-def run_solve(cfg, pin, precision, precond, deflate, weights, steps, warmup, maxiter, profile):
+def run_solve(cfg, pin, precision, precond, deflate, weights, steps, warmup, maxiter, profile, net_nside=None):
     """One timed pipeline: create -> uploads -> pixelize -> setup -> solve to a
     true residual <= 1e-6.  Returns timings, per-iteration time and kernel
     times of `steps` iterations measured after `warmup`."""
@@ -111,7 +111,7 @@ def run_solve(cfg, pin, precision, precond, deflate, weights, steps, warmup, max
           "learned": sanepic_amd.PRECOND_LEARNED}[precond]
     un = None
     if precond == "learned":
-        un = sanepic_amd.UNet(cfg.nside)
+        un = sanepic_amd.UNet(net_nside or cfg.nside)
         un.set_params(weights)  # loading the trained network is not part of the solve
     t0 = time.perf_counter()
     if hasattr(cfg, "seglen"):  # per-detector segment lengths
@@ -161,7 +161,7 @@ def run_solve(cfg, pin, precision, precond, deflate, weights, steps, warmup, max
     its = len(hist)  # the first iteration whose true residual confirmed 1e-6
     out = dict(precision=precision, precond=precond, deflate=deflate, npix=npix, deferred=ndef,
                upload_s=t_up - t0, pixelize_s=t_pix - t_up, setup_s=t_setup - t_pix,
-               begin_s=t_begin - t_setup, iterations=its, converged=converged,
+               begin_s=t_begin - t_setup, solve_s=t_end - t_begin, iterations=its, converged=converged,
                final_true_residual=true_res,
                time_to_solution_s=(t_end - t0) if converged else None,
                device_bytes=ctx.device_bytes(), ms_per_step=ms_per_step, kernel_times=kt)
@@ -187,10 +187,14 @@ def main():
     ap.add_argument("--workload", default="c1")
     ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
     ap.add_argument("--precond", default="learned", choices=["learned", "block", "diag"])
+    ap.add_argument("--no-second", action="store_true",
+                    help="skip the second pipeline (mixed block-Jacobi beside a learned headline, the "
+                         "learned solver beside any other)")
     ap.add_argument("--no-deflate", action="store_true")
     ap.add_argument("--no-baseline", action="store_true", help="skip the FP64 block-Jacobi solve")
     ap.add_argument("--maxiter", type=int, default=600)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--weights", default=None, help="trained network (default: the workload's committed one)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run a 1 GiB device copy first (counter calibration)")
     args = ap.parse_args()
@@ -217,13 +221,12 @@ def main():
         f"({cfg.ndet} det, {ntot} samples, NSIDE {cfg.nside}), in pinned host memory")
 
     precond, weights, record = args.precond, None, None
-    if precond == "learned":
-        path = train_precond.weights_path(args.workload)
-        if os.path.exists(path):
-            weights, record = train_precond.load_weights(path)
-        else:
-            log(f"no trained network at {path}: block-Jacobi instead")
-            precond = "block"
+    path = args.weights or train_precond.weights_path(args.workload)
+    if os.path.exists(path):
+        weights, record = train_precond.load_weights(path)
+    if precond == "learned" and weights is None:
+        log(f"no trained network at {path}: block-Jacobi instead")
+        precond = "block"
     deflate = not args.no_deflate
 
     base = None
@@ -231,7 +234,17 @@ def main():
         base = run_solve(cfg, pin, "fp64", "block", False, None, args.steps, args.warmup,
                          args.maxiter, profile=False)
     head = run_solve(cfg, pin, args.precision, precond, deflate, weights, args.steps, args.warmup,
-                     args.maxiter, profile=True)
+                     args.maxiter, profile=True, net_nside=(record or {}).get("net_nside"))
+    learned = second_block = None
+    if not args.no_second:
+        if precond != "learned" and weights is not None:
+            # the same pipeline with the trained network (reported beside the headline)
+            learned = run_solve(cfg, pin, args.precision, "learned", deflate, weights, 2, 1, args.maxiter,
+                                profile=False, net_nside=record.get("net_nside"))
+        elif precond == "learned":
+            # the same pipeline with block-Jacobi: what the network buys
+            second_block = run_solve(cfg, pin, args.precision, "block", deflate, None, 2, 1, args.maxiter,
+                                     profile=False)
     pin.close()
 
     kt = head["kernel_times"]
@@ -283,11 +296,25 @@ def main():
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
                      "traffic": None, "bytes_per_launch": alg, "avg_launch_ms": avg_ms},
     }
-    if "learned_stats" in head:
-        out["learned"] = {"stats": head["learned_stats"],
-                          "training": {k: record[k] for k in (
-                              "pairs", "steps", "loss", "harvest_seeds", "harvest_seconds",
-                              "train_seconds", "holdout_block_jacobi", "holdout_trained")}}
+    lrun = head if "learned_stats" in head else learned
+    if lrun is not None:
+        out["learned"] = {"solver": f"{args.precision} operator, flexible PCG with the trained network"
+                                    + (" + monopole deflation" if deflate else ""),
+                          "time_to_1e-6_s": lrun["time_to_solution_s"],
+                          "iterations_to_1e-6": lrun["iterations"],
+                          "ms_per_iteration_in_solve": lrun["solve_s"] * 1e3 / max(lrun["iterations"], 1),
+                          "stats": lrun["learned_stats"],
+                          "training": {k: record.get(k) for k in (
+                              "net_nside", "pairs", "steps", "rounds", "round_pairs", "round_steps", "loss", "noise_gain",
+                              "harvest_seeds", "on_policy_seeds", "holdout_block_jacobi", "holdout_trained",
+                              "validation_block_jacobi_iterations", "validation_iterations", "kept_phase",
+                              "total_seconds")}}
+    if second_block is not None:
+        out["block_jacobi_same_precision"] = {
+            "solver": f"{args.precision} operator, block-Jacobi PCG" + (" + monopole deflation" if deflate else ""),
+            "time_to_1e-6_s": second_block["time_to_solution_s"],
+            "iterations_to_1e-6": second_block["iterations"],
+            "ms_per_iteration_in_solve": second_block["solve_s"] * 1e3 / max(second_block["iterations"], 1)}
     if base is not None:
         out["baseline"] = {"solver": "FP64 operator, plain block-Jacobi PCG",
                            "time_to_1e-6_s": base["time_to_solution_s"],

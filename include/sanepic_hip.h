@@ -207,6 +207,9 @@ typedef struct sanepic_solve_opts {
                        A-orthogonalised against the last fcg_window directions
                        (truncated flexible CG, FCG(m); 1 is the Polak-Ribiere
                        form; 0 selects the default 16; at most 32). */
+  int32_t learned_iters; /* SANEPIC_PRECOND_LEARNED only: use the network
+                       for the first learned_iters preconditioner applications
+                       and block-Jacobi after them (0: throughout). */
 } sanepic_solve_opts;
 
 /* Start the PCG of sanepic.cc:1316-1410 from x0 (compressed 3 npix, NULL for
@@ -282,8 +285,11 @@ int sanepic_unet_forward(sanepic_unet *u, const float *x, float *y);
 int sanepic_unet_backward(sanepic_unet *u, const float *x, const float *dy,
                           float *y, float *grad);
 
-/* Attach a network to a context after sanepic_setup (same device, nside).
- * The context does not own it; detach with u = NULL before destroying it. */
+/* Attach a network to a context after sanepic_setup (same device).  The
+ * network's nside is the map's or coarser by a power of two c: it then sees
+ * the residual's features averaged over c x c face pixels, and its output is
+ * added at every pixel of the block.  The context does not own the network;
+ * detach with u = NULL before destroying it. */
 int sanepic_learn_attach(sanepic_ctx *ctx, sanepic_unet *u);
 /* Face position f*S*S + y*S + x (nested face f, row y, column x) of every
  * seen pixel, pos[npix]. */
@@ -292,7 +298,7 @@ int sanepic_learn_face_index(sanepic_ctx *ctx, int64_t *pos);
 int sanepic_ring_to_face(int64_t nside, int64_t n, const int64_t *ring,
                          int64_t *pos);
 /* The network input of residual r (compressed 3 npix, block layout):
- * feat[12][S][S][10], bf16-rounded; sigma optional. */
+ * feat[12][S][S][10] at the network's nside S, bf16-rounded; sigma optional. */
 int sanepic_learn_features(sanepic_ctx *ctx, const double *r, float *feat,
                            double *sigma);
 
@@ -303,18 +309,31 @@ typedef struct sanepic_harvest_opts {
   double keep_crit;   /* keep iterates with |r|^2/|b|^2 >= keep_crit */
   double stop_crit;   /* solve to |r|^2/|b|^2 <= stop_crit for x* */
   int64_t capacity;   /* pair store size (allocated on first use) */
+  int32_t policy;     /* SANEPIC_HARVEST_BLOCK3 or SANEPIC_HARVEST_LEARNED */
+  double noise_gain;  /* amplitude of the noise term relative to the
+                         detectors' own noise (0 selects 1); > 1 makes the
+                         right-hand sides noise-dominated */
 } sanepic_harvest_opts;
 
-/* Generate training pairs (r_k, A^-1 r_k = x* - x_k) from a block-Jacobi
- * PCG solve of a synthetic right-hand side: a Gaussian I/Q/U sky (100, 10,
- * 10 uK) seen through A plus P^T N^-1 n for noise n drawn from the
- * detectors' own spectra.  The context's data b is left untouched.
- * Appends up to capacity - stored pairs; *added (may be NULL) gets the
- * count, *iters the solver iterations. */
+enum {
+  SANEPIC_HARVEST_BLOCK3 = 0,  /* block-Jacobi PCG iterates */
+  SANEPIC_HARVEST_LEARNED = 1  /* iterates of the attached network's own
+                                  flexible PCG (FCG(16), fallback growth 2):
+                                  the residuals the trained solver meets */
+};
+
+/* Generate training pairs (r_k, A^-1 r_k = x* - x_k) from a PCG solve of a
+ * synthetic right-hand side: a Gaussian I/Q/U sky (100, 10, 10 uK) seen
+ * through A plus P^T N^-1 n for noise n drawn from the detectors' own
+ * spectra.  The context's data b is left untouched.  Appends up to
+ * capacity - stored pairs; *added (may be NULL) gets the count, *iters the
+ * solver iterations. */
 int sanepic_learn_harvest(sanepic_ctx *ctx, const sanepic_harvest_opts *opts,
                           int64_t *added, int32_t *iters);
 int sanepic_learn_npairs(sanepic_ctx *ctx, int64_t *n);
 int sanepic_learn_clear(sanepic_ctx *ctx);
+/* Keep the first n stored pairs (n <= stored). */
+int sanepic_learn_truncate(sanepic_ctx *ctx, int64_t n);
 /* Pair k in block layout (r and e of 3 npix each). */
 int sanepic_learn_get_pair(sanepic_ctx *ctx, int64_t k, double *r, double *e);
 
@@ -349,6 +368,10 @@ int sanepic_learn_train(sanepic_ctx *ctx, const sanepic_train_opts *opts,
 /* Loss (SANEPIC_LOSS_*) of stored pair k under the current parameters. */
 int sanepic_learn_eval(sanepic_ctx *ctx, int64_t k, int32_t loss_kind,
                        double *loss);
+/* Loss of stored pair k and its gradient with respect to every network
+ * parameter, grad[nparams] (no update; loss may be NULL). */
+int sanepic_learn_grad(sanepic_ctx *ctx, int64_t k, int32_t loss_kind,
+                       double *loss, float *grad);
 /* Flexible-PCG statistics since solve_begin: steps taken with the network,
  * steps redone with block-Jacobi, network outputs rejected as non-descent. */
 int sanepic_learn_stats(sanepic_ctx *ctx, int64_t *learned, int64_t *fallback,

@@ -12,6 +12,9 @@ kernels:
   hand-written backward kernels must reproduce.
 * ``features(...)``: the 10 input channels of a residual.
 * ``block_jacobi(blocks, cnt)``: the guarded 3x3 inverse used as M.
+* ``pool(img, c)`` / ``coarse_pos(pos, nside, c)``: a network coarser than
+  the map by c sees the mean over c x c face pixels (unseen pixels zero) and
+  its output at network position coarse_pos(pos) is added to each pixel.
 
 The parameter layout is the library's: per convolution W[cout][9][cin]
 (tap t = 3*dy + dx, offsets dy - 1 and dx - 1 on rows and columns) then
@@ -194,3 +197,17 @@ def features(r_blk, blocks, cnt, pos, nside):
     out = np.zeros((12 * nside * nside, NFEAT))
     out[pos] = f
     return out.reshape(12, nside, nside, NFEAT), sigma
+
+
+def pool(img, c):
+    """Mean over c x c blocks of face images img[12][S][S][ch]."""
+    f, S, _, ch = img.shape
+    return img.reshape(f, S // c, c, S // c, c, ch).mean(axis=(2, 4))
+
+
+def coarse_pos(pos, nside, c):
+    """Network position (f, y // c, x // c) of face position f*S*S + y*S + x."""
+    pos = np.asarray(pos, dtype=np.int64)
+    f, y, x = pos // (nside * nside), (pos // nside) % nside, pos % nside
+    Sn = nside // c
+    return (f * Sn + y // c) * Sn + x // c

@@ -20,6 +20,7 @@ CSRC = os.path.join(HERE, "csrc")
 FP64, MIXED = 0, 1
 PRECOND_DIAG, PRECOND_BLOCK3, PRECOND_LEARNED = 0, 1, 2
 LOSS_ANORM, LOSS_BLOCK, LOSS_RESIDUAL = 0, 1, 2
+HARVEST_BLOCK3, HARVEST_LEARNED = 0, 1
 
 # The header's function list; tests check the library exports all of them.
 ABI_FUNCTIONS = [
@@ -35,8 +36,8 @@ ABI_FUNCTIONS = [
     "sanepic_unet_set_params", "sanepic_unet_forward", "sanepic_unet_backward",
     "sanepic_learn_attach", "sanepic_learn_face_index", "sanepic_ring_to_face",
     "sanepic_learn_features", "sanepic_learn_harvest", "sanepic_learn_npairs",
-    "sanepic_learn_clear", "sanepic_learn_get_pair", "sanepic_learn_train", "sanepic_learn_eval",
-    "sanepic_learn_stats", "sanepic_host_alloc", "sanepic_host_free", "sanepic_set_pointing_async",
+    "sanepic_learn_clear", "sanepic_learn_truncate", "sanepic_learn_get_pair", "sanepic_learn_train",
+    "sanepic_learn_eval", "sanepic_learn_grad", "sanepic_learn_stats", "sanepic_host_alloc", "sanepic_host_free", "sanepic_set_pointing_async",
     "sanepic_set_tod_async", "sanepic_autotune", "sanepic_set_fft_split", "sanepic_create_uneven",
     "sanepic_get_seglen", "sanepic_set_pointing_range", "sanepic_set_tod_range",
 ]
@@ -59,7 +60,8 @@ class UnevenConfig(ctypes.Structure):
 class SolveOpts(ctypes.Structure):
     _fields_ = [("precond", ctypes.c_int32), ("refresh", ctypes.c_int32),
                 ("deflate", ctypes.c_int32), ("fallback_growth", ctypes.c_double),
-                ("graph", ctypes.c_int32), ("fcg_window", ctypes.c_int32)]
+                ("graph", ctypes.c_int32), ("fcg_window", ctypes.c_int32),
+                ("learned_iters", ctypes.c_int32)]
 
 
 class IterInfo(ctypes.Structure):
@@ -70,7 +72,7 @@ class IterInfo(ctypes.Structure):
 class HarvestOpts(ctypes.Structure):
     _fields_ = [("seed", ctypes.c_uint64), ("deflate", ctypes.c_int32), ("maxiter", ctypes.c_int32),
                 ("keep_crit", ctypes.c_double), ("stop_crit", ctypes.c_double),
-                ("capacity", ctypes.c_int64)]
+                ("capacity", ctypes.c_int64), ("policy", ctypes.c_int32), ("noise_gain", ctypes.c_double)]
 
 
 class TrainOpts(ctypes.Structure):
@@ -145,9 +147,11 @@ def lib():
                                       ctypes.POINTER(ctypes.c_int32)],
             "sanepic_learn_npairs": [P, i64p],
             "sanepic_learn_clear": [P],
+            "sanepic_learn_truncate": [P, ctypes.c_int64],
             "sanepic_learn_get_pair": [P, i64, d_p, d_p],
             "sanepic_learn_train": [P, ctypes.POINTER(TrainOpts), f_p],
             "sanepic_learn_eval": [P, i64, ctypes.c_int32, d_p],
+            "sanepic_learn_grad": [P, i64, ctypes.c_int32, d_p, f_p],
             "sanepic_learn_stats": [P, i64p, i64p, i64p],
             "sanepic_host_alloc": [ctypes.c_uint64, ctypes.POINTER(P)],
             "sanepic_host_free": [P],
@@ -498,9 +502,9 @@ class Context:
         return out.value
 
     def solve_begin(self, precond=PRECOND_DIAG, refresh=10, x0=None, deflate=False, fallback_growth=0.0,
-                    graph=False, fcg_window=0):
+                    graph=False, fcg_window=0, learned_iters=0):
         opts = SolveOpts(precond, refresh, 1 if deflate else 0, fallback_growth, 1 if graph else 0,
-                         fcg_window)
+                         fcg_window, learned_iters)
         xp = None if x0 is None else _dp(_f64(x0))
         self._x0_keep = None if x0 is None else _f64(x0)
         self._chk(lib().sanepic_solve_begin(self.h, ctypes.byref(opts), xp), "solve_begin")
@@ -573,15 +577,21 @@ class Context:
     def features(self, r):
         """Network input of residual r (block layout) and its sigma."""
         r = _f64(r)
-        S = self.nside
+        un = getattr(self, "_unet", None)
+        S = un.nside if un is not None else self.nside  # the network's (the map's or coarser)
         feat = np.empty((12, S, S, UNet.NFEAT), dtype=np.float32)
         sig = ctypes.c_double()
         self._chk(lib().sanepic_learn_features(self.h, _dp(r), _fp(feat), ctypes.byref(sig)),
                   "learn_features")
         return feat, sig.value
 
-    def harvest(self, seed, capacity, deflate=True, maxiter=400, keep_crit=1e-12, stop_crit=1e-17):
-        o = HarvestOpts(seed, 1 if deflate else 0, maxiter, keep_crit, stop_crit, capacity)
+    def harvest(self, seed, capacity, deflate=True, maxiter=400, keep_crit=1e-12, stop_crit=1e-17,
+                policy=None, noise_gain=1.0):
+        """Append training pairs from one solve of a synthetic right-hand side
+        (seed); policy HARVEST_BLOCK3 (default) or HARVEST_LEARNED (the attached
+        network's own flexible PCG).  Returns (pairs added, iterations)."""
+        o = HarvestOpts(seed, 1 if deflate else 0, maxiter, keep_crit, stop_crit, capacity,
+                        HARVEST_BLOCK3 if policy is None else policy, noise_gain)
         added = ctypes.c_int64()
         iters = ctypes.c_int32()
         self._chk(lib().sanepic_learn_harvest(self.h, ctypes.byref(o), ctypes.byref(added),
@@ -595,6 +605,9 @@ class Context:
 
     def clear_pairs(self):
         self._chk(lib().sanepic_learn_clear(self.h), "learn_clear")
+
+    def truncate_pairs(self, n):
+        self._chk(lib().sanepic_learn_truncate(self.h, n), "learn_truncate")
 
     def pair(self, k):
         r = np.empty(3 * self.npix)
@@ -614,6 +627,15 @@ class Context:
         out = ctypes.c_double()
         self._chk(lib().sanepic_learn_eval(self.h, k, loss, ctypes.byref(out)), "learn_eval")
         return out.value
+
+    def grad_pair(self, k, loss=LOSS_RESIDUAL):
+        """(loss, gradient w.r.t. the attached network's parameters) of pair k."""
+        n = ctypes.c_int64()
+        self._chk(lib().sanepic_unet_nparams(self._unet.h, ctypes.byref(n)), "unet_nparams")
+        g = np.empty(n.value, dtype=np.float32)
+        out = ctypes.c_double()
+        self._chk(lib().sanepic_learn_grad(self.h, k, loss, ctypes.byref(out), _fp(g)), "learn_grad")
+        return out.value, g
 
     def learn_stats(self):
         a, b, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()

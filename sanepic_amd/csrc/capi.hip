@@ -164,6 +164,8 @@ OpArgs<double> op_args<double>(sanepic_ctx *c) {
   a.twlo = c->twlo;
   a.twhi = c->twhi;
   a.chi2part = c->chi2part;
+  a.xcd = c->xcd_blocks ? 1 : 0;
+  a.zpos = c->zsorted ? c->rank : nullptr;
   return a;
 }
 
@@ -182,6 +184,11 @@ OpArgs<float> op_args<float>(sanepic_ctx *c) {
   a.twlo = c->twlo32;
   a.twhi = c->twhi32;
   a.chi2part = c->chi2part;
+  a.xcd = c->xcd_blocks ? 1 : 0;
+  // zpos maps a sample to its slot in the [ntot] z buffers above; a pass
+  // that writes another buffer must clear it
+  a.zpos = c->zsorted ? c->rank : nullptr;
+  a.waves = c->col_waves;
   return a;
 }
 
@@ -239,6 +246,7 @@ int run_scatter(sanepic_ctx *c, const T *z, double *q, int mask) {
   else
     s.cs_sorted = c->cs_sorted32;
   s.z = z;
+  s.sorted = (c->zsorted && c->rank) ? 1 : 0;
   s.cnt = c->cnt;
   s.q = q;
   s.partial = c->partial;
@@ -259,13 +267,29 @@ int apply_A(sanepic_ctx *c, const double *x, double *q, int mask, bool chi2, boo
   if (use64) {
     OpArgs<double> a = op_args<double>(c);
     a.map = x;
-    ACHK(c, run_filter<double>(c, a, SRC_GATHER, mode, &nb));
+    int src = SRC_GATHER;
+    if (c->pregather) {
+      ACHK(c, dalloc(c, c->px64, c->ntot));
+      ProfScope ps(c, 0);
+      LCHK(c, launch_gather_px<double>(a, c->px64, c->ntot, c->st));
+      a.px = c->px64;
+      src = SRC_PX;
+    }
+    ACHK(c, run_filter<double>(c, a, src, mode, &nb));
     ACHK(c, run_scatter<double>(c, c->z, q, mask));
   } else {
     LCHK(c, launch_d2f(x, c->map32, 3 * c->npix, c->st));
     OpArgs<float> a = op_args<float>(c);
     a.map = c->map32;
-    ACHK(c, run_filter<float>(c, a, SRC_GATHER, mode, &nb));
+    int src = SRC_GATHER;
+    if (c->pregather) {
+      ACHK(c, dalloc(c, c->px32, c->ntot));
+      ProfScope ps(c, 0);
+      LCHK(c, launch_gather_px<float>(a, c->px32, c->ntot, c->st));
+      a.px = c->px32;
+      src = SRC_PX;
+    }
+    ACHK(c, run_filter<float>(c, a, src, mode, &nb));
     ACHK(c, run_scatter<float>(c, c->z32, q, mask));
   }
   if (chi2) LCHK(c, launch_reduce_chi2(c->chi2part, nb, c->sc, c->st));
@@ -471,6 +495,10 @@ int sanepic_create(const sanepic_config *cfg, sanepic_ctx **out) {
   c->nside = cfg->nside;
   c->ntot = ntot;
   c->chunk = chunk_pairs();
+  if (const char *e = getenv("SANEPIC_PREGATHER")) c->pregather = atoi(e) != 0;
+  if (const char *e = getenv("SANEPIC_XCD_BLOCKS")) c->xcd_blocks = atoi(e) != 0;
+  if (const char *e = getenv("SANEPIC_ZSORT")) c->zsorted = atoi(e) != 0;
+  if (const char *e = getenv("SANEPIC_COL_WAVES")) c->col_waves = atoi(e);
   if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
     g_create_error = "hipStreamCreate failed";
     delete c;
@@ -801,6 +829,10 @@ int sanepic_pixelize(sanepic_ctx *c, int64_t *npix_out, int64_t *n_deferred) {
     dfree(c, keys_out, ntot);
     dfree(c, vals_in, ntot);
   }
+  if (c->zsorted) {
+    ACHK(c, dalloc(c, c->rank, ntot));
+    LCHK(c, launch_invperm(c->perm, c->rank, ntot, c->st));
+  }
   ACHK(c, dalloc(c, c->cs_sorted, ntot));
   LCHK(c, launch_gather_cs(c->perm, c->cs, c->cs_sorted, ntot, c->st));
   if (c->csfp) {
@@ -1032,7 +1064,8 @@ int sanepic_setup(sanepic_ctx *c) {
     OpArgs<double> a = op_args<double>(c);
     a.g = c->gauto;
     a.wt = dwabs;
-    a.z = c->nauto;
+    a.z = c->nauto;   // [ndet][N] in time order: not the [ntot] pixel-sorted z
+    a.zpos = nullptr;
     a.mode = ROW_SRC_WT | ROW_INV;
     if (c->single) {
       LCHK(c, launch_single<double>(a, SRC_GATHER, c->st, nullptr));
@@ -1169,6 +1202,9 @@ int sanepic_solve_begin(sanepic_ctx *c, const sanepic_solve_opts *opts, const do
     const int m = opts ? opts->fcg_window : 0;
     if (m < 0 || m > kFcgMax) return fail(c, SANEPIC_EINVAL, "fcg_window must be in 0..32");
     c->fcg_m = m > 0 ? m : 16;
+    const int li = opts ? opts->learned_iters : 0;
+    if (li < 0) return fail(c, SANEPIC_EINVAL, "learned_iters must be >= 0");
+    c->learned_iters = li;
   }
   if (c->precond != SANEPIC_PRECOND_DIAG && c->precond != SANEPIC_PRECOND_BLOCK3 &&
       c->precond != SANEPIC_PRECOND_LEARNED)

@@ -53,6 +53,11 @@ LearnArgs learn_args(sanepic_ctx *c) {
   LearnArgs a{};
   a.npix = c->npix;
   a.fidx = c->fidx;
+  a.coarse = c->lcoarse;
+  a.Sn = c->nside / c->lcoarse;
+  a.inv = c->linv;
+  a.fpx = c->lfpx;
+  a.gpx = c->lgpx;
   a.cnt = c->cnt;
   a.M = c->M;
   a.blk = c->pairs6;
@@ -173,7 +178,8 @@ int fcg_direction(sanepic_ctx *c, bool use_net) {
 // Network loss (and with train, gradient + Adam) on stored pair k; the
 // block-Jacobi blocks must be in M.  The A-norm loss uses r, zf and qx as
 // scratch (the solver state is invalidated by training).
-int pair_step(sanepic_ctx *c, int64_t k, int kind, bool train, float lr, const sanepic_train_opts *o) {
+int pair_step(sanepic_ctx *c, int64_t k, int kind, bool train, float lr, const sanepic_train_opts *o,
+              bool adam = true) {
   const int64_t n3 = 3 * c->npix;
   const float *rk = c->pairs + 2 * k * n3, *ek = c->pairs + (2 * k + 1) * n3;
   LearnArgs la = learn_args(c);
@@ -200,7 +206,7 @@ int pair_step(sanepic_ctx *c, int64_t k, int kind, bool train, float lr, const s
   }
   if (train) {
     if (unet::net_backward(n)) return fail(c, SANEPIC_EHIP, "network backward: " + n.err);
-    if (unet::net_adam(n, lr, o->beta1, o->beta2, o->eps, o->weight_decay))
+    if (adam && unet::net_adam(n, lr, o->beta1, o->beta2, o->eps, o->weight_decay))
       return fail(c, SANEPIC_EHIP, "Adam: " + n.err);
   }
   return 0;
@@ -283,11 +289,14 @@ int fpcg_begin(sanepic_ctx *c) {
   return 0;
 }
 
-int fpcg_step(sanepic_ctx *c, sanepic_iter_info *info) {
+int fpcg_step(sanepic_ctx *c, sanepic_iter_info *info, float *store, double *tail) {
   FArgs fa = f_args_ring(c);
   FScalars h{};
+  const int64_t n3 = 3 * c->npix;
   ACHK(c, apply_A(c, fa.d, fa.q, 1, false, false));
   LCHK(c, f_launch_alpha(fa, c->st));
+  if (store) LCHK(c, launch_fstep_store(fa.d, c->fs, store, n3, c->st));
+  if (tail) LCHK(c, launch_fstep_tail(fa.d, c->fs, tail, n3, c->st));
   LCHK(c, f_launch_update(fa, c->st));
   ACHK(c, d2h(c, &h, c->fs, sizeof(h)));
   // The exact line search always lowers the A-norm error, so the step is
@@ -301,7 +310,8 @@ int fpcg_step(sanepic_ctx *c, sanepic_iter_info *info) {
     ACHK(c, apply_A(c, c->x, c->qx, 1, true, true));
     LCHK(c, f_launch_refresh(fa, c->st));
   }
-  ACHK(c, fcg_direction(c, !fallback));
+  const bool net_turn = c->learned_iters == 0 || c->iter + 1 < c->learned_iters;
+  ACHK(c, fcg_direction(c, !fallback && net_turn));
   CGScalars hs{};
   ACHK(c, d2h(c, &h, c->fs, sizeof(h)));
   ACHK(c, d2h(c, &hs, c->sc, sizeof(hs)));
@@ -445,18 +455,30 @@ int sanepic_learn_attach(sanepic_ctx *c, sanepic_unet *u) {
     return SANEPIC_OK;
   }
   if (u->attached && u->attached != c) return fail(c, SANEPIC_EINVAL, "network attached to another context");
-  if (u->net.nside != c->nside) return fail(c, SANEPIC_EINVAL, "network nside differs from the map's");
+  const int64_t npix = c->npix, S = c->nside, Sn = u->net.nside;
+  if (Sn > S || S % Sn != 0)
+    return fail(c, SANEPIC_EINVAL, "the network's nside must divide the map's (equal or coarser)");
   if (u->device != c->cfg.device) return fail(c, SANEPIC_EINVAL, "network lives on another device");
-  const int64_t npix = c->npix, S = c->nside;
+  const int64_t cf = S / Sn;
   std::vector<int32_t> pos((size_t)npix);
+  std::vector<int32_t> inv(cf > 1 ? (size_t)(12 * S * S) : 0, -1);
   for (int64_t p = 0; p < npix; p++) {
     int f, x, y;
     if (host_ring2xyf((long)S, (long)c->h_pixlist[p], &f, &x, &y))
       return fail(c, SANEPIC_EINVAL, "the learned preconditioner needs nside a power of two");
-    pos[(size_t)p] = (int32_t)(((int64_t)f * S + y) * S + x);
+    pos[(size_t)p] = (int32_t)(((int64_t)f * Sn + y / cf) * Sn + x / cf);
+    if (cf > 1) inv[(size_t)(((int64_t)f * S + y) * S + x)] = (int32_t)p;
   }
   ACHK(c, dalloc(c, c->fidx, npix));
   HIPCHK(c, hipMemcpyAsync(c->fidx, pos.data(), npix * sizeof(int32_t), hipMemcpyHostToDevice, c->st));
+  c->lcoarse = (int)cf;
+  if (cf > 1) {
+    ACHK(c, dalloc(c, c->linv, 12 * S * S));
+    ACHK(c, dalloc(c, c->lfpx, 12 * npix));
+    ACHK(c, dalloc(c, c->lgpx, 3 * npix));
+    HIPCHK(c, hipMemcpyAsync(c->linv, inv.data(), inv.size() * sizeof(int32_t), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));  // inv is a host temporary
+  }
   std::vector<double> p8((size_t)(8 * npix));
   ACHK(c, d2h(c, p8.data(), c->pairs6, p8.size() * sizeof(double)));
   double s = 0.0;
@@ -525,6 +547,12 @@ int sanepic_learn_features(sanepic_ctx *c, const double *r, float *feat, double 
 int sanepic_learn_harvest(sanepic_ctx *c, const sanepic_harvest_opts *o, int64_t *added, int32_t *iters) {
   ACHK(c, check_ready(c, true));
   if (!o || o->maxiter <= 0 || o->capacity <= 0) return fail(c, SANEPIC_EINVAL, "bad harvest options");
+  if (o->policy != SANEPIC_HARVEST_BLOCK3 && o->policy != SANEPIC_HARVEST_LEARNED)
+    return fail(c, SANEPIC_EINVAL, "unknown harvest policy");
+  if (!(o->noise_gain >= 0.0) || !std::isfinite(o->noise_gain))
+    return fail(c, SANEPIC_EINVAL, "noise_gain must be >= 0");
+  const bool learned = o->policy == SANEPIC_HARVEST_LEARNED;
+  if (learned) ACHK(c, need_net(c));
   const int64_t npix = c->npix, n3 = 3 * npix;
   if (!c->pairs) {
     ACHK(c, dalloc(c, c->pairs, 2 * o->capacity * n3));
@@ -539,10 +567,13 @@ int sanepic_learn_harvest(sanepic_ctx *c, const sanepic_harvest_opts *o, int64_t
   LCHK(c, launch_rng_map(c->m_train, c->cnt, npix, o->seed, 1e-4, 1e-5, c->st));
   ACHK(c, apply_A(c, c->m_train, c->b_train, 1, false, true));
   ACHK(c, noise_rhs(c, o->seed * 2654435761ULL + 0x5bd1e995ULL, c->qx));
-  LCHK(c, launch_axpy(c->b_train, c->qx, 1.0, n3, c->st));
-  // plain block-Jacobi PCG (optionally deflated) on b'
+  LCHK(c, launch_axpy(c->b_train, c->qx, o->noise_gain > 0.0 ? o->noise_gain : 1.0, n3, c->st));
+  // PCG (optionally deflated) on b'; the solver settings of the context are
+  // restored afterwards (its solve state is not: solve_begin again)
   double *b_data = c->b;
   const bool defl_save = c->deflate;
+  const int refresh_save = c->refresh, m_save = c->fcg_m, precond_save = c->precond, li_save = c->learned_iters;
+  const double growth_save = c->fgrowth;
   c->b = c->b_train;
   c->deflate = o->deflate != 0;
   c->begun = false;
@@ -563,10 +594,24 @@ int sanepic_learn_harvest(sanepic_ctx *c, const sanepic_harvest_opts *o, int64_t
       LCHK(c, cg_launch_coarse_start(a, c->st));
     }
     CGArgs a = cg_args(c);
-    LCHK(c, cg_launch_init(a, c->st));
-    CGScalars h{};
-    ACHK(c, d2h(c, &h, c->sc, sizeof(h)));
-    double crit = h.var00 > 0 ? h.var_n / h.var00 : 0.0;
+    double crit;
+    if (learned) {
+      c->refresh = 10;
+      c->fcg_m = 16;
+      c->fgrowth = 4.0;
+      c->learned_iters = 0;
+      c->precond = SANEPIC_PRECOND_LEARNED;
+      c->iter = 0;
+      ACHK(c, fpcg_begin(c));
+      FScalars h{};
+      ACHK(c, d2h(c, &h, c->fs, sizeof(h)));
+      crit = h.var00 > 0 ? h.rr / h.var00 : 0.0;
+    } else {
+      LCHK(c, cg_launch_init(a, c->st));
+      CGScalars h{};
+      ACHK(c, d2h(c, &h, c->sc, sizeof(h)));
+      crit = h.var00 > 0 ? h.var_n / h.var00 : 0.0;
+    }
     bool keeping = true;
     for (k = 0; k < o->maxiter; k++) {
       int64_t slot = -1;
@@ -576,23 +621,30 @@ int sanepic_learn_harvest(sanepic_ctx *c, const sanepic_harvest_opts *o, int64_t
       } else {
         keeping = false;
       }
-      ACHK(c, apply_A(c, c->d, c->q, 1, false, true));
-      LCHK(c, cg_launch_alpha(a, c->st));
-      if (slot >= 0) {
-        LCHK(c, launch_step_store(c->d, c->sc, c->pairs + (2 * slot + 1) * n3, n3, c->st));
-        kept++;
+      float *store = slot >= 0 ? c->pairs + (2 * slot + 1) * n3 : nullptr;
+      if (slot >= 0) kept++;
+      if (learned) {
+        sanepic_iter_info info{};
+        ACHK(c, fpcg_step(c, &info, store, store ? nullptr : c->tail));
+        crit = info.crit;
       } else {
-        LCHK(c, launch_step_tail(c->d, c->sc, c->tail, n3, c->st));
+        ACHK(c, apply_A(c, c->d, c->q, 1, false, true));
+        LCHK(c, cg_launch_alpha(a, c->st));
+        if (store)
+          LCHK(c, launch_step_store(c->d, c->sc, store, n3, c->st));
+        else
+          LCHK(c, launch_step_tail(c->d, c->sc, c->tail, n3, c->st));
+        const bool refresh = (k % 10) == 0;
+        LCHK(c, cg_launch_update(a, refresh ? 1 : 0, c->st));
+        if (refresh) {
+          ACHK(c, apply_A(c, c->x, c->q, 1, false, true));
+          LCHK(c, cg_launch_refresh_r(a, c->st));
+        }
+        LCHK(c, cg_launch_dir(a, c->st));
+        CGScalars h{};
+        ACHK(c, d2h(c, &h, c->sc, sizeof(h)));
+        crit = h.var00 > 0 ? h.var_n / h.var00 : 0.0;
       }
-      const bool refresh = (k % 10) == 0;
-      LCHK(c, cg_launch_update(a, refresh ? 1 : 0, c->st));
-      if (refresh) {
-        ACHK(c, apply_A(c, c->x, c->q, 1, false, true));
-        LCHK(c, cg_launch_refresh_r(a, c->st));
-      }
-      LCHK(c, cg_launch_dir(a, c->st));
-      ACHK(c, d2h(c, &h, c->sc, sizeof(h)));
-      crit = h.var00 > 0 ? h.var_n / h.var00 : 0.0;
       if (crit <= o->stop_crit) {
         k++;
         break;
@@ -607,6 +659,12 @@ int sanepic_learn_harvest(sanepic_ctx *c, const sanepic_harvest_opts *o, int64_t
   const int rc = body();
   c->b = b_data;
   c->deflate = defl_save;
+  c->refresh = refresh_save;
+  c->fcg_m = m_save;
+  c->fgrowth = growth_save;
+  c->precond = precond_save;
+  c->learned_iters = li_save;
+  c->begun = false;
   if (rc) return rc;
   c->npairs += kept;
   if (added) *added = kept;
@@ -623,6 +681,13 @@ int sanepic_learn_npairs(sanepic_ctx *c, int64_t *n) {
 int sanepic_learn_clear(sanepic_ctx *c) {
   if (!c) return SANEPIC_EINVAL;
   c->npairs = 0;
+  return SANEPIC_OK;
+}
+
+int sanepic_learn_truncate(sanepic_ctx *c, int64_t n) {
+  if (!c) return SANEPIC_EINVAL;
+  if (n < 0 || n > c->npairs) return fail(c, SANEPIC_EINVAL, "truncate beyond the stored pairs");
+  c->npairs = n;
   return SANEPIC_OK;
 }
 
@@ -689,6 +754,23 @@ int sanepic_learn_eval(sanepic_ctx *c, int64_t k, int32_t kind, double *loss) {
   FScalars h{};
   ACHK(c, d2h(c, &h, c->fs, sizeof(h)));
   *loss = h.loss;
+  return SANEPIC_OK;
+}
+
+int sanepic_learn_grad(sanepic_ctx *c, int64_t k, int32_t kind, double *loss, float *grad) {
+  ACHK(c, check_ready(c, true));
+  ACHK(c, need_net(c));
+  if (k < 0 || k >= c->npairs || !grad) return fail(c, SANEPIC_EINVAL, "pair index out of range");
+  if (kind != SANEPIC_LOSS_ANORM && kind != SANEPIC_LOSS_BLOCK && kind != SANEPIC_LOSS_RESIDUAL)
+    return fail(c, SANEPIC_EINVAL, "unknown loss");
+  ACHK(c, prepare_blocks(c));
+  ACHK(c, clean_net_inputs(c));
+  c->begun = false;
+  ACHK(c, pair_step(c, k, kind, true, 0.f, nullptr, false));
+  FScalars h{};
+  ACHK(c, d2h(c, &h, c->fs, sizeof(h)));
+  if (loss) *loss = h.loss;
+  ACHK(c, d2h(c, grad, c->un->net.g, (size_t)c->un->net.nparam * sizeof(float)));
   return SANEPIC_OK;
 }
 

@@ -61,6 +61,13 @@ struct sanepic_ctx {
   int *zero_flag = nullptr;
   int iter = 0, refresh = 10, precond = 0;
   int64_t chunk = 0;  // pairs per four-step chunk (0: all at once)
+  bool pregather = false;            // P x in a separate coalesced pass before the column FFT
+  bool xcd_blocks = true;            // XCD-contiguous numbering of the four-step blocks
+  bool zsorted = false;              // the filter output z is written in pixel-sorted order
+  int col_waves = 3;                 // occupancy target of the FP32 gather column pass (0, 3, 4)
+  uint32_t *rank = nullptr;          // [ntot] sorted position of each sample (zsorted)
+  double *px64 = nullptr;
+  float *px32 = nullptr;
   bool deflate = false;
   double *az = nullptr;
   bool begun = false, setup_done = false, pixelized = false;
@@ -84,7 +91,11 @@ struct sanepic_ctx {
   bool use_graph = false;
   // learned preconditioner (flexible PCG)
   sanepic_unet *un = nullptr;
-  int32_t *fidx = nullptr;          // [npix] face position of each seen pixel
+  int32_t *fidx = nullptr;          // [npix] network face position of each seen pixel
+  int lcoarse = 1;                   // map nside / network nside
+  int32_t *linv = nullptr;           // [12 nside^2] seen pixel at each map face position, -1 (lcoarse > 1)
+  float *lfpx = nullptr;             // [npix][12] unpooled features (lcoarse > 1)
+  float *lgpx = nullptr;             // [npix][3] unpooled output gradients (lcoarse > 1)
   double b00ref = 1.0, wnorm = 1.0; // feature / loss normalisation
   FScalars *fs = nullptr;
   double *zf = nullptr, *qx = nullptr;
@@ -93,6 +104,7 @@ struct sanepic_ctx {
   // FCG(m) rings: directions and their images under A
   double *Dring = nullptr, *Qring = nullptr, *fpart = nullptr;
   int fcg_m = 16, fcg_alloc_m = 0, fcg_slot = 0;
+  int learned_iters = 0;             // network for the first this many z (0: always)
   int64_t fcg_stored = 0;           // directions stored since the start
   bool last_z_learned = false;
   int64_t n_learned = 0, n_fallback = 0, n_rejected = 0;
@@ -198,6 +210,8 @@ void set_create_error(const std::string &m);  // sanepic_last_error(NULL) of thi
 // Flexible PCG with the attached network (capi_learn.hip): start after x0 and
 // the deflation set-up (q = A x0), and one iteration.
 int fpcg_begin(sanepic_ctx *c);
-int fpcg_step(sanepic_ctx *c, sanepic_iter_info *info);
+// One flexible-PCG iteration; with store (FP32) or tail (FP64) set, the
+// step alpha d is written there / added there (training-pair harvest).
+int fpcg_step(sanepic_ctx *c, sanepic_iter_info *info, float *store = nullptr, double *tail = nullptr);
 
 }  // namespace sctx

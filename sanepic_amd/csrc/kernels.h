@@ -56,6 +56,7 @@ enum : int {
   SRC_GATHER = 0,  // Ps = I + cos2psi Q + sin2psi U from the map
   SRC_TOD = 1,     // the TOD (right-hand side)
   SRC_RNG = 2,     // white Gaussian samples from (seed, sample index) (FP64 only)
+  SRC_PX = 3,      // P x already evaluated in time order (launch_gather_px)
 };
 
 // Counter-based standard normal deviate: the SplitMix64 finalizer (Steele,
@@ -91,19 +92,26 @@ struct OpArgs {
   const V *cs;          // [ntot] cos 2psi, sin 2psi
   const T *map;         // [npix][3] interleaved I, Q, U
   const double *tod;    // [ntot] (SRC_TOD)
+  const T *px;          // [ntot] (SRC_PX)
   V *buf;               // [npairs][N] four-step work buffer
   V *dspec;             // [npairs][N] stored data spectrum
   const T *wt;          // [ndet][N] weights in row layout: wt[k2*N1 + k1] = W[k2 + N2 k1]
   T *z;                 // [ntot] time-domain output
+  const uint32_t *zpos; // [ntot] pixel-sorted position of each sample (z kept sorted) or null
   const V *tw4096;      // w_4096^j
   const V *twlo, *twhi; // w_N^j = twlo[j & 4095] * twhi[j >> 12]
   double *chi2part;     // per-block chi2 partials
   int64_t chi2off;      // first partial slot of this launch
   int mode;
   uint64_t seed;        // SRC_RNG stream
+  int xcd;              // XCD-contiguous block numbering (four-step passes)
+  int waves;            // occupancy target of the FP32 gather column pass (0: compiler's choice)
 };
 
 template <typename T> int launch_col_fwd(const OpArgs<T> &a, int src, hipStream_t s);
+// px[g] = map[pix[g]].I + cos 2psi map.Q + sin 2psi map.U for every sample, in
+// time order (one coalesced pass; consecutive samples share map records)
+template <typename T> int launch_gather_px(const OpArgs<T> &a, T *px, int64_t ntot, hipStream_t s);
 template <typename T> int launch_row(const OpArgs<T> &a, hipStream_t s, int64_t *nblocks);
 template <typename T> int launch_col_inv(const OpArgs<T> &a, hipStream_t s);
 template <typename T> int launch_single(const OpArgs<T> &a, int src, hipStream_t s, int64_t *nblocks);
@@ -129,7 +137,8 @@ struct ScatterArgs {
   Items it;
   const uint32_t *perm;   // sorted entry -> sample
   const V *cs_sorted;     // sorted entry -> cos 2psi, sin 2psi
-  const T *z;             // [ntot] time-domain values
+  const T *z;             // [ntot] time-domain values (in sorted order when `sorted`)
+  int sorted;             // z is stored pixel-sorted: entry j at z[j]
   const int32_t *cnt;     // [npix] samples per pixel (mask: cnt < 2)
   double *q;              // [npix][3] output
   double *partial;        // [nslots][3]
@@ -170,6 +179,7 @@ int launch_tod_nanfill(double *tod, int64_t ntot, int64_t ns, hipStream_t s);
 int launch_histogram(const int32_t *pix, int64_t n, int32_t *counts, hipStream_t s);
 int launch_compress_pix(int32_t *pix, int64_t n, const int32_t *indpix, hipStream_t s);
 int launch_iota_u32(uint32_t *v, int64_t n, hipStream_t s);
+int launch_invperm(const uint32_t *perm, uint32_t *rank, int64_t n, hipStream_t s);  // rank[perm[j]] = j
 
 // CG on interleaved [npix][3] FP64 vectors with FP64 scalars on the device.
 struct CGScalars {

@@ -24,7 +24,15 @@ constexpr int kFcgMax = 32;
 
 struct LearnArgs {
   int64_t npix;
-  const int32_t *fidx;  // [npix] face position f*S*S + y*S + x of each compressed pixel
+  // [npix] position f*Sn*Sn + y*Sn + x of each compressed pixel in the
+  // network's face images (Sn = the network's nside; a map finer by the
+  // factor `coarse` puts coarse x coarse pixels on one network position)
+  const int32_t *fidx;
+  int coarse;           // map nside / network nside (1: one pixel per position)
+  int64_t Sn;           // network nside
+  const int32_t *inv;   // [12 (coarse Sn)^2] compressed pixel at each map face position or -1 (coarse > 1)
+  float *fpx;           // [npix][12] per-pixel features before pooling (coarse > 1)
+  float *gpx;           // [npix][3] per-pixel output gradients before pooling (coarse > 1)
   const int32_t *cnt;   // [npix] samples per pixel (Q/U mask: cnt < 2)
   const double *M;      // [npix][6] block-Jacobi (inverse diagonal blocks)
   const double *blk;    // [npix][8] diagonal blocks of A: b00 b01 b02 b11 b12 b22 . .
@@ -34,6 +42,9 @@ struct LearnArgs {
   double *part;         // [512][8] reduction partials
 };
 
+// With coarse > 1 the network sees the map averaged over coarse x coarse
+// blocks of face pixels (unseen pixels count as zero) and its output is
+// applied to every pixel of the block; gradients are summed over the block.
 // sigma = sqrt(mean over the 3 npix entries of (M r)^2), guarded against 0
 template <typename T> int learn_sigma(const LearnArgs &a, const T *r, hipStream_t s);
 // X0[fidx[p]][0..15] = (M r / sigma, log-hits, block features, zero padding)
@@ -98,6 +109,8 @@ int launch_axpy(double *y, const double *x, double a, int64_t n, hipStream_t s);
 int launch_d2f_vec(const double *src, float *dst, int64_t n, hipStream_t s);              // dst = (float) src
 int launch_step_store(const double *d, const CGScalars *sc, float *dst, int64_t n, hipStream_t s);  // dst = alpha d
 int launch_step_tail(const double *d, const CGScalars *sc, double *tail, int64_t n, hipStream_t s); // tail += alpha d
+int launch_fstep_store(const double *d, const FScalars *fs, float *dst, int64_t n, hipStream_t s);  // flexible: alpha d
+int launch_fstep_tail(const double *d, const FScalars *fs, double *tail, int64_t n, hipStream_t s);
 int launch_tail_fold(double *tail, float *slot, int64_t n, hipStream_t s);  // tail += slot; slot = (float) tail
 int launch_f2d_block(const float *inter, double *blk, int64_t npix, hipStream_t s);  // interleaved f32 -> block f64
 

@@ -118,6 +118,13 @@ __global__ __launch_bounds__(RBS) void k_features(LearnArgs a, const T *r, uint1
   f[9] = (float)(b[5] * ib);
 #pragma unroll
   for (int c = 10; c < 16; c++) f[c] = 0.f;
+  if (a.coarse > 1) {
+    float4 *dst = reinterpret_cast<float4 *>(a.fpx + 12 * p);
+    dst[0] = make_float4(f[0], f[1], f[2], f[3]);
+    dst[1] = make_float4(f[4], f[5], f[6], f[7]);
+    dst[2] = make_float4(f[8], f[9], 0.f, 0.f);
+    return;
+  }
   uint32_t w[8];
 #pragma unroll
   for (int i = 0; i < 8; i++) w[i] = (uint32_t)unet::f2bf(f[2 * i]) | ((uint32_t)unet::f2bf(f[2 * i + 1]) << 16);
@@ -138,6 +145,80 @@ __global__ __launch_bounds__(RBS) void k_apply_out(LearnArgs a, const double *r,
   z[3 * p] = zb[0] + s * (double)o.x;
   z[3 * p + 1] = masked ? 0.0 : zb[1] + s * (double)o.y;
   z[3 * p + 2] = masked ? 0.0 : zb[2] + s * (double)o.z;
+}
+
+// Output gradient of pixel p: bf16 at its network position, or FP32 per
+// pixel for the block sum of k_pool_grad.
+__device__ __forceinline__ void put_grad(const LearnArgs &a, int64_t p, bool masked, double g0, double g1, double g2,
+                                         uint16_t *gO) {
+  if (a.coarse > 1) {
+    float *d = a.gpx + 3 * p;
+    d[0] = (float)g0;
+    d[1] = masked ? 0.f : (float)g1;
+    d[2] = masked ? 0.f : (float)g2;
+    return;
+  }
+  const int64_t pos = (int64_t)a.fidx[p] * 16;
+  gO[pos] = unet::f2bf((float)g0);
+  gO[pos + 1] = masked ? (uint16_t)0 : unet::f2bf((float)g1);
+  gO[pos + 2] = masked ? (uint16_t)0 : unet::f2bf((float)g2);
+}
+
+// Network position q of a coarse map: the mean of the features of the
+// coarse x coarse face pixels it covers (unseen pixels are zero).
+__global__ __launch_bounds__(RBS) void k_pool_feat(LearnArgs a, uint16_t *X0) {
+  const int64_t q = (int64_t)blockIdx.x * RBS + threadIdx.x;
+  const int64_t Sn = a.Sn, c = a.coarse, S = Sn * c;
+  if (q >= 12 * Sn * Sn) return;
+  const int64_t f = q / (Sn * Sn), cy = (q / Sn) % Sn, cx = q % Sn;
+  float acc[10];
+#pragma unroll
+  for (int k = 0; k < 10; k++) acc[k] = 0.f;
+  for (int64_t dy = 0; dy < c; dy++) {
+    const int32_t *row = a.inv + (f * S + cy * c + dy) * S + cx * c;
+    for (int64_t dx = 0; dx < c; dx++) {
+      const int32_t p = row[dx];
+      if (p < 0) continue;
+      const float *v = a.fpx + 12 * (int64_t)p;
+#pragma unroll
+      for (int k = 0; k < 10; k++) acc[k] += v[k];
+    }
+  }
+  const float inv_area = 1.f / (float)(c * c);
+  uint32_t w[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    const float lo = 2 * i < 10 ? acc[2 * i] * inv_area : 0.f;
+    const float hi = 2 * i + 1 < 10 ? acc[2 * i + 1] * inv_area : 0.f;
+    w[i] = (uint32_t)unet::f2bf(lo) | ((uint32_t)unet::f2bf(hi) << 16);
+  }
+  uint4 *dst = reinterpret_cast<uint4 *>(X0 + q * 16);
+  dst[0] = make_uint4(w[0], w[1], w[2], w[3]);
+  dst[1] = make_uint4(w[4], w[5], w[6], w[7]);
+}
+
+// Gradient w.r.t. network position q: the sum over the pixels it covers
+// (the output is applied to each of them).
+__global__ __launch_bounds__(RBS) void k_pool_grad(LearnArgs a, uint16_t *gO) {
+  const int64_t q = (int64_t)blockIdx.x * RBS + threadIdx.x;
+  const int64_t Sn = a.Sn, c = a.coarse, S = Sn * c;
+  if (q >= 12 * Sn * Sn) return;
+  const int64_t f = q / (Sn * Sn), cy = (q / Sn) % Sn, cx = q % Sn;
+  float acc[3] = {0.f, 0.f, 0.f};
+  for (int64_t dy = 0; dy < c; dy++) {
+    const int32_t *row = a.inv + (f * S + cy * c + dy) * S + cx * c;
+    for (int64_t dx = 0; dx < c; dx++) {
+      const int32_t p = row[dx];
+      if (p < 0) continue;
+      const float *g = a.gpx + 3 * (int64_t)p;
+      acc[0] += g[0];
+      acc[1] += g[1];
+      acc[2] += g[2];
+    }
+  }
+  gO[q * 16] = unet::f2bf(acc[0]);
+  gO[q * 16 + 1] = unet::f2bf(acc[1]);
+  gO[q * 16 + 2] = unet::f2bf(acc[2]);
 }
 
 __global__ __launch_bounds__(RBS) void k_loss_part(LearnArgs a, const float *r, const float *e, const float *OUT,
@@ -163,11 +244,7 @@ __global__ __launch_bounds__(RBS) void k_loss_part(LearnArgs a, const float *r, 
     BD[1] = (b[1] * D[0] + b[3] * D[1] + b[4] * D[2]) * ir;
     BD[2] = (b[2] * D[0] + b[4] * D[1] + b[5] * D[2]) * ir;
     acc[0] += D[0] * BD[0] + D[1] * BD[1] + D[2] * BD[2];
-    if (write_grad) {
-      gO[pos] = unet::f2bf((float)(gscale * BD[0]));
-      gO[pos + 1] = masked ? (uint16_t)0 : unet::f2bf((float)(gscale * BD[1]));
-      gO[pos + 2] = masked ? (uint16_t)0 : unet::f2bf((float)(gscale * BD[2]));
-    }
+    if (write_grad) put_grad(a, p, masked, gscale * BD[0], gscale * BD[1], gscale * BD[2], gO);
   }
   store_partials<1>(acc, a.part);
 }
@@ -209,11 +286,8 @@ __global__ __launch_bounds__(RBS) void k_anorm_grad(LearnArgs a, const double *q
   const int64_t p = (int64_t)blockIdx.x * RBS + threadIdx.x;
   if (p >= a.npix) return;
   const double g = 2.0 * a.fs->sigma / a.fs->pad[0];
-  const int64_t pos = (int64_t)a.fidx[p] * 16;
-  const bool masked = a.cnt[p] < 2;
-  gO[pos] = unet::f2bf((float)(g * (q[3 * p] - r[3 * p])));
-  gO[pos + 1] = masked ? (uint16_t)0 : unet::f2bf((float)(g * (q[3 * p + 1] - r[3 * p + 1])));
-  gO[pos + 2] = masked ? (uint16_t)0 : unet::f2bf((float)(g * (q[3 * p + 2] - r[3 * p + 2])));
+  put_grad(a, p, a.cnt[p] < 2, g * (q[3 * p] - r[3 * p]), g * (q[3 * p + 1] - r[3 * p + 1]),
+           g * (q[3 * p + 2] - r[3 * p + 2]), gO);
 }
 
 // Residual loss of one pair: L = |A z - r|^2 / |r|^2 (the residual the step
@@ -244,11 +318,7 @@ __global__ __launch_bounds__(RBS) void k_rloss_grad(LearnArgs a, const double *v
   const int64_t p = (int64_t)blockIdx.x * RBS + threadIdx.x;
   if (p >= a.npix) return;
   const double g = 2.0 * a.fs->sigma / a.fs->pad[0];
-  const int64_t pos = (int64_t)a.fidx[p] * 16;
-  const bool masked = a.cnt[p] < 2;
-  gO[pos] = unet::f2bf((float)(g * v[3 * p]));
-  gO[pos + 1] = masked ? (uint16_t)0 : unet::f2bf((float)(g * v[3 * p + 1]));
-  gO[pos + 2] = masked ? (uint16_t)0 : unet::f2bf((float)(g * v[3 * p + 2]));
+  put_grad(a, p, a.cnt[p] < 2, g * v[3 * p], g * v[3 * p + 1], g * v[3 * p + 2], gO);
 }
 
 __global__ __launch_bounds__(RBS) void k_f2d_vec(const float *s, double *d, int64_t n) {
@@ -501,6 +571,16 @@ __global__ __launch_bounds__(RBS) void k_step_tail(const double *d, const CGScal
   if (i < n) tail[i] += sc->alpha * d[i];
 }
 
+__global__ __launch_bounds__(RBS) void k_fstep_store(const double *d, const FScalars *fs, float *dst, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * RBS + threadIdx.x;
+  if (i < n) dst[i] = (float)(fs->alpha * d[i]);
+}
+
+__global__ __launch_bounds__(RBS) void k_fstep_tail(const double *d, const FScalars *fs, double *tail, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * RBS + threadIdx.x;
+  if (i < n) tail[i] += fs->alpha * d[i];
+}
+
 __global__ __launch_bounds__(RBS) void k_tail_fold(double *tail, float *slot, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * RBS + threadIdx.x;
   if (i < n) {
@@ -534,6 +614,7 @@ template int learn_sigma<float>(const LearnArgs &, const float *, hipStream_t);
 template <typename T>
 int learn_features(const LearnArgs &a, const T *r, uint16_t *X0, hipStream_t s) {
   if (a.npix > 0) hipLaunchKernelGGL(k_features<T>, dim3(nb(a.npix)), dim3(RBS), 0, s, a, r, X0);
+  if (a.coarse > 1) hipLaunchKernelGGL(k_pool_feat, dim3(nb(12 * a.Sn * a.Sn)), dim3(RBS), 0, s, a, X0);
   return ok();
 }
 template int learn_features<double>(const LearnArgs &, const double *, uint16_t *, hipStream_t);
@@ -548,6 +629,8 @@ int learn_loss(const LearnArgs &a, const float *r, const float *e, const float *
                hipStream_t s) {
   hipLaunchKernelGGL(k_loss_part, dim3(RGRID), dim3(RBS), 0, s, a, r, e, OUT, gO, write_grad);
   hipLaunchKernelGGL(k_loss_final, dim3(1), dim3(RBS), 0, s, a);
+  if (write_grad && a.coarse > 1)
+    hipLaunchKernelGGL(k_pool_grad, dim3(nb(12 * a.Sn * a.Sn)), dim3(RBS), 0, s, a, gO);
   return ok();
 }
 
@@ -556,6 +639,8 @@ int learn_anorm_loss(const LearnArgs &a, const double *z, const double *q, const
   hipLaunchKernelGGL(k_anorm_part, dim3(RGRID), dim3(RBS), 0, s, a, z, q, r, e);
   hipLaunchKernelGGL(k_anorm_final, dim3(1), dim3(RBS), 0, s, a);
   if (write_grad && a.npix > 0) hipLaunchKernelGGL(k_anorm_grad, dim3(nb(a.npix)), dim3(RBS), 0, s, a, q, r, gO);
+  if (write_grad && a.coarse > 1)
+    hipLaunchKernelGGL(k_pool_grad, dim3(nb(12 * a.Sn * a.Sn)), dim3(RBS), 0, s, a, gO);
   return ok();
 }
 
@@ -567,6 +652,7 @@ int learn_rloss(const LearnArgs &a, const double *q, const double *r, double *u,
 
 int learn_rloss_grad(const LearnArgs &a, const double *v, uint16_t *gO, hipStream_t s) {
   if (a.npix > 0) hipLaunchKernelGGL(k_rloss_grad, dim3(nb(a.npix)), dim3(RBS), 0, s, a, v, gO);
+  if (a.coarse > 1) hipLaunchKernelGGL(k_pool_grad, dim3(nb(12 * a.Sn * a.Sn)), dim3(RBS), 0, s, a, gO);
   return ok();
 }
 
@@ -652,6 +738,16 @@ int launch_d2f_vec(const double *src, float *dst, int64_t n, hipStream_t s) {
 
 int launch_step_store(const double *d, const CGScalars *sc, float *dst, int64_t n, hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(k_step_store, dim3(nb(n)), dim3(RBS), 0, s, d, sc, dst, n);
+  return ok();
+}
+
+int launch_fstep_store(const double *d, const FScalars *fs, float *dst, int64_t n, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_fstep_store, dim3(nb(n)), dim3(RBS), 0, s, d, fs, dst, n);
+  return ok();
+}
+
+int launch_fstep_tail(const double *d, const FScalars *fs, double *tail, int64_t n, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_fstep_tail, dim3(nb(n)), dim3(RBS), 0, s, d, fs, tail, n);
   return ok();
 }
 

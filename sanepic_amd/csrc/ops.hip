@@ -53,6 +53,7 @@ template <typename T, int SRC>
 __device__ __forceinline__ T load_src(const OpArgs<T> &a, int64_t gi) {
   if constexpr (SRC == SRC_GATHER) return gather_one<T>(a, gi);
   else if constexpr (SRC == SRC_RNG) return (T)normal_at(a.seed, (uint64_t)gi);
+  else if constexpr (SRC == SRC_PX) return a.px[gi];
   else return (T)a.tod[gi];
 }
 
@@ -73,16 +74,40 @@ __device__ __forceinline__ double block_sum256(double v, double *red) {
   return s;
 }
 
+// Workgroups are dispatched to the 8 XCDs round-robin (block b on XCD b % 8),
+// each XCD with its own L2.  With a.xcd set, XCD x takes one contiguous range
+// of logical blocks instead, so neighbouring column blocks -- which read and
+// write the two halves of the same 128-byte lines -- share an L2.
+__device__ __forceinline__ int64_t logical_block(int xcd) {
+  const int64_t b = blockIdx.x;
+  if (!xcd) return b;
+  const int64_t nb = gridDim.x, q = nb >> 3, r = nb & 7, x = b & 7, i = b >> 3;
+  return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
+// Output slot of sample g: its position in the pixel-sorted order when the
+// context keeps z sorted (P^T then reads z contiguously), else g itself.
+template <typename T>
+__device__ __forceinline__ int64_t zslot(const OpArgs<T> &a, int64_t g) {
+  return a.zpos ? (int64_t)a.zpos[g] : g;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_gather_px(OpArgs<T> a, T *px, int64_t ntot) {
+  const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (g < ntot) px[g] = gather_one<T>(a, g);
+}
+
 // ---------------------------------------------------------------- column fwd
 template <typename T, int LOG2N2, int SRC>
-__global__ __launch_bounds__(256) void k_col_fwd(OpArgs<T> a) {
+__device__ __forceinline__ void col_fwd_body(const OpArgs<T> &a) {
   using V = VT<T>;
   constexpr int N2 = 1 << LOG2N2, TS = N2 / 16, B = 256 / TS;
   __shared__ V lds[LdsSize<LOG2N2, B>::value];
   const int tid = threadIdx.x, b = tid % B, t = tid / B;
   const int64_t N1 = a.g.N1;
   const int64_t ncb = N1 / B;
-  const int64_t pr = blockIdx.x / ncb, cb = blockIdx.x % ncb;
+  const int64_t lb = logical_block(a.xcd), pr = lb / ncb, cb = lb % ncb;
   int det;
   int64_t ga, gb;
   pair_info(a.g, pr, det, ga, gb);
@@ -102,6 +127,18 @@ __global__ __launch_bounds__(256) void k_col_fwd(OpArgs<T> a) {
     const int64_t k2 = t + TS * s;
     out[k2 * N1 + n1] = cmul(v[s], twN<V, -1>(a.twlo, a.twhi, n1 * k2));
   }
+}
+
+template <typename T, int LOG2N2, int SRC>
+__global__ __launch_bounds__(256) void k_col_fwd(OpArgs<T> a) {
+  col_fwd_body<T, LOG2N2, SRC>(a);
+}
+
+// The same pass held to WAVES waves per SIMD (fewer registers per lane, more
+// gathers in flight per CU) for the FP32 map gather.
+template <typename T, int LOG2N2, int SRC, int WAVES>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_col_fwd_w(OpArgs<T> a) {
+  col_fwd_body<T, LOG2N2, SRC>(a);
 }
 
 // ------------------------------------------------- mixed-radix column passes
@@ -152,7 +189,7 @@ __global__ __launch_bounds__(256) void k_col_fwd_m(OpArgs<T> a) {
   const int tid = threadIdx.x, b = tid % B, t = tid / B;
   const int64_t N1 = a.g.N1;
   const int64_t ncb = N1 / B;
-  const int64_t pr = blockIdx.x / ncb, cb = blockIdx.x % ncb;
+  const int64_t lb = logical_block(a.xcd), pr = lb / ncb, cb = lb % ncb;
   int det;
   int64_t ga, gb;
   pair_info(a.g, pr, det, ga, gb);
@@ -198,7 +235,7 @@ __global__ __launch_bounds__(256) void k_col_inv_m(OpArgs<T> a) {
   const int tid = threadIdx.x, b = tid % B, t = tid / B;
   const int64_t N1 = a.g.N1;
   const int64_t ncb = N1 / B;
-  const int64_t pr = blockIdx.x / ncb, cb = blockIdx.x % ncb;
+  const int64_t lb = logical_block(a.xcd), pr = lb / ncb, cb = lb % ncb;
   int det;
   int64_t ga, gb;
   pair_info(a.g, pr, det, ga, gb);
@@ -226,8 +263,8 @@ __global__ __launch_bounds__(256) void k_col_inv_m(OpArgs<T> a) {
 #pragma unroll
     for (int s = 0; s < 16; s++) {
       const int64_t idx = n1 + N1 * (int64_t)(j + M * (t + TS * s));
-      a.z[ga + idx] = v[s].x;
-      if (gb >= 0) a.z[gb + idx] = v[s].y;
+      a.z[zslot(a, ga + idx)] = v[s].x;
+      if (gb >= 0) a.z[zslot(a, gb + idx)] = v[s].y;
     }
   }
 }
@@ -242,7 +279,7 @@ __global__ __launch_bounds__(256) void k_row(OpArgs<T> a) {
   const int tid = threadIdx.x, b = tid % B, t = tid / B;
   const int64_t N = a.g.N, N2 = a.g.N2;
   const int64_t nrb = N2 / B;
-  const int64_t pr = blockIdx.x / nrb, rb = blockIdx.x % nrb;
+  const int64_t lb = logical_block(a.xcd), pr = lb / nrb, rb = lb % nrb;
   const int64_t k2 = rb * B + b;
   int det;
   int64_t ga, gb;
@@ -296,7 +333,7 @@ __global__ __launch_bounds__(256) void k_row(OpArgs<T> a) {
   }
   if (mode & ROW_CHI2) {
     const double tot = block_sum256(chi, red);
-    if (tid == 0) a.chi2part[a.chi2off + blockIdx.x] = tot;
+    if (tid == 0) a.chi2part[a.chi2off + lb] = tot;
   }
 }
 
@@ -309,7 +346,7 @@ __global__ __launch_bounds__(256) void k_col_inv(OpArgs<T> a) {
   const int tid = threadIdx.x, b = tid % B, t = tid / B;
   const int64_t N1 = a.g.N1;
   const int64_t ncb = N1 / B;
-  const int64_t pr = blockIdx.x / ncb, cb = blockIdx.x % ncb;
+  const int64_t lb = logical_block(a.xcd), pr = lb / ncb, cb = lb % ncb;
   int det;
   int64_t ga, gb;
   pair_info(a.g, pr, det, ga, gb);
@@ -322,8 +359,8 @@ __global__ __launch_bounds__(256) void k_col_inv(OpArgs<T> a) {
 #pragma unroll
   for (int s = 0; s < 16; s++) {
     const int64_t idx = n1 + N1 * (int64_t)(t + TS * s);
-    a.z[ga + idx] = v[s].x;
-    if (gb >= 0) a.z[gb + idx] = v[s].y;
+    a.z[zslot(a, ga + idx)] = v[s].x;
+    if (gb >= 0) a.z[zslot(a, gb + idx)] = v[s].y;
   }
 }
 
@@ -392,8 +429,8 @@ __global__ __launch_bounds__(256) void k_single(OpArgs<T> a) {
 #pragma unroll
       for (int s = 0; s < 16; s++) {
         const int64_t idx = t + TS * s;
-        a.z[ga + idx] = v[s].x;
-        if (gb >= 0) a.z[gb + idx] = v[s].y;
+        a.z[zslot(a, ga + idx)] = v[s].x;
+        if (gb >= 0) a.z[zslot(a, gb + idx)] = v[s].y;
       }
     }
   }
@@ -421,6 +458,7 @@ int launch_col_fwd_m(const OpArgs<T> &a, int src, hipStream_t s) {
   case L:                                                                               \
     if (src == SRC_GATHER) return launch1(k_col_fwd_m<T, L, M, SRC_GATHER>, nb, s, a);  \
     if (src == SRC_TOD) return launch1(k_col_fwd_m<T, L, M, SRC_TOD>, nb, s, a);        \
+    if (src == SRC_PX) return launch1(k_col_fwd_m<T, L, M, SRC_PX>, nb, s, a);          \
     if constexpr (sizeof(T) == 8) return launch1(k_col_fwd_m<T, L, M, SRC_RNG>, nb, s, a); \
     return -2;
   switch (a.g.log2N2) {
@@ -451,8 +489,15 @@ int launch_col_fwd(const OpArgs<T> &a, int src, hipStream_t s) {
   const int64_t nb = a.g.npairs * (a.g.N / 4096);
 #define CF(L)                                                                      \
   case L:                                                                          \
+    if constexpr (sizeof(T) == 4) {                                                \
+      if (src == SRC_GATHER && a.waves == 3)                                       \
+        return launch1(k_col_fwd_w<T, L, SRC_GATHER, 3>, nb, s, a);                \
+      if (src == SRC_GATHER && a.waves == 4)                                       \
+        return launch1(k_col_fwd_w<T, L, SRC_GATHER, 4>, nb, s, a);                \
+    }                                                                              \
     if (src == SRC_GATHER) return launch1(k_col_fwd<T, L, SRC_GATHER>, nb, s, a);  \
     if (src == SRC_TOD) return launch1(k_col_fwd<T, L, SRC_TOD>, nb, s, a);        \
+    if (src == SRC_PX) return launch1(k_col_fwd<T, L, SRC_PX>, nb, s, a);          \
     if constexpr (sizeof(T) == 8) return launch1(k_col_fwd<T, L, SRC_RNG>, nb, s, a); \
     return -2;
   switch (a.g.log2N2) {
@@ -500,6 +545,7 @@ int launch_single(const OpArgs<T> &a, int src, hipStream_t s, int64_t *nblocks) 
   case L:                                                                         \
     if (src == SRC_GATHER) return launch1(k_single<T, L, SRC_GATHER>, nb, s, a);  \
     if (src == SRC_TOD) return launch1(k_single<T, L, SRC_TOD>, nb, s, a);        \
+    if (src == SRC_PX) return launch1(k_single<T, L, SRC_PX>, nb, s, a);          \
     if constexpr (sizeof(T) == 8) return launch1(k_single<T, L, SRC_RNG>, nb, s, a); \
     return -2;
   switch (log2n) {
@@ -509,6 +555,15 @@ int launch_single(const OpArgs<T> &a, int src, hipStream_t s, int64_t *nblocks) 
 #undef SP
 }
 
+template <typename T>
+int launch_gather_px(const OpArgs<T> &a, T *px, int64_t ntot, hipStream_t s) {
+  if (ntot <= 0) return 0;
+  hipLaunchKernelGGL(k_gather_px<T>, dim3((unsigned)((ntot + 255) / 256)), dim3(256), 0, s, a, px, ntot);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template int launch_gather_px<double>(const OpArgs<double> &, double *, int64_t, hipStream_t);
+template int launch_gather_px<float>(const OpArgs<float> &, float *, int64_t, hipStream_t);
 template int launch_col_fwd<double>(const OpArgs<double> &, int, hipStream_t);
 template int launch_row<double>(const OpArgs<double> &, hipStream_t, int64_t *);
 template int launch_col_inv<double>(const OpArgs<double> &, hipStream_t);

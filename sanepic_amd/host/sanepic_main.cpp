@@ -332,7 +332,11 @@ int main(int argc, char *argv[]) {
         fprintf(stderr, "SANEPIC_PRECOND=learned needs SANEPIC_UNET_WEIGHTS=<parameter file>\n");
         exit(1);
       }
-      rc = sanepic_unet_create(nside, cfg.device, &unet);
+      // the network's own nside (a power of two dividing the map's; default
+      // min(nside, 64), as python -m sanepic_amd.train_precond trains it)
+      long unside = nside < 64 ? nside : 64;
+      if (const char *e = getenv("SANEPIC_UNET_NSIDE")) unside = atol(e);
+      rc = sanepic_unet_create(unside, cfg.device, &unet);
       if (rc) {
         fprintf(stderr, "sanepic_unet_create: %s\n", sanepic_unet_last_error(nullptr));
         exit(1);

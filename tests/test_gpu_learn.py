@@ -212,3 +212,25 @@ def test_training_gain_over_block_jacobi(problem):
     dA = (d @ ctx.apply(d)) / (xb @ ctx.apply(xb))
     print(f"  relative A-norm difference of the two maps {dA:.2e}", flush=True)
     assert dA <= 1e-6
+    # on-policy pairs: iterates of the trained network's own flexible PCG,
+    # with the same exactness A e_k = r_k as block-Jacobi pairs
+    ctx.clear_pairs()
+    added, iters = ctx.harvest(20000, capacity=3000, policy=sanepic_amd.HARVEST_LEARNED)
+    print(f"  on-policy harvest: {added} pairs from a {iters}-iteration solve", flush=True)
+    assert 5 < added <= iters
+    npix = ctx.npix
+    cnt = ctx.get_setup()[0][:npix] // 2
+    r0 = np.linalg.norm(ctx.pair(0)[0])  # ~ |b'|: the solve starts from x = 0
+    for k in (0, 1, added // 2):
+        r, e = ctx.pair(k)
+        err = rel(_mask(ctx.apply(e), cnt, npix), r)
+        # truncation bound |r*| / |r_k| (solved to |r*|^2/|b|^2 <= 1e-17)
+        bound = np.sqrt(1e-17) * r0 / np.linalg.norm(r)
+        print(f"  on-policy pair {k}: |A e - r| / |r| = {err:.2e} (bound {bound:.2e})", flush=True)
+        assert err < 3 * bound + 1e-6
+    ctx.truncate_pairs(1)
+    assert ctx.npairs() == 1
+    with pytest.raises(sanepic_amd.SanepicError):
+        ctx.truncate_pairs(2)
+    with pytest.raises(sanepic_amd.SanepicError):
+        ctx.harvest(20001, capacity=3000, policy=7)

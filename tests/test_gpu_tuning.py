@@ -100,3 +100,24 @@ def test_graph_replay_matches_eager(data, precision):
     assert np.array_equal(c0, c1) and np.array_equal(a0, a1)
     assert np.array_equal(x0, x1)
     ctx.close()
+
+
+@pytest.mark.parametrize("env", ["SANEPIC_ZSORT", "SANEPIC_XCD_BLOCKS"])
+def test_layout_options_are_bit_identical(data, env, monkeypatch):
+    """Writing the filter output in pixel-sorted order (read contiguously by
+    P^T) and the XCD-contiguous block numbering change where values are
+    stored and which L2 serves a block, not any sum: A x, b and chi2 are
+    bit-identical with the option off."""
+    cfg, ds, P = data
+    print("layout option", env, flush=True)
+    out = []
+    for v in ("0", "1"):
+        monkeypatch.setenv(env, v)
+        for prec in (sanepic_amd.FP64, sanepic_amd.MIXED):
+            ctx = _ctx(cfg, ds, precision=prec)
+            x = np.random.default_rng(11).standard_normal(3 * ctx.npix)
+            out.append((ctx.apply(x), ctx.rhs(), ctx.chi2(x) if prec == sanepic_amd.FP64 else 0.0))
+            ctx.close()
+    for a, b in zip(out[:2], out[2:]):
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and a[2] == b[2]
+    assert rel(out[2][0], P.apply_A(np.random.default_rng(11).standard_normal(3 * out[2][0].size // 3))) < 1e-11

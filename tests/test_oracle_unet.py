@@ -132,3 +132,28 @@ def test_planted_tap_flip_is_detected():
     q[o:o + 9 * ci * co] = p[o:o + 9 * ci * co].reshape(co, 9, ci)[:, ::-1, :].reshape(-1)
     y3, _ = ou.forward(q, x)
     assert rel(y3.numpy()[..., :3], y.numpy()[..., :3]) > 20 * sens
+
+
+def test_pool_and_coarse_pos_against_loops():
+    """Pooling is the block mean written out with loops; nested pooling
+    composes; coarse_pos agrees with the block a face pixel falls in."""
+    rng = np.random.default_rng(5)
+    S, c = 8, 4
+    img = rng.standard_normal((12, S, S, 3))
+    ref = np.zeros((12, S // c, S // c, 3))
+    for f in range(12):
+        for Y in range(S // c):
+            for X in range(S // c):
+                ref[f, Y, X] = img[f, Y * c:(Y + 1) * c, X * c:(X + 1) * c].sum(axis=(0, 1)) / (c * c)
+    assert np.allclose(ou.pool(img, c), ref, rtol=1e-14, atol=0)
+    assert np.allclose(ou.pool(ou.pool(img, 2), 2), ou.pool(img, 4), rtol=1e-13, atol=1e-15)
+    pos = np.arange(12 * S * S)
+    cp = ou.coarse_pos(pos, S, c)
+    f, y, x = pos // (S * S), (pos // S) % S, pos % S
+    assert np.array_equal(cp, (f * (S // c) + y // c) * (S // c) + x // c)
+    # upsampling by coarse_pos is the transpose of the block sum (= c^2 pool)
+    v = rng.standard_normal(12 * (S // c) ** 2)
+    up = v[cp]
+    lhs = up @ img[..., 0].reshape(-1)
+    rhs = v @ (c * c * ou.pool(img[..., :1], c)).reshape(-1)
+    assert abs(lhs - rhs) <= 1e-12 * abs(rhs)
