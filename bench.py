@@ -6,15 +6,18 @@ scan, white + 1/f noise (fknee 0.1 Hz, index 1).  Synthetic inputs are
 generated in host RAM first (untimed) and copied into pinned host buffers,
 the form a caller hands to the library.
 
-Two complete solves are timed, each from inputs in host RAM to a true
-relative residual |b - A x| / |b| <= 1e-6 (H2D, pointing, ang2pix, sort, b,
-preconditioner, CG; file I/O excluded):
+Three complete solves are timed, each from inputs in pinned host memory to
+a true relative residual |b - A x| / |b| <= 1e-6 (H2D, pointing, ang2pix,
+sort, b, preconditioner, CG; file I/O excluded):
   baseline   FP64 operator, plain 3x3 block-Jacobi PCG
   headline   FP32 operator with FP64 CG scalars and FP64 residual refresh,
              flexible PCG with the learned U-Net preconditioner (trained
              offline by sanepic_amd.train_precond on disjoint seeds) and the
              monopole deflation; block-Jacobi mixed PCG if no trained
-             weights exist for the workload.
+             weights exist for the workload
+  second     the headline's pipeline with block-Jacobi instead of the
+             network ("block_jacobi_same_precision"), or with the network
+             when the headline is not the learned solver ("learned")
 Reported:
   value            TOD samples per second per CG iteration of the headline
                    solve (W warm-up iterations, then exactly K timed)

@@ -19,6 +19,8 @@
 #include <cstring>
 #include <numeric>
 #include <random>
+#include <thread>
+#include <atomic>
 #include <vector>
 
 #include "ctx.h"
@@ -462,13 +464,25 @@ int sanepic_learn_attach(sanepic_ctx *c, sanepic_unet *u) {
   const int64_t cf = S / Sn;
   std::vector<int32_t> pos((size_t)npix);
   std::vector<int32_t> inv(cf > 1 ? (size_t)(12 * S * S) : 0, -1);
-  for (int64_t p = 0; p < npix; p++) {
-    int f, x, y;
-    if (host_ring2xyf((long)S, (long)c->h_pixlist[p], &f, &x, &y))
-      return fail(c, SANEPIC_EINVAL, "the learned preconditioner needs nside a power of two");
-    pos[(size_t)p] = (int32_t)(((int64_t)f * Sn + y / cf) * Sn + x / cf);
-    if (cf > 1) inv[(size_t)(((int64_t)f * S + y) * S + x)] = (int32_t)p;
-  }
+  // ring -> nested face geometry of every seen pixel, on host threads
+  // (distinct pixels write distinct entries)
+  std::atomic<bool> bad{false};
+  const int nth = (int)std::max<int64_t>(1, std::min<int64_t>(16, npix / 65536));
+  std::vector<std::thread> pool;
+  for (int w = 0; w < nth; w++)
+    pool.emplace_back([&, w]() {
+      for (int64_t p = w; p < npix; p += nth) {
+        int f, x, y;
+        if (host_ring2xyf((long)S, (long)c->h_pixlist[p], &f, &x, &y)) {
+          bad = true;
+          return;
+        }
+        pos[(size_t)p] = (int32_t)(((int64_t)f * Sn + y / cf) * Sn + x / cf);
+        if (cf > 1) inv[(size_t)(((int64_t)f * S + y) * S + x)] = (int32_t)p;
+      }
+    });
+  for (auto &t : pool) t.join();
+  if (bad) return fail(c, SANEPIC_EINVAL, "the learned preconditioner needs nside a power of two");
   ACHK(c, dalloc(c, c->fidx, npix));
   HIPCHK(c, hipMemcpyAsync(c->fidx, pos.data(), npix * sizeof(int32_t), hipMemcpyHostToDevice, c->st));
   c->lcoarse = (int)cf;
@@ -479,16 +493,19 @@ int sanepic_learn_attach(sanepic_ctx *c, sanepic_unet *u) {
     HIPCHK(c, hipMemcpyAsync(c->linv, inv.data(), inv.size() * sizeof(int32_t), hipMemcpyHostToDevice, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));  // inv is a host temporary
   }
-  std::vector<double> p8((size_t)(8 * npix));
-  ACHK(c, d2h(c, p8.data(), c->pairs6, p8.size() * sizeof(double)));
-  double s = 0.0;
-  for (int64_t p = 0; p < npix; p++) s += p8[(size_t)(8 * p)];
-  c->b00ref = (npix > 0 && s > 0) ? s / (double)npix : 1.0;
-  c->wnorm = npix > 0 ? 1.0 / (double)npix : 1.0;  // sum_p b00_p / b00ref = npix
   ACHK(c, dalloc(c, c->fs, 1));
   ACHK(c, dalloc(c, c->zf, 3 * npix));
   ACHK(c, dalloc(c, c->qx, 3 * npix));
   HIPCHK(c, hipMemsetAsync(c->fs, 0, sizeof(FScalars), c->st));
+  {
+    LearnArgs la = learn_args(c);
+    LCHK(c, learn_b00_sum(la, c->st));
+    FScalars h{};
+    ACHK(c, d2h(c, &h, c->fs, sizeof(h)));
+    const double s = h.pad[1];
+    c->b00ref = (npix > 0 && s > 0) ? s / (double)npix : 1.0;
+  }
+  c->wnorm = npix > 0 ? 1.0 / (double)npix : 1.0;  // sum_p b00_p / b00ref = npix
   HIPCHK(c, hipStreamSynchronize(u->net.st));
   u->net.st = c->st;
   u->attached = c;

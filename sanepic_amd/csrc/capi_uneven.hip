@@ -302,6 +302,14 @@ int sanepic_set_tod_range(sanepic_ctx *c, int64_t idet, int64_t t0, int64_t n, c
   double *dst = s->tod + (int64_t)c->det_local[(size_t)idet] * len + t0;
   if (queued) {
     if (!c->up_tod) HIPCHK(c, hipStreamCreateWithFlags(&c->up_tod, hipStreamNonBlocking));
+    if (!c->tod_after_pt && c->up_pt) {  // the pointing queued so far goes first
+      hipEvent_t ev = nullptr;
+      HIPCHK(c, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+      HIPCHK(c, hipEventRecord(ev, c->up_pt));
+      HIPCHK(c, hipStreamWaitEvent(c->up_tod, ev, 0));
+      HIPCHK(c, hipEventDestroy(ev));
+      c->tod_after_pt = true;
+    }
     HIPCHK(c, hipMemcpyAsync(dst, tod, (size_t)n * sizeof(double), hipMemcpyHostToDevice, c->up_tod));
   } else {
     HIPCHK(c, hipMemcpy(dst, tod, (size_t)n * sizeof(double), hipMemcpyHostToDevice));

@@ -62,6 +62,9 @@ struct sanepic_ctx {
   int iter = 0, refresh = 10, precond = 0;
   int64_t chunk = 0;  // pairs per four-step chunk (0: all at once)
   bool pregather = false;            // P x in a separate coalesced pass before the column FFT
+  bool tod_after_pt = false;         // TOD uploads queued behind the pointing uploads
+  void *h_pin = nullptr;             // pinned host staging written by kernels (pixel counts)
+  int64_t h_pin_bytes = 0;
   bool xcd_blocks = true;            // XCD-contiguous numbering of the four-step blocks
   bool zsorted = false;              // the filter output z is written in pixel-sorted order
   int col_waves = 3;                 // occupancy target of the FP32 gather column pass (0, 3, 4)
@@ -76,6 +79,9 @@ struct sanepic_ctx {
   double kt_ms[kProfClasses] = {0};
   int64_t kt_n[kProfClasses] = {0};
   std::vector<void *> allocs;
+  // temporaries released at the next flush_frees: hipFree waits for the whole
+  // device, including uploads still in flight on the upload streams
+  std::vector<std::pair<void *, int64_t>> later;
   // Per-detector segment lengths: each group of detectors with one length is
   // a uniform sub-context that borrows this context's boresight timeline and
   // stream; the parent holds the union of seen pixels and the solver.
@@ -177,6 +183,27 @@ inline void dfree(sanepic_ctx *c, T *&p, int64_t n) {
     c->bytes -= (int64_t)((size_t)std::max<int64_t>(n, 1) * sizeof(T));
   }
   p = nullptr;
+}
+
+// Drops a temporary now and frees it at the next flush_frees (after the
+// set-up's own synchronisation, when nothing else is in flight).
+template <typename T>
+inline void dfree_later(sanepic_ctx *c, T *&p, int64_t n) {
+  if (!p) return;
+  if (std::find(c->allocs.begin(), c->allocs.end(), (void *)p) != c->allocs.end())
+    c->later.emplace_back((void *)p, (int64_t)((size_t)std::max<int64_t>(n, 1) * sizeof(T)));
+  p = nullptr;
+}
+
+inline void flush_frees(sanepic_ctx *c) {
+  for (auto &f : c->later) {
+    auto it = std::find(c->allocs.begin(), c->allocs.end(), f.first);
+    if (it == c->allocs.end()) continue;
+    c->allocs.erase(it);
+    (void)hipFree(f.first);
+    c->bytes -= f.second;
+  }
+  c->later.clear();
 }
 
 #define ACHK(c, expr)        \

@@ -139,6 +139,7 @@ struct ScatterArgs {
   const V *cs_sorted;     // sorted entry -> cos 2psi, sin 2psi
   const T *z;             // [ntot] time-domain values (in sorted order when `sorted`)
   int sorted;             // z is stored pixel-sorted: entry j at z[j]
+  int xcd;                // XCD-contiguous item ranges (neighbouring pixels share an L2)
   const int32_t *cnt;     // [npix] samples per pixel (mask: cnt < 2)
   double *q;              // [npix][3] output
   double *partial;        // [nslots][3]
@@ -162,6 +163,7 @@ struct PairSumArgs {
   const int64_t *off;     // [npix+1] pixel offsets into the sorted list
   const double *nauto;    // [ndet][ns] c2r(|SpN|)
   int64_t ns, nseg;
+  int ns_shift;           // log2(ns) when ns is a power of two, else -1
   double *out;            // [npix][8]
   double *partial;        // [nslots][8]
 };
@@ -179,6 +181,13 @@ int launch_tod_nanfill(double *tod, int64_t ntot, int64_t ns, hipStream_t s);
 int launch_histogram(const int32_t *pix, int64_t n, int32_t *counts, hipStream_t s);
 int launch_compress_pix(int32_t *pix, int64_t n, const int32_t *indpix, hipStream_t s);
 int launch_iota_u32(uint32_t *v, int64_t n, hipStream_t s);
+// Noise weights in the four-step row layout from the half spectrum
+// spn[det][0..N/2]: wt[det][k2*N1 + k1] = spn[det][min(k, N-k)], k = k2 + N2*k1;
+// wt32 (may be null) gets the float copy.
+int launch_build_weights(const double *spn, int64_t ndet, int64_t N, int64_t N1, int64_t N2, double *wt,
+                         float *wt32, hipStream_t s);
+int launch_abs(const double *src, double *dst, int64_t n, hipStream_t s);  // dst = |src|
+int launch_copy_i32(const int32_t *src, int32_t *dst, int64_t n, hipStream_t s);  // dst may be mapped host memory
 int launch_invperm(const uint32_t *perm, uint32_t *rank, int64_t n, hipStream_t s);  // rank[perm[j]] = j
 
 // CG on interleaved [npix][3] FP64 vectors with FP64 scalars on the device.

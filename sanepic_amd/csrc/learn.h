@@ -49,6 +49,8 @@ struct LearnArgs {
 template <typename T> int learn_sigma(const LearnArgs &a, const T *r, hipStream_t s);
 // X0[fidx[p]][0..15] = (M r / sigma, log-hits, block features, zero padding)
 template <typename T> int learn_features(const LearnArgs &a, const T *r, uint16_t *X0, hipStream_t s);
+// fs->pad[1] = sum over pixels of b00 (fixed-order reduction)
+int learn_b00_sum(const LearnArgs &a, hipStream_t s);
 // z = M r + sigma * OUT[fidx[p]][0..2], Q/U zero where masked
 int learn_apply_out(const LearnArgs &a, const double *r, const float *OUT, double *z, hipStream_t s);
 // Training loss of one pair (r, e): L = wnorm / sigma^2 * sum_p D_p^T (B_p / b00ref) D_p

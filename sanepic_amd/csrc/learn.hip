@@ -71,6 +71,19 @@ __device__ __forceinline__ void mvec(const double *M, const double (&r)[3], doub
 
 #define PIX_LOOP(p, n) for (int64_t p = (int64_t)blockIdx.x * RBS + threadIdx.x; p < (n); p += (int64_t)RGRID * RBS)
 
+// sum over pixels of b00 -> fs->pad[1] (feature and loss normalisation)
+__global__ __launch_bounds__(RBS) void k_b00_part(LearnArgs a) {
+  double acc[1] = {0.0};
+  PIX_LOOP(p, a.npix) acc[0] += a.blk[8 * p];
+  store_partials<1>(acc, a.part);
+}
+
+__global__ __launch_bounds__(RBS) void k_b00_final(LearnArgs a) {
+  double acc[NP];
+  sum_partials(a.part, acc);
+  if (threadIdx.x == 0) a.fs->pad[1] = acc[0];
+}
+
 // ------------------------------------------------------------- features
 template <typename T>
 __global__ __launch_bounds__(RBS) void k_sigma_part(LearnArgs a, const T *r) {
@@ -619,6 +632,12 @@ int learn_features(const LearnArgs &a, const T *r, uint16_t *X0, hipStream_t s) 
 }
 template int learn_features<double>(const LearnArgs &, const double *, uint16_t *, hipStream_t);
 template int learn_features<float>(const LearnArgs &, const float *, uint16_t *, hipStream_t);
+
+int learn_b00_sum(const LearnArgs &a, hipStream_t s) {
+  hipLaunchKernelGGL(k_b00_part, dim3(RGRID), dim3(RBS), 0, s, a);
+  hipLaunchKernelGGL(k_b00_final, dim3(1), dim3(RBS), 0, s, a);
+  return ok();
+}
 
 int learn_apply_out(const LearnArgs &a, const double *r, const float *OUT, double *z, hipStream_t s) {
   if (a.npix > 0) hipLaunchKernelGGL(k_apply_out, dim3(nb(a.npix)), dim3(RBS), 0, s, a, r, OUT, z);

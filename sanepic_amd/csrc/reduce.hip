@@ -24,11 +24,21 @@ __device__ __forceinline__ T red16(T v) {
   return v;
 }
 
+// Blocks go to the 8 XCDs round-robin; with xcd set, XCD x takes one
+// contiguous range of logical blocks, so pixels processed at the same time on
+// one L2 are neighbours in the map (their samples share z lines).
+__device__ __forceinline__ int64_t logical_block(int xcd) {
+  const int64_t b = blockIdx.x;
+  if (!xcd) return b;
+  const int64_t nb = gridDim.x, q = nb >> 3, r = nb & 7, x = b & 7, i = b >> 3;
+  return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_scatter(ScatterArgs<T> a) {
   using V = VT<T>;
   const int grp = threadIdx.x >> 4, lane = threadIdx.x & 15;
-  const int64_t item = (int64_t)blockIdx.x * 16 + grp;
+  const int64_t item = logical_block(a.xcd) * 16 + grp;
   if (item >= a.it.n) return;
   const int p = a.it.pix[item];
   const int64_t e0 = a.it.beg[item];
@@ -154,19 +164,24 @@ __global__ __launch_bounds__(256) void k_pairsum(PairSumArgs a) {
   const int64_t e0 = a.it.beg[item];
   const int n = a.it.len[item];
   const int64_t P0 = a.off[p], P1 = a.off[p + 1];
-  const int64_t ns = a.ns, half = ns / 2;
+  // sample indices are < 2^31: 32-bit block/offset arithmetic (a shift when
+  // ns is a power of two)
+  const uint32_t ns = (uint32_t)a.ns, half = ns / 2;
+  const int sh = a.ns_shift;
+  auto block_of = [&](uint32_t g) -> uint32_t { return sh >= 0 ? g >> sh : g / ns; };
   double acc[7] = {0, 0, 0, 0, 0, 0, 0};
   for (int k = lane; k < n; k += 16) {
     const int64_t j = e0 + k;
-    const int64_t g = a.perm[j];
-    const int64_t blk = g / ns, t = g % ns;
-    const double *Na = a.nauto + (blk / a.nseg) * ns;
+    const uint32_t g = a.perm[j];
+    const uint32_t blk = block_of(g), t = g - blk * ns;
+    const double *Na = a.nauto + (int64_t)(blk / (uint32_t)a.nseg) * ns;
     const double2 cj = a.cs_sorted[j];
     double u0 = 0.0, u1 = 0.0, u2 = 0.0, ut = 0.0;
     for (int64_t jj = j; jj >= P0; jj--) {
-      const int64_t gg = a.perm[jj];
-      if (gg / ns != blk) break;
-      const int64_t dt = t - gg % ns;
+      const uint32_t gg = a.perm[jj];
+      const uint32_t bb = block_of(gg);
+      if (bb != blk) break;
+      const uint32_t dt = t - (gg - bb * ns);
       const double w = Na[dt];
       const double2 ck = a.cs_sorted[jj];
       u0 += w;
@@ -175,9 +190,10 @@ __global__ __launch_bounds__(256) void k_pairsum(PairSumArgs a) {
       if (dt < half) ut += w;
     }
     for (int64_t jj = j + 1; jj < P1; jj++) {
-      const int64_t gg = a.perm[jj];
-      if (gg / ns != blk) break;
-      const int64_t dt = gg % ns - t;
+      const uint32_t gg = a.perm[jj];
+      const uint32_t bb = block_of(gg);
+      if (bb != blk) break;
+      const uint32_t dt = (gg - bb * ns) - t;
       const double w = Na[dt];
       const double2 ck = a.cs_sorted[jj];
       u0 += w;
@@ -277,6 +293,28 @@ __global__ void k_compress(int32_t *pix, int64_t n, const int32_t *indpix) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) pix[i] = indpix[pix[i]];
 }
+__global__ void k_build_weights(const double *spn, int64_t ndet, int64_t N, int64_t N1, int64_t N2, double *wt,
+                                float *wt32) {
+  const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (g >= ndet * N) return;
+  const int64_t det = g / N, i = g - det * N;
+  const int64_t k2 = i / N1, k1 = i - k2 * N1;
+  const int64_t k = k2 + N2 * k1;
+  const double v = spn[det * (N / 2 + 1) + (k < N - k ? k : N - k)];
+  wt[g] = v;
+  if (wt32) wt32[g] = (float)v;
+}
+
+__global__ void k_copy_i32(const int32_t *src, int32_t *dst, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+__global__ void k_abs(const double *src, double *dst, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) dst[i] = fabs(src[i]);
+}
+
 __global__ void k_invperm(const uint32_t *perm, uint32_t *rank, int64_t n) {
   const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (j < n) rank[perm[j]] = (uint32_t)j;
@@ -369,6 +407,22 @@ int launch_compress_pix(int32_t *pix, int64_t n, const int32_t *indpix, hipStrea
   hipLaunchKernelGGL(k_compress, dim3(nblk(n)), dim3(256), 0, s, pix, n, indpix);
   return ok();
 }
+int launch_build_weights(const double *spn, int64_t ndet, int64_t N, int64_t N1, int64_t N2, double *wt,
+                         float *wt32, hipStream_t s) {
+  if (ndet * N > 0) hipLaunchKernelGGL(k_build_weights, dim3(nblk(ndet * N)), dim3(256), 0, s, spn, ndet, N, N1, N2, wt, wt32);
+  return ok();
+}
+
+int launch_copy_i32(const int32_t *src, int32_t *dst, int64_t n, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_copy_i32, dim3(nblk(n)), dim3(256), 0, s, src, dst, n);
+  return ok();
+}
+
+int launch_abs(const double *src, double *dst, int64_t n, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_abs, dim3(nblk(n)), dim3(256), 0, s, src, dst, n);
+  return ok();
+}
+
 int launch_invperm(const uint32_t *perm, uint32_t *rank, int64_t n, hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(k_invperm, dim3(nblk(n)), dim3(256), 0, s, perm, rank, n);
   return ok();
