@@ -210,6 +210,10 @@ typedef struct sanepic_solve_opts {
   int32_t learned_iters; /* SANEPIC_PRECOND_LEARNED only: use the network
                        for the first learned_iters preconditioner applications
                        and block-Jacobi after them (0: throughout). */
+  int32_t refresh_fp32; /* SANEPIC_MIXED only: the explicit residual every
+                       `refresh` iterations uses the FP32 operator too (no
+                       chi2 then; the true residual reachable is ~1e-7 of
+                       |b|, the FP32 operator's own error).  0: FP64 refresh. */
 } sanepic_solve_opts;
 
 /* Start the PCG of sanepic.cc:1316-1410 from x0 (compressed 3 npix, NULL for

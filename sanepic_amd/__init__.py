@@ -61,7 +61,7 @@ class SolveOpts(ctypes.Structure):
     _fields_ = [("precond", ctypes.c_int32), ("refresh", ctypes.c_int32),
                 ("deflate", ctypes.c_int32), ("fallback_growth", ctypes.c_double),
                 ("graph", ctypes.c_int32), ("fcg_window", ctypes.c_int32),
-                ("learned_iters", ctypes.c_int32)]
+                ("learned_iters", ctypes.c_int32), ("refresh_fp32", ctypes.c_int32)]
 
 
 class IterInfo(ctypes.Structure):
@@ -363,13 +363,15 @@ class Context:
                   "set_tod_range")
 
     def load_uneven(self, cfg, ds, queued=False):
-        """Feeds a synth.make_uneven_dataset (or PinnedInputs of one) dataset."""
+        """Feeds a synth.make_uneven_dataset dataset (dict) or a PinnedInputs of one."""
+        get = (lambda k: ds[k]) if isinstance(ds, dict) else (lambda k: getattr(ds, k))
         for k in range(self.ndet):
             lel, xel = cfg.offsets_deg[k]
-            self.set_detector(k, lel, xel, cfg.det_angles_deg[k], ds["ispf"][k])
-        self.set_pointing_range(0, ds["ra"], ds["dec"], ds["psi"], queued)
+            self.set_detector(k, lel, xel, cfg.det_angles_deg[k], get("ispf")[k])
+        self.set_pointing_range(0, get("ra"), get("dec"), get("psi"), queued)
+        tod = get("tod")
         for k in range(self.ndet):
-            self.set_tod_range(k, 0, ds["tod"][k], queued)
+            self.set_tod_range(k, 0, tod[k], queued)
 
     def close(self):
         if self.h:
@@ -502,9 +504,9 @@ class Context:
         return out.value
 
     def solve_begin(self, precond=PRECOND_DIAG, refresh=10, x0=None, deflate=False, fallback_growth=0.0,
-                    graph=False, fcg_window=0, learned_iters=0):
+                    graph=False, fcg_window=0, learned_iters=0, refresh_fp32=False):
         opts = SolveOpts(precond, refresh, 1 if deflate else 0, fallback_growth, 1 if graph else 0,
-                         fcg_window, learned_iters)
+                         fcg_window, learned_iters, 1 if refresh_fp32 else 0)
         xp = None if x0 is None else _dp(_f64(x0))
         self._x0_keep = None if x0 is None else _f64(x0)
         self._chk(lib().sanepic_solve_begin(self.h, ctypes.byref(opts), xp), "solve_begin")

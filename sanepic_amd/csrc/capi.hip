@@ -17,6 +17,8 @@
 #include <tuple>
 #include <unordered_map>
 #include <vector>
+#include <thread>
+#include <atomic>
 
 #include "../../include/sanepic_hip.h"
 #include "ctx.h"
@@ -253,8 +255,8 @@ int run_scatter(sanepic_ctx *c, const T *z, double *q, int mask) {
 
 // q = P^T N^-1 P x (x, q interleaved, device).  chi2 -> sc->chi2 when asked.
 int apply_A(sanepic_ctx *c, const double *x, double *q, int mask, bool chi2, bool fp64) {
+  ProfScope whole(c, 4);  // groups: the parent times the whole operator (all groups on its stream)
   if (!c->sub.empty()) return group_apply_A(c, x, q, mask, chi2, fp64);
-  ProfScope whole(c, 4);
   const int mode = ROW_FWD | ROW_FILTER | ROW_INV | (chi2 ? ROW_CHI2 : 0);
   int64_t nb = 0;
   const bool use64 = fp64 || c->cfg.precision == SANEPIC_FP64;
@@ -262,6 +264,11 @@ int apply_A(sanepic_ctx *c, const double *x, double *q, int mask, bool chi2, boo
   if (use64) {
     OpArgs<double> a = op_args<double>(c);
     a.map = x;
+    if (c->pix_nested) {
+      ACHK(c, dalloc(c, c->map64, 3 * c->npix));
+      LCHK(c, launch_map_slots(x, c->mpos, c->map64, c->npix, c->st));
+      a.map = c->map64;
+    }
     int src = SRC_GATHER;
     if (c->pregather) {
       ACHK(c, dalloc(c, c->px64, c->ntot));
@@ -273,7 +280,10 @@ int apply_A(sanepic_ctx *c, const double *x, double *q, int mask, bool chi2, boo
     ACHK(c, run_filter<double>(c, a, src, mode, &nb));
     ACHK(c, run_scatter<double>(c, c->z, q, mask));
   } else {
-    LCHK(c, launch_d2f(x, c->map32, 3 * c->npix, c->st));
+    if (c->pix_nested)
+      LCHK(c, launch_map_slots(x, c->mpos, c->map32, c->npix, c->st));
+    else
+      LCHK(c, launch_d2f(x, c->map32, 3 * c->npix, c->st));
     OpArgs<float> a = op_args<float>(c);
     a.map = c->map32;
     int src = SRC_GATHER;
@@ -357,7 +367,7 @@ int enqueue_iteration(sanepic_ctx *c, bool refresh) {
   LCHK(c, cg_launch_alpha(a, c->st));
   LCHK(c, cg_launch_update(a, refresh ? 1 : 0, c->st));
   if (refresh) {
-    ACHK(c, apply_A(c, c->x, c->q, 1, true, true));
+    ACHK(c, apply_A(c, c->x, c->q, 1, !c->refresh32, !c->refresh32));
     LCHK(c, cg_launch_refresh_r(a, c->st));
   }
   LCHK(c, cg_launch_dir(a, c->st));
@@ -494,6 +504,7 @@ int sanepic_create(const sanepic_config *cfg, sanepic_ctx **out) {
   if (const char *e = getenv("SANEPIC_XCD_BLOCKS")) c->xcd_blocks = atoi(e) != 0;
   if (const char *e = getenv("SANEPIC_ZSORT")) c->zsorted = atoi(e) != 0;
   if (const char *e = getenv("SANEPIC_COL_WAVES")) c->col_waves = atoi(e);
+  if (const char *e = getenv("SANEPIC_NESTED_MAP")) c->nested_map = atoi(e) != 0;
   if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
     g_create_error = "hipStreamCreate failed";
     delete c;
@@ -804,15 +815,41 @@ int sanepic_pixelize(sanepic_ctx *c, int64_t *npix_out, int64_t *n_deferred) {
   LCHK(c, launch_copy_i32(counts, hc_dev, nsky, c->st));
   HIPCHK(c, hipStreamSynchronize(c->st));
   const int32_t *hc = (const int32_t *)c->h_pin;
-  c->h_indpix.assign(nsky, -1);
-  c->h_pixlist.clear();
-  c->h_cnt.clear();
-  for (int64_t i = 0; i < nsky; i++)
-    if (hc[i] != 0) {
-      c->h_indpix[i] = (int32_t)c->h_pixlist.size();
-      c->h_pixlist.push_back((int32_t)i);
-      c->h_cnt.push_back(hc[i]);
-    }
+  // ascending compaction of the seen pixels (904-915) on host threads: count
+  // per chunk, offsets, then each chunk fills its own range
+  {
+    const int nth = (int)std::max<int64_t>(1, std::min<int64_t>(16, nsky / (1 << 18)));
+    std::vector<int64_t> seen((size_t)nth + 1, 0);
+    auto chunk = [&](int w) { return std::make_pair(nsky * w / nth, nsky * (w + 1) / nth); };
+    std::vector<std::thread> pool;
+    for (int w = 0; w < nth; w++)
+      pool.emplace_back([&, w]() {
+        int64_t n = 0;
+        for (int64_t i = chunk(w).first; i < chunk(w).second; i++) n += hc[i] != 0;
+        seen[(size_t)w + 1] = n;
+      });
+    for (auto &t : pool) t.join();
+    pool.clear();
+    for (int w = 0; w < nth; w++) seen[(size_t)w + 1] += seen[(size_t)w];
+    c->h_indpix.resize((size_t)nsky);
+    c->h_pixlist.resize((size_t)seen[(size_t)nth]);
+    c->h_cnt.resize((size_t)seen[(size_t)nth]);
+    for (int w = 0; w < nth; w++)
+      pool.emplace_back([&, w]() {
+        int64_t k = seen[(size_t)w];
+        for (int64_t i = chunk(w).first; i < chunk(w).second; i++) {
+          if (hc[i] != 0) {
+            c->h_indpix[(size_t)i] = (int32_t)k;
+            c->h_pixlist[(size_t)k] = (int32_t)i;
+            c->h_cnt[(size_t)k] = hc[i];
+            k++;
+          } else {
+            c->h_indpix[(size_t)i] = -1;
+          }
+        }
+      });
+    for (auto &t : pool) t.join();
+  }
   const int64_t npix = (int64_t)c->h_pixlist.size();
   c->npix = npix;
   HIPCHK(c, hipMemcpy(counts, c->h_indpix.data(), nsky * sizeof(int32_t), hipMemcpyHostToDevice));
@@ -927,7 +964,7 @@ int sanepic_get_pixels(sanepic_ctx *c, int64_t idet, int64_t iseg, int64_t *out)
     return fail(c, SANEPIC_EINVAL, "index out of range");
   std::vector<int32_t> h(c->ns);
   ACHK(c, d2h(c, h.data(), c->pix + (idet * c->nseg + iseg) * c->ns, c->ns * 4));
-  for (int64_t i = 0; i < c->ns; i++) out[i] = c->h_pixlist[h[i]];
+  for (int64_t i = 0; i < c->ns; i++) out[i] = c->h_pixlist[c->pix_nested ? c->h_slot2pix[h[i]] : h[i]];
   return SANEPIC_OK;
 }
 
@@ -1043,6 +1080,55 @@ int sanepic_autotune(sanepic_ctx *c, const char *cache, int32_t *log2_n1, double
   return SANEPIC_OK;
 }
 
+namespace {
+
+// Orders the gather map by nested pixel index (a space-filling curve on each
+// base face) and renumbers the per-sample pixel stream to the map slots.  The
+// sums of the operator are unchanged (only where P reads its values moves).
+// Needs nside a power of two; otherwise the map stays in ring order.
+int nest_gather_map(sanepic_ctx *c) {
+  const int64_t npix = c->npix, S = c->nside;
+  if (npix <= 0 || (S & (S - 1)) != 0) return 0;
+  std::vector<int32_t> byidx((size_t)(12 * S * S), -1);
+  std::atomic<bool> bad{false};
+  const int nth = (int)std::max<int64_t>(1, std::min<int64_t>(16, npix / 65536));
+  std::vector<std::thread> pool;
+  for (int w = 0; w < nth; w++)
+    pool.emplace_back([&, w]() {
+      for (int64_t p = w; p < npix; p += nth) {
+        int f, x, y;
+        if (host_ring2xyf((long)S, (long)c->h_pixlist[p], &f, &x, &y)) {
+          bad = true;
+          return;
+        }
+        uint64_t m = 0;  // bit-interleaved (x, y): the nested index within the face
+        for (int b = 0; b < 31; b++)
+          m |= (uint64_t)((x >> b) & 1) << (2 * b) | (uint64_t)((y >> b) & 1) << (2 * b + 1);
+        byidx[(size_t)((int64_t)f * S * S + (int64_t)m)] = (int32_t)p;
+      }
+    });
+  for (auto &t : pool) t.join();
+  if (bad) return 0;
+  std::vector<int32_t> mpos((size_t)npix);
+  c->h_slot2pix.assign((size_t)npix, 0);
+  int32_t slot = 0;
+  for (int32_t p : byidx)
+    if (p >= 0) {
+      mpos[(size_t)p] = slot;
+      c->h_slot2pix[(size_t)slot] = p;
+      slot++;
+    }
+  if (slot != npix) return fail(c, SANEPIC_EINVAL, "nested ordering lost pixels");
+  ACHK(c, dalloc(c, c->mpos, npix));
+  HIPCHK(c, hipMemcpyAsync(c->mpos, mpos.data(), npix * sizeof(int32_t), hipMemcpyHostToDevice, c->st));
+  LCHK(c, launch_compress_pix(c->pix, c->ntot, c->mpos, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));  // mpos is a host temporary
+  c->pix_nested = true;
+  return 0;
+}
+
+}  // namespace
+
 int sanepic_setup(sanepic_ctx *c) {
   ACHK(c, check_ready(c, false));
   if (c->setup_done) return fail(c, SANEPIC_EINVAL, "sanepic_setup already ran");
@@ -1122,6 +1208,7 @@ int sanepic_setup(sanepic_ctx *c) {
     ACHK(c, dalloc(c, c->z32, ntot));
     ACHK(c, dalloc(c, c->map32, 3 * npix));
   }
+  if (c->nested_map && !c->pix_nested) ACHK(c, nest_gather_map(c));
   HIPCHK(c, hipStreamSynchronize(c->st));
   flush_frees(c);
   c->setup_done = true;
@@ -1220,6 +1307,7 @@ int sanepic_solve_begin(sanepic_ctx *c, const sanepic_solve_opts *opts, const do
     const int li = opts ? opts->learned_iters : 0;
     if (li < 0) return fail(c, SANEPIC_EINVAL, "learned_iters must be >= 0");
     c->learned_iters = li;
+    c->refresh32 = opts && opts->refresh_fp32 && c->cfg.precision == SANEPIC_MIXED;
   }
   if (c->precond != SANEPIC_PRECOND_DIAG && c->precond != SANEPIC_PRECOND_BLOCK3 &&
       c->precond != SANEPIC_PRECOND_LEARNED)
@@ -1364,6 +1452,7 @@ int sanepic_kernel_time(sanepic_ctx *c, int32_t which, double *ms, int64_t *coun
   double tot = c->kt_ms[which];
   int64_t n = c->kt_n[which];
   for (sanepic_ctx *s : c->sub) {  // groups: launches of every group
+    if (which == 4) break;         // the whole operator is timed once, on the parent
     double m = 0.0;
     int64_t k = 0;
     const int rc = sanepic_kernel_time(s, which, &m, &k);

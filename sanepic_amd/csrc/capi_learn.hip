@@ -309,7 +309,7 @@ int fpcg_step(sanepic_ctx *c, sanepic_iter_info *info, float *store, double *tai
   if (fallback) c->n_fallback++;
   const bool refresh = (c->iter % c->refresh) == 0;
   if (refresh) {
-    ACHK(c, apply_A(c, c->x, c->qx, 1, true, true));
+    ACHK(c, apply_A(c, c->x, c->qx, 1, !c->refresh32, !c->refresh32));
     LCHK(c, f_launch_refresh(fa, c->st));
   }
   const bool net_turn = c->learned_iters == 0 || c->iter + 1 < c->learned_iters;
@@ -590,6 +590,8 @@ int sanepic_learn_harvest(sanepic_ctx *c, const sanepic_harvest_opts *o, int64_t
   double *b_data = c->b;
   const bool defl_save = c->deflate;
   const int refresh_save = c->refresh, m_save = c->fcg_m, precond_save = c->precond, li_save = c->learned_iters;
+  const bool r32_save = c->refresh32;
+  c->refresh32 = false;  // pairs need the exact (FP64) residual refresh
   const double growth_save = c->fgrowth;
   c->b = c->b_train;
   c->deflate = o->deflate != 0;
@@ -681,6 +683,7 @@ int sanepic_learn_harvest(sanepic_ctx *c, const sanepic_harvest_opts *o, int64_t
   c->fgrowth = growth_save;
   c->precond = precond_save;
   c->learned_iters = li_save;
+  c->refresh32 = r32_save;
   c->begun = false;
   if (rc) return rc;
   c->npairs += kept;

@@ -88,9 +88,9 @@ int group_pixelize(sanepic_ctx *c, int64_t *npix_out, int64_t *ndef_out) {
     ACHK(c, dalloc(c, c->gmap[k], s->npix));
     HIPCHK(c, hipMemcpy(c->gmap[k], gm.data(), s->npix * sizeof(int32_t), hipMemcpyHostToDevice));
   }
-  dfree(c, c->ra, c->tl_len);
-  dfree(c, c->dec, c->tl_len);
-  dfree(c, c->psi, c->tl_len);
+  dfree_later(c, c->ra, c->tl_len);  // hipFree would wait for the TOD uploads in flight
+  dfree_later(c, c->dec, c->tl_len);
+  dfree_later(c, c->psi, c->tl_len);
   c->n_deferred = ndef;
   c->pixelized = true;
   if (npix_out) *npix_out = c->npix;
@@ -134,6 +134,7 @@ int group_setup(sanepic_ctx *c) {
   ACHK(c, dalloc(c, c->nvar, 3 * npix));
   ACHK(c, dalloc(c, c->part, 8 * 512));
   HIPCHK(c, hipStreamSynchronize(c->st));
+  flush_frees(c);
   c->setup_done = true;
   return SANEPIC_OK;
 }

@@ -63,6 +63,14 @@ struct sanepic_ctx {
   int64_t chunk = 0;  // pairs per four-step chunk (0: all at once)
   bool pregather = false;            // P x in a separate coalesced pass before the column FFT
   bool tod_after_pt = false;         // TOD uploads queued behind the pointing uploads
+  // Gather maps in nested order: after set-up, pix holds each sample's slot
+  // in a map whose compressed pixels are ordered by nested index (pixels that
+  // a scan run crosses share cache lines); mpos[p] is pixel p's slot.
+  bool nested_map = true;
+  bool pix_nested = false;
+  int32_t *mpos = nullptr;           // [npix]
+  double *map64 = nullptr;           // [npix][3] FP64 gather map in slot order
+  std::vector<int32_t> h_slot2pix;   // slot -> compressed pixel
   void *h_pin = nullptr;             // pinned host staging written by kernels (pixel counts)
   int64_t h_pin_bytes = 0;
   bool xcd_blocks = true;            // XCD-contiguous numbering of the four-step blocks
@@ -111,6 +119,7 @@ struct sanepic_ctx {
   double *Dring = nullptr, *Qring = nullptr, *fpart = nullptr;
   int fcg_m = 16, fcg_alloc_m = 0, fcg_slot = 0;
   int learned_iters = 0;             // network for the first this many z (0: always)
+  bool refresh32 = false;            // mixed precision: explicit residual with the FP32 operator
   int64_t fcg_stored = 0;           // directions stored since the start
   bool last_z_learned = false;
   int64_t n_learned = 0, n_fallback = 0, n_rejected = 0;

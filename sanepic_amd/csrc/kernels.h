@@ -188,6 +188,9 @@ int launch_build_weights(const double *spn, int64_t ndet, int64_t N, int64_t N1,
                          float *wt32, hipStream_t s);
 int launch_abs(const double *src, double *dst, int64_t n, hipStream_t s);  // dst = |src|
 int launch_copy_i32(const int32_t *src, int32_t *dst, int64_t n, hipStream_t s);  // dst may be mapped host memory
+// Gather map in slot order: map[3 slot[p] + k] = x[3 p + k] (interleaved x).
+int launch_map_slots(const double *x, const int32_t *slot, double *map, int64_t npix, hipStream_t s);
+int launch_map_slots(const double *x, const int32_t *slot, float *map, int64_t npix, hipStream_t s);
 int launch_invperm(const uint32_t *perm, uint32_t *rank, int64_t n, hipStream_t s);  // rank[perm[j]] = j
 
 // CG on interleaved [npix][3] FP64 vectors with FP64 scalars on the device.

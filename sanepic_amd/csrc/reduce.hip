@@ -310,6 +310,16 @@ __global__ void k_copy_i32(const int32_t *src, int32_t *dst, int64_t n) {
   if (i < n) dst[i] = src[i];
 }
 
+template <typename T>
+__global__ void k_map_slots(const double *x, const int32_t *slot, T *map, int64_t npix) {
+  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= npix) return;
+  T *m = map + 3 * (int64_t)slot[p];
+  m[0] = (T)x[3 * p];
+  m[1] = (T)x[3 * p + 1];
+  m[2] = (T)x[3 * p + 2];
+}
+
 __global__ void k_abs(const double *src, double *dst, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (i < n) dst[i] = fabs(src[i]);
@@ -415,6 +425,16 @@ int launch_build_weights(const double *spn, int64_t ndet, int64_t N, int64_t N1,
 
 int launch_copy_i32(const int32_t *src, int32_t *dst, int64_t n, hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(k_copy_i32, dim3(nblk(n)), dim3(256), 0, s, src, dst, n);
+  return ok();
+}
+
+int launch_map_slots(const double *x, const int32_t *slot, double *map, int64_t npix, hipStream_t s) {
+  if (npix > 0) hipLaunchKernelGGL(k_map_slots<double>, dim3(nblk(npix)), dim3(256), 0, s, x, slot, map, npix);
+  return ok();
+}
+
+int launch_map_slots(const double *x, const int32_t *slot, float *map, int64_t npix, hipStream_t s) {
+  if (npix > 0) hipLaunchKernelGGL(k_map_slots<float>, dim3(nblk(npix)), dim3(256), 0, s, x, slot, map, npix);
   return ok();
 }
 

@@ -102,12 +102,13 @@ def test_graph_replay_matches_eager(data, precision):
     ctx.close()
 
 
-@pytest.mark.parametrize("env", ["SANEPIC_ZSORT", "SANEPIC_XCD_BLOCKS"])
+@pytest.mark.parametrize("env", ["SANEPIC_ZSORT", "SANEPIC_XCD_BLOCKS", "SANEPIC_NESTED_MAP"])
 def test_layout_options_are_bit_identical(data, env, monkeypatch):
     """Writing the filter output in pixel-sorted order (read contiguously by
-    P^T) and the XCD-contiguous block numbering change where values are
-    stored and which L2 serves a block, not any sum: A x, b and chi2 are
-    bit-identical with the option off."""
+    P^T), the XCD-contiguous block numbering and the nested-order gather map
+    change where values are stored and which L2 serves a block, not any sum:
+    A x, b and chi2 are bit-identical with the option off, and the per-sample
+    pixels reported back are the same."""
     cfg, ds, P = data
     print("layout option", env, flush=True)
     out = []
@@ -116,8 +117,10 @@ def test_layout_options_are_bit_identical(data, env, monkeypatch):
         for prec in (sanepic_amd.FP64, sanepic_amd.MIXED):
             ctx = _ctx(cfg, ds, precision=prec)
             x = np.random.default_rng(11).standard_normal(3 * ctx.npix)
-            out.append((ctx.apply(x), ctx.rhs(), ctx.chi2(x) if prec == sanepic_amd.FP64 else 0.0))
+            out.append((ctx.apply(x), ctx.rhs(), ctx.chi2(x) if prec == sanepic_amd.FP64 else 0.0,
+                        ctx.pixels(1, 2)))
             ctx.close()
     for a, b in zip(out[:2], out[2:]):
         assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and a[2] == b[2]
+        assert np.array_equal(a[3], b[3])
     assert rel(out[2][0], P.apply_A(np.random.default_rng(11).standard_normal(3 * out[2][0].size // 3))) < 1e-11
