@@ -571,12 +571,14 @@ void sanepic_destroy(sanepic_ctx *c) {
   }
   drop_graphs(c);
   if (c->up_pt) (void)hipStreamSynchronize(c->up_pt);
-  if (c->up_tod) (void)hipStreamSynchronize(c->up_tod);
+  for (hipStream_t s : c->up_tod)
+    if (s) (void)hipStreamSynchronize(s);
   for (void *p : c->allocs) (void)hipFree(p);
   if (c->h_pin) (void)hipHostFree(c->h_pin);
   if (c->st && !c->borrowed) (void)hipStreamDestroy(c->st);
   if (c->up_pt) (void)hipStreamDestroy(c->up_pt);
-  if (c->up_tod) (void)hipStreamDestroy(c->up_tod);
+  for (hipStream_t s : c->up_tod)
+    if (s) (void)hipStreamDestroy(s);
   delete c;
 }
 
@@ -665,19 +667,10 @@ int sanepic_set_tod_async(sanepic_ctx *c, int64_t idet, int64_t iseg, const doub
   if (c->setup_done) return fail(c, SANEPIC_EINVAL, "TOD already consumed by sanepic_setup");
   if (!tod) return fail(c, SANEPIC_EINVAL, "null TOD buffer");
   ACHK(c, dalloc(c, c->tod, c->ntot));
-  ACHK(c, upload_stream(c, &c->up_tod));
-  if (!c->tod_after_pt && c->up_pt) {
-    // the pointing queued so far goes first at full bandwidth: pixelisation
-    // needs it, the TOD only at set-up
-    hipEvent_t ev = nullptr;
-    HIPCHK(c, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-    HIPCHK(c, hipEventRecord(ev, c->up_pt));
-    HIPCHK(c, hipStreamWaitEvent(c->up_tod, ev, 0));
-    HIPCHK(c, hipEventDestroy(ev));
-    c->tod_after_pt = true;
-  }
+  hipStream_t st = nullptr;
+  ACHK(c, tod_stream(c, idet * c->nseg + iseg, &st));
   HIPCHK(c, hipMemcpyAsync(c->tod + (idet * c->nseg + iseg) * c->ns, tod, (size_t)c->ns * sizeof(double),
-                           hipMemcpyHostToDevice, c->up_tod));
+                           hipMemcpyHostToDevice, st));
   c->tod_ok[idet * c->nseg + iseg] = 1;
   return SANEPIC_OK;
 }
@@ -1133,7 +1126,7 @@ int sanepic_setup(sanepic_ctx *c) {
   ACHK(c, check_ready(c, false));
   if (c->setup_done) return fail(c, SANEPIC_EINVAL, "sanepic_setup already ran");
   if (!c->sub.empty()) return group_setup(c);
-  if (c->up_tod) HIPCHK(c, hipStreamSynchronize(c->up_tod));
+  ACHK(c, sync_tod_uploads(c));
   for (size_t i = 0; i < c->tod_ok.size(); i++)
     if (!c->tod_ok[i]) return fail(c, SANEPIC_EINVAL, "TOD block " + std::to_string(i) + " not set");
   const int64_t npix = c->npix, ntot = c->ntot, N = c->ns;

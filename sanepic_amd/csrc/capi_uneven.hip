@@ -99,7 +99,7 @@ int group_pixelize(sanepic_ctx *c, int64_t *npix_out, int64_t *ndef_out) {
 }
 
 int group_setup(sanepic_ctx *c) {
-  if (c->up_tod) HIPCHK(c, hipStreamSynchronize(c->up_tod));
+  ACHK(c, sync_tod_uploads(c));
   for (int64_t d = 0; d < c->ndet; d++) {
     const sanepic_ctx *s = c->sub[(size_t)c->det_group[(size_t)d]];
     if (c->tod_set[(size_t)d] < s->nseg * s->ns)
@@ -302,16 +302,9 @@ int sanepic_set_tod_range(sanepic_ctx *c, int64_t idet, int64_t t0, int64_t n, c
   if (rc) return fail(c, rc, s->err);
   double *dst = s->tod + (int64_t)c->det_local[(size_t)idet] * len + t0;
   if (queued) {
-    if (!c->up_tod) HIPCHK(c, hipStreamCreateWithFlags(&c->up_tod, hipStreamNonBlocking));
-    if (!c->tod_after_pt && c->up_pt) {  // the pointing queued so far goes first
-      hipEvent_t ev = nullptr;
-      HIPCHK(c, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-      HIPCHK(c, hipEventRecord(ev, c->up_pt));
-      HIPCHK(c, hipStreamWaitEvent(c->up_tod, ev, 0));
-      HIPCHK(c, hipEventDestroy(ev));
-      c->tod_after_pt = true;
-    }
-    HIPCHK(c, hipMemcpyAsync(dst, tod, (size_t)n * sizeof(double), hipMemcpyHostToDevice, c->up_tod));
+    hipStream_t st = nullptr;
+    ACHK(c, tod_stream(c, idet, &st));
+    HIPCHK(c, hipMemcpyAsync(dst, tod, (size_t)n * sizeof(double), hipMemcpyHostToDevice, st));
   } else {
     HIPCHK(c, hipMemcpy(dst, tod, (size_t)n * sizeof(double), hipMemcpyHostToDevice));
   }

@@ -20,7 +20,10 @@ struct sanepic_ctx {
   sanepic_config cfg{};
   std::string err;
   hipStream_t st = nullptr;
-  hipStream_t up_pt = nullptr, up_tod = nullptr;  // upload streams (queued set_* calls)
+  // upload streams of the queued set_* calls: pointing, and the TOD spread
+  // over several streams (one copy engine each), queued behind the pointing
+  static constexpr int kTodStreams = 4;
+  hipStream_t up_pt = nullptr, up_tod[kTodStreams] = {};
   int64_t ns = 0, nseg = 0, ndet = 0, ntot = 0, nside = 0, npix = -1;
   PairGeom g{}, gauto{};
   bool single = false;
@@ -62,7 +65,6 @@ struct sanepic_ctx {
   int iter = 0, refresh = 10, precond = 0;
   int64_t chunk = 0;  // pairs per four-step chunk (0: all at once)
   bool pregather = false;            // P x in a separate coalesced pass before the column FFT
-  bool tod_after_pt = false;         // TOD uploads queued behind the pointing uploads
   // Gather maps in nested order: after set-up, pix holds each sample's slot
   // in a map whose compressed pixels are ordered by nested index (pixels that
   // a scan run crosses share cache lines); mpos[p] is pixel p's slot.
@@ -213,6 +215,39 @@ inline void flush_frees(sanepic_ctx *c) {
     c->bytes -= f.second;
   }
   c->later.clear();
+}
+
+// TOD upload stream k (mod kTodStreams), created on first use behind the
+// pointing uploads queued so far (pixelisation needs those first).
+inline int tod_stream(sanepic_ctx *c, int64_t k, hipStream_t *out) {
+  hipStream_t &s = c->up_tod[k % sanepic_ctx::kTodStreams];
+  if (!s) {
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(c, SANEPIC_EHIP, "hipStreamCreate failed");
+    }
+    if (c->up_pt) {
+      hipEvent_t ev = nullptr;
+      if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess || hipEventRecord(ev, c->up_pt) != hipSuccess ||
+          hipStreamWaitEvent(s, ev, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        if (ev) (void)hipEventDestroy(ev);
+        return fail(c, SANEPIC_EHIP, "upload ordering failed");
+      }
+      (void)hipEventDestroy(ev);
+    }
+  }
+  *out = s;
+  return 0;
+}
+
+inline int sync_tod_uploads(sanepic_ctx *c) {
+  for (hipStream_t s : c->up_tod)
+    if (s && hipStreamSynchronize(s) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(c, SANEPIC_EHIP, "TOD upload failed");
+    }
+  return 0;
 }
 
 #define ACHK(c, expr)        \

@@ -194,8 +194,12 @@ __global__ __launch_bounds__(256) void k_col_fwd_m(OpArgs<T> a) {
   int64_t ga, gb;
   pair_info(a.g, pr, det, ga, gb);
   const int64_t n1 = cb * B + b;
-  V zz[M][16];
-#pragma unroll
+  V *out = a.buf + pr * a.g.N;
+  // Sub-FFT j's output v goes to row P j + v of this column.  A thread later
+  // reads back exactly the rows it wrote (rows P j + t + TS i for every j),
+  // so the M x 16 values pass through its own buffer rows instead of
+  // registers, with no barrier.
+#pragma unroll 1
   for (int j = 0; j < M; j++) {
     V v[16];
 #pragma unroll
@@ -208,16 +212,16 @@ __global__ __launch_bounds__(256) void k_col_fwd_m(OpArgs<T> a) {
     if (j) __syncthreads();  // the previous sub-FFT's last LDS reads
     fft_reg<V, -1, LOG2P, B>(v, t, lds + b, a.tw4096);
 #pragma unroll
-    for (int i = 0; i < 16; i++) zz[j][i] = v[i];
+    for (int i = 0; i < 16; i++) out[((int64_t)P * j + t + TS * i) * N1 + n1] = v[i];
   }
-  V *out = a.buf + pr * a.g.N;
-#pragma unroll
+#pragma unroll 4
   for (int s = 0; s < 16; s++) {
     const int64_t vv = t + TS * s;
     V y[M], Y[M];
-    y[0] = zz[0][s];
 #pragma unroll
-    for (int j = 1; j < M; j++) y[j] = cmul(zz[j][s], twN<V, -1>(a.twlo, a.twhi, (int64_t)j * vv * N1));
+    for (int j = 0; j < M; j++) y[j] = out[((int64_t)P * j + vv) * N1 + n1];
+#pragma unroll
+    for (int j = 1; j < M; j++) y[j] = cmul(y[j], twN<V, -1>(a.twlo, a.twhi, (int64_t)j * vv * N1));
     dft_m<V, M, -1>(y, Y);
 #pragma unroll
     for (int q = 0; q < M; q++) {
@@ -240,24 +244,27 @@ __global__ __launch_bounds__(256) void k_col_inv_m(OpArgs<T> a) {
   int64_t ga, gb;
   pair_info(a.g, pr, det, ga, gb);
   const int64_t n1 = cb * B + b;
-  const V *in = a.buf + pr * a.g.N;
-  V zz[M][16];
-#pragma unroll
+  V *io = a.buf + pr * a.g.N;
+  // Radix-M DFT of rows vv + P q, written back in place as sub-column j's
+  // input at row P j + vv: each thread reads and writes the same rows of
+  // its own column, so no barrier is needed before the sub-FFTs read them.
+#pragma unroll 4
   for (int s = 0; s < 16; s++) {
     const int64_t vv = t + TS * s;
     V y[M], Z[M];
 #pragma unroll
-    for (int q = 0; q < M; q++) y[q] = in[(vv + (int64_t)P * q) * N1 + n1];
+    for (int q = 0; q < M; q++) y[q] = io[(vv + (int64_t)P * q) * N1 + n1];
     dft_m<V, M, +1>(y, Z);
-    zz[0][s] = Z[0];
+    io[vv * N1 + n1] = Z[0];
 #pragma unroll
-    for (int j = 1; j < M; j++) zz[j][s] = cmul(Z[j], twN<V, +1>(a.twlo, a.twhi, (int64_t)j * vv * N1));
+    for (int j = 1; j < M; j++)
+      io[((int64_t)P * j + vv) * N1 + n1] = cmul(Z[j], twN<V, +1>(a.twlo, a.twhi, (int64_t)j * vv * N1));
   }
-#pragma unroll
+#pragma unroll 1
   for (int j = 0; j < M; j++) {
     V v[16];
 #pragma unroll
-    for (int i = 0; i < 16; i++) v[i] = zz[j][i];
+    for (int i = 0; i < 16; i++) v[i] = io[((int64_t)P * j + t + TS * i) * N1 + n1];
     if (j) __syncthreads();
     fft_reg<V, +1, LOG2P, B>(v, t, lds + b, a.tw4096);
 #pragma unroll
