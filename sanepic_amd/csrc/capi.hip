@@ -127,16 +127,14 @@ int build_weights(sanepic_ctx *c) {
   if (c->wt) return 0;
   const int64_t N = c->ns, H = N / 2 + 1;
   // the half spectra go up; the row-layout transposition runs on the device
-  std::vector<double> half((size_t)(c->ndet * H));
-  for (int64_t det = 0; det < c->ndet; det++)
-    std::copy(c->spn.data() + det * N, c->spn.data() + det * N + H, half.data() + det * H);
   double *dspn = nullptr;
   ACHK(c, dalloc(c, dspn, c->ndet * H));
-  HIPCHK(c, hipMemcpyAsync(dspn, half.data(), half.size() * sizeof(double), hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipMemcpyAsync(dspn, c->spn.data(), (size_t)(c->ndet * H) * sizeof(double), hipMemcpyHostToDevice,
+                           c->st));
   ACHK(c, dalloc(c, c->wt, c->ndet * N));
   if (c->cfg.precision == SANEPIC_MIXED) ACHK(c, dalloc(c, c->wt32, c->ndet * N));
   LCHK(c, launch_build_weights(dspn, c->ndet, N, c->g.N1, c->g.N2, c->wt, c->wt32, c->st));
-  HIPCHK(c, hipStreamSynchronize(c->st));  // `half` is a host temporary
+  HIPCHK(c, hipStreamSynchronize(c->st));
   dfree_later(c, dspn, c->ndet * H);
   return 0;
 }
@@ -144,6 +142,48 @@ int build_weights(sanepic_ctx *c) {
 }  // namespace
 
 namespace sctx {
+
+// Ascending compaction of the seen pixels (904-915) on host threads: count
+// per chunk, offsets, then each chunk fills its own range.
+template <typename C>
+void compact_seen(const C *hc, int64_t nsky, std::vector<int32_t> &indpix, std::vector<int32_t> &pixlist,
+                  std::vector<int32_t> &cnt) {
+  const int nth = (int)std::max<int64_t>(1, std::min<int64_t>(16, nsky / (1 << 18)));
+  std::vector<int64_t> seen((size_t)nth + 1, 0);
+  auto lo = [&](int w) { return nsky * w / nth; };
+  std::vector<std::thread> pool;
+  for (int w = 0; w < nth; w++)
+    pool.emplace_back([&, w]() {
+      int64_t n = 0;
+      for (int64_t i = lo(w); i < lo(w + 1); i++) n += hc[i] != 0;
+      seen[(size_t)w + 1] = n;
+    });
+  for (auto &t : pool) t.join();
+  pool.clear();
+  for (int w = 0; w < nth; w++) seen[(size_t)w + 1] += seen[(size_t)w];
+  indpix.resize((size_t)nsky);
+  pixlist.resize((size_t)seen[(size_t)nth]);
+  cnt.resize((size_t)seen[(size_t)nth]);
+  for (int w = 0; w < nth; w++)
+    pool.emplace_back([&, w]() {
+      int64_t k = seen[(size_t)w];
+      for (int64_t i = lo(w); i < lo(w + 1); i++) {
+        if (hc[i] != 0) {
+          indpix[(size_t)i] = (int32_t)k;
+          pixlist[(size_t)k] = (int32_t)i;
+          cnt[(size_t)k] = (int32_t)hc[i];
+          k++;
+        } else {
+          indpix[(size_t)i] = -1;
+        }
+      }
+    });
+  for (auto &t : pool) t.join();
+}
+template void compact_seen<int32_t>(const int32_t *, int64_t, std::vector<int32_t> &, std::vector<int32_t> &,
+                                    std::vector<int32_t> &);
+template void compact_seen<int64_t>(const int64_t *, int64_t, std::vector<int32_t> &, std::vector<int32_t> &,
+                                    std::vector<int32_t> &);
 
 template <>
 OpArgs<double> op_args<double>(sanepic_ctx *c) {
@@ -537,7 +577,7 @@ int sanepic_create(const sanepic_config *cfg, sanepic_ctx **out) {
   c->detc.resize(c->ndet);
   c->offx.resize(c->ndet);
   c->offy.resize(c->ndet);
-  c->spn.assign((size_t)c->ndet * ns, 0.0);
+  c->spn.assign((size_t)c->ndet * (ns / 2 + 1), 0.0);  // half spectra
   c->det_ok.assign(c->ndet, 0);
   c->pt_ok.assign(c->nseg, 0);
   c->tod_ok.assign((size_t)c->ndet * c->nseg, 0);
@@ -603,7 +643,7 @@ int sanepic_set_detector(sanepic_ctx *c, int64_t idet, double lel_deg, double xe
                  &d.cos_th, &d.drp2);
   c->offx[idet] = offx;
   c->offy[idet] = offy;
-  host_spn_row(ispf, nssp, ns, c->spn.data() + idet * ns);
+  host_spn_row(ispf, nssp, ns, ns / 2 + 1, c->spn.data() + idet * (ns / 2 + 1));
   c->det_ok[idet] = 1;
   return SANEPIC_OK;
 }
@@ -687,6 +727,39 @@ int sanepic_set_tod(sanepic_ctx *c, int64_t idet, int64_t iseg, const double *to
   c->tod_ok[idet * c->nseg + iseg] = 1;
   return SANEPIC_OK;
 }
+
+namespace {
+
+// Pinned host staging of at least `bytes` (kept for the context's lifetime:
+// freeing pinned memory would wait for the uploads in flight).
+int need_pinned(sanepic_ctx *c, int64_t bytes) {
+  if (c->h_pin && c->h_pin_bytes >= bytes) return 0;
+  if (c->h_pin) (void)hipHostFree(c->h_pin);
+  c->h_pin = nullptr;
+  c->h_pin_bytes = 0;
+  HIPCHK(c, hipHostMalloc(&c->h_pin, (size_t)bytes, hipHostMallocDefault));
+  c->h_pin_bytes = bytes;
+  return 0;
+}
+
+// Host -> device copy during the set-up: through the pinned staging and a
+// copy kernel that reads host memory, so it does not queue behind the input
+// uploads still running on the copy engines.
+int h2d_staged(sanepic_ctx *c, void *dst, const void *src, int64_t bytes) {
+  if (bytes <= 0) return 0;
+  if (bytes % 4 != 0 || !c->h_pin || c->h_pin_bytes < bytes) {
+    HIPCHK(c, hipMemcpy(dst, src, (size_t)bytes, hipMemcpyHostToDevice));
+    return 0;
+  }
+  std::memcpy(c->h_pin, src, (size_t)bytes);
+  int32_t *hs = nullptr;
+  HIPCHK(c, hipHostGetDevicePointer((void **)&hs, c->h_pin, 0));
+  LCHK(c, launch_copy_i32(hs, (int32_t *)dst, bytes / 4, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));  // the staging is reused by the next copy
+  return 0;
+}
+
+}  // namespace
 
 int sanepic_pixelize(sanepic_ctx *c, int64_t *npix_out, int64_t *n_deferred) {
   if (!c) return SANEPIC_EINVAL;
@@ -796,63 +869,23 @@ int sanepic_pixelize(sanepic_ctx *c, int64_t *npix_out, int64_t *n_deferred) {
   LCHK(c, launch_histogram(c->pix, ntot, counts, c->st));
   // the counts come back through a kernel writing pinned host memory: a DMA
   // copy would queue behind the TOD upload still running on the copy engines
-  if (!c->h_pin || c->h_pin_bytes < nsky * (int64_t)sizeof(int32_t)) {
-    if (c->h_pin) (void)hipHostFree(c->h_pin);
-    c->h_pin = nullptr;
-    c->h_pin_bytes = 0;
-    HIPCHK(c, hipHostMalloc(&c->h_pin, (size_t)nsky * sizeof(int32_t), hipHostMallocDefault));
-    c->h_pin_bytes = nsky * (int64_t)sizeof(int32_t);
-  }
+  ACHK(c, need_pinned(c, nsky * (int64_t)sizeof(int64_t)));  // also stages the uploads below
   int32_t *hc_dev = nullptr;
   HIPCHK(c, hipHostGetDevicePointer((void **)&hc_dev, c->h_pin, 0));
   LCHK(c, launch_copy_i32(counts, hc_dev, nsky, c->st));
   HIPCHK(c, hipStreamSynchronize(c->st));
   const int32_t *hc = (const int32_t *)c->h_pin;
-  // ascending compaction of the seen pixels (904-915) on host threads: count
-  // per chunk, offsets, then each chunk fills its own range
-  {
-    const int nth = (int)std::max<int64_t>(1, std::min<int64_t>(16, nsky / (1 << 18)));
-    std::vector<int64_t> seen((size_t)nth + 1, 0);
-    auto chunk = [&](int w) { return std::make_pair(nsky * w / nth, nsky * (w + 1) / nth); };
-    std::vector<std::thread> pool;
-    for (int w = 0; w < nth; w++)
-      pool.emplace_back([&, w]() {
-        int64_t n = 0;
-        for (int64_t i = chunk(w).first; i < chunk(w).second; i++) n += hc[i] != 0;
-        seen[(size_t)w + 1] = n;
-      });
-    for (auto &t : pool) t.join();
-    pool.clear();
-    for (int w = 0; w < nth; w++) seen[(size_t)w + 1] += seen[(size_t)w];
-    c->h_indpix.resize((size_t)nsky);
-    c->h_pixlist.resize((size_t)seen[(size_t)nth]);
-    c->h_cnt.resize((size_t)seen[(size_t)nth]);
-    for (int w = 0; w < nth; w++)
-      pool.emplace_back([&, w]() {
-        int64_t k = seen[(size_t)w];
-        for (int64_t i = chunk(w).first; i < chunk(w).second; i++) {
-          if (hc[i] != 0) {
-            c->h_indpix[(size_t)i] = (int32_t)k;
-            c->h_pixlist[(size_t)k] = (int32_t)i;
-            c->h_cnt[(size_t)k] = hc[i];
-            k++;
-          } else {
-            c->h_indpix[(size_t)i] = -1;
-          }
-        }
-      });
-    for (auto &t : pool) t.join();
-  }
+  compact_seen(hc, nsky, c->h_indpix, c->h_pixlist, c->h_cnt);
   const int64_t npix = (int64_t)c->h_pixlist.size();
   c->npix = npix;
-  HIPCHK(c, hipMemcpy(counts, c->h_indpix.data(), nsky * sizeof(int32_t), hipMemcpyHostToDevice));
+  ACHK(c, h2d_staged(c, counts, c->h_indpix.data(), nsky * (int64_t)sizeof(int32_t)));
   LCHK(c, launch_compress_pix(c->pix, ntot, counts, c->st));
   ACHK(c, dalloc(c, c->cnt, npix));
-  HIPCHK(c, hipMemcpy(c->cnt, c->h_cnt.data(), npix * sizeof(int32_t), hipMemcpyHostToDevice));
+  ACHK(c, h2d_staged(c, c->cnt, c->h_cnt.data(), npix * (int64_t)sizeof(int32_t)));
   std::vector<int64_t> off(npix + 1, 0);
   for (int64_t p = 0; p < npix; p++) off[p + 1] = off[p] + c->h_cnt[p];
   ACHK(c, dalloc(c, c->off, npix + 1));
-  HIPCHK(c, hipMemcpy(c->off, off.data(), (npix + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
+  ACHK(c, h2d_staged(c, c->off, off.data(), (npix + 1) * (int64_t)sizeof(int64_t)));
   HIPCHK(c, hipStreamSynchronize(c->st));
   dfree_later(c, counts, nsky);
   // stable sort of samples by compressed pixel -> perm
@@ -897,6 +930,10 @@ int sanepic_pixelize(sanepic_ctx *c, int64_t *npix_out, int64_t *n_deferred) {
   // P^T work items: pixels, split into <= kChunk-entry chunks
   std::vector<int32_t> ipix, ilen, islot, spix, sfirst, scount;
   std::vector<int64_t> ibeg;
+  ipix.reserve((size_t)npix + 1024);
+  ilen.reserve((size_t)npix + 1024);
+  islot.reserve((size_t)npix + 1024);
+  ibeg.reserve((size_t)npix + 1024);
   int32_t nslots = 0;
   for (int64_t p = 0; p < npix; p++) {
     const int64_t n = c->h_cnt[p];
@@ -926,10 +963,11 @@ int sanepic_pixelize(sanepic_ctx *c, int64_t *npix_out, int64_t *n_deferred) {
   ACHK(c, dalloc(c, c->sp_pix, nsp));
   ACHK(c, dalloc(c, c->sp_first, nsp));
   ACHK(c, dalloc(c, c->sp_count, nsp));
-  HIPCHK(c, hipMemcpy(c->it_pix, ipix.data(), nit * 4, hipMemcpyHostToDevice));
-  HIPCHK(c, hipMemcpy(c->it_beg, ibeg.data(), nit * 8, hipMemcpyHostToDevice));
-  HIPCHK(c, hipMemcpy(c->it_len, ilen.data(), nit * 4, hipMemcpyHostToDevice));
-  HIPCHK(c, hipMemcpy(c->it_slot, islot.data(), nit * 4, hipMemcpyHostToDevice));
+  ACHK(c, need_pinned(c, nit * 8));
+  ACHK(c, h2d_staged(c, c->it_pix, ipix.data(), nit * 4));
+  ACHK(c, h2d_staged(c, c->it_beg, ibeg.data(), nit * 8));
+  ACHK(c, h2d_staged(c, c->it_len, ilen.data(), nit * 4));
+  ACHK(c, h2d_staged(c, c->it_slot, islot.data(), nit * 4));
   if (nsp) {
     HIPCHK(c, hipMemcpy(c->sp_pix, spix.data(), nsp * 4, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->sp_first, sfirst.data(), nsp * 4, hipMemcpyHostToDevice));

@@ -68,15 +68,7 @@ int group_pixelize(sanepic_ctx *c, int64_t *npix_out, int64_t *ndef_out) {
   std::vector<int64_t> hc((size_t)nsky, 0);
   for (sanepic_ctx *s : c->sub)
     for (int64_t p = 0; p < s->npix; p++) hc[(size_t)s->h_pixlist[p]] += s->h_cnt[p];
-  c->h_indpix.assign((size_t)nsky, -1);
-  c->h_pixlist.clear();
-  c->h_cnt.clear();
-  for (int64_t i = 0; i < nsky; i++)
-    if (hc[(size_t)i]) {
-      c->h_indpix[(size_t)i] = (int32_t)c->h_pixlist.size();
-      c->h_pixlist.push_back((int32_t)i);
-      c->h_cnt.push_back((int32_t)hc[(size_t)i]);
-    }
+  compact_seen(hc.data(), nsky, c->h_indpix, c->h_pixlist, c->h_cnt);
   c->npix = (int64_t)c->h_pixlist.size();
   ACHK(c, dalloc(c, c->cnt, c->npix));
   HIPCHK(c, hipMemcpy(c->cnt, c->h_cnt.data(), c->npix * sizeof(int32_t), hipMemcpyHostToDevice));

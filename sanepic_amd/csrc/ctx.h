@@ -258,6 +258,9 @@ inline int sync_tod_uploads(sanepic_ctx *c) {
 
 
 // Filter / scatter passes and the operator (capi.hip).
+template <typename C>
+void compact_seen(const C *hc, int64_t nsky, std::vector<int32_t> &indpix, std::vector<int32_t> &pixlist,
+                  std::vector<int32_t> &cnt);
 template <typename T> OpArgs<T> op_args(sanepic_ctx *c);
 template <> OpArgs<double> op_args<double>(sanepic_ctx *c);
 template <> OpArgs<float> op_args<float>(sanepic_ctx *c);

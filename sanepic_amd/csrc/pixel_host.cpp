@@ -8,6 +8,8 @@
 // the reference's pixel whenever libm is glibc.
 #include <algorithm>
 #include <cmath>
+#include <thread>
+#include <vector>
 #include <cstdint>
 
 #include "pixel_host.h"
@@ -88,10 +90,21 @@ void host_det_const(double lel, double xel, double angl, double *offx, double *o
   *drp2 = -std::sin(*th_) * std::cos(*ph_);
 }
 
-void host_spn_row(const double *ispf, long nssp, long ns, double *row) {
-  // 533-537, including the truncating index expression.
-  for (long ii = 0; ii < ns; ii++) row[ii] = ispf[long(double(ii) / nssp * ns)] / double(ns) / double(ns);
-  row[0] = ispf[1] * 1e-6 / double(ns) / double(ns);
+void host_spn_row(const double *ispf, long nssp, long ns, long count, double *row) {
+  // 533-537, including the truncating index expression; long rows are
+  // split over host threads (each entry is independent)
+  auto fill = [&](long a, long b) {
+    for (long ii = a; ii < b; ii++) row[ii] = ispf[long(double(ii) / nssp * ns)] / double(ns) / double(ns);
+  };
+  const long nth = count >= (1L << 18) ? 8 : 1;
+  if (nth == 1) {
+    fill(0, count);
+  } else {
+    std::vector<std::thread> pool;
+    for (long w = 0; w < nth; w++) pool.emplace_back(fill, count * w / nth, count * (w + 1) / nth);
+    for (auto &t : pool) t.join();
+  }
+  if (count > 0) row[0] = ispf[1] * 1e-6 / double(ns) / double(ns);
 }
 
 namespace {

@@ -11,8 +11,10 @@ long host_sample_pixel(long nside, double offx, double offy, double ra, double d
 void host_det_const(double lel, double xel, double angl, double *offx, double *offy, double *psidet,
                     double *th_, double *ph_, double *sin_th, double *cos_th, double *drp2);
 
-// SpN_all diagonal row of one detector (533-537).
-void host_spn_row(const double *ispf, long nssp, long ns, double *row);
+// SpN_all diagonal row of one detector (533-537): its first `count` entries
+// (the weights are even, W[k] = SpN[min(k, ns - k)], so count = ns/2 + 1
+// serves the whole spectrum).
+void host_spn_row(const double *ispf, long nssp, long ns, long count, double *row);
 
 // Nested base-face coordinates of a RING pixel (nside a power of two): face
 // f in 0..11 and the pixel's column x and row y inside the face, from the
