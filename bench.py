@@ -105,7 +105,7 @@ def cpu_baseline(wl, budget_s=20.0):
 
 
 def run_solve(cfg, pin, precision, precond, deflate, weights, steps, warmup, maxiter, profile, net_nside=None,
-              refresh_fp32=False):
+              refresh_fp32=False, refresh=10):
     """One timed pipeline: create -> uploads -> pixelize -> setup -> solve to a
     true residual <= 1e-6.  Returns timings, per-iteration time and kernel
     times of `steps` iterations measured after `warmup`."""
@@ -131,7 +131,7 @@ def run_solve(cfg, pin, precision, precond, deflate, weights, steps, warmup, max
     t_setup = time.perf_counter()
     if un is not None:
         ctx.learn_attach(un)
-    ctx.solve_begin(pc, 10, None, deflate, refresh_fp32=refresh_fp32)
+    ctx.solve_begin(pc, refresh, None, deflate, refresh_fp32=refresh_fp32)
     t_begin = time.perf_counter()
     # time to solution: iterate until crit <= 1e-12, then confirm the true
     # residual with an explicit A x
@@ -152,7 +152,7 @@ def run_solve(cfg, pin, precision, precond, deflate, weights, steps, warmup, max
     # `steps` iterations with the kernel timers on
     kt, ms_per_step = None, None
     if profile:
-        ctx.solve_begin(pc, 10, None, deflate, refresh_fp32=refresh_fp32)
+        ctx.solve_begin(pc, refresh, None, deflate, refresh_fp32=refresh_fp32)
         for _ in range(warmup):
             ctx.step()
         ctx.set_profiling(True)
@@ -195,6 +195,9 @@ def main():
                     help="skip the second pipeline (mixed block-Jacobi beside a learned headline, the "
                          "learned solver beside any other)")
     ap.add_argument("--no-deflate", action="store_true")
+    ap.add_argument("--mixed-refresh", type=int, default=20,
+                    help="FP64 explicit residual every this many iterations in the mixed-precision pipelines "
+                         "(the FP64 baseline keeps the reference's 10)")
     ap.add_argument("--no-baseline", action="store_true", help="skip the FP64 block-Jacobi solve")
     ap.add_argument("--maxiter", type=int, default=600)
     ap.add_argument("--no-cpu", action="store_true")
@@ -237,18 +240,19 @@ def main():
     if not args.no_baseline:
         base = run_solve(cfg, pin, "fp64", "block", False, None, args.steps, args.warmup,
                          args.maxiter, profile=False)
+    rf = args.mixed_refresh if args.precision == "mixed" else 10
     head = run_solve(cfg, pin, args.precision, precond, deflate, weights, args.steps, args.warmup,
-                     args.maxiter, profile=True, net_nside=(record or {}).get("net_nside"))
+                     args.maxiter, profile=True, net_nside=(record or {}).get("net_nside"), refresh=rf)
     learned = second_block = None
     if not args.no_second:
         if precond != "learned" and weights is not None:
             # the same pipeline with the trained network (reported beside the headline)
             learned = run_solve(cfg, pin, args.precision, "learned", deflate, weights, 2, 1, args.maxiter,
-                                profile=False, net_nside=record.get("net_nside"))
+                                profile=False, net_nside=record.get("net_nside"), refresh=rf)
         elif precond == "learned":
             # the same pipeline with block-Jacobi: what the network buys
             second_block = run_solve(cfg, pin, args.precision, "block", deflate, None, 2, 1, args.maxiter,
-                                     profile=False)
+                                     profile=False, refresh=rf)
     pin.close()
 
     kt = head["kernel_times"]
@@ -290,7 +294,7 @@ def main():
                                + (" + monopole deflation" if deflate else ""),
                    "nside": cfg.nside, "ndet": cfg.ndet, "samples": ntot,
                    "segment": sorted(set(lens)) if uneven else cfg.ns, "npix": head["npix"], "precond": precond,
-                   "deflate": deflate, "precision": args.precision},
+                   "deflate": deflate, "precision": args.precision, "fp64_refresh_every": rf},
         "time_to_1e-6_s": tts,
         "iterations_to_1e-6": head["iterations"] if head["converged"] else None,
         "final_true_residual": head["final_true_residual"],
