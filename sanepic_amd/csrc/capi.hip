@@ -545,6 +545,7 @@ int sanepic_create(const sanepic_config *cfg, sanepic_ctx **out) {
   if (const char *e = getenv("SANEPIC_ZSORT")) c->zsorted = atoi(e) != 0;
   if (const char *e = getenv("SANEPIC_COL_WAVES")) c->col_waves = atoi(e);
   if (const char *e = getenv("SANEPIC_NESTED_MAP")) c->nested_map = atoi(e) != 0;
+  if (const char *e = getenv("SANEPIC_ITEM_ORDER")) c->items_nested = atoi(e) != 0;
   if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
     g_create_error = "hipStreamCreate failed";
     delete c;
@@ -730,6 +731,47 @@ int sanepic_set_tod(sanepic_ctx *c, int64_t idet, int64_t iseg, const double *to
 
 namespace {
 
+// NESTED-index order of the seen pixels (a space-filling curve on each base
+// face): h_mpos[p] = slot of compressed pixel p, h_slot2pix the inverse.
+// Leaves both empty when nside is not a power of two.
+int nested_order(sanepic_ctx *c) {
+  c->h_mpos.clear();
+  c->h_slot2pix.clear();
+  const int64_t npix = c->npix, S = c->nside;
+  if (npix <= 0 || (S & (S - 1)) != 0) return 0;
+  std::vector<int32_t> byidx((size_t)(12 * S * S), -1);
+  std::atomic<bool> bad{false};
+  const int nth = (int)std::max<int64_t>(1, std::min<int64_t>(16, npix / 65536));
+  std::vector<std::thread> pool;
+  for (int w = 0; w < nth; w++)
+    pool.emplace_back([&, w]() {
+      for (int64_t p = w; p < npix; p += nth) {
+        int f, x, y;
+        if (host_ring2xyf((long)S, (long)c->h_pixlist[p], &f, &x, &y)) {
+          bad = true;
+          return;
+        }
+        uint64_t m = 0;  // bit-interleaved (x, y): the nested index within the face
+        for (int b = 0; b < 31; b++)
+          m |= (uint64_t)((x >> b) & 1) << (2 * b) | (uint64_t)((y >> b) & 1) << (2 * b + 1);
+        byidx[(size_t)((int64_t)f * S * S + (int64_t)m)] = (int32_t)p;
+      }
+    });
+  for (auto &t : pool) t.join();
+  if (bad) return 0;
+  c->h_mpos.assign((size_t)npix, 0);
+  c->h_slot2pix.assign((size_t)npix, 0);
+  int32_t slot = 0;
+  for (int32_t p : byidx)
+    if (p >= 0) {
+      c->h_mpos[(size_t)p] = slot;
+      c->h_slot2pix[(size_t)slot] = p;
+      slot++;
+    }
+  if (slot != npix) return fail(c, SANEPIC_EINVAL, "nested ordering lost pixels");
+  return 0;
+}
+
 // Pinned host staging of at least `bytes` (kept for the context's lifetime:
 // freeing pinned memory would wait for the uploads in flight).
 int need_pinned(sanepic_ctx *c, int64_t bytes) {
@@ -878,6 +920,7 @@ int sanepic_pixelize(sanepic_ctx *c, int64_t *npix_out, int64_t *n_deferred) {
   compact_seen(hc, nsky, c->h_indpix, c->h_pixlist, c->h_cnt);
   const int64_t npix = (int64_t)c->h_pixlist.size();
   c->npix = npix;
+  if (c->nested_map || c->items_nested) ACHK(c, nested_order(c));
   ACHK(c, h2d_staged(c, counts, c->h_indpix.data(), nsky * (int64_t)sizeof(int32_t)));
   LCHK(c, launch_compress_pix(c->pix, ntot, counts, c->st));
   ACHK(c, dalloc(c, c->cnt, npix));
@@ -935,7 +978,12 @@ int sanepic_pixelize(sanepic_ctx *c, int64_t *npix_out, int64_t *n_deferred) {
   islot.reserve((size_t)npix + 1024);
   ibeg.reserve((size_t)npix + 1024);
   int32_t nslots = 0;
-  for (int64_t p = 0; p < npix; p++) {
+  // items in NESTED order when enabled: pixels reduced at the same time are
+  // neighbours on the sky, whose runs of z share cache lines (each pixel's
+  // sums are unchanged)
+  const bool nest_items = c->items_nested && !c->h_slot2pix.empty();
+  for (int64_t q = 0; q < npix; q++) {
+    const int64_t p = nest_items ? c->h_slot2pix[(size_t)q] : q;
     const int64_t n = c->h_cnt[p];
     if (n <= kChunk) {
       ipix.push_back((int32_t)p);
@@ -1118,42 +1166,12 @@ namespace {
 // sums of the operator are unchanged (only where P reads its values moves).
 // Needs nside a power of two; otherwise the map stays in ring order.
 int nest_gather_map(sanepic_ctx *c) {
-  const int64_t npix = c->npix, S = c->nside;
-  if (npix <= 0 || (S & (S - 1)) != 0) return 0;
-  std::vector<int32_t> byidx((size_t)(12 * S * S), -1);
-  std::atomic<bool> bad{false};
-  const int nth = (int)std::max<int64_t>(1, std::min<int64_t>(16, npix / 65536));
-  std::vector<std::thread> pool;
-  for (int w = 0; w < nth; w++)
-    pool.emplace_back([&, w]() {
-      for (int64_t p = w; p < npix; p += nth) {
-        int f, x, y;
-        if (host_ring2xyf((long)S, (long)c->h_pixlist[p], &f, &x, &y)) {
-          bad = true;
-          return;
-        }
-        uint64_t m = 0;  // bit-interleaved (x, y): the nested index within the face
-        for (int b = 0; b < 31; b++)
-          m |= (uint64_t)((x >> b) & 1) << (2 * b) | (uint64_t)((y >> b) & 1) << (2 * b + 1);
-        byidx[(size_t)((int64_t)f * S * S + (int64_t)m)] = (int32_t)p;
-      }
-    });
-  for (auto &t : pool) t.join();
-  if (bad) return 0;
-  std::vector<int32_t> mpos((size_t)npix);
-  c->h_slot2pix.assign((size_t)npix, 0);
-  int32_t slot = 0;
-  for (int32_t p : byidx)
-    if (p >= 0) {
-      mpos[(size_t)p] = slot;
-      c->h_slot2pix[(size_t)slot] = p;
-      slot++;
-    }
-  if (slot != npix) return fail(c, SANEPIC_EINVAL, "nested ordering lost pixels");
+  if (c->h_mpos.empty()) return 0;
+  const int64_t npix = c->npix;
   ACHK(c, dalloc(c, c->mpos, npix));
-  HIPCHK(c, hipMemcpyAsync(c->mpos, mpos.data(), npix * sizeof(int32_t), hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipMemcpyAsync(c->mpos, c->h_mpos.data(), npix * sizeof(int32_t), hipMemcpyHostToDevice, c->st));
   LCHK(c, launch_compress_pix(c->pix, c->ntot, c->mpos, c->st));
-  HIPCHK(c, hipStreamSynchronize(c->st));  // mpos is a host temporary
+  HIPCHK(c, hipStreamSynchronize(c->st));
   c->pix_nested = true;
   return 0;
 }
@@ -1240,6 +1258,7 @@ int sanepic_setup(sanepic_ctx *c) {
     ACHK(c, dalloc(c, c->map32, 3 * npix));
   }
   if (c->nested_map && !c->pix_nested) ACHK(c, nest_gather_map(c));
+  if (!c->nested_map) c->h_slot2pix.clear();  // pixels stay in ring order (get_pixels)
   HIPCHK(c, hipStreamSynchronize(c->st));
   flush_frees(c);
   c->setup_done = true;

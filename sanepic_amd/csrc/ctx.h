@@ -73,6 +73,8 @@ struct sanepic_ctx {
   int32_t *mpos = nullptr;           // [npix]
   double *map64 = nullptr;           // [npix][3] FP64 gather map in slot order
   std::vector<int32_t> h_slot2pix;   // slot -> compressed pixel
+  std::vector<int32_t> h_mpos;       // compressed pixel -> slot
+  bool items_nested = true;          // P^T items in NESTED pixel order
   void *h_pin = nullptr;             // pinned host staging written by kernels (pixel counts)
   int64_t h_pin_bytes = 0;
   bool xcd_blocks = true;            // XCD-contiguous numbering of the four-step blocks

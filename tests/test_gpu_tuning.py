@@ -102,7 +102,7 @@ def test_graph_replay_matches_eager(data, precision):
     ctx.close()
 
 
-@pytest.mark.parametrize("env", ["SANEPIC_ZSORT", "SANEPIC_XCD_BLOCKS", "SANEPIC_NESTED_MAP"])
+@pytest.mark.parametrize("env", ["SANEPIC_ZSORT", "SANEPIC_XCD_BLOCKS", "SANEPIC_NESTED_MAP", "SANEPIC_ITEM_ORDER"])
 def test_layout_options_are_bit_identical(data, env, monkeypatch):
     """Writing the filter output in pixel-sorted order (read contiguously by
     P^T), the XCD-contiguous block numbering and the nested-order gather map
