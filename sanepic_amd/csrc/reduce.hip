@@ -170,44 +170,63 @@ __global__ __launch_bounds__(256) void k_pairsum(PairSumArgs a) {
   const int sh = a.ns_shift;
   auto block_of = [&](uint32_t g) -> uint32_t { return sh >= 0 ? g >> sh : g / ns; };
   double acc[7] = {0, 0, 0, 0, 0, 0, 0};
-  for (int k = lane; k < n; k += 16) {
-    const int64_t j = e0 + k;
+  // The block is symmetric (w depends on |dt|), so entry j takes itself and
+  // the later entries k > j of its group once, with the off-diagonal terms
+  // doubled: w (a_j a_k^T + a_k a_j^T).  The forward scan issues its loads
+  // four entries at a time.
+  for (int kk = lane; kk < n; kk += 16) {
+    const int64_t j = e0 + kk;
     const uint32_t g = a.perm[j];
     const uint32_t blk = block_of(g), t = g - blk * ns;
     const double *Na = a.nauto + (int64_t)(blk / (uint32_t)a.nseg) * ns;
     const double2 cj = a.cs_sorted[j];
     double u0 = 0.0, u1 = 0.0, u2 = 0.0, ut = 0.0;
-    for (int64_t jj = j; jj >= P0; jj--) {
-      const uint32_t gg = a.perm[jj];
-      const uint32_t bb = block_of(gg);
-      if (bb != blk) break;
-      const uint32_t dt = t - (gg - bb * ns);
-      const double w = Na[dt];
-      const double2 ck = a.cs_sorted[jj];
-      u0 += w;
-      u1 += w * ck.x;
-      u2 += w * ck.y;
-      if (dt < half) ut += w;
+    int64_t k = j + 1;
+    for (; k + 3 < P1; k += 4) {
+      uint32_t gg[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) gg[u] = a.perm[k + u];
+      bool inside = true;
+#pragma unroll
+      for (int u = 0; u < 4; u++) inside = inside && block_of(gg[u]) == blk;
+      if (!inside) break;
+      double w[4];
+      double2 ck[4];
+      uint32_t dt[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        dt[u] = (gg[u] - blk * ns) - t;
+        w[u] = Na[dt[u]];
+        ck[u] = a.cs_sorted[k + u];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        u0 += w[u];
+        u1 += w[u] * ck[u].x;
+        u2 += w[u] * ck[u].y;
+        if (dt[u] < half) ut += w[u];
+      }
     }
-    for (int64_t jj = j + 1; jj < P1; jj++) {
-      const uint32_t gg = a.perm[jj];
+    for (; k < P1; k++) {
+      const uint32_t gg = a.perm[k];
       const uint32_t bb = block_of(gg);
       if (bb != blk) break;
       const uint32_t dt = (gg - bb * ns) - t;
       const double w = Na[dt];
-      const double2 ck = a.cs_sorted[jj];
+      const double2 ck = a.cs_sorted[k];
       u0 += w;
       u1 += w * ck.x;
       u2 += w * ck.y;
       if (dt < half) ut += w;
     }
-    acc[0] += u0;
-    acc[1] += u1;
-    acc[2] += u2;
-    acc[3] += cj.x * u1;
-    acc[4] += cj.x * u2;
-    acc[5] += cj.y * u2;
-    acc[6] += ut;
+    const double w0 = Na[0];
+    acc[0] += w0 + 2.0 * u0;
+    acc[1] += w0 * cj.x + (u1 + cj.x * u0);
+    acc[2] += w0 * cj.y + (u2 + cj.y * u0);
+    acc[3] += w0 * cj.x * cj.x + 2.0 * cj.x * u1;
+    acc[4] += w0 * cj.x * cj.y + (cj.x * u2 + cj.y * u1);
+    acc[5] += w0 * cj.y * cj.y + 2.0 * cj.y * u2;
+    acc[6] += (half > 0 ? w0 : 0.0) + 2.0 * ut;
   }
 #pragma unroll
   for (int v = 0; v < 7; v++) acc[v] = red16(acc[v]);
