@@ -68,7 +68,7 @@ def log(*a):
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
-    ap.add_argument("--workload", default="c1", choices=sorted(synth.WORKLOADS))
+    ap.add_argument("--workload", default="c1", choices=sorted(synth.WORKLOADS) + sorted(synth.UNEVEN_WORKLOADS))
     ap.add_argument("--pairs", type=int, default=1500, help="block-Jacobi pairs")
     ap.add_argument("--holdout", type=int, default=100)
     ap.add_argument("--steps", type=int, default=6000)
@@ -95,12 +95,18 @@ def main(argv=None):
     out = a.out or weights_path(a.workload)
     t0 = time.time()
     cfg = synth.workload_config(a.workload)
-    ds = synth.make_dataset(cfg, workers=min(16, os.cpu_count() or 1))
-    log(f"workload {a.workload}: {cfg.ndet} det x {cfg.nseg * cfg.ns} samples, NSIDE {cfg.nside} "
-        f"({time.time() - t0:.1f} s)")
+    uneven = a.workload in synth.UNEVEN_WORKLOADS
+    workers = min(16, os.cpu_count() or 1)
+    ds = synth.make_uneven_dataset(cfg, workers) if uneven else synth.make_dataset(cfg, workers)
+    nsamp = sum(cfg.used(d) for d in range(cfg.ndet)) if uneven else cfg.ndet * cfg.nseg * cfg.ns
+    log(f"workload {a.workload}: {cfg.ndet} det, {nsamp} samples, NSIDE {cfg.nside} ({time.time() - t0:.1f} s)")
     prec = sanepic_amd.FP64 if a.precision == "fp64" else sanepic_amd.MIXED
-    ctx = sanepic_amd.Context(cfg.nside, cfg.ndet, cfg.nseg, cfg.ns, precision=prec)
-    ctx.load(cfg, ds)
+    if uneven:  # per-detector segment lengths
+        ctx = sanepic_amd.Context.uneven(cfg.nside, cfg.seglen, cfg.nsamp, precision=prec)
+        ctx.load_uneven(cfg, ds)
+    else:
+        ctx = sanepic_amd.Context(cfg.nside, cfg.ndet, cfg.nseg, cfg.ns, precision=prec)
+        ctx.load(cfg, ds)
     del ds
     npix, _ = ctx.pixelize()
     ctx.setup()
@@ -185,8 +191,8 @@ def main(argv=None):
     os.makedirs(os.path.dirname(out), exist_ok=True)
     p.astype("<f4").tofile(out)
     rec = dict(workload=a.workload, nside=cfg.nside, net_nside=net_nside, ndet=cfg.ndet,
-               samples_per_det=cfg.nseg * cfg.ns,
-               segment=cfg.ns, npix=npix, nparams=int(p.size), pairs=n, holdout=a.holdout,
+               samples=nsamp, segment=sorted(set(cfg.seglen)) if uneven else cfg.ns, npix=npix,
+               nparams=int(p.size), pairs=n, holdout=a.holdout,
                harvest_seeds=[a.seed0, seed - 1], on_policy_seeds=[a.seed0 + 5000, rseed - 1],
                steps=a.steps, rounds=a.rounds, round_pairs=a.round_pairs, round_steps=a.round_steps,
                lr=a.lr, round_lr=a.round_lr, lr_final=a.lr_final,
