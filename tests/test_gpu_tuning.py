@@ -102,11 +102,13 @@ def test_graph_replay_matches_eager(data, precision):
     ctx.close()
 
 
-@pytest.mark.parametrize("env", ["SANEPIC_ZSORT", "SANEPIC_XCD_BLOCKS", "SANEPIC_NESTED_MAP", "SANEPIC_ITEM_ORDER"])
+@pytest.mark.parametrize("env", ["SANEPIC_ZSORT", "SANEPIC_XCD_BLOCKS", "SANEPIC_NESTED_MAP", "SANEPIC_ITEM_ORDER",
+                                 "SANEPIC_PREGATHER"])
 def test_layout_options_are_bit_identical(data, env, monkeypatch):
     """Writing the filter output in pixel-sorted order (read contiguously by
-    P^T), the XCD-contiguous block numbering and the nested-order gather map
-    change where values are stored and which L2 serves a block, not any sum:
+    P^T), the XCD-contiguous block numbering, the nested-order gather map and
+    items, and gathering P x in a pass of its own change where values are
+    stored and which L2 serves a block, not any sum:
     A x, b and chi2 are bit-identical with the option off, and the per-sample
     pixels reported back are the same."""
     cfg, ds, P = data
