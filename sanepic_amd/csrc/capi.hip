@@ -970,40 +970,30 @@ int sanepic_pixelize(sanepic_ctx *c, int64_t *npix_out, int64_t *n_deferred) {
     LCHK(c, launch_cs_to_float(c->cs, c->cs32, ntot, c->st));
     LCHK(c, launch_gather_cs_f(c->perm, c->cs, c->cs_sorted32, ntot, c->st));
   }
-  // P^T work items: pixels, split into <= kChunk-entry chunks
-  std::vector<int32_t> ipix, ilen, islot, spix, sfirst, scount;
-  std::vector<int64_t> ibeg;
-  ipix.reserve((size_t)npix + 1024);
-  ilen.reserve((size_t)npix + 1024);
-  islot.reserve((size_t)npix + 1024);
-  ibeg.reserve((size_t)npix + 1024);
-  int32_t nslots = 0;
-  // items in NESTED order when enabled: pixels reduced at the same time are
-  // neighbours on the sky, whose runs of z share cache lines (each pixel's
-  // sums are unchanged)
+  // P^T work items: pixels, split into <= kChunk-entry chunks, built on the
+  // device (two prefix sums and a fill); the host only lists the rare split
+  // pixels.  Items in NESTED order when enabled: pixels reduced at the same
+  // time are neighbours on the sky, whose runs of z share cache lines (each
+  // pixel's sums are unchanged).
   const bool nest_items = c->items_nested && !c->h_slot2pix.empty();
+  std::vector<int32_t> spix, sfirst, scount;
+  int64_t nit = 0;
+  int32_t nslots = 0;
   for (int64_t q = 0; q < npix; q++) {
     const int64_t p = nest_items ? c->h_slot2pix[(size_t)q] : q;
-    const int64_t n = c->h_cnt[p];
+    const int64_t n = c->h_cnt[(size_t)p];
     if (n <= kChunk) {
-      ipix.push_back((int32_t)p);
-      ibeg.push_back(off[p]);
-      ilen.push_back((int32_t)n);
-      islot.push_back(-1);
-    } else {
-      spix.push_back((int32_t)p);
-      sfirst.push_back(nslots);
-      int32_t k = 0;
-      for (int64_t e = 0; e < n; e += kChunk, k++) {
-        ipix.push_back((int32_t)p);
-        ibeg.push_back(off[p] + e);
-        ilen.push_back((int32_t)std::min<int64_t>(kChunk, n - e));
-        islot.push_back(nslots++);
-      }
-      scount.push_back(k);
+      nit++;
+      continue;
     }
+    const int32_t k = (int32_t)((n + kChunk - 1) / kChunk);
+    spix.push_back((int32_t)p);
+    sfirst.push_back(nslots);
+    scount.push_back(k);
+    nslots += k;
+    nit += k;
   }
-  const int64_t nit = (int64_t)ipix.size(), nsp = (int64_t)spix.size();
+  const int64_t nsp = (int64_t)spix.size();
   ACHK(c, dalloc(c, c->it_pix, nit));
   ACHK(c, dalloc(c, c->it_beg, nit));
   ACHK(c, dalloc(c, c->it_len, nit));
@@ -1011,11 +1001,32 @@ int sanepic_pixelize(sanepic_ctx *c, int64_t *npix_out, int64_t *n_deferred) {
   ACHK(c, dalloc(c, c->sp_pix, nsp));
   ACHK(c, dalloc(c, c->sp_first, nsp));
   ACHK(c, dalloc(c, c->sp_count, nsp));
-  ACHK(c, need_pinned(c, nit * 8));
-  ACHK(c, h2d_staged(c, c->it_pix, ipix.data(), nit * 4));
-  ACHK(c, h2d_staged(c, c->it_beg, ibeg.data(), nit * 8));
-  ACHK(c, h2d_staged(c, c->it_len, ilen.data(), nit * 4));
-  ACHK(c, h2d_staged(c, c->it_slot, islot.data(), nit * 4));
+  if (npix > 0) {
+    int32_t *order = nullptr, *kcnt = nullptr, *scnt = nullptr, *kofs = nullptr, *sofs = nullptr;
+    if (nest_items) {
+      ACHK(c, dalloc(c, order, npix));
+      ACHK(c, h2d_staged(c, order, c->h_slot2pix.data(), npix * (int64_t)sizeof(int32_t)));
+    }
+    ACHK(c, dalloc(c, kcnt, npix));
+    ACHK(c, dalloc(c, scnt, npix));
+    ACHK(c, dalloc(c, kofs, npix));
+    ACHK(c, dalloc(c, sofs, npix));
+    LCHK(c, launch_item_count(order, c->cnt, npix, kChunk, kcnt, scnt, c->st));
+    size_t tb = 0;
+    HIPCHK(c, hipcub::DeviceScan::ExclusiveSum(nullptr, tb, kcnt, kofs, (int)npix, c->st));
+    char *tmp = nullptr;
+    ACHK(c, dalloc(c, tmp, (int64_t)tb));
+    HIPCHK(c, hipcub::DeviceScan::ExclusiveSum(tmp, tb, kcnt, kofs, (int)npix, c->st));
+    HIPCHK(c, hipcub::DeviceScan::ExclusiveSum(tmp, tb, scnt, sofs, (int)npix, c->st));
+    LCHK(c, launch_item_fill(order, c->cnt, c->off, kofs, sofs, npix, kChunk, c->it_pix, c->it_beg, c->it_len,
+                             c->it_slot, c->st));
+    dfree_later(c, tmp, (int64_t)tb);
+    dfree_later(c, order, npix);
+    dfree_later(c, kcnt, npix);
+    dfree_later(c, scnt, npix);
+    dfree_later(c, kofs, npix);
+    dfree_later(c, sofs, npix);
+  }
   if (nsp) {
     HIPCHK(c, hipMemcpy(c->sp_pix, spix.data(), nsp * 4, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->sp_first, sfirst.data(), nsp * 4, hipMemcpyHostToDevice));

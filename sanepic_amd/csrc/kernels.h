@@ -188,6 +188,16 @@ int launch_build_weights(const double *spn, int64_t ndet, int64_t N, int64_t N1,
                          float *wt32, hipStream_t s);
 int launch_abs(const double *src, double *dst, int64_t n, hipStream_t s);  // dst = |src|
 int launch_copy_i32(const int32_t *src, int32_t *dst, int64_t n, hipStream_t s);  // dst may be mapped host memory
+// P^T work items on the device.  Pixels are taken in the order order[q]
+// (null: q); a pixel of n entries makes ceil(n / chunk) items (one if
+// n <= chunk), split pixels' items numbered consecutively as partial slots.
+// count: kcnt[q] items and scnt[q] slots (0 unless split); fill: with kofs,
+// sofs the exclusive prefix sums of those.
+int launch_item_count(const int32_t *order, const int32_t *cnt, int64_t npix, int chunk, int32_t *kcnt,
+                      int32_t *scnt, hipStream_t s);
+int launch_item_fill(const int32_t *order, const int32_t *cnt, const int64_t *off, const int32_t *kofs,
+                     const int32_t *sofs, int64_t npix, int chunk, int32_t *it_pix, int64_t *it_beg,
+                     int32_t *it_len, int32_t *it_slot, hipStream_t s);
 // Gather map in slot order: map[3 slot[p] + k] = x[3 p + k] (interleaved x).
 int launch_map_slots(const double *x, const int32_t *slot, double *map, int64_t npix, hipStream_t s);
 int launch_map_slots(const double *x, const int32_t *slot, float *map, int64_t npix, hipStream_t s);

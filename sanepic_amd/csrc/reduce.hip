@@ -324,6 +324,35 @@ __global__ void k_build_weights(const double *spn, int64_t ndet, int64_t N, int6
   if (wt32) wt32[g] = (float)v;
 }
 
+__global__ void k_item_count(const int32_t *order, const int32_t *cnt, int64_t npix, int chunk, int32_t *kcnt,
+                             int32_t *scnt) {
+  const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (q >= npix) return;
+  const int32_t p = order ? order[q] : (int32_t)q;
+  const int32_t n = cnt[p];
+  const int32_t k = n <= chunk ? 1 : (n + chunk - 1) / chunk;
+  kcnt[q] = k;
+  scnt[q] = n > chunk ? k : 0;
+}
+
+__global__ void k_item_fill(const int32_t *order, const int32_t *cnt, const int64_t *off, const int32_t *kofs,
+                            const int32_t *sofs, int64_t npix, int chunk, int32_t *it_pix, int64_t *it_beg,
+                            int32_t *it_len, int32_t *it_slot) {
+  const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (q >= npix) return;
+  const int32_t p = order ? order[q] : (int32_t)q;
+  const int32_t n = cnt[p];
+  const bool split = n > chunk;
+  const int32_t k = split ? (n + chunk - 1) / chunk : 1;
+  for (int32_t i = 0; i < k; i++) {
+    const int64_t it = (int64_t)kofs[q] + i;
+    it_pix[it] = p;
+    it_beg[it] = off[p] + (int64_t)i * chunk;
+    it_len[it] = split ? min(chunk, n - i * chunk) : n;
+    it_slot[it] = split ? sofs[q] + i : -1;
+  }
+}
+
 __global__ void k_copy_i32(const int32_t *src, int32_t *dst, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (i < n) dst[i] = src[i];
@@ -439,6 +468,21 @@ int launch_compress_pix(int32_t *pix, int64_t n, const int32_t *indpix, hipStrea
 int launch_build_weights(const double *spn, int64_t ndet, int64_t N, int64_t N1, int64_t N2, double *wt,
                          float *wt32, hipStream_t s) {
   if (ndet * N > 0) hipLaunchKernelGGL(k_build_weights, dim3(nblk(ndet * N)), dim3(256), 0, s, spn, ndet, N, N1, N2, wt, wt32);
+  return ok();
+}
+
+int launch_item_count(const int32_t *order, const int32_t *cnt, int64_t npix, int chunk, int32_t *kcnt,
+                      int32_t *scnt, hipStream_t s) {
+  if (npix > 0) hipLaunchKernelGGL(k_item_count, dim3(nblk(npix)), dim3(256), 0, s, order, cnt, npix, chunk, kcnt, scnt);
+  return ok();
+}
+
+int launch_item_fill(const int32_t *order, const int32_t *cnt, const int64_t *off, const int32_t *kofs,
+                     const int32_t *sofs, int64_t npix, int chunk, int32_t *it_pix, int64_t *it_beg,
+                     int32_t *it_len, int32_t *it_slot, hipStream_t s) {
+  if (npix > 0)
+    hipLaunchKernelGGL(k_item_fill, dim3(nblk(npix)), dim3(256), 0, s, order, cnt, off, kofs, sofs, npix, chunk,
+                       it_pix, it_beg, it_len, it_slot);
   return ok();
 }
 
