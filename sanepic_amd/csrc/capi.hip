@@ -909,26 +909,60 @@ int sanepic_pixelize(sanepic_ctx *c, int64_t *npix_out, int64_t *n_deferred) {
   ACHK(c, dalloc(c, counts, nsky));
   HIPCHK(c, hipMemsetAsync(counts, 0, nsky * sizeof(int32_t), c->st));
   LCHK(c, launch_histogram(c->pix, ntot, counts, c->st));
-  // the counts come back through a kernel writing pinned host memory: a DMA
-  // copy would queue behind the TOD upload still running on the copy engines
-  ACHK(c, need_pinned(c, nsky * (int64_t)sizeof(int64_t)));  // also stages the uploads below
-  int32_t *hc_dev = nullptr;
-  HIPCHK(c, hipHostGetDevicePointer((void **)&hc_dev, c->h_pin, 0));
-  LCHK(c, launch_copy_i32(counts, hc_dev, nsky, c->st));
+  // seen pixels compacted on the device (904-915); the host gets the compact
+  // pixel list and hit counts through a kernel writing pinned host memory (a
+  // DMA copy would queue behind the TOD upload still on the copy engines)
+  int32_t *flag = nullptr, *pos = nullptr, *indpix = nullptr, *dlist = nullptr;
+  ACHK(c, dalloc(c, flag, nsky));
+  ACHK(c, dalloc(c, pos, nsky));
+  ACHK(c, dalloc(c, indpix, nsky));
+  LCHK(c, launch_seen_flag(counts, nsky, flag, c->st));
+  {
+    size_t tb = 0;
+    HIPCHK(c, hipcub::DeviceScan::ExclusiveSum(nullptr, tb, flag, pos, (int)nsky, c->st));
+    char *tmp = nullptr;
+    ACHK(c, dalloc(c, tmp, (int64_t)tb));
+    HIPCHK(c, hipcub::DeviceScan::ExclusiveSum(tmp, tb, flag, pos, (int)nsky, c->st));
+    dfree_later(c, tmp, (int64_t)tb);
+  }
+  ACHK(c, need_pinned(c, nsky * (int64_t)sizeof(int64_t)));
+  int32_t *hpin = nullptr;
+  HIPCHK(c, hipHostGetDevicePointer((void **)&hpin, c->h_pin, 0));
+  LCHK(c, launch_copy_i32(pos + (nsky - 1), hpin, 1, c->st));
+  LCHK(c, launch_copy_i32(flag + (nsky - 1), hpin + 1, 1, c->st));
   HIPCHK(c, hipStreamSynchronize(c->st));
-  const int32_t *hc = (const int32_t *)c->h_pin;
-  compact_seen(hc, nsky, c->h_indpix, c->h_pixlist, c->h_cnt);
-  const int64_t npix = (int64_t)c->h_pixlist.size();
+  const int64_t npix = (int64_t)((const int32_t *)c->h_pin)[0] + ((const int32_t *)c->h_pin)[1];
   c->npix = npix;
-  if (c->nested_map || c->items_nested) ACHK(c, nested_order(c));
-  ACHK(c, h2d_staged(c, counts, c->h_indpix.data(), nsky * (int64_t)sizeof(int32_t)));
-  LCHK(c, launch_compress_pix(c->pix, ntot, counts, c->st));
   ACHK(c, dalloc(c, c->cnt, npix));
-  ACHK(c, h2d_staged(c, c->cnt, c->h_cnt.data(), npix * (int64_t)sizeof(int32_t)));
-  std::vector<int64_t> off(npix + 1, 0);
-  for (int64_t p = 0; p < npix; p++) off[p + 1] = off[p] + c->h_cnt[p];
-  ACHK(c, dalloc(c, c->off, npix + 1));
-  ACHK(c, h2d_staged(c, c->off, off.data(), (npix + 1) * (int64_t)sizeof(int64_t)));
+  ACHK(c, dalloc(c, dlist, npix));
+  LCHK(c, launch_seen_compact(counts, pos, nsky, indpix, dlist, c->cnt, c->st));
+  LCHK(c, launch_compress_pix(c->pix, ntot, indpix, c->st));
+  // offsets of each pixel's run in the sorted sample list (64-bit prefix sum)
+  {
+    int64_t *wide = nullptr;
+    ACHK(c, dalloc(c, wide, npix + 1));
+    ACHK(c, dalloc(c, c->off, npix + 1));
+    LCHK(c, launch_widen(c->cnt, npix, wide, c->st));
+    size_t tb = 0;
+    HIPCHK(c, hipcub::DeviceScan::ExclusiveSum(nullptr, tb, wide, c->off, (int)(npix + 1), c->st));
+    char *tmp = nullptr;
+    ACHK(c, dalloc(c, tmp, (int64_t)tb));
+    HIPCHK(c, hipcub::DeviceScan::ExclusiveSum(tmp, tb, wide, c->off, (int)(npix + 1), c->st));
+    dfree_later(c, tmp, (int64_t)tb);
+    dfree_later(c, wide, npix + 1);
+  }
+  // host copies of the pixel list and counts
+  LCHK(c, launch_copy_i32(dlist, hpin, npix, c->st));
+  LCHK(c, launch_copy_i32(c->cnt, hpin + npix, npix, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  c->h_pixlist.assign((const int32_t *)c->h_pin, (const int32_t *)c->h_pin + npix);
+  c->h_cnt.assign((const int32_t *)c->h_pin + npix, (const int32_t *)c->h_pin + 2 * npix);
+  c->h_indpix.clear();  // built from h_pixlist when asked for (sanepic_get_indpix)
+  dfree_later(c, flag, nsky);
+  dfree_later(c, pos, nsky);
+  dfree_later(c, indpix, nsky);
+  dfree_later(c, dlist, npix);
+  if (c->nested_map || c->items_nested) ACHK(c, nested_order(c));
   HIPCHK(c, hipStreamSynchronize(c->st));
   dfree_later(c, counts, nsky);
   // stable sort of samples by compressed pixel -> perm
@@ -1060,6 +1094,10 @@ int sanepic_get_pixels(sanepic_ctx *c, int64_t idet, int64_t iseg, int64_t *out)
 
 int sanepic_get_indpix(sanepic_ctx *c, int64_t *indpix) {
   ACHK(c, check_ready(c, false));
+  if (c->h_indpix.empty()) {  // the compaction ran on the device
+    c->h_indpix.assign((size_t)(12 * c->nside * c->nside), -1);
+    for (size_t k = 0; k < c->h_pixlist.size(); k++) c->h_indpix[(size_t)c->h_pixlist[k]] = (int32_t)k;
+  }
   for (size_t i = 0; i < c->h_indpix.size(); i++) indpix[i] = c->h_indpix[i];
   return SANEPIC_OK;
 }

@@ -188,6 +188,13 @@ int launch_build_weights(const double *spn, int64_t ndet, int64_t N, int64_t N1,
                          float *wt32, hipStream_t s);
 int launch_abs(const double *src, double *dst, int64_t n, hipStream_t s);  // dst = |src|
 int launch_copy_i32(const int32_t *src, int32_t *dst, int64_t n, hipStream_t s);  // dst may be mapped host memory
+// Seen-pixel compaction on the device (904-915): f[i] = counts[i] > 0; with
+// pos = exclusive prefix sum of f, indpix[i] = pos[i] or -1, pixlist[pos] = i,
+// cnt[pos] = counts[i]; wide[k] = cnt[k] as int64 (wide[n] = 0) for the offsets.
+int launch_seen_flag(const int32_t *counts, int64_t nsky, int32_t *f, hipStream_t s);
+int launch_seen_compact(const int32_t *counts, const int32_t *pos, int64_t nsky, int32_t *indpix,
+                        int32_t *pixlist, int32_t *cnt, hipStream_t s);
+int launch_widen(const int32_t *cnt, int64_t n, int64_t *wide, hipStream_t s);
 // P^T work items on the device.  Pixels are taken in the order order[q]
 // (null: q); a pixel of n entries makes ceil(n / chunk) items (one if
 // n <= chunk), split pixels' items numbered consecutively as partial slots.

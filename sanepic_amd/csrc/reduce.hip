@@ -163,7 +163,7 @@ __global__ __launch_bounds__(256) void k_pairsum(PairSumArgs a) {
   const int p = a.it.pix[item];
   const int64_t e0 = a.it.beg[item];
   const int n = a.it.len[item];
-  const int64_t P0 = a.off[p], P1 = a.off[p + 1];
+  const int64_t P1 = a.off[p + 1];  // the pixel's entries end here (a group never crosses it)
   // sample indices are < 2^31: 32-bit block/offset arithmetic (a shift when
   // ns is a power of two)
   const uint32_t ns = (uint32_t)a.ns, half = ns / 2;
@@ -324,6 +324,32 @@ __global__ void k_build_weights(const double *spn, int64_t ndet, int64_t N, int6
   if (wt32) wt32[g] = (float)v;
 }
 
+__global__ void k_seen_flag(const int32_t *counts, int64_t nsky, int32_t *f) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < nsky) f[i] = counts[i] > 0 ? 1 : 0;
+}
+
+__global__ void k_seen_compact(const int32_t *counts, const int32_t *pos, int64_t nsky, int32_t *indpix,
+                               int32_t *pixlist, int32_t *cnt) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= nsky) return;
+  const int32_t c = counts[i];
+  if (c > 0) {
+    const int32_t k = pos[i];
+    indpix[i] = k;
+    pixlist[k] = (int32_t)i;
+    cnt[k] = c;
+  } else {
+    indpix[i] = -1;
+  }
+}
+
+__global__ void k_widen(const int32_t *cnt, int64_t n, int64_t *wide) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) wide[i] = cnt[i];
+  if (i == n) wide[i] = 0;
+}
+
 __global__ void k_item_count(const int32_t *order, const int32_t *cnt, int64_t npix, int chunk, int32_t *kcnt,
                              int32_t *scnt) {
   const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -468,6 +494,23 @@ int launch_compress_pix(int32_t *pix, int64_t n, const int32_t *indpix, hipStrea
 int launch_build_weights(const double *spn, int64_t ndet, int64_t N, int64_t N1, int64_t N2, double *wt,
                          float *wt32, hipStream_t s) {
   if (ndet * N > 0) hipLaunchKernelGGL(k_build_weights, dim3(nblk(ndet * N)), dim3(256), 0, s, spn, ndet, N, N1, N2, wt, wt32);
+  return ok();
+}
+
+int launch_seen_flag(const int32_t *counts, int64_t nsky, int32_t *f, hipStream_t s) {
+  if (nsky > 0) hipLaunchKernelGGL(k_seen_flag, dim3(nblk(nsky)), dim3(256), 0, s, counts, nsky, f);
+  return ok();
+}
+
+int launch_seen_compact(const int32_t *counts, const int32_t *pos, int64_t nsky, int32_t *indpix,
+                        int32_t *pixlist, int32_t *cnt, hipStream_t s) {
+  if (nsky > 0)
+    hipLaunchKernelGGL(k_seen_compact, dim3(nblk(nsky)), dim3(256), 0, s, counts, pos, nsky, indpix, pixlist, cnt);
+  return ok();
+}
+
+int launch_widen(const int32_t *cnt, int64_t n, int64_t *wide, hipStream_t s) {
+  hipLaunchKernelGGL(k_widen, dim3(nblk(n + 1)), dim3(256), 0, s, cnt, n, wide);
   return ok();
 }
 
