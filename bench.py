@@ -200,6 +200,10 @@ def main():
                          "(the FP64 baseline keeps the reference's 10)")
     ap.add_argument("--no-baseline", action="store_true", help="skip the FP64 block-Jacobi solve")
     ap.add_argument("--maxiter", type=int, default=600)
+    ap.add_argument("--settle", type=float, default=5.0,
+                    help="idle seconds before each pipeline after the first, outside the timed regions: a "
+                         "context created right after another released ~100 GB of device memory has stalled "
+                         "for up to 3 s inside its set-up, so every pipeline starts from an idle device")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--weights", default=None, help="trained network (default: the workload's committed one)")
     ap.add_argument("--calib-copy", action="store_true",
@@ -241,10 +245,13 @@ def main():
         base = run_solve(cfg, pin, "fp64", "block", False, None, args.steps, args.warmup,
                          args.maxiter, profile=False)
     rf = args.mixed_refresh if args.precision == "mixed" else 10
+    if base is not None:
+        time.sleep(args.settle)
     head = run_solve(cfg, pin, args.precision, precond, deflate, weights, args.steps, args.warmup,
                      args.maxiter, profile=True, net_nside=(record or {}).get("net_nside"), refresh=rf)
     learned = second_block = None
     if not args.no_second:
+        time.sleep(args.settle)
         if precond != "learned" and weights is not None:
             # the same pipeline with the trained network (reported beside the headline)
             learned = run_solve(cfg, pin, args.precision, "learned", deflate, weights, 2, 1, args.maxiter,
@@ -300,6 +307,7 @@ def main():
         "final_true_residual": head["final_true_residual"],
         "setup_s": head["upload_s"] + head["pixelize_s"] + head["setup_s"] + head["begin_s"],
         "phases_s": {k: head[k] for k in ("upload_s", "pixelize_s", "setup_s", "begin_s")},
+        "idle_before_each_pipeline_s": args.settle,
         "operator_ms": apply_ms,
         "operator_frac_hbm": b_alg / (apply_ms * 1e-3) / HBM_PEAK,
         "device_bytes": head["device_bytes"],

@@ -532,6 +532,7 @@ int sanepic_create(const sanepic_config *cfg, sanepic_ctx **out) {
     g_create_error = "hipSetDevice failed";
     return SANEPIC_EHIP;
   }
+  trace_mark("create");
   sanepic_ctx *c = new sanepic_ctx();
   c->cfg = *cfg;
   c->ns = ns;
@@ -615,6 +616,7 @@ void sanepic_destroy(sanepic_ctx *c) {
   for (hipStream_t s : c->up_tod)
     if (s) (void)hipStreamSynchronize(s);
   for (void *p : c->allocs) (void)hipFree(p);
+  if (!c->borrowed) trace_mark("destroy: device memory freed");
   if (c->h_pin) (void)hipHostFree(c->h_pin);
   if (c->st && !c->borrowed) (void)hipStreamDestroy(c->st);
   if (c->up_pt) (void)hipStreamDestroy(c->up_pt);
@@ -811,7 +813,9 @@ int sanepic_pixelize(sanepic_ctx *c, int64_t *npix_out, int64_t *n_deferred) {
       if (!c->det_ok[(size_t)i]) return fail(c, SANEPIC_EINVAL, "detector " + std::to_string(i) + " not set");
     return group_pixelize(c, npix_out, n_deferred);
   }
+  trace_mark("pixelize: enter");
   if (c->up_pt) HIPCHK(c, hipStreamSynchronize(c->up_pt));
+  trace_mark("pixelize: pointing uploads done");
   if (c->pixelized) return fail(c, SANEPIC_EINVAL, "sanepic_pixelize already ran");
   for (int64_t i = 0; i < c->ndet; i++)
     if (!c->det_ok[i]) return fail(c, SANEPIC_EINVAL, "detector " + std::to_string(i) + " not set");
@@ -1070,6 +1074,7 @@ int sanepic_pixelize(sanepic_ctx *c, int64_t *npix_out, int64_t *n_deferred) {
   c->nslots = nslots;
   ACHK(c, dalloc(c, c->partial, 8 * std::max<int64_t>(1, nslots)));
   HIPCHK(c, hipStreamSynchronize(c->st));
+  trace_mark("pixelize: done");
   c->pixelized = true;
   if (npix_out) *npix_out = npix;
   if (n_deferred) *n_deferred = c->n_deferred;
@@ -1231,11 +1236,14 @@ int sanepic_setup(sanepic_ctx *c) {
   ACHK(c, check_ready(c, false));
   if (c->setup_done) return fail(c, SANEPIC_EINVAL, "sanepic_setup already ran");
   if (!c->sub.empty()) return group_setup(c);
+  trace_mark("setup: enter");
   ACHK(c, sync_tod_uploads(c));
+  trace_mark("setup: TOD uploads done");
   for (size_t i = 0; i < c->tod_ok.size(); i++)
     if (!c->tod_ok[i]) return fail(c, SANEPIC_EINVAL, "TOD block " + std::to_string(i) + " not set");
   const int64_t npix = c->npix, ntot = c->ntot, N = c->ns;
   ACHK(c, build_tables(c));
+  trace_mark("setup: noise tables");
   ACHK(c, dalloc(c, c->z, ntot));
   ACHK(c, dalloc(c, c->dspec, c->g.npairs * N));
   if (!c->single) ACHK(c, dalloc(c, c->buf, c->g.npairs * N));
@@ -1252,6 +1260,7 @@ int sanepic_setup(sanepic_ctx *c) {
     ACHK(c, run_scatter<double>(c, c->z, c->b_raw, 0));
   }
   HIPCHK(c, hipStreamSynchronize(c->st));
+  trace_mark("setup: P^T N^-1 d");
   dfree(c, c->tod, ntot);
   // noise autocorrelation N_ = c2r(|SpN|) per detector (2698-2706)
   {
@@ -1309,7 +1318,9 @@ int sanepic_setup(sanepic_ctx *c) {
   if (c->nested_map && !c->pix_nested) ACHK(c, nest_gather_map(c));
   if (!c->nested_map) c->h_slot2pix.clear();  // pixels stay in ring order (get_pixels)
   HIPCHK(c, hipStreamSynchronize(c->st));
+  trace_mark("setup: pair sums, vectors");
   flush_frees(c);
+  trace_mark("setup: deferred frees");
   c->setup_done = true;
   return SANEPIC_OK;
 }

@@ -84,6 +84,7 @@ int group_pixelize(sanepic_ctx *c, int64_t *npix_out, int64_t *ndef_out) {
   dfree_later(c, c->dec, c->tl_len);
   dfree_later(c, c->psi, c->tl_len);
   c->n_deferred = ndef;
+  trace_mark("group pixelize: union done");
   c->pixelized = true;
   if (npix_out) *npix_out = c->npix;
   if (ndef_out) *ndef_out = ndef;
@@ -91,7 +92,9 @@ int group_pixelize(sanepic_ctx *c, int64_t *npix_out, int64_t *ndef_out) {
 }
 
 int group_setup(sanepic_ctx *c) {
+  trace_mark("group setup: enter");
   ACHK(c, sync_tod_uploads(c));
+  trace_mark("group setup: TOD uploads done");
   for (int64_t d = 0; d < c->ndet; d++) {
     const sanepic_ctx *s = c->sub[(size_t)c->det_group[(size_t)d]];
     if (c->tod_set[(size_t)d] < s->nseg * s->ns)
@@ -126,7 +129,9 @@ int group_setup(sanepic_ctx *c) {
   ACHK(c, dalloc(c, c->nvar, 3 * npix));
   ACHK(c, dalloc(c, c->part, 8 * 512));
   HIPCHK(c, hipStreamSynchronize(c->st));
+  trace_mark("group setup: union sums");
   flush_frees(c);
+  trace_mark("group setup: deferred frees");
   c->setup_done = true;
   return SANEPIC_OK;
 }

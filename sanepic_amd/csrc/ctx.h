@@ -4,6 +4,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -241,6 +244,23 @@ inline int tod_stream(sanepic_ctx *c, int64_t k, hipStream_t *out) {
   }
   *out = s;
   return 0;
+}
+
+// Set-up phase trace on stderr when SANEPIC_TRACE is set: the host time
+// since the previous mark of the calling thread under `what`, and the free
+// device memory of the current device.
+inline void trace_mark(const char *what) {
+  static const bool on = getenv("SANEPIC_TRACE") != nullptr;
+  if (!on) return;
+  using clk = std::chrono::steady_clock;
+  thread_local clk::time_point last = clk::now();
+  const clk::time_point now = clk::now();
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess) (void)hipGetLastError();
+  fprintf(stderr, "[sanepic trace] %-30s %8.1f ms  free %.1f of %.1f GiB\n", what,
+          std::chrono::duration<double, std::milli>(now - last).count(), fr / 1073741824.0,
+          tot / 1073741824.0);
+  last = now;
 }
 
 inline int sync_tod_uploads(sanepic_ctx *c) {
