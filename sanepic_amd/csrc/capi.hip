@@ -199,6 +199,8 @@ OpArgs<double> op_args<double>(sanepic_ctx *c) {
   a.tw4096 = c->tw;
   a.twlo = c->twlo;
   a.twhi = c->twhi;
+  a.twlo64 = c->twlo;
+  a.twhi64 = c->twhi;
   a.chi2part = c->chi2part;
   a.xcd = c->xcd_blocks ? 1 : 0;
   a.zpos = c->zsorted ? c->rank : nullptr;
@@ -219,6 +221,8 @@ OpArgs<float> op_args<float>(sanepic_ctx *c) {
   a.tw4096 = c->tw32;
   a.twlo = c->twlo32;
   a.twhi = c->twhi32;
+  a.twlo64 = c->twlo;
+  a.twhi64 = c->twhi;
   a.chi2part = c->chi2part;
   a.xcd = c->xcd_blocks ? 1 : 0;
   // zpos maps a sample to its slot in the [ntot] z buffers above; a pass

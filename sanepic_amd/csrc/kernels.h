@@ -100,6 +100,7 @@ struct OpArgs {
   const uint32_t *zpos; // [ntot] pixel-sorted position of each sample (z kept sorted) or null
   const V *tw4096;      // w_4096^j
   const V *twlo, *twhi; // w_N^j = twlo[j & 4095] * twhi[j >> 12]
+  const double2 *twlo64, *twhi64;  // the same tables in FP64 (runs of outer twiddles)
   double *chi2part;     // per-block chi2 partials
   int64_t chi2off;      // first partial slot of this launch
   int mode;
