@@ -239,6 +239,12 @@ __global__ __launch_bounds__(256) void k_col_fwd_m(OpArgs<T> a) {
 #pragma unroll
     for (int i = 0; i < 16; i++) out[((int64_t)P * j + t + TS * i) * N1 + n1] = v[i];
   }
+  // output twiddle w_N^{n1 k2}, k2 = t + TS s + P q: a run over s times
+  // w_N^{n1 P q}, both in FP64
+  TwRun tr = tw_run<-1>(a.twlo64, a.twhi64, n1 * t, n1 * TS);
+  double2 cq[M];
+#pragma unroll
+  for (int q = 0; q < M; q++) cq[q] = twN<double2, -1>(a.twlo64, a.twhi64, n1 * P * q);
 #pragma unroll 4
   for (int s = 0; s < 16; s++) {
     const int64_t vv = t + TS * s;
@@ -248,10 +254,13 @@ __global__ __launch_bounds__(256) void k_col_fwd_m(OpArgs<T> a) {
 #pragma unroll
     for (int j = 1; j < M; j++) y[j] = cmul(y[j], twN<V, -1>(a.twlo, a.twhi, (int64_t)j * vv * N1));
     dft_m<V, M, -1>(y, Y);
+    const double2 r = tr.w;
+    tr.w = cmul(tr.w, tr.step);
 #pragma unroll
     for (int q = 0; q < M; q++) {
       const int64_t k2 = vv + (int64_t)P * q;
-      out[k2 * N1 + n1] = cmul(Y[q], twN<V, -1>(a.twlo, a.twhi, n1 * k2));
+      const double2 w = cmul(r, cq[q]);
+      out[k2 * N1 + n1] = cmul(Y[q], mk<V>((decltype(V::x))w.x, (decltype(V::x))w.y));
     }
   }
 }
