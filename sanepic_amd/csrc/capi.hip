@@ -212,7 +212,8 @@ OpArgs<float> op_args<float>(sanepic_ctx *c) {
   OpArgs<float> a{};
   a.g = c->g;
   a.pix = c->pix;
-  a.cs = c->cs32;
+  a.cs = nullptr;
+  a.ang = c->ang32;
   a.tod = c->tod;
   a.buf = c->buf32;
   a.dspec = nullptr;
@@ -284,7 +285,7 @@ int run_scatter(sanepic_ctx *c, const T *z, double *q, int mask) {
   if constexpr (sizeof(T) == 8)
     s.cs_sorted = c->cs_sorted;
   else
-    s.cs_sorted = c->cs_sorted32;
+    s.ang_sorted = c->ang_sorted32;
   s.z = z;
   s.sorted = (c->zsorted && c->rank) ? 1 : 0;
   s.xcd = 0;  // measured: contiguous item ranges per XCD make P^T slower (4.49 -> 5.16 ms)
@@ -1007,10 +1008,10 @@ int sanepic_pixelize(sanepic_ctx *c, int64_t *npix_out, int64_t *n_deferred) {
     dfree_later(c, c->csfp, ntot);
   }
   if (c->cfg.precision == SANEPIC_MIXED) {
-    ACHK(c, dalloc(c, c->cs32, ntot));
-    ACHK(c, dalloc(c, c->cs_sorted32, ntot));
-    LCHK(c, launch_cs_to_float(c->cs, c->cs32, ntot, c->st));
-    LCHK(c, launch_gather_cs_f(c->perm, c->cs, c->cs_sorted32, ntot, c->st));
+    ACHK(c, dalloc(c, c->ang32, ntot));
+    ACHK(c, dalloc(c, c->ang_sorted32, ntot));
+    LCHK(c, launch_cs_to_turn(c->cs, c->ang32, ntot, c->st));
+    LCHK(c, launch_gather_turn(c->perm, c->cs, c->ang_sorted32, ntot, c->st));
   }
   // P^T work items: pixels, split into <= kChunk-entry chunks, built on the
   // device (two prefix sums and a fill); the host only lists the rare split

@@ -41,7 +41,7 @@ struct sanepic_ctx {
   DetConst *d_det = nullptr;
   int32_t *pix = nullptr, *cnt = nullptr;
   double2 *cs = nullptr, *csfp = nullptr, *cs_sorted = nullptr, *csfp_sorted = nullptr;
-  float2 *cs32 = nullptr, *cs_sorted32 = nullptr;
+  uint32_t *ang32 = nullptr, *ang_sorted32 = nullptr;  // FP32 operator: 2psi as fractions of a turn
   int64_t *off = nullptr;
   uint32_t *perm = nullptr;
   int32_t *it_pix = nullptr, *it_len = nullptr, *it_slot = nullptr, *sp_pix = nullptr,

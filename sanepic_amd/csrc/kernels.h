@@ -89,7 +89,8 @@ struct OpArgs {
   using V = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
   PairGeom g;
   const int32_t *pix;   // [ntot] compressed pixel
-  const V *cs;          // [ntot] cos 2psi, sin 2psi
+  const V *cs;          // [ntot] cos 2psi, sin 2psi (FP64 operator)
+  const uint32_t *ang;  // [ntot] 2psi as a fraction of a turn (FP32 operator, turn.h)
   const T *map;         // [npix][3] interleaved I, Q, U
   const double *tod;    // [ntot] (SRC_TOD)
   const T *px;          // [ntot] (SRC_PX)
@@ -137,7 +138,8 @@ struct ScatterArgs {
   using V = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
   Items it;
   const uint32_t *perm;   // sorted entry -> sample
-  const V *cs_sorted;     // sorted entry -> cos 2psi, sin 2psi
+  const V *cs_sorted;     // sorted entry -> cos 2psi, sin 2psi (FP64 operator)
+  const uint32_t *ang_sorted;  // sorted entry -> 2psi as a fraction of a turn (FP32 operator)
   const T *z;             // [ntot] time-domain values (in sorted order when `sorted`)
   int sorted;             // z is stored pixel-sorted: entry j at z[j]
   int xcd;                // XCD-contiguous item ranges (neighbouring pixels share an L2)
@@ -172,8 +174,9 @@ int launch_pairsum(const PairSumArgs &a, hipStream_t s);
 
 // Gathers sorted copies: dst[j] = src[perm[j]].
 int launch_gather_cs(const uint32_t *perm, const double2 *src, double2 *dst, int64_t n, hipStream_t s);
-int launch_gather_cs_f(const uint32_t *perm, const double2 *src, float2 *dst, int64_t n, hipStream_t s);
-int launch_cs_to_float(const double2 *src, float2 *dst, int64_t n, hipStream_t s);
+// dst[i] = the angle of src[perm[i]] (or of src[i]) as a fraction of a turn (turn.h)
+int launch_gather_turn(const uint32_t *perm, const double2 *src, uint32_t *dst, int64_t n, hipStream_t s);
+int launch_cs_to_turn(const double2 *src, uint32_t *dst, int64_t n, hipStream_t s);
 
 // TOD NaN fill (784-790) on the device, per segment.
 int launch_tod_nanfill(double *tod, int64_t ntot, int64_t ns, hipStream_t s);

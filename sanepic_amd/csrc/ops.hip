@@ -21,6 +21,7 @@
 
 #include "fft_core.h"
 #include "kernels.h"
+#include "turn.h"
 
 using namespace sfft;
 
@@ -44,7 +45,9 @@ __device__ __forceinline__ void pair_info(const PairGeom &g, int64_t pr, int &de
 template <typename T>
 __device__ __forceinline__ T gather_one(const OpArgs<T> &a, int64_t gi) {
   const int p = a.pix[gi];
-  const VT<T> c = a.cs[gi];
+  VT<T> c;
+  if constexpr (sizeof(T) == 4) c = cs_of_turn(a.ang[gi]);
+  else c = a.cs[gi];
   const T *m = a.map + 3 * (int64_t)p;
   return m[0] + (m[1] * c.x + m[2] * c.y);
 }

@@ -12,6 +12,7 @@
 #include <type_traits>
 
 #include "kernels.h"
+#include "turn.h"
 
 namespace {
 
@@ -32,6 +33,13 @@ __device__ __forceinline__ int64_t logical_block(int xcd) {
   if (!xcd) return b;
   const int64_t nb = gridDim.x, q = nb >> 3, r = nb & 7, x = b & 7, i = b >> 3;
   return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
+// (cos 2psi, sin 2psi) of sorted entry j: stored pairs in FP64, turns in FP32
+template <typename T>
+__device__ __forceinline__ VT<T> sorted_cs(const ScatterArgs<T> &a, int64_t j) {
+  if constexpr (sizeof(T) == 4) return cs_of_turn(a.ang_sorted[j]);
+  else return a.cs_sorted[j];
 }
 
 template <typename T>
@@ -61,7 +69,7 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterArgs<T> a) {
       for (int u = 0; u < 4; u++) zv[u] = a.z[pp[u]];
     }
 #pragma unroll
-    for (int u = 0; u < 4; u++) c[u] = a.cs_sorted[j + 16 * u];
+    for (int u = 0; u < 4; u++) c[u] = sorted_cs(a, j + 16 * u);
 #pragma unroll
     for (int u = 0; u < 4; u++) {
       s0 += zv[u];
@@ -72,7 +80,7 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterArgs<T> a) {
   for (; k < n; k += 16) {
     const int64_t j = e0 + k;
     const T zv = a.sorted ? a.z[j] : a.z[a.perm[j]];
-    const V c = a.cs_sorted[j];
+    const V c = sorted_cs(a, j);
     s0 += zv;
     s1 += zv * c.x;
     s2 += zv * c.y;
@@ -243,16 +251,16 @@ __global__ void k_gather_cs(const uint32_t *perm, const double2 *src, double2 *d
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) dst[i] = src[perm[i]];
 }
-__global__ void k_gather_cs_f(const uint32_t *perm, const double2 *src, float2 *dst, int64_t n) {
+__global__ void k_gather_turn(const uint32_t *perm, const double2 *src, uint32_t *dst, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
     const double2 v = src[perm[i]];
-    dst[i] = make_float2((float)v.x, (float)v.y);
+    dst[i] = turn_of(v.x, v.y);
   }
 }
-__global__ void k_cs_to_float(const double2 *src, float2 *dst, int64_t n) {
+__global__ void k_cs_to_turn(const double2 *src, uint32_t *dst, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) dst[i] = make_float2((float)src[i].x, (float)src[i].y);
+  if (i < n) dst[i] = turn_of(src[i].x, src[i].y);
 }
 
 __device__ __forceinline__ bool isnan_(double x) { return !(x > 0) && !(x <= 0.0); }
@@ -470,13 +478,13 @@ int launch_gather_cs(const uint32_t *perm, const double2 *src, double2 *dst, int
   hipLaunchKernelGGL(k_gather_cs, dim3(nblk(n)), dim3(256), 0, s, perm, src, dst, n);
   return ok();
 }
-int launch_gather_cs_f(const uint32_t *perm, const double2 *src, float2 *dst, int64_t n,
+int launch_gather_turn(const uint32_t *perm, const double2 *src, uint32_t *dst, int64_t n,
                        hipStream_t s) {
-  hipLaunchKernelGGL(k_gather_cs_f, dim3(nblk(n)), dim3(256), 0, s, perm, src, dst, n);
+  hipLaunchKernelGGL(k_gather_turn, dim3(nblk(n)), dim3(256), 0, s, perm, src, dst, n);
   return ok();
 }
-int launch_cs_to_float(const double2 *src, float2 *dst, int64_t n, hipStream_t s) {
-  hipLaunchKernelGGL(k_cs_to_float, dim3(nblk(n)), dim3(256), 0, s, src, dst, n);
+int launch_cs_to_turn(const double2 *src, uint32_t *dst, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_cs_to_turn, dim3(nblk(n)), dim3(256), 0, s, src, dst, n);
   return ok();
 }
 int launch_tod_nanfill(double *tod, int64_t ntot, int64_t ns, hipStream_t s) {
