@@ -49,11 +49,20 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def stream_bytes_per_sample(precision):
-    # compulsory per-sample stream of the operator: compressed pixel (int32)
-    # plus cos 2psi and sin 2psi in the operator's precision
-    s = 8 if precision == "fp64" else 4
-    return 4 + 2 * s
+def class_bytes_per_sample(precision):
+    """Algorithmic HBM bytes per TOD sample of each operator kernel class
+    (reads + writes of its streams; the map gathers hit the caches).
+    FP64: pixel int32, (cos, sin) 2psi as two doubles, complex FP64 pair
+    buffer (8 bytes per sample), FP64 z.  FP32: pixel int32, 2psi as a 32-bit
+    fraction of a turn, complex FP32 pair buffer (4 bytes per sample), FP32 z."""
+    fp64 = precision == "fp64"
+    ang = 16 if fp64 else 4
+    w = 8 if fp64 else 4
+    z = 8 if fp64 else 4
+    return [4 + ang + w,   # gather + column FFT: pixel, angle in; pair buffer out
+            2 * w,         # row FFT + filter: pair buffer in and out
+            w + z,         # inverse column FFT: pair buffer in, z out
+            4 + z + ang]   # pixel-sorted P^T: permutation, z (gathered), sorted angle
 
 
 def calibration_copy(nbytes=1 << 30):
@@ -270,8 +279,8 @@ def main():
     # launches each kernel class once per detector group
     n_ops = max(kt[4][1], 1)
     avg_ms = tot_ms / n_ops
-    bps = stream_bytes_per_sample(args.precision)
-    alg = ntot * bps  # compulsory stream bytes of the kernel class per operator application
+    cbs = class_bytes_per_sample(args.precision)
+    alg = ntot * cbs[dom]  # algorithmic bytes of the dominant kernel class per operator application
     achieved = alg / (avg_ms * 1e-3)
     for k in range(4):
         if kt[k][1]:
@@ -279,7 +288,7 @@ def main():
     apply_ms = kt[4][0] / max(kt[4][1], 1)
     s = 8 if args.precision == "fp64" else 4
     lens = cfg.seglen if uneven else [cfg.ns] * cfg.ndet
-    b_alg = ntot * bps + sum(L // 2 + 1 for L in lens) * s + 12 * 3 * head["npix"] * 8
+    b_alg = ntot * sum(cbs) + sum(L // 2 + 1 for L in lens) * s + 12 * 3 * head["npix"] * 8
     tts = head["time_to_solution_s"]
     out = {
         "metric": "time-to-1e-6 residual (s) + TOD samples/s per CG iter, NSIDE512 16det, 1 GPU",
