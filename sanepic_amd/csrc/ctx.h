@@ -78,6 +78,7 @@ struct sanepic_ctx {
   std::vector<int32_t> h_slot2pix;   // slot -> compressed pixel
   std::vector<int32_t> h_mpos;       // compressed pixel -> slot
   bool items_nested = true;          // P^T items in NESTED pixel order
+  int scatter_lanes = 16;            // P^T lanes per item (from the mean item length at pixelisation)
   void *h_pin = nullptr;             // pinned host staging written by kernels (pixel counts)
   int64_t h_pin_bytes = 0;
   bool xcd_blocks = true;            // XCD-contiguous numbering of the four-step blocks

@@ -143,6 +143,7 @@ struct ScatterArgs {
   const T *z;             // [ntot] time-domain values (in sorted order when `sorted`)
   int sorted;             // z is stored pixel-sorted: entry j at z[j]
   int xcd;                // XCD-contiguous item ranges (neighbouring pixels share an L2)
+  int lanes;              // lanes per item: 4, 8 or 16 (fewer for short items; 0 means 16)
   const int32_t *cnt;     // [npix] samples per pixel (mask: cnt < 2)
   double *q;              // [npix][3] output
   double *partial;        // [nslots][3]

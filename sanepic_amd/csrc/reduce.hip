@@ -25,6 +25,15 @@ __device__ __forceinline__ T red16(T v) {
   return v;
 }
 
+// Sum over the L aligned lanes of a group (fixed tree: the result is the
+// same on every lane and independent of the schedule).
+template <int L, typename T>
+__device__ __forceinline__ T red_lanes(T v) {
+#pragma unroll
+  for (int o = L / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, L);
+  return v;
+}
+
 // Blocks go to the 8 XCDs round-robin; with xcd set, XCD x takes one
 // contiguous range of logical blocks, so pixels processed at the same time on
 // one L2 are neighbours in the map (their samples share z lines).
@@ -42,11 +51,15 @@ __device__ __forceinline__ VT<T> sorted_cs(const ScatterArgs<T> &a, int64_t j) {
   else return a.cs_sorted[j];
 }
 
-template <typename T>
+// L lanes own one item (256 / L items per block): 16 for long items; with
+// short items (per-detector-length groups hold few detectors) fewer lanes
+// keep several entries per lane in flight.  The summation order depends on L
+// only, which the host derives from the data, so results are deterministic.
+template <typename T, int L>
 __global__ __launch_bounds__(256) void k_scatter(ScatterArgs<T> a) {
   using V = VT<T>;
-  const int grp = threadIdx.x >> 4, lane = threadIdx.x & 15;
-  const int64_t item = logical_block(a.xcd) * 16 + grp;
+  const int grp = threadIdx.x / L, lane = threadIdx.x % L;
+  const int64_t item = logical_block(a.xcd) * (256 / L) + grp;
   if (item >= a.it.n) return;
   const int p = a.it.pix[item];
   const int64_t e0 = a.it.beg[item];
@@ -54,22 +67,22 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterArgs<T> a) {
   T s0 = 0, s1 = 0, s2 = 0;
   int k = lane;
   // four entries per lane in flight (their loads first), summed in entry order
-  for (; k + 48 < n; k += 64) {
+  for (; k + 3 * L < n; k += 4 * L) {
     const int64_t j = e0 + k;
     T zv[4];
     V c[4];
     if (a.sorted) {
 #pragma unroll
-      for (int u = 0; u < 4; u++) zv[u] = a.z[j + 16 * u];
+      for (int u = 0; u < 4; u++) zv[u] = a.z[j + L * u];
     } else {
       uint32_t pp[4];
 #pragma unroll
-      for (int u = 0; u < 4; u++) pp[u] = a.perm[j + 16 * u];
+      for (int u = 0; u < 4; u++) pp[u] = a.perm[j + L * u];
 #pragma unroll
       for (int u = 0; u < 4; u++) zv[u] = a.z[pp[u]];
     }
 #pragma unroll
-    for (int u = 0; u < 4; u++) c[u] = sorted_cs(a, j + 16 * u);
+    for (int u = 0; u < 4; u++) c[u] = sorted_cs(a, j + L * u);
 #pragma unroll
     for (int u = 0; u < 4; u++) {
       s0 += zv[u];
@@ -77,7 +90,7 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterArgs<T> a) {
       s2 += zv[u] * c[u].y;
     }
   }
-  for (; k < n; k += 16) {
+  for (; k < n; k += L) {
     const int64_t j = e0 + k;
     const T zv = a.sorted ? a.z[j] : a.z[a.perm[j]];
     const V c = sorted_cs(a, j);
@@ -85,9 +98,9 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterArgs<T> a) {
     s1 += zv * c.x;
     s2 += zv * c.y;
   }
-  s0 = red16(s0);
-  s1 = red16(s1);
-  s2 = red16(s2);
+  s0 = red_lanes<L>(s0);
+  s1 = red_lanes<L>(s1);
+  s2 = red_lanes<L>(s2);
   if (lane == 0) {
     const int slot = a.it.slot[item];
     if (slot < 0) {
@@ -444,8 +457,14 @@ inline int ok() { return hipGetLastError() == hipSuccess ? 0 : -1; }
 
 template <typename T>
 int launch_scatter(const ScatterArgs<T> &a, hipStream_t s) {
-  if (a.it.n > 0)
-    hipLaunchKernelGGL(k_scatter<T>, dim3(nblk(a.it.n, 16)), dim3(256), 0, s, a);
+  if (a.it.n > 0) {
+    if (a.lanes == 4)
+      hipLaunchKernelGGL((k_scatter<T, 4>), dim3(nblk(a.it.n, 64)), dim3(256), 0, s, a);
+    else if (a.lanes == 8)
+      hipLaunchKernelGGL((k_scatter<T, 8>), dim3(nblk(a.it.n, 32)), dim3(256), 0, s, a);
+    else
+      hipLaunchKernelGGL((k_scatter<T, 16>), dim3(nblk(a.it.n, 16)), dim3(256), 0, s, a);
+  }
   if (a.it.nsplit > 0)
     hipLaunchKernelGGL(k_fix<3>, dim3(nblk(a.it.nsplit)), dim3(256), 0, s, a.it, a.partial, a.q,
                        a.cnt, a.mask);
