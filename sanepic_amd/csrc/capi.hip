@@ -17,8 +17,6 @@
 #include <tuple>
 #include <unordered_map>
 #include <vector>
-#include <thread>
-#include <atomic>
 
 #include "../../include/sanepic_hip.h"
 #include "ctx.h"
@@ -142,48 +140,6 @@ int build_weights(sanepic_ctx *c) {
 }  // namespace
 
 namespace sctx {
-
-// Ascending compaction of the seen pixels (904-915) on host threads: count
-// per chunk, offsets, then each chunk fills its own range.
-template <typename C>
-void compact_seen(const C *hc, int64_t nsky, std::vector<int32_t> &indpix, std::vector<int32_t> &pixlist,
-                  std::vector<int32_t> &cnt) {
-  const int nth = (int)std::max<int64_t>(1, std::min<int64_t>(16, nsky / (1 << 18)));
-  std::vector<int64_t> seen((size_t)nth + 1, 0);
-  auto lo = [&](int w) { return nsky * w / nth; };
-  std::vector<std::thread> pool;
-  for (int w = 0; w < nth; w++)
-    pool.emplace_back([&, w]() {
-      int64_t n = 0;
-      for (int64_t i = lo(w); i < lo(w + 1); i++) n += hc[i] != 0;
-      seen[(size_t)w + 1] = n;
-    });
-  for (auto &t : pool) t.join();
-  pool.clear();
-  for (int w = 0; w < nth; w++) seen[(size_t)w + 1] += seen[(size_t)w];
-  indpix.resize((size_t)nsky);
-  pixlist.resize((size_t)seen[(size_t)nth]);
-  cnt.resize((size_t)seen[(size_t)nth]);
-  for (int w = 0; w < nth; w++)
-    pool.emplace_back([&, w]() {
-      int64_t k = seen[(size_t)w];
-      for (int64_t i = lo(w); i < lo(w + 1); i++) {
-        if (hc[i] != 0) {
-          indpix[(size_t)i] = (int32_t)k;
-          pixlist[(size_t)k] = (int32_t)i;
-          cnt[(size_t)k] = (int32_t)hc[i];
-          k++;
-        } else {
-          indpix[(size_t)i] = -1;
-        }
-      }
-    });
-  for (auto &t : pool) t.join();
-}
-template void compact_seen<int32_t>(const int32_t *, int64_t, std::vector<int32_t> &, std::vector<int32_t> &,
-                                    std::vector<int32_t> &);
-template void compact_seen<int64_t>(const int64_t *, int64_t, std::vector<int32_t> &, std::vector<int32_t> &,
-                                    std::vector<int32_t> &);
 
 template <>
 OpArgs<double> op_args<double>(sanepic_ctx *c) {
@@ -399,6 +355,20 @@ template int run_filter<double>(sanepic_ctx *, OpArgs<double>, int, int, int64_t
 template int run_filter<float>(sanepic_ctx *, OpArgs<float>, int, int, int64_t *);
 template int run_scatter<double>(sanepic_ctx *, const double *, double *, int);
 template int run_scatter<float>(sanepic_ctx *, const float *, double *, int);
+
+// The groups of a per-detector-length context run one after another on the
+// host and share their parent's staging buffer (one allocation instead of one
+// per group).
+int need_pinned(sanepic_ctx *c, int64_t bytes) {
+  sanepic_ctx *o = pin_ctx(c);
+  if (o->h_pin && o->h_pin_bytes >= bytes) return 0;
+  if (o->h_pin) (void)hipHostFree(o->h_pin);
+  o->h_pin = nullptr;
+  o->h_pin_bytes = 0;
+  HIPCHK(c, hipHostMalloc(&o->h_pin, (size_t)bytes, hipHostMallocDefault));
+  o->h_pin_bytes = bytes;
+  return 0;
+}
 
 }  // namespace sctx
 
@@ -740,55 +710,46 @@ int sanepic_set_tod(sanepic_ctx *c, int64_t idet, int64_t iseg, const double *to
 namespace {
 
 // NESTED-index order of the seen pixels (a space-filling curve on each base
-// face): h_mpos[p] = slot of compressed pixel p, h_slot2pix the inverse.
-// Leaves both empty when nside is not a power of two.
-int nested_order(sanepic_ctx *c) {
-  c->h_mpos.clear();
+// face), built on the device from the compact RING pixel list: each pixel's
+// NESTED index is its sort key.  Leaves c->mpos (pixel -> slot), c->d_slot2pix
+// (slot -> pixel, released once the P^T items are built) and c->h_slot2pix;
+// nothing when nside is not a power of two or 12 nside^2 exceeds 32 bits.
+int nested_order(sanepic_ctx *c, const int32_t *dlist) {
   c->h_slot2pix.clear();
   const int64_t npix = c->npix, S = c->nside;
-  if (npix <= 0 || (S & (S - 1)) != 0) return 0;
-  std::vector<int32_t> byidx((size_t)(12 * S * S), -1);
-  std::atomic<bool> bad{false};
-  const int nth = (int)std::max<int64_t>(1, std::min<int64_t>(16, npix / 65536));
-  std::vector<std::thread> pool;
-  for (int w = 0; w < nth; w++)
-    pool.emplace_back([&, w]() {
-      for (int64_t p = w; p < npix; p += nth) {
-        int f, x, y;
-        if (host_ring2xyf((long)S, (long)c->h_pixlist[p], &f, &x, &y)) {
-          bad = true;
-          return;
-        }
-        uint64_t m = 0;  // bit-interleaved (x, y): the nested index within the face
-        for (int b = 0; b < 31; b++)
-          m |= (uint64_t)((x >> b) & 1) << (2 * b) | (uint64_t)((y >> b) & 1) << (2 * b + 1);
-        byidx[(size_t)((int64_t)f * S * S + (int64_t)m)] = (int32_t)p;
-      }
-    });
-  for (auto &t : pool) t.join();
-  if (bad) return 0;
-  c->h_mpos.assign((size_t)npix, 0);
-  c->h_slot2pix.assign((size_t)npix, 0);
-  int32_t slot = 0;
-  for (int32_t p : byidx)
-    if (p >= 0) {
-      c->h_mpos[(size_t)p] = slot;
-      c->h_slot2pix[(size_t)slot] = p;
-      slot++;
-    }
-  if (slot != npix) return fail(c, SANEPIC_EINVAL, "nested ordering lost pixels");
-  return 0;
-}
-
-// Pinned host staging of at least `bytes` (kept for the context's lifetime:
-// freeing pinned memory would wait for the uploads in flight).
-int need_pinned(sanepic_ctx *c, int64_t bytes) {
-  if (c->h_pin && c->h_pin_bytes >= bytes) return 0;
-  if (c->h_pin) (void)hipHostFree(c->h_pin);
-  c->h_pin = nullptr;
-  c->h_pin_bytes = 0;
-  HIPCHK(c, hipHostMalloc(&c->h_pin, (size_t)bytes, hipHostMallocDefault));
-  c->h_pin_bytes = bytes;
+  if (npix <= 0 || (S & (S - 1)) != 0 || 12 * S * S > (int64_t)UINT32_MAX) return 0;
+  uint32_t *key = nullptr, *key_out = nullptr, *ids = nullptr;
+  int32_t *s2p = nullptr, *mpos = nullptr;
+  ACHK(c, dalloc(c, key, npix));
+  ACHK(c, dalloc(c, key_out, npix));
+  ACHK(c, dalloc(c, ids, npix));
+  ACHK(c, dalloc(c, s2p, npix));
+  ACHK(c, dalloc(c, mpos, npix));
+  LCHK(c, launch_nest_key(dlist, npix, S, key, c->st));
+  LCHK(c, launch_iota_u32(ids, npix, c->st));
+  const int end_bit = ilog2(12 * S * S);
+  size_t tb = 0;
+  HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(nullptr, tb, (const uint32_t *)key, key_out, (const uint32_t *)ids,
+                                               (uint32_t *)s2p, (int)npix, 0, end_bit, c->st));
+  char *tmp = nullptr;
+  ACHK(c, dalloc(c, tmp, (int64_t)tb));
+  HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(tmp, tb, (const uint32_t *)key, key_out, (const uint32_t *)ids,
+                                               (uint32_t *)s2p, (int)npix, 0, end_bit, c->st));
+  LCHK(c, launch_invperm((const uint32_t *)s2p, (uint32_t *)mpos, npix, c->st));
+  // slot -> pixel on the host (pixels read back in RING order) through pinned memory
+  ACHK(c, need_pinned(c, npix * (int64_t)sizeof(int32_t)));
+  const int32_t *pinned = (const int32_t *)pin_ctx(c)->h_pin;
+  int32_t *hpin = nullptr;
+  HIPCHK(c, hipHostGetDevicePointer((void **)&hpin, (void *)pinned, 0));
+  LCHK(c, launch_copy_i32(s2p, hpin, npix, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  c->h_slot2pix.assign(pinned, pinned + npix);
+  c->mpos = mpos;
+  c->d_slot2pix = s2p;
+  dfree_later(c, tmp, (int64_t)tb);
+  dfree_later(c, key, npix);
+  dfree_later(c, key_out, npix);
+  dfree_later(c, ids, npix);
   return 0;
 }
 
@@ -797,13 +758,14 @@ int need_pinned(sanepic_ctx *c, int64_t bytes) {
 // uploads still running on the copy engines.
 int h2d_staged(sanepic_ctx *c, void *dst, const void *src, int64_t bytes) {
   if (bytes <= 0) return 0;
-  if (bytes % 4 != 0 || !c->h_pin || c->h_pin_bytes < bytes) {
+  sanepic_ctx *o = pin_ctx(c);
+  if (bytes % 4 != 0 || !o->h_pin || o->h_pin_bytes < bytes) {
     HIPCHK(c, hipMemcpy(dst, src, (size_t)bytes, hipMemcpyHostToDevice));
     return 0;
   }
-  std::memcpy(c->h_pin, src, (size_t)bytes);
+  std::memcpy(o->h_pin, src, (size_t)bytes);
   int32_t *hs = nullptr;
-  HIPCHK(c, hipHostGetDevicePointer((void **)&hs, c->h_pin, 0));
+  HIPCHK(c, hipHostGetDevicePointer((void **)&hs, o->h_pin, 0));
   LCHK(c, launch_copy_i32(hs, (int32_t *)dst, bytes / 4, c->st));
   HIPCHK(c, hipStreamSynchronize(c->st));  // the staging is reused by the next copy
   return 0;
@@ -935,13 +897,15 @@ int sanepic_pixelize(sanepic_ctx *c, int64_t *npix_out, int64_t *n_deferred) {
     HIPCHK(c, hipcub::DeviceScan::ExclusiveSum(tmp, tb, flag, pos, (int)nsky, c->st));
     dfree_later(c, tmp, (int64_t)tb);
   }
-  ACHK(c, need_pinned(c, nsky * (int64_t)sizeof(int64_t)));
+  // room for the compact pixel list and counts: 2 npix values, npix <= min(nsky, ntot)
+  ACHK(c, need_pinned(c, 2 * std::min(nsky, ntot) * (int64_t)sizeof(int32_t) + 16));
+  const int32_t *pinned = (const int32_t *)pin_ctx(c)->h_pin;
   int32_t *hpin = nullptr;
-  HIPCHK(c, hipHostGetDevicePointer((void **)&hpin, c->h_pin, 0));
+  HIPCHK(c, hipHostGetDevicePointer((void **)&hpin, (void *)pinned, 0));
   LCHK(c, launch_copy_i32(pos + (nsky - 1), hpin, 1, c->st));
   LCHK(c, launch_copy_i32(flag + (nsky - 1), hpin + 1, 1, c->st));
   HIPCHK(c, hipStreamSynchronize(c->st));
-  const int64_t npix = (int64_t)((const int32_t *)c->h_pin)[0] + ((const int32_t *)c->h_pin)[1];
+  const int64_t npix = (int64_t)pinned[0] + pinned[1];
   c->npix = npix;
   ACHK(c, dalloc(c, c->cnt, npix));
   ACHK(c, dalloc(c, dlist, npix));
@@ -965,14 +929,19 @@ int sanepic_pixelize(sanepic_ctx *c, int64_t *npix_out, int64_t *n_deferred) {
   LCHK(c, launch_copy_i32(dlist, hpin, npix, c->st));
   LCHK(c, launch_copy_i32(c->cnt, hpin + npix, npix, c->st));
   HIPCHK(c, hipStreamSynchronize(c->st));
-  c->h_pixlist.assign((const int32_t *)c->h_pin, (const int32_t *)c->h_pin + npix);
-  c->h_cnt.assign((const int32_t *)c->h_pin + npix, (const int32_t *)c->h_pin + 2 * npix);
+  c->h_pixlist.assign(pinned, pinned + npix);
+  c->h_cnt.assign(pinned + npix, pinned + 2 * npix);
   c->h_indpix.clear();  // built from h_pixlist when asked for (sanepic_get_indpix)
   dfree_later(c, flag, nsky);
   dfree_later(c, pos, nsky);
   dfree_later(c, indpix, nsky);
-  dfree_later(c, dlist, npix);
-  if (c->nested_map || c->items_nested) ACHK(c, nested_order(c));
+  trace_mark("pixelize: pixel list on host");
+  if (c->nested_map || c->items_nested) ACHK(c, nested_order(c, dlist));
+  trace_mark("pixelize: nested order");
+  if (c->pin_owner)
+    c->d_pixlist = dlist;  // a group: its parent builds the union of the groups' pixels from it
+  else
+    dfree_later(c, dlist, npix);
   HIPCHK(c, hipStreamSynchronize(c->st));
   dfree_later(c, counts, nsky);
   // stable sort of samples by compressed pixel -> perm
@@ -996,6 +965,7 @@ int sanepic_pixelize(sanepic_ctx *c, int64_t *npix_out, int64_t *n_deferred) {
     dfree_later(c, keys_out, ntot);
     dfree_later(c, vals_in, ntot);
   }
+  trace_mark("pixelize: sample sort");
   if (c->zsorted) {
     ACHK(c, dalloc(c, c->rank, ntot));
     LCHK(c, launch_invperm(c->perm, c->rank, ntot, c->st));
@@ -1019,25 +989,39 @@ int sanepic_pixelize(sanepic_ctx *c, int64_t *npix_out, int64_t *n_deferred) {
   // pixels.  Items in NESTED order when enabled: pixels reduced at the same
   // time are neighbours on the sky, whose runs of z share cache lines (each
   // pixel's sums are unchanged).
-  const bool nest_items = c->items_nested && !c->h_slot2pix.empty();
+  trace_mark("pixelize: sorted angles");
+  const bool nest_items = c->items_nested && c->d_slot2pix != nullptr;
+  // item count in pixel order (it does not depend on the order); the rare
+  // split pixels then listed in item order with their partial slots
   std::vector<int32_t> spix, sfirst, scount;
   int64_t nit = 0;
   int32_t nslots = 0;
-  for (int64_t q = 0; q < npix; q++) {
-    const int64_t p = nest_items ? c->h_slot2pix[(size_t)q] : q;
+  for (int64_t p = 0; p < npix; p++) {
     const int64_t n = c->h_cnt[(size_t)p];
     if (n <= kChunk) {
       nit++;
-      continue;
+    } else {
+      nit += (n + kChunk - 1) / kChunk;
+      spix.push_back((int32_t)p);
     }
-    const int32_t k = (int32_t)((n + kChunk - 1) / kChunk);
-    spix.push_back((int32_t)p);
+  }
+  if (nest_items && !spix.empty()) {
+    std::vector<uint8_t> split((size_t)npix, 0);
+    for (int32_t p : spix) split[(size_t)p] = 1;
+    spix.clear();
+    for (int64_t q = 0; q < npix; q++) {
+      const int32_t p = c->h_slot2pix[(size_t)q];
+      if (split[(size_t)p]) spix.push_back(p);
+    }
+  }
+  for (int32_t p : spix) {
+    const int32_t k = (int32_t)((c->h_cnt[(size_t)p] + kChunk - 1) / kChunk);
     sfirst.push_back(nslots);
     scount.push_back(k);
     nslots += k;
-    nit += k;
   }
   const int64_t nsp = (int64_t)spix.size();
+  trace_mark("pixelize: item counts (host)");
   ACHK(c, dalloc(c, c->it_pix, nit));
   ACHK(c, dalloc(c, c->it_beg, nit));
   ACHK(c, dalloc(c, c->it_len, nit));
@@ -1046,11 +1030,8 @@ int sanepic_pixelize(sanepic_ctx *c, int64_t *npix_out, int64_t *n_deferred) {
   ACHK(c, dalloc(c, c->sp_first, nsp));
   ACHK(c, dalloc(c, c->sp_count, nsp));
   if (npix > 0) {
-    int32_t *order = nullptr, *kcnt = nullptr, *scnt = nullptr, *kofs = nullptr, *sofs = nullptr;
-    if (nest_items) {
-      ACHK(c, dalloc(c, order, npix));
-      ACHK(c, h2d_staged(c, order, c->h_slot2pix.data(), npix * (int64_t)sizeof(int32_t)));
-    }
+    const int32_t *order = nest_items ? c->d_slot2pix : nullptr;
+    int32_t *kcnt = nullptr, *scnt = nullptr, *kofs = nullptr, *sofs = nullptr;
     ACHK(c, dalloc(c, kcnt, npix));
     ACHK(c, dalloc(c, scnt, npix));
     ACHK(c, dalloc(c, kofs, npix));
@@ -1065,7 +1046,6 @@ int sanepic_pixelize(sanepic_ctx *c, int64_t *npix_out, int64_t *n_deferred) {
     LCHK(c, launch_item_fill(order, c->cnt, c->off, kofs, sofs, npix, kChunk, c->it_pix, c->it_beg, c->it_len,
                              c->it_slot, c->st));
     dfree_later(c, tmp, (int64_t)tb);
-    dfree_later(c, order, npix);
     dfree_later(c, kcnt, npix);
     dfree_later(c, scnt, npix);
     dfree_later(c, kofs, npix);
@@ -1076,6 +1056,7 @@ int sanepic_pixelize(sanepic_ctx *c, int64_t *npix_out, int64_t *n_deferred) {
     HIPCHK(c, hipMemcpy(c->sp_first, sfirst.data(), nsp * 4, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->sp_count, scount.data(), nsp * 4, hipMemcpyHostToDevice));
   }
+  dfree_later(c, c->d_slot2pix, npix);
   c->items = Items{c->it_pix, c->it_beg, c->it_len, c->it_slot, nit, c->sp_pix, c->sp_first, c->sp_count, nsp};
   c->nslots = nslots;
   // lanes per P^T item: 16 keep four entries each in flight from 64 entries
@@ -1238,10 +1219,7 @@ namespace {
 // sums of the operator are unchanged (only where P reads its values moves).
 // Needs nside a power of two; otherwise the map stays in ring order.
 int nest_gather_map(sanepic_ctx *c) {
-  if (c->h_mpos.empty()) return 0;
-  const int64_t npix = c->npix;
-  ACHK(c, dalloc(c, c->mpos, npix));
-  HIPCHK(c, hipMemcpyAsync(c->mpos, c->h_mpos.data(), npix * sizeof(int32_t), hipMemcpyHostToDevice, c->st));
+  if (!c->mpos) return 0;
   LCHK(c, launch_compress_pix(c->pix, c->ntot, c->mpos, c->st));
   HIPCHK(c, hipStreamSynchronize(c->st));
   c->pix_nested = true;
@@ -1334,7 +1312,10 @@ int sanepic_setup(sanepic_ctx *c) {
     ACHK(c, dalloc(c, c->map32, 3 * npix));
   }
   if (c->nested_map && !c->pix_nested) ACHK(c, nest_gather_map(c));
-  if (!c->nested_map) c->h_slot2pix.clear();  // pixels stay in ring order (get_pixels)
+  if (!c->nested_map) {  // pixels stay in ring order (get_pixels)
+    c->h_slot2pix.clear();
+    dfree_later(c, c->mpos, npix);
+  }
   HIPCHK(c, hipStreamSynchronize(c->st));
   trace_mark("setup: pair sums, vectors");
   flush_frees(c);

@@ -16,6 +16,7 @@
 // the parent's Q/U mask.  Every step is a kernel launch on one stream, so
 // the whole iteration can be captured in a HIP graph.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <map>
@@ -63,23 +64,61 @@ int group_pixelize(sanepic_ctx *c, int64_t *npix_out, int64_t *ndef_out) {
     if (rc) return fail(c, rc, s->err);
     ndef += nd;
   }
-  // union of the groups' seen pixels, ascending RING order (904-915)
+  // union of the groups' seen pixels, ascending RING order (904-915), on the
+  // device: the groups' hit counts added into one sky-sized array (a group
+  // lists each pixel once and the groups run in order on one stream), then
+  // the flag / prefix sum / compaction of a single context
   const int64_t nsky = 12 * c->nside * c->nside;
-  std::vector<int64_t> hc((size_t)nsky, 0);
-  for (sanepic_ctx *s : c->sub)
-    for (int64_t p = 0; p < s->npix; p++) hc[(size_t)s->h_pixlist[p]] += s->h_cnt[p];
-  compact_seen(hc.data(), nsky, c->h_indpix, c->h_pixlist, c->h_cnt);
-  c->npix = (int64_t)c->h_pixlist.size();
-  ACHK(c, dalloc(c, c->cnt, c->npix));
-  HIPCHK(c, hipMemcpy(c->cnt, c->h_cnt.data(), c->npix * sizeof(int32_t), hipMemcpyHostToDevice));
+  int64_t nsum = 0;
+  for (sanepic_ctx *s : c->sub) nsum += s->npix;
+  int32_t *counts = nullptr, *flag = nullptr, *pos = nullptr, *indpix = nullptr, *dlist = nullptr;
+  ACHK(c, dalloc(c, counts, nsky));
+  ACHK(c, dalloc(c, flag, nsky));
+  ACHK(c, dalloc(c, pos, nsky));
+  ACHK(c, dalloc(c, indpix, nsky));
+  HIPCHK(c, hipMemsetAsync(counts, 0, nsky * sizeof(int32_t), c->st));
+  for (sanepic_ctx *s : c->sub) LCHK(c, launch_add_at_i32(counts, s->d_pixlist, s->cnt, s->npix, c->st));
+  LCHK(c, launch_seen_flag(counts, nsky, flag, c->st));
+  {
+    size_t tb = 0;
+    HIPCHK(c, hipcub::DeviceScan::ExclusiveSum(nullptr, tb, flag, pos, (int)nsky, c->st));
+    char *tmp = nullptr;
+    ACHK(c, dalloc(c, tmp, (int64_t)tb));
+    HIPCHK(c, hipcub::DeviceScan::ExclusiveSum(tmp, tb, flag, pos, (int)nsky, c->st));
+    dfree_later(c, tmp, (int64_t)tb);
+  }
+  ACHK(c, need_pinned(c, 2 * std::min(nsky, nsum) * (int64_t)sizeof(int32_t) + 16));
+  const int32_t *pinned = (const int32_t *)pin_ctx(c)->h_pin;
+  int32_t *hpin = nullptr;
+  HIPCHK(c, hipHostGetDevicePointer((void **)&hpin, (void *)pinned, 0));
+  LCHK(c, launch_copy_i32(pos + (nsky - 1), hpin, 1, c->st));
+  LCHK(c, launch_copy_i32(flag + (nsky - 1), hpin + 1, 1, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  const int64_t npix = (int64_t)pinned[0] + pinned[1];
+  c->npix = npix;
+  ACHK(c, dalloc(c, c->cnt, npix));
+  ACHK(c, dalloc(c, dlist, npix));
+  LCHK(c, launch_seen_compact(counts, pos, nsky, indpix, dlist, c->cnt, c->st));
+  // group pixel -> union pixel: each group's RING list through the union index
   c->gmap.assign(c->sub.size(), nullptr);
   for (size_t k = 0; k < c->sub.size(); k++) {
     sanepic_ctx *s = c->sub[k];
-    std::vector<int32_t> gm((size_t)s->npix);
-    for (int64_t p = 0; p < s->npix; p++) gm[(size_t)p] = c->h_indpix[(size_t)s->h_pixlist[p]];
     ACHK(c, dalloc(c, c->gmap[k], s->npix));
-    HIPCHK(c, hipMemcpy(c->gmap[k], gm.data(), s->npix * sizeof(int32_t), hipMemcpyHostToDevice));
+    LCHK(c, launch_copy_i32(s->d_pixlist, c->gmap[k], s->npix, c->st));
+    LCHK(c, launch_compress_pix(c->gmap[k], s->npix, indpix, c->st));
   }
+  LCHK(c, launch_copy_i32(dlist, hpin, npix, c->st));
+  LCHK(c, launch_copy_i32(c->cnt, hpin + npix, npix, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  c->h_pixlist.assign(pinned, pinned + npix);
+  c->h_cnt.assign(pinned + npix, pinned + 2 * npix);
+  c->h_indpix.clear();  // built from h_pixlist when asked for (sanepic_get_indpix)
+  for (sanepic_ctx *s : c->sub) dfree_later(s, s->d_pixlist, s->npix);
+  dfree_later(c, counts, nsky);
+  dfree_later(c, flag, nsky);
+  dfree_later(c, pos, nsky);
+  dfree_later(c, indpix, nsky);
+  dfree_later(c, dlist, npix);
   dfree_later(c, c->ra, c->tl_len);  // hipFree would wait for the TOD uploads in flight
   dfree_later(c, c->dec, c->tl_len);
   dfree_later(c, c->psi, c->tl_len);
@@ -242,11 +281,7 @@ int sanepic_create_uneven(const sanepic_uneven_config *cfg, sanepic_ctx **out) {
     (void)hipStreamDestroy(s->st);
     s->st = c->st;
     s->borrowed = true;
-    // NESTED-order gather map and items cost each group a host pass over
-    // 12 nside^2 sky pixels (~70 ms at NSIDE 1024, x16 groups at 64 detectors)
-    // for ~0.5 ms per iteration: groups keep RING order
-    s->nested_map = false;
-    s->items_nested = false;
+    s->pin_owner = c;
     for (size_t j = 0; j < gdet[g].size(); j++) {
       c->det_group[(size_t)gdet[g][j]] = (int32_t)g;
       c->det_local[(size_t)gdet[g][j]] = (int32_t)j;

@@ -76,11 +76,13 @@ struct sanepic_ctx {
   int32_t *mpos = nullptr;           // [npix]
   double *map64 = nullptr;           // [npix][3] FP64 gather map in slot order
   std::vector<int32_t> h_slot2pix;   // slot -> compressed pixel
-  std::vector<int32_t> h_mpos;       // compressed pixel -> slot
+  int32_t *d_slot2pix = nullptr;     // the same on the device, until the P^T items are built
+  int32_t *d_pixlist = nullptr;      // a group's compact RING pixel list, until its parent's union is built
   bool items_nested = true;          // P^T items in NESTED pixel order
   int scatter_lanes = 16;            // P^T lanes per item (from the mean item length at pixelisation)
   void *h_pin = nullptr;             // pinned host staging written by kernels (pixel counts)
   int64_t h_pin_bytes = 0;
+  sanepic_ctx *pin_owner = nullptr;  // a group context stages through its parent's buffer
   bool xcd_blocks = true;            // XCD-contiguous numbering of the four-step blocks
   bool zsorted = false;              // the filter output z is written in pixel-sorted order
   int col_waves = 3;                 // occupancy target of the FP32 gather column pass (0, 3, 4)
@@ -264,6 +266,9 @@ inline void trace_mark(const char *what) {
   last = now;
 }
 
+// The context whose pinned staging buffer a call uses.
+inline sanepic_ctx *pin_ctx(sanepic_ctx *c) { return c->pin_owner ? c->pin_owner : c; }
+
 inline int sync_tod_uploads(sanepic_ctx *c) {
   for (hipStream_t s : c->up_tod)
     if (s && hipStreamSynchronize(s) != hipSuccess) {
@@ -281,9 +286,9 @@ inline int sync_tod_uploads(sanepic_ctx *c) {
 
 
 // Filter / scatter passes and the operator (capi.hip).
-template <typename C>
-void compact_seen(const C *hc, int64_t nsky, std::vector<int32_t> &indpix, std::vector<int32_t> &pixlist,
-                  std::vector<int32_t> &cnt);
+// Pinned host staging of at least `bytes` (pin_ctx(c)->h_pin; kept for the
+// context's lifetime, since freeing pinned memory waits for uploads in flight).
+int need_pinned(sanepic_ctx *c, int64_t bytes);
 template <typename T> OpArgs<T> op_args(sanepic_ctx *c);
 template <> OpArgs<double> op_args<double>(sanepic_ctx *c);
 template <> OpArgs<float> op_args<float>(sanepic_ctx *c);

@@ -214,6 +214,10 @@ int launch_item_fill(const int32_t *order, const int32_t *cnt, const int64_t *of
 int launch_map_slots(const double *x, const int32_t *slot, double *map, int64_t npix, hipStream_t s);
 int launch_map_slots(const double *x, const int32_t *slot, float *map, int64_t npix, hipStream_t s);
 int launch_invperm(const uint32_t *perm, uint32_t *rank, int64_t n, hipStream_t s);  // rank[perm[j]] = j
+// dst[idx[i]] += val[i] for distinct idx within the launch
+int launch_add_at_i32(int32_t *dst, const int32_t *idx, const int32_t *val, int64_t n, hipStream_t s);
+// key[i] = NESTED index of RING pixel ring[i] (power-of-two nside, 12 nside^2 < 2^32)
+int launch_nest_key(const int32_t *ring, int64_t n, int64_t nside, uint32_t *key, hipStream_t s);
 
 // CG on interleaved [npix][3] FP64 vectors with FP64 scalars on the device.
 struct CGScalars {

@@ -420,6 +420,85 @@ __global__ void k_abs(const double *src, double *dst, int64_t n) {
   if (i < n) dst[i] = fabs(src[i]);
 }
 
+__device__ __forceinline__ int64_t isqrt_dev(int64_t v) {
+  int64_t r = (int64_t)sqrt((double)v);
+  while (r * r > v) r--;
+  while ((r + 1) * (r + 1) <= v) r++;
+  return r;
+}
+
+// NESTED index of RING pixel `pix` at power-of-two nside `ns`: the pixel's
+// centre (z, phi) from its ring and position in the ring, then the face and
+// in-face (x, y) of that direction, bit-interleaved.  Centres lie half a pixel
+// from every face and pixel boundary, so FP64 rounding cannot move them.
+__device__ uint32_t ring_to_nest(int64_t ns, int64_t pix) {
+  constexpr double kPi = 3.14159265358979323846;
+  const int64_t npix = 12 * ns * ns, ncap = 2 * ns * (ns - 1);
+  double z, phi;
+  if (pix < ncap) {
+    const int64_t ir = (1 + isqrt_dev(1 + 2 * pix)) >> 1;
+    const int64_t iphi = pix + 1 - 2 * ir * (ir - 1);
+    z = 1.0 - (double)(ir * ir) / (3.0 * (double)(ns * ns));
+    phi = ((double)iphi - 0.5) * kPi / (2.0 * (double)ir);
+  } else if (pix < npix - ncap) {
+    const int64_t ip = pix - ncap;
+    const int64_t ir = ip / (4 * ns) + ns;
+    const int64_t iphi = ip % (4 * ns) + 1;
+    const double fodd = ((ir + ns) & 1) ? 1.0 : 0.5;
+    z = (double)(2 * ns - ir) * 2.0 / (3.0 * (double)ns);
+    phi = ((double)iphi - fodd) * kPi / (2.0 * (double)ns);
+  } else {
+    const int64_t ip = npix - pix;
+    const int64_t ir = (1 + isqrt_dev(2 * ip - 1)) >> 1;
+    const int64_t iphi = 4 * ir + 1 - (ip - 2 * ir * (ir - 1));
+    z = -1.0 + (double)(ir * ir) / (3.0 * (double)(ns * ns));
+    phi = ((double)iphi - 0.5) * kPi / (2.0 * (double)ir);
+  }
+  const double za = fabs(z);
+  double tt = fmod(phi * (2.0 / kPi), 4.0);
+  if (tt < 0.0) tt += 4.0;
+  int64_t f, x, y;
+  if (za <= 2.0 / 3.0) {
+    const double t1 = (double)ns * (0.5 + tt), t2 = (double)ns * (z * 0.75);
+    const int64_t jp = (int64_t)(t1 - t2), jm = (int64_t)(t1 + t2);
+    const int64_t ifp = jp / ns, ifm = jm / ns;
+    f = ifp == ifm ? (ifp | 4) : (ifp < ifm ? ifp : ifm + 8);
+    x = jm & (ns - 1);
+    y = ns - (jp & (ns - 1)) - 1;
+  } else {
+    const int64_t ntt = tt < 3.0 ? (int64_t)tt : 3;
+    const double tp = tt - (double)ntt;
+    const double tmp = (double)ns * sqrt(3.0 * (1.0 - za));
+    int64_t jp = (int64_t)(tp * tmp), jm = (int64_t)((1.0 - tp) * tmp);
+    jp = jp < ns - 1 ? jp : ns - 1;
+    jm = jm < ns - 1 ? jm : ns - 1;
+    if (z >= 0) {
+      f = ntt;
+      x = ns - jm - 1;
+      y = ns - jp - 1;
+    } else {
+      f = ntt + 8;
+      x = jp;
+      y = jm;
+    }
+  }
+  uint64_t m = 0;
+  for (int b = 0; b < 31 && (int64_t(1) << b) < ns; b++)
+    m |= (uint64_t)((x >> b) & 1) << (2 * b) | (uint64_t)((y >> b) & 1) << (2 * b + 1);
+  return (uint32_t)(f * ns * ns + (int64_t)m);
+}
+
+// dst[idx[i]] += val[i]; the indices of one launch are distinct
+__global__ void k_add_at_i32(int32_t *dst, const int32_t *idx, const int32_t *val, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) dst[idx[i]] += val[i];
+}
+
+__global__ void k_nest_key(const int32_t *ring, int64_t n, int64_t ns, uint32_t *key) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) key[i] = ring_to_nest(ns, ring[i]);
+}
+
 __global__ void k_invperm(const uint32_t *perm, uint32_t *rank, int64_t n) {
   const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (j < n) rank[perm[j]] = (uint32_t)j;
@@ -573,6 +652,16 @@ int launch_map_slots(const double *x, const int32_t *slot, float *map, int64_t n
 
 int launch_abs(const double *src, double *dst, int64_t n, hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(k_abs, dim3(nblk(n)), dim3(256), 0, s, src, dst, n);
+  return ok();
+}
+
+int launch_add_at_i32(int32_t *dst, const int32_t *idx, const int32_t *val, int64_t n, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_add_at_i32, dim3(nblk(n)), dim3(256), 0, s, dst, idx, val, n);
+  return ok();
+}
+
+int launch_nest_key(const int32_t *ring, int64_t n, int64_t nside, uint32_t *key, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_nest_key, dim3(nblk(n)), dim3(256), 0, s, ring, n, nside, key);
   return ok();
 }
 
