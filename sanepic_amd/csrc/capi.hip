@@ -1221,22 +1221,16 @@ namespace {
 int nest_gather_map(sanepic_ctx *c) {
   if (!c->mpos) return 0;
   LCHK(c, launch_compress_pix(c->pix, c->ntot, c->mpos, c->st));
-  HIPCHK(c, hipStreamSynchronize(c->st));
   c->pix_nested = true;
   return 0;
 }
 
 }  // namespace
 
-int sanepic_setup(sanepic_ctx *c) {
-  ACHK(c, check_ready(c, false));
-  if (c->setup_done) return fail(c, SANEPIC_EINVAL, "sanepic_setup already ran");
-  if (!c->sub.empty()) return group_setup(c);
-  trace_mark("setup: enter");
-  ACHK(c, sync_tod_uploads(c));
-  trace_mark("setup: TOD uploads done");
-  for (size_t i = 0; i < c->tod_ok.size(); i++)
-    if (!c->tod_ok[i]) return fail(c, SANEPIC_EINVAL, "TOD block " + std::to_string(i) + " not set");
+extern "C++" {
+namespace sctx {
+
+int setup_pointing(sanepic_ctx *c) {
   const int64_t npix = c->npix, ntot = c->ntot, N = c->ns;
   ACHK(c, build_tables(c));
   trace_mark("setup: noise tables");
@@ -1248,16 +1242,6 @@ int sanepic_setup(sanepic_ctx *c) {
   c->nchi2 = nchi;
   ACHK(c, dalloc(c, c->b_raw, 3 * npix));
   ACHK(c, dalloc(c, c->b, 3 * npix));
-  // b = P^T N^-1 d  (1069-1125); the data spectrum is kept for chi2
-  LCHK(c, launch_tod_nanfill(c->tod, ntot, c->ns, c->st));
-  {
-    OpArgs<double> a = op_args<double>(c);
-    ACHK(c, run_filter<double>(c, a, SRC_TOD, ROW_FWD | ROW_STORE_D | ROW_FILTER | ROW_INV, nullptr));
-    ACHK(c, run_scatter<double>(c, c->z, c->b_raw, 0));
-  }
-  HIPCHK(c, hipStreamSynchronize(c->st));
-  trace_mark("setup: P^T N^-1 d");
-  dfree(c, c->tod, ntot);
   // noise autocorrelation N_ = c2r(|SpN|) per detector (2698-2706)
   {
     double *dwabs = nullptr;
@@ -1295,9 +1279,6 @@ int sanepic_setup(sanepic_ctx *c) {
   pa.out = c->pairs6;
   pa.partial = c->partial;
   LCHK(c, launch_pairsum(pa, c->st));
-  // masked right-hand side
-  HIPCHK(c, hipMemcpyAsync(c->b, c->b_raw, 3 * npix * sizeof(double), hipMemcpyDeviceToDevice, c->st));
-  LCHK(c, cg_launch_mask(c->b, c->cnt, npix, c->st));
   // CG vectors
   ACHK(c, dalloc(c, c->x, 3 * npix));
   ACHK(c, dalloc(c, c->r, 3 * npix));
@@ -1316,8 +1297,43 @@ int sanepic_setup(sanepic_ctx *c) {
     c->h_slot2pix.clear();
     dfree_later(c, c->mpos, npix);
   }
+  trace_mark("setup: pair sums, vectors (queued)");
+  return 0;
+}
+
+int setup_data(sanepic_ctx *c) {
+  const int64_t npix = c->npix, ntot = c->ntot;
+  // b = P^T N^-1 d  (1069-1125); the data spectrum is kept for chi2
+  LCHK(c, launch_tod_nanfill(c->tod, ntot, c->ns, c->st));
+  {
+    OpArgs<double> a = op_args<double>(c);
+    ACHK(c, run_filter<double>(c, a, SRC_TOD, ROW_FWD | ROW_STORE_D | ROW_FILTER | ROW_INV, nullptr));
+    ACHK(c, run_scatter<double>(c, c->z, c->b_raw, 0));
+  }
+  // masked right-hand side
+  HIPCHK(c, hipMemcpyAsync(c->b, c->b_raw, 3 * npix * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+  LCHK(c, cg_launch_mask(c->b, c->cnt, npix, c->st));
   HIPCHK(c, hipStreamSynchronize(c->st));
-  trace_mark("setup: pair sums, vectors");
+  trace_mark("setup: P^T N^-1 d");
+  dfree(c, c->tod, ntot);
+  return 0;
+}
+
+}  // namespace sctx
+}  // extern "C++"
+
+int sanepic_setup(sanepic_ctx *c) {
+  ACHK(c, check_ready(c, false));
+  if (c->setup_done) return fail(c, SANEPIC_EINVAL, "sanepic_setup already ran");
+  if (!c->sub.empty()) return group_setup(c);
+  trace_mark("setup: enter");
+  for (size_t i = 0; i < c->tod_ok.size(); i++)
+    if (!c->tod_ok[i]) return fail(c, SANEPIC_EINVAL, "TOD block " + std::to_string(i) + " not set");
+  // the work that needs no TOD first, while the TOD upload may still run
+  ACHK(c, setup_pointing(c));
+  ACHK(c, sync_tod_uploads(c));
+  trace_mark("setup: TOD uploads done");
+  ACHK(c, setup_data(c));
   flush_frees(c);
   trace_mark("setup: deferred frees");
   c->setup_done = true;

@@ -132,17 +132,24 @@ int group_pixelize(sanepic_ctx *c, int64_t *npix_out, int64_t *ndef_out) {
 
 int group_setup(sanepic_ctx *c) {
   trace_mark("group setup: enter");
-  ACHK(c, sync_tod_uploads(c));
-  trace_mark("group setup: TOD uploads done");
   for (int64_t d = 0; d < c->ndet; d++) {
     const sanepic_ctx *s = c->sub[(size_t)c->det_group[(size_t)d]];
     if (c->tod_set[(size_t)d] < s->nseg * s->ns)
       return fail(c, SANEPIC_EINVAL, "TOD of detector " + std::to_string(d) + " not fully set");
   }
+  // every group's TOD-free set-up first, while the TOD upload may still run
   for (sanepic_ctx *s : c->sub) {
     std::fill(s->tod_ok.begin(), s->tod_ok.end(), 1);
-    const int rc = sanepic_setup(s);
+    const int rc = setup_pointing(s);
     if (rc) return fail(c, rc, s->err);
+  }
+  ACHK(c, sync_tod_uploads(c));
+  trace_mark("group setup: TOD uploads done");
+  for (sanepic_ctx *s : c->sub) {
+    const int rc = setup_data(s);
+    if (rc) return fail(c, rc, s->err);
+    flush_frees(s);
+    s->setup_done = true;
   }
   const int64_t npix = c->npix;
   ACHK(c, dalloc(c, c->b_raw, 3 * npix));

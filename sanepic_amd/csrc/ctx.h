@@ -289,6 +289,11 @@ inline int sync_tod_uploads(sanepic_ctx *c) {
 // Pinned host staging of at least `bytes` (pin_ctx(c)->h_pin; kept for the
 // context's lifetime, since freeing pinned memory waits for uploads in flight).
 int need_pinned(sanepic_ctx *c, int64_t bytes);
+// Set-up in two halves: everything that needs no TOD (noise tables, noise
+// autocorrelation, crosslinks, pair sums, the gather map, CG buffers), queued
+// while the TOD upload may still run; then b = P^T N^-1 d from the TOD.
+int setup_pointing(sanepic_ctx *c);
+int setup_data(sanepic_ctx *c);
 template <typename T> OpArgs<T> op_args(sanepic_ctx *c);
 template <> OpArgs<double> op_args<double>(sanepic_ctx *c);
 template <> OpArgs<float> op_args<float>(sanepic_ctx *c);
