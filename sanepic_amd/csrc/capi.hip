@@ -753,24 +753,6 @@ int nested_order(sanepic_ctx *c, const int32_t *dlist) {
   return 0;
 }
 
-// Host -> device copy during the set-up: through the pinned staging and a
-// copy kernel that reads host memory, so it does not queue behind the input
-// uploads still running on the copy engines.
-int h2d_staged(sanepic_ctx *c, void *dst, const void *src, int64_t bytes) {
-  if (bytes <= 0) return 0;
-  sanepic_ctx *o = pin_ctx(c);
-  if (bytes % 4 != 0 || !o->h_pin || o->h_pin_bytes < bytes) {
-    HIPCHK(c, hipMemcpy(dst, src, (size_t)bytes, hipMemcpyHostToDevice));
-    return 0;
-  }
-  std::memcpy(o->h_pin, src, (size_t)bytes);
-  int32_t *hs = nullptr;
-  HIPCHK(c, hipHostGetDevicePointer((void **)&hs, o->h_pin, 0));
-  LCHK(c, launch_copy_i32(hs, (int32_t *)dst, bytes / 4, c->st));
-  HIPCHK(c, hipStreamSynchronize(c->st));  // the staging is reused by the next copy
-  return 0;
-}
-
 }  // namespace
 
 int sanepic_pixelize(sanepic_ctx *c, int64_t *npix_out, int64_t *n_deferred) {
