@@ -248,6 +248,8 @@ int run_scatter(sanepic_ctx *c, const T *z, double *q, int mask) {
   s.lanes = c->scatter_lanes;
   s.cnt = c->cnt;
   s.q = q;
+  s.qidx = c->gsrc;
+  s.accumulate = c->gsrc ? 1 : 0;
   s.partial = c->partial;
   s.mask = mask;
   ProfScope ps(c, 3);
@@ -266,9 +268,9 @@ int apply_A(sanepic_ctx *c, const double *x, double *q, int mask, bool chi2, boo
   if (use64) {
     OpArgs<double> a = op_args<double>(c);
     a.map = x;
-    if (c->pix_nested) {
+    if (c->pix_nested || c->gsrc) {
       ACHK(c, dalloc(c, c->map64, 3 * c->npix));
-      LCHK(c, launch_map_slots(x, c->mpos, c->map64, c->npix, c->st));
+      LCHK(c, launch_map_slots(x, c->gsrc, c->pix_nested ? c->mpos : nullptr, c->map64, c->npix, c->st));
       a.map = c->map64;
     }
     int src = SRC_GATHER;
@@ -282,8 +284,8 @@ int apply_A(sanepic_ctx *c, const double *x, double *q, int mask, bool chi2, boo
     ACHK(c, run_filter<double>(c, a, src, mode, &nb));
     ACHK(c, run_scatter<double>(c, c->z, q, mask));
   } else {
-    if (c->pix_nested)
-      LCHK(c, launch_map_slots(x, c->mpos, c->map32, c->npix, c->st));
+    if (c->pix_nested || c->gsrc)
+      LCHK(c, launch_map_slots(x, c->gsrc, c->pix_nested ? c->mpos : nullptr, c->map32, c->npix, c->st));
     else
       LCHK(c, launch_d2f(x, c->map32, 3 * c->npix, c->st));
     OpArgs<float> a = op_args<float>(c);

@@ -32,12 +32,15 @@ namespace sctx {
 int group_apply_A(sanepic_ctx *c, const double *x, double *q, int mask, bool chi2, bool fp64) {
   LCHK(c, launch_fill(q, 0.0, 3 * c->npix, c->st));
   if (chi2) LCHK(c, launch_fill(&c->sc->chi2, 0.0, 1, c->st));
+  // each group reads its pixels of x through gmap into its gather map and its
+  // P^T adds into q at the same rows (a group lists a pixel once; the groups
+  // run in order on one stream)
   for (size_t k = 0; k < c->sub.size(); k++) {
     sanepic_ctx *s = c->sub[k];
-    LCHK(c, launch_group_gather(x, c->gmap[k], s->x, s->npix, 3, c->st));
-    const int rc = apply_A(s, s->x, s->q, 0, chi2, fp64);
+    s->gsrc = c->gmap[k];
+    const int rc = apply_A(s, x, q, 0, chi2, fp64);
+    s->gsrc = nullptr;
     if (rc) return fail(c, rc, s->err);
-    LCHK(c, launch_group_scatter_add(q, c->gmap[k], s->q, s->npix, 3, c->st));
     if (chi2) LCHK(c, launch_add_scalar(&c->sc->chi2, &s->sc->chi2, c->st));
   }
   if (mask) LCHK(c, cg_launch_mask(q, c->cnt, c->npix, c->st));

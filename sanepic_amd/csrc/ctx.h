@@ -106,6 +106,9 @@ struct sanepic_ctx {
   std::vector<sanepic_ctx *> sub;
   std::vector<int32_t> det_group, det_local;   // per detector
   std::vector<int32_t *> gmap;                 // per group: group pixel -> pixel
+  // a group during its parent's operator: its pixels' rows in the parent's x
+  // and q (the group reads x and adds into q directly)
+  const int32_t *gsrc = nullptr;
   int64_t tl_len = 0;                          // boresight timeline samples
   std::vector<std::pair<int64_t, int64_t>> tl_set;  // pointing ranges uploaded
   std::vector<int64_t> tod_set;                // samples uploaded per detector

@@ -145,7 +145,9 @@ struct ScatterArgs {
   int xcd;                // XCD-contiguous item ranges (neighbouring pixels share an L2)
   int lanes;              // lanes per item: 4, 8 or 16 (fewer for short items; 0 means 16)
   const int32_t *cnt;     // [npix] samples per pixel (mask: cnt < 2)
-  double *q;              // [npix][3] output
+  double *q;              // [npix][3] output (pixel p at qidx[p] when qidx is set)
+  const int32_t *qidx;    // output row of each pixel, or null (a detector group into its parent's map)
+  int accumulate;         // add into q instead of writing it
   double *partial;        // [nslots][3]
   int mask;               // apply the Q/U mask
 };
@@ -210,9 +212,12 @@ int launch_item_count(const int32_t *order, const int32_t *cnt, int64_t npix, in
 int launch_item_fill(const int32_t *order, const int32_t *cnt, const int64_t *off, const int32_t *kofs,
                      const int32_t *sofs, int64_t npix, int chunk, int32_t *it_pix, int64_t *it_beg,
                      int32_t *it_len, int32_t *it_slot, hipStream_t s);
-// Gather map in slot order: map[3 slot[p] + k] = x[3 p + k] (interleaved x).
-int launch_map_slots(const double *x, const int32_t *slot, double *map, int64_t npix, hipStream_t s);
-int launch_map_slots(const double *x, const int32_t *slot, float *map, int64_t npix, hipStream_t s);
+// Gather map in slot order: map[3 slot[p] + k] = x[3 src[p] + k] (interleaved x;
+// src reads a detector group's pixels from its parent's map; null = identity).
+int launch_map_slots(const double *x, const int32_t *src, const int32_t *slot, double *map, int64_t npix,
+                     hipStream_t s);
+int launch_map_slots(const double *x, const int32_t *src, const int32_t *slot, float *map, int64_t npix,
+                     hipStream_t s);
 int launch_invperm(const uint32_t *perm, uint32_t *rank, int64_t n, hipStream_t s);  // rank[perm[j]] = j
 // dst[idx[i]] += val[i] for distinct idx within the launch
 int launch_add_at_i32(int32_t *dst, const int32_t *idx, const int32_t *val, int64_t n, hipStream_t s);

@@ -109,10 +109,16 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterArgs<T> a) {
         q1 = 0.0;
         q2 = 0.0;
       }
-      double *q = a.q + 3 * (int64_t)p;
-      q[0] = q0;
-      q[1] = q1;
-      q[2] = q2;
+      double *q = a.q + 3 * (int64_t)(a.qidx ? a.qidx[p] : p);
+      if (a.accumulate) {
+        q[0] += q0;
+        q[1] += q1;
+        q[2] += q2;
+      } else {
+        q[0] = q0;
+        q[1] = q1;
+        q[2] = q2;
+      }
     } else {
       double *pp = a.partial + 3 * (int64_t)slot;
       pp[0] = s0;
@@ -124,7 +130,8 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterArgs<T> a) {
 
 template <int NV>
 __global__ __launch_bounds__(256) void k_fix(Items it, const double *partial, double *out,
-                                             const int32_t *cnt, int mask) {
+                                             const int32_t *cnt, int mask, const int32_t *oidx,
+                                             int accumulate) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= it.nsplit) return;
   const int p = it.split_pix[i];
@@ -142,8 +149,9 @@ __global__ __launch_bounds__(256) void k_fix(Items it, const double *partial, do
       acc[2] = 0.0;
     }
   }
+  double *o = out + (int64_t)(oidx ? oidx[p] : p) * NV;
 #pragma unroll
-  for (int v = 0; v < NV; v++) out[(int64_t)p * NV + v] = acc[v];
+  for (int v = 0; v < NV; v++) o[v] = accumulate ? o[v] + acc[v] : acc[v];
 }
 
 __global__ __launch_bounds__(256) void k_crosslink(Items it, const double2 *cs, double *out,
@@ -405,14 +413,16 @@ __global__ void k_copy_i32(const int32_t *src, int32_t *dst, int64_t n) {
   if (i < n) dst[i] = src[i];
 }
 
+// map[slot[p]] = x[src[p]] per pixel p (3 values; a null index is the identity)
 template <typename T>
-__global__ void k_map_slots(const double *x, const int32_t *slot, T *map, int64_t npix) {
+__global__ void k_map_slots(const double *x, const int32_t *src, const int32_t *slot, T *map, int64_t npix) {
   const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (p >= npix) return;
-  T *m = map + 3 * (int64_t)slot[p];
-  m[0] = (T)x[3 * p];
-  m[1] = (T)x[3 * p + 1];
-  m[2] = (T)x[3 * p + 2];
+  const double *xs = x + 3 * (int64_t)(src ? src[p] : p);
+  T *m = map + 3 * (int64_t)(slot ? slot[p] : p);
+  m[0] = (T)xs[0];
+  m[1] = (T)xs[1];
+  m[2] = (T)xs[2];
 }
 
 __global__ void k_abs(const double *src, double *dst, int64_t n) {
@@ -546,7 +556,7 @@ int launch_scatter(const ScatterArgs<T> &a, hipStream_t s) {
   }
   if (a.it.nsplit > 0)
     hipLaunchKernelGGL(k_fix<3>, dim3(nblk(a.it.nsplit)), dim3(256), 0, s, a.it, a.partial, a.q,
-                       a.cnt, a.mask);
+                       a.cnt, a.mask, a.qidx, a.accumulate);
   return ok();
 }
 template int launch_scatter<double>(const ScatterArgs<double> &, hipStream_t);
@@ -559,7 +569,7 @@ int launch_crosslink(const Items &it, const double2 *cs_sorted, double *out, dou
                        partial);
   if (it.nsplit > 0)
     hipLaunchKernelGGL(k_fix<2>, dim3(nblk(it.nsplit)), dim3(256), 0, s, it, partial, out,
-                       (const int32_t *)nullptr, 0);
+                       (const int32_t *)nullptr, 0, (const int32_t *)nullptr, 0);
   return ok();
 }
 
@@ -567,7 +577,7 @@ int launch_pairsum(const PairSumArgs &a, hipStream_t s) {
   if (a.it.n > 0) hipLaunchKernelGGL(k_pairsum, dim3(nblk(a.it.n, 16)), dim3(256), 0, s, a);
   if (a.it.nsplit > 0)
     hipLaunchKernelGGL(k_fix<8>, dim3(nblk(a.it.nsplit)), dim3(256), 0, s, a.it, a.partial, a.out,
-                       (const int32_t *)nullptr, 0);
+                       (const int32_t *)nullptr, 0, (const int32_t *)nullptr, 0);
   return ok();
 }
 
@@ -640,13 +650,15 @@ int launch_copy_i32(const int32_t *src, int32_t *dst, int64_t n, hipStream_t s) 
   return ok();
 }
 
-int launch_map_slots(const double *x, const int32_t *slot, double *map, int64_t npix, hipStream_t s) {
-  if (npix > 0) hipLaunchKernelGGL(k_map_slots<double>, dim3(nblk(npix)), dim3(256), 0, s, x, slot, map, npix);
+int launch_map_slots(const double *x, const int32_t *src, const int32_t *slot, double *map, int64_t npix,
+                     hipStream_t s) {
+  if (npix > 0) hipLaunchKernelGGL(k_map_slots<double>, dim3(nblk(npix)), dim3(256), 0, s, x, src, slot, map, npix);
   return ok();
 }
 
-int launch_map_slots(const double *x, const int32_t *slot, float *map, int64_t npix, hipStream_t s) {
-  if (npix > 0) hipLaunchKernelGGL(k_map_slots<float>, dim3(nblk(npix)), dim3(256), 0, s, x, slot, map, npix);
+int launch_map_slots(const double *x, const int32_t *src, const int32_t *slot, float *map, int64_t npix,
+                     hipStream_t s) {
+  if (npix > 0) hipLaunchKernelGGL(k_map_slots<float>, dim3(nblk(npix)), dim3(256), 0, s, x, src, slot, map, npix);
   return ok();
 }
 
