@@ -265,9 +265,8 @@ int launch_block_to_inter(const double *blk, double *inter, int64_t npix, hipStr
 int launch_inter_to_block(const double *inter, double *blk, int64_t npix, hipStream_t s);
 int launch_d2f(const double *a, float *b, int64_t n, hipStream_t s);
 
-// Detector groups (contexts with per-detector segment lengths): move
-// [npix][width] vectors between the parent's pixels and a group's pixels.
-int launch_group_gather(const double *x, const int32_t *gmap, double *xg, int64_t npg, int width, hipStream_t s);
+// Detector groups (contexts with per-detector segment lengths): add a group's
+// [npix][width] vector into the parent's pixels.
 int launch_group_scatter_add(double *x, const int32_t *gmap, const double *xg, int64_t npg, int width,
                              hipStream_t s);
 int launch_fill(double *v, double val, int64_t n, hipStream_t s);  // graph-safe zero/constant fill

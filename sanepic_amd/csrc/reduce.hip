@@ -717,14 +717,6 @@ int launch_set_values(int32_t *dst, const int64_t *idx, const int32_t *val, int6
 // is the parent's pixel of the group's pixel p (injective, so the scatter-add
 // has no conflicts).
 namespace {
-__global__ void k_group_gather(const double *x, const int32_t *gmap, double *xg, int64_t npg, int width) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= npg * width) return;
-  const int64_t p = i / width;
-  const int c = (int)(i % width);
-  xg[i] = x[(int64_t)gmap[p] * width + c];
-}
-
 __global__ void k_group_scatter_add(double *x, const int32_t *gmap, const double *xg, int64_t npg, int width) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= npg * width) return;
@@ -742,11 +734,6 @@ __global__ void k_add_scalar(double *dst, const double *src) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *dst += *src;
 }
 }  // namespace
-
-int launch_group_gather(const double *x, const int32_t *gmap, double *xg, int64_t npg, int width, hipStream_t s) {
-  if (npg > 0) hipLaunchKernelGGL(k_group_gather, dim3(nblk(npg * width)), dim3(256), 0, s, x, gmap, xg, npg, width);
-  return ok();
-}
 
 int launch_group_scatter_add(double *x, const int32_t *gmap, const double *xg, int64_t npg, int width,
                              hipStream_t s) {
