@@ -219,8 +219,8 @@ def main():
                     help="run a 1 GiB device copy first (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py runs on one GPU only: there is no multi-GPU path (sharding the (detector, "
+              "segment-pair) blocks over GPUs is a design, not built)", file=sys.stderr)
         sys.exit(2)
 
     import sanepic_amd

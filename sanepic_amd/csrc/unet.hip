@@ -1,5 +1,5 @@
 // unet.hip -- forward and backward kernels of the preconditioner U-Net on
-// gfx950 matrix cores, and the host-side network driver.
+// gfx950 matrix cores, and the host-side code that runs the network.
 //
 // Convolution = implicit GEMM on v_mfma_f32_16x16x32_bf16: M = output
 // channels, N = 16 pixels of one face row, K = 9 taps x input channels

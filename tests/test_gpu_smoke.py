@@ -1,4 +1,4 @@
-"""The driver's smoke entry point (one small solve on device 0 checked
+"""The smoke entry point (one small solve on device 0 checked
 against the CPU oracle) must pass as committed."""
 import pytest
 
