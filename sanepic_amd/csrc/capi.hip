@@ -596,6 +596,7 @@ void sanepic_destroy(sanepic_ctx *c) {
   for (hipStream_t s : c->up_tod)
     if (s) (void)hipStreamSynchronize(s);
   for (void *p : c->allocs) (void)hipFree(p);
+  release_pool(c);
   if (!c->borrowed) trace_mark("destroy: device memory freed");
   if (c->h_pin) (void)hipHostFree(c->h_pin);
   if (c->st && !c->borrowed) (void)hipStreamDestroy(c->st);
@@ -1059,6 +1060,7 @@ int sanepic_pixelize(sanepic_ctx *c, int64_t *npix_out, int64_t *n_deferred) {
   }
   ACHK(c, dalloc(c, c->partial, 8 * std::max<int64_t>(1, nslots)));
   HIPCHK(c, hipStreamSynchronize(c->st));
+  if (c->pin_owner) flush_frees(c);  // a group: its temporaries serve the next group
   trace_mark("pixelize: done");
   c->pixelized = true;
   if (npix_out) *npix_out = npix;

@@ -180,6 +180,7 @@ int group_setup(sanepic_ctx *c) {
   HIPCHK(c, hipStreamSynchronize(c->st));
   trace_mark("group setup: union sums");
   flush_frees(c);
+  release_pool(c);
   trace_mark("group setup: deferred frees");
   c->setup_done = true;
   return SANEPIC_OK;

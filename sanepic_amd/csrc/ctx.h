@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <string>
 #include <tuple>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/sanepic_hip.h"
@@ -83,6 +84,11 @@ struct sanepic_ctx {
   void *h_pin = nullptr;             // pinned host staging written by kernels (pixel counts)
   int64_t h_pin_bytes = 0;
   sanepic_ctx *pin_owner = nullptr;  // a group context stages through its parent's buffer
+  // A group's temporaries go back to its parent's pool when the group flushes
+  // them, and the next group's allocations take pool blocks first: the groups
+  // run one after another, so a few blocks serve all of them.
+  std::vector<std::pair<void *, int64_t>> pool;   // parent: free device blocks and their sizes
+  std::unordered_map<void *, int64_t> pool_cap;    // parent: size of every block its groups allocated
   bool xcd_blocks = true;            // XCD-contiguous numbering of the four-step blocks
   bool zsorted = false;              // the filter output z is written in pixel-sorted order
   int col_waves = 3;                 // occupancy target of the FP32 gather column pass (0, 3, 4)
@@ -180,6 +186,23 @@ template <typename T>
 inline int dalloc(sanepic_ctx *c, T *&p, int64_t n) {
   if (p) return 0;
   if (n <= 0) n = 1;
+  const int64_t bytes = (int64_t)((size_t)n * sizeof(T));
+  if (sanepic_ctx *o = c->pin_owner) {  // a group: the smallest pool block that fits, within 2x or 1 MB
+    size_t best = o->pool.size();
+    for (size_t i = 0; i < o->pool.size(); i++) {
+      const int64_t cap = o->pool[i].second;
+      if (cap >= bytes && cap <= std::max<int64_t>(2 * bytes, bytes + (1 << 20)) &&
+          (best == o->pool.size() || cap < o->pool[best].second))
+        best = i;
+    }
+    if (best < o->pool.size()) {
+      p = (T *)o->pool[best].first;
+      o->pool.erase(o->pool.begin() + (int64_t)best);
+      c->allocs.push_back((void *)p);
+      c->bytes += bytes;
+      return 0;
+    }
+  }
   void *v = nullptr;
   hipError_t e = hipMalloc(&v, (size_t)n * sizeof(T));
   if (e != hipSuccess) {
@@ -190,7 +213,8 @@ inline int dalloc(sanepic_ctx *c, T *&p, int64_t n) {
   }
   p = (T *)v;
   c->allocs.push_back(v);
-  c->bytes += (int64_t)((size_t)n * sizeof(T));
+  c->bytes += bytes;
+  if (c->pin_owner) c->pin_owner->pool_cap[v] = bytes;
   return 0;
 }
 
@@ -222,10 +246,24 @@ inline void flush_frees(sanepic_ctx *c) {
     auto it = std::find(c->allocs.begin(), c->allocs.end(), f.first);
     if (it == c->allocs.end()) continue;
     c->allocs.erase(it);
-    (void)hipFree(f.first);
     c->bytes -= f.second;
+    sanepic_ctx *o = c->pin_owner;
+    auto cap = o ? o->pool_cap.find(f.first) : std::unordered_map<void *, int64_t>::iterator();
+    if (o && cap != o->pool_cap.end())
+      o->pool.emplace_back(f.first, cap->second);  // a group's temporary: back to the parent's pool
+    else
+      (void)hipFree(f.first);
   }
   c->later.clear();
+}
+
+// Frees the parent's pool of group temporaries.
+inline void release_pool(sanepic_ctx *c) {
+  for (auto &b : c->pool) {
+    c->pool_cap.erase(b.first);
+    (void)hipFree(b.first);
+  }
+  c->pool.clear();
 }
 
 // TOD upload stream k (mod kTodStreams), created on first use behind the
