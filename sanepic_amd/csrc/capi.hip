@@ -374,6 +374,26 @@ int need_pinned(sanepic_ctx *c, int64_t bytes) {
   return 0;
 }
 
+// Host copies of the compact RING pixel list and the slot -> pixel order,
+// through pinned memory and a copy kernel (a DMA copy would queue behind
+// uploads still in flight), where the host has none yet and the device does.
+int fetch_host_order(sanepic_ctx *c) {
+  const int64_t npix = c->npix;
+  const bool list = c->h_pixlist.empty() && c->d_pixlist && npix > 0;
+  const bool order = c->h_slot2pix.empty() && c->d_slot2pix && npix > 0;
+  if (!list && !order) return 0;
+  ACHK(c, need_pinned(c, 2 * npix * (int64_t)sizeof(int32_t)));
+  const int32_t *pinned = (const int32_t *)pin_ctx(c)->h_pin;
+  int32_t *hpin = nullptr;
+  HIPCHK(c, hipHostGetDevicePointer((void **)&hpin, (void *)pinned, 0));
+  if (list) LCHK(c, launch_copy_i32(c->d_pixlist, hpin, npix, c->st));
+  if (order) LCHK(c, launch_copy_i32(c->d_slot2pix, hpin + npix, npix, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  if (list) c->h_pixlist.assign(pinned, pinned + npix);
+  if (order) c->h_slot2pix.assign(pinned + npix, pinned + 2 * npix);
+  return 0;
+}
+
 }  // namespace sctx
 
 namespace {
@@ -741,16 +761,11 @@ int nested_order(sanepic_ctx *c, const int32_t *dlist) {
   HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(tmp, tb, (const uint32_t *)key, key_out, (const uint32_t *)ids,
                                                (uint32_t *)s2p, (int)npix, 0, end_bit, c->st));
   LCHK(c, launch_invperm((const uint32_t *)s2p, (uint32_t *)mpos, npix, c->st));
-  // slot -> pixel on the host (pixels read back in RING order) through pinned memory
-  ACHK(c, need_pinned(c, npix * (int64_t)sizeof(int32_t)));
-  const int32_t *pinned = (const int32_t *)pin_ctx(c)->h_pin;
-  int32_t *hpin = nullptr;
-  HIPCHK(c, hipHostGetDevicePointer((void **)&hpin, (void *)pinned, 0));
-  LCHK(c, launch_copy_i32(s2p, hpin, npix, c->st));
-  HIPCHK(c, hipStreamSynchronize(c->st));
-  c->h_slot2pix.assign(pinned, pinned + npix);
   c->mpos = mpos;
   c->d_slot2pix = s2p;
+  // slot -> pixel on the host (pixels read back in RING order); a group
+  // copies it when first needed
+  if (!c->pin_owner) ACHK(c, fetch_host_order(c));
   dfree_later(c, tmp, (int64_t)tb);
   dfree_later(c, key, npix);
   dfree_later(c, key_out, npix);
@@ -912,11 +927,12 @@ int sanepic_pixelize(sanepic_ctx *c, int64_t *npix_out, int64_t *n_deferred) {
     dfree_later(c, tmp, (int64_t)tb);
     dfree_later(c, wide, npix + 1);
   }
-  // host copies of the pixel list and counts
-  LCHK(c, launch_copy_i32(dlist, hpin, npix, c->st));
+  // host copies of the pixel list and counts (a group copies its pixel list
+  // when first needed: its parent builds the union on the device)
+  if (!c->pin_owner) LCHK(c, launch_copy_i32(dlist, hpin, npix, c->st));
   LCHK(c, launch_copy_i32(c->cnt, hpin + npix, npix, c->st));
   HIPCHK(c, hipStreamSynchronize(c->st));
-  c->h_pixlist.assign(pinned, pinned + npix);
+  if (!c->pin_owner) c->h_pixlist.assign(pinned, pinned + npix);
   c->h_cnt.assign(pinned + npix, pinned + 2 * npix);
   c->h_indpix.clear();  // built from h_pixlist when asked for (sanepic_get_indpix)
   dfree_later(c, flag, nsky);
@@ -993,6 +1009,7 @@ int sanepic_pixelize(sanepic_ctx *c, int64_t *npix_out, int64_t *n_deferred) {
     }
   }
   if (nest_items && !spix.empty()) {
+    ACHK(c, fetch_host_order(c));
     std::vector<uint8_t> split((size_t)npix, 0);
     for (int32_t p : spix) split[(size_t)p] = 1;
     spix.clear();
@@ -1043,7 +1060,7 @@ int sanepic_pixelize(sanepic_ctx *c, int64_t *npix_out, int64_t *n_deferred) {
     HIPCHK(c, hipMemcpy(c->sp_first, sfirst.data(), nsp * 4, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->sp_count, scount.data(), nsp * 4, hipMemcpyHostToDevice));
   }
-  dfree_later(c, c->d_slot2pix, npix);
+  if (!c->pin_owner) dfree_later(c, c->d_slot2pix, npix);  // a group keeps it for fetch_host_order
   c->items = Items{c->it_pix, c->it_beg, c->it_len, c->it_slot, nit, c->sp_pix, c->sp_first, c->sp_count, nsp};
   c->nslots = nslots;
   // lanes per P^T item: 16 keep four entries each in flight from 64 entries
@@ -1078,6 +1095,7 @@ int sanepic_get_pixels(sanepic_ctx *c, int64_t idet, int64_t iseg, int64_t *out)
   }
   if (idet < 0 || idet >= c->ndet || iseg < 0 || iseg >= c->nseg)
     return fail(c, SANEPIC_EINVAL, "index out of range");
+  ACHK(c, fetch_host_order(c));
   std::vector<int32_t> h(c->ns);
   ACHK(c, d2h(c, h.data(), c->pix + (idet * c->nseg + iseg) * c->ns, c->ns * 4));
   for (int64_t i = 0; i < c->ns; i++) out[i] = c->h_pixlist[c->pix_nested ? c->h_slot2pix[h[i]] : h[i]];

@@ -116,7 +116,6 @@ int group_pixelize(sanepic_ctx *c, int64_t *npix_out, int64_t *ndef_out) {
   c->h_pixlist.assign(pinned, pinned + npix);
   c->h_cnt.assign(pinned + npix, pinned + 2 * npix);
   c->h_indpix.clear();  // built from h_pixlist when asked for (sanepic_get_indpix)
-  for (sanepic_ctx *s : c->sub) dfree_later(s, s->d_pixlist, s->npix);
   dfree_later(c, counts, nsky);
   dfree_later(c, flag, nsky);
   dfree_later(c, pos, nsky);

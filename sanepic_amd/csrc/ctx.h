@@ -77,8 +77,8 @@ struct sanepic_ctx {
   int32_t *mpos = nullptr;           // [npix]
   double *map64 = nullptr;           // [npix][3] FP64 gather map in slot order
   std::vector<int32_t> h_slot2pix;   // slot -> compressed pixel
-  int32_t *d_slot2pix = nullptr;     // the same on the device, until the P^T items are built
-  int32_t *d_pixlist = nullptr;      // a group's compact RING pixel list, until its parent's union is built
+  int32_t *d_slot2pix = nullptr;     // the same on the device (until the P^T items are built; kept by a group)
+  int32_t *d_pixlist = nullptr;      // a group's compact RING pixel list (the union and read-back use it)
   bool items_nested = true;          // P^T items in NESTED pixel order
   int scatter_lanes = 16;            // P^T lanes per item (from the mean item length at pixelisation)
   void *h_pin = nullptr;             // pinned host staging written by kernels (pixel counts)
@@ -330,6 +330,9 @@ inline int sync_tod_uploads(sanepic_ctx *c) {
 // Pinned host staging of at least `bytes` (pin_ctx(c)->h_pin; kept for the
 // context's lifetime, since freeing pinned memory waits for uploads in flight).
 int need_pinned(sanepic_ctx *c, int64_t bytes);
+// Host copies of the pixel list and slot order where only the device has them
+// (detector groups make them when first needed).
+int fetch_host_order(sanepic_ctx *c);
 // Set-up in two halves: everything that needs no TOD (noise tables, noise
 // autocorrelation, crosslinks, pair sums, the gather map, CG buffers), queued
 // while the TOD upload may still run; then b = P^T N^-1 d from the TOD.
