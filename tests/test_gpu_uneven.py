@@ -141,3 +141,25 @@ def test_uneven_rejects_uniform_uploads(case):
     ctx.close()
     with pytest.raises(sanepic_amd.SanepicError):
         sanepic_amd.Context.uneven(32, [8192, 7 * 8192], NSAMP * 3)
+
+
+@pytest.mark.parametrize("env", ["SANEPIC_NESTED_MAP", "SANEPIC_ITEM_ORDER"])
+def test_uneven_layout_options_are_bit_identical(case, env, monkeypatch):
+    """The groups' NESTED-order gather maps and P^T items (a group reads its
+    pixels of the parent's x into its own map and adds its P^T into the
+    parent's q) move where values are stored, not any sum: A x, b, chi2 and
+    the pixels read back are bit-identical in RING order."""
+    cfg, ds, probs, ring = case
+    print("uneven layout option", env, flush=True)
+    out = []
+    for v in ("0", "1"):
+        monkeypatch.setenv(env, v)
+        for prec in (sanepic_amd.FP64, sanepic_amd.MIXED):
+            ctx = _ctx(cfg, ds, precision=prec)
+            x = np.random.default_rng(11).standard_normal(3 * ctx.npix)
+            out.append((ctx.apply(x), ctx.rhs(), ctx.chi2(x) if prec == sanepic_amd.FP64 else 0.0,
+                        ctx.pixels(1, 0)))
+            ctx.close()
+    for a, b in zip(out[:2], out[2:]):
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and a[2] == b[2]
+        assert np.array_equal(a[3], b[3])
