@@ -57,6 +57,9 @@ def lib():
             "or_problem_get_setup": (None, [_V, _lp, _dp, _dp]),
             "or_problem_solve": (_I, [_V, _dp, _I, _D, _I, _dp, _dp, _dp, _dp, _dp, _dp]),
             "or_expand_map": (None, [_L, _lp, _dp, _dp]),
+            "or_exact_blocks": (None, [_V, _dp]),
+            "or_problem_set_dipole": (_I, [_V, _dp]),
+            "or_problem_solve_pcg": (_I, [_V, _dp, _I, _D, _I, _I, _I, _dp, _dp, _dp, _dp, _dp]),
             "or_version": (_I, []),
         }
         for name, (res, args) in sig.items():
@@ -187,6 +190,12 @@ class Problem:
         assert tod.size == self.ndet * self.nseg * self.ns
         lib().or_problem_load_tod(self.h, _d(tod))
 
+    def set_dipole(self, odip):
+        """Orbital dipole [ndet, nseg * ns] out of the data spectra (after load_tod)."""
+        odip = _f64(odip)
+        assert odip.size == self.ndet * self.nseg * self.ns
+        assert lib().or_problem_set_dipole(self.h, _d(odip)) == 0
+
     def apply_A(self, x):
         x = _f64(x)
         y = np.empty(3 * self.npix)
@@ -237,3 +246,29 @@ class Problem:
         lib().or_problem_get_setup(self.h, _l(hits), _d(mp), _d(sc))
         return dict(S=S, Mptot=Mpt, b=b, crit=crit[:it], crit2=crit2[:it], chi2=chi2[:it],
                     niter=it, hits=hits, sumCross=sc)
+
+    def exact_blocks(self):
+        """[npix, 6] diagonal 3x3 blocks of A (B00 B01 B02 B11 B12 B22), all lags."""
+        out = np.empty(6 * self.npix)
+        lib().or_exact_blocks(self.h, _d(out))
+        return out.reshape(self.npix, 6)
+
+    def solve_pcg(self, kind=1, deflate=False, x0=None, maxiter=2000, stop=1e-15, refresh=10):
+        """The GPU solver's PCG: kind 0 = the reference's diagonal, 1 = 3x3
+        block-Jacobi; deflate = A-DEF2 with the I monopole as coarse vector."""
+        n3 = 3 * self.npix
+        S = np.empty(n3)
+        b = np.empty(n3)
+        M6 = np.zeros(6 * self.npix)
+        crit = np.zeros(maxiter + 1)
+        crit2 = np.zeros(maxiter + 1)
+        x0p = None if x0 is None else _d(_f64(x0))
+        it = lib().or_problem_solve_pcg(self.h, x0p, maxiter, stop, refresh, int(kind),
+                                        1 if deflate else 0, _d(S), _d(crit), _d(crit2), _d(b),
+                                        _d(M6))
+        if it < 0:
+            raise RuntimeError("oracle: Mp has a zero I element")
+        hits = np.empty(2 * self.npix, dtype=np.int64)
+        lib().or_problem_get_setup(self.h, _l(hits), None, None)
+        return dict(S=S, b=b, crit=crit[:it], crit2=crit2[:it], niter=it, hits=hits,
+                    M6=M6.reshape(self.npix, 6))

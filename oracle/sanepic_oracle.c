@@ -14,10 +14,13 @@
  *   or_ang2pix_ring       HEALPix RING pixel              2764-2843
  *   or_problem_pixelize   pointing loop + pixelisation    722-773, 848-951
  *   or_problem_load_tod   TOD NaN fix + r2c -> fdatas0    775-803
+ *   or_problem_set_dipole orbital dipole out of fdatas0   700-720, 1055-1067
  *   or_deproject          P (map -> TOD)                  2845-2873
  *   or_apply_A            write_tfAS + do_PtNd            2390-2445, 2489-2637
  *   or_diag_corr          compute_diagPtNPCorr            2680-2762
  *   or_problem_solve      masks + diagonal PCG            1069-1761
+ *   or_exact_blocks       exact 3x3 diagonal blocks of A  (no counterpart)
+ *   or_problem_solve_pcg  block-Jacobi / A-DEF2 PCG       (no counterpart)
  *   or_chi2               chi2 of a map                   1514-1538
  *
  * FFT: the reference calls FFTW3 (version unpinned in the reference; its
@@ -474,6 +477,7 @@ typedef struct or_problem {
   long npix;
   long n_bad_pixel;             /* samples that took the previous pixel */
   cplx *fdatas0;                /* [ndet][nseg][ns/2+1] */
+  cplx *fodipole;               /* [ndet][nseg][ns/2+1] or NULL */
   double *b;                    /* PNdtot, 3 npix */
   double chi20;
   long *hits;                   /* 2 npix (doubled, as the reference) */
@@ -515,6 +519,7 @@ void or_problem_free(or_problem *P) {
   free(P->psipFP);
   free(P->indpix);
   free(P->fdatas0);
+  free(P->fodipole);
   free(P->b);
   free(P->hits);
   free(P->Mp);
@@ -557,32 +562,45 @@ int or_problem_pixelize(or_problem *P, const double *ra, const double *dec,
   P->datatopix = (long *)malloc(sizeof(long) * ntot);
   P->psip = (double *)malloc(sizeof(double) * ntot);
   P->psipFP = (double *)malloc(sizeof(double) * ntot);
-  double *psis = (double *)malloc(sizeof(double) * ns);
-  double *thp = (double *)malloc(sizeof(double) * ns);
-  for (long iseg = 0; iseg < nseg; iseg++) {
-    memcpy(psis, psi + iseg * ns, sizeof(double) * ns);
-    or_nan_fill(psis, ns);
-    for (long idet = 0; idet < ndet; idet++) {
-      for (long ii = 0; ii < ns; ii++) {
-        double thetap_i = OR_D2PI / 4.0 - dec[iseg * ns + ii];
-        double phip_i = ra[iseg * ns + ii];
-        double psip_i = psis[ii];
-        double th_o, ph_o, ps_o;
-        or_pointingshift(P->offx[idet], P->offy[idet], 1, &thetap_i, &phip_i,
-                         &psip_i, &th_o, &ph_o, &ps_o);
-        long k = IDX(P, idet, iseg, ii);
-        ra_d[k] = ph_o;
-        dec_d[k] = OR_D2PI / 4.0 - th_o;
-        P->psip[k] = ps_o + P->psidet[idet];
-        P->psipFP[k] = P->psip[k];
-        if (P->hwp)
-          P->psip[k] = -(ps_o + P->psidet[idet]) +
-                       2.0 * P->omega * (double)(ii + ns * iseg);
+  /* Every sample is independent up to the previous-pixel substitution, so
+   * the pointing and ang2pix run over segments in parallel and the
+   * substitution follows in the reference's sample order. */
+#pragma omp parallel num_threads(P->nthreads)
+  {
+    double *psis = (double *)malloc(sizeof(double) * ns);
+#pragma omp for schedule(dynamic)
+    for (long iseg = 0; iseg < nseg; iseg++) {
+      memcpy(psis, psi + iseg * ns, sizeof(double) * ns);
+      or_nan_fill(psis, ns);
+      for (long idet = 0; idet < ndet; idet++) {
+        for (long ii = 0; ii < ns; ii++) {
+          double thetap_i = OR_D2PI / 4.0 - dec[iseg * ns + ii];
+          double phip_i = ra[iseg * ns + ii];
+          double psip_i = psis[ii];
+          double th_o, ph_o, ps_o;
+          or_pointingshift(P->offx[idet], P->offy[idet], 1, &thetap_i, &phip_i,
+                           &psip_i, &th_o, &ph_o, &ps_o);
+          long k = IDX(P, idet, iseg, ii);
+          ra_d[k] = ph_o;
+          dec_d[k] = OR_D2PI / 4.0 - th_o;
+          P->psip[k] = ps_o + P->psidet[idet];
+          P->psipFP[k] = P->psip[k];
+          if (P->hwp)
+            P->psip[k] = -(ps_o + P->psidet[idet]) +
+                         2.0 * P->omega * (double)(ii + ns * iseg);
+          double theta = OR_D2PI / 4.0 - dec_d[k];
+          double phi = fmod(ra_d[k] + OR_D2PI, OR_D2PI);
+          long v = -1;
+          /* NaN angles make the reference's ang2pix undefined; they and any
+           * out-of-range result take the previous pixel (862-872). */
+          if (theta == theta && phi == phi)
+            v = or_ang2pix_ring(P->nside, theta, phi);
+          P->datatopix[k] = v;
+        }
       }
     }
+    free(psis);
   }
-  free(psis);
-  free(thp);
   long *pixon = (long *)calloc(npixsky, sizeof(long));
   long ipr = 0;
   P->n_bad_pixel = 0;
@@ -590,15 +608,8 @@ int or_problem_pixelize(or_problem *P, const double *ra, const double *dec,
     for (long iseg = 0; iseg < nseg; iseg++)
       for (long ii = 0; ii < ns; ii++) {
         long k = IDX(P, idet, iseg, ii);
-        double theta = OR_D2PI / 4.0 - dec_d[k];
-        double phi = fmod(ra_d[k] + OR_D2PI, OR_D2PI);
         long prev = ipr;
-        long v = -1;
-        /* NaN angles make the reference's ang2pix undefined; they and any
-         * out-of-range result take the previous pixel (862-872). */
-        if (theta == theta && phi == phi)
-          v = or_ang2pix_ring(P->nside, theta, phi);
-        ipr = v;
+        ipr = P->datatopix[k];
         if (ipr < 0 || ipr > npixsky - 1) {
           ipr = prev;
           P->n_bad_pixel++;
@@ -676,6 +687,34 @@ int or_problem_load_tod(or_problem *P, const double *tod) {
                   work);
     }
     free(data);
+    free(work);
+  }
+  return 0;
+}
+
+/* Orbital dipole (-o): fodipole = r2c(orbdata) per (det, seg) block, read
+ * without NaN substitution (702-720), then fdatas0 -= fodipole (1055-1067).
+ * odip: [ndet][nseg][ns]; call after or_problem_load_tod. */
+int or_problem_set_dipole(or_problem *P, const double *odip) {
+  long ns = P->ns, nh = ns / 2 + 1, nblk = P->ndet * P->nseg;
+  if (!P->fdatas0)
+    return -1;
+  free(P->fodipole);
+  P->fodipole = (cplx *)malloc(sizeof(cplx) * nh * nblk);
+#pragma omp parallel num_threads(P->nthreads)
+  {
+    cplx *work = (cplx *)malloc(sizeof(cplx) * 2 * ns);
+#pragma omp for schedule(dynamic)
+    for (long blk = 0; blk < nblk; blk++) {
+      long idet = blk / P->nseg, iseg = blk % P->nseg;
+      cplx *F = P->fodipole + (idet * P->nseg + iseg) * nh;
+      cplx *D = P->fdatas0 + (idet * P->nseg + iseg) * nh;
+      or_rfft_r2c(P->plan, odip + IDX(P, idet, iseg, 0), F, work);
+      for (long ii = 0; ii < nh; ii++) {
+        D[ii].re -= F[ii].re;
+        D[ii].im -= F[ii].im;
+      }
+    }
     free(work);
   }
   return 0;
@@ -1077,6 +1116,337 @@ int or_problem_solve(or_problem *P, const double *x0, int maxiter, double stop,
   free(Mpt);
   free(d);
   free(q);
+  return iter;
+}
+
+/* ======================================================================== */
+/* Exact 3x3 diagonal blocks and the block-Jacobi / A-DEF2 PCG               */
+/* ======================================================================== */
+
+/* blocks[npix][6] = {B00, B01, B02, B11, B12, B22}: the diagonal 3x3 block
+ * of A = P^T N^-1 P at each seen pixel, unmasked.  N^-1 of a (det, seg)
+ * block is the circulant c2r(SpN r2c(.)) (2549-2581), i.e. the matrix
+ * kern[(i - j) mod ns] with kern = c2r(SpN[0..ns/2]) (unnormalised); the
+ * block is the brute-force sum over every ordered pair of samples (i, j) of
+ * the pixel in one (det, seg) block of kern[(t_i - t_j) mod ns] w_i w_j^T,
+ * w = (1, cos 2psi, sin 2psi) -- all lags, not the reference's |dt| < ns/2
+ * truncation of compute_diagPtNPCorr (2680-2762), which builds only the I
+ * entry.  O(sum of squared hits per block): test sizes only.  Pinned by
+ * tests/test_oracle_blocks.py against columns of or_apply_A. */
+void or_exact_blocks(or_problem *P, double *blocks) {
+  long ns = P->ns, nh = ns / 2 + 1, npix = P->npix;
+  int nt = P->nthreads;
+  long nblk = P->ndet * P->nseg;
+  double *kern = (double *)malloc(sizeof(double) * ns * P->ndet);
+  {
+    cplx *Nk_ = (cplx *)malloc(sizeof(cplx) * nh);
+    cplx *work = (cplx *)malloc(sizeof(cplx) * 2 * ns);
+    for (long idet = 0; idet < P->ndet; idet++) {
+      for (long k = 0; k < nh; k++) {
+        Nk_[k].re = P->spn[idet * ns + k];
+        Nk_[k].im = 0.0;
+      }
+      or_rfft_c2r(P->plan, Nk_, kern + idet * ns, work);
+    }
+    free(Nk_);
+    free(work);
+  }
+  double *priv = (double *)calloc((size_t)nt * 6 * npix, sizeof(double));
+#pragma omp parallel num_threads(nt)
+  {
+    int tid = 0;
+#ifdef _OPENMP
+    tid = omp_get_thread_num();
+#endif
+    double *mine = priv + (size_t)tid * 6 * npix;
+    or_pt *pts = (or_pt *)malloc(sizeof(or_pt) * ns);
+#pragma omp for schedule(dynamic)
+    for (long blk = 0; blk < nblk; blk++) {
+      long idet = blk / P->nseg, iseg = blk % P->nseg;
+      const double *kr = kern + idet * ns;
+      for (long ii = 0; ii < ns; ii++) {
+        pts[ii].pix = P->indpix[P->datatopix[IDX(P, idet, iseg, ii)]];
+        pts[ii].t = ii;
+      }
+      qsort(pts, ns, sizeof(or_pt), or_pt_cmp);
+      long c0 = 0;
+      while (c0 < ns) {
+        long c1 = c0;
+        while (c1 < ns && pts[c1].pix == pts[c0].pix)
+          c1++;
+        long p = pts[c0].pix;
+        if (p >= 0) {
+          double acc[6] = {0, 0, 0, 0, 0, 0};
+          for (long i = c0; i < c1; i++) {
+            double ai = 2.0 * P->psip[IDX(P, idet, iseg, pts[i].t)];
+            double wi[3] = {1.0, cos(ai), sin(ai)};
+            for (long j = c0; j < c1; j++) {
+              double aj = 2.0 * P->psip[IDX(P, idet, iseg, pts[j].t)];
+              long lag = ((pts[i].t - pts[j].t) % ns + ns) % ns;
+              double kv = kr[lag];
+              double wj[3] = {1.0, cos(aj), sin(aj)};
+              acc[0] += kv * wi[0] * wj[0];
+              acc[1] += kv * wi[0] * wj[1];
+              acc[2] += kv * wi[0] * wj[2];
+              acc[3] += kv * wi[1] * wj[1];
+              acc[4] += kv * wi[1] * wj[2];
+              acc[5] += kv * wi[2] * wj[2];
+            }
+          }
+          for (int k = 0; k < 6; k++)
+            mine[6 * p + k] += acc[k];
+        }
+        c0 = c1;
+      }
+    }
+    free(pts);
+  }
+  for (long i = 0; i < 6 * npix; i++) {
+    double s = 0.0;
+    for (int t = 0; t < nt; t++)
+      s += priv[(size_t)t * 6 * npix + i];
+    blocks[i] = s;
+  }
+  free(priv);
+  free(kern);
+}
+
+/* Inverse of a symmetric 3x3 block by Gauss-Jordan elimination with partial
+ * pivoting (m6 in, inv6 out, both {00, 01, 02, 11, 12, 22}). */
+static void or_inv3(const double *m6, double *inv6) {
+  double a[3][6] = {{m6[0], m6[1], m6[2], 1, 0, 0},
+                    {m6[1], m6[3], m6[4], 0, 1, 0},
+                    {m6[2], m6[4], m6[5], 0, 0, 1}};
+  for (int c = 0; c < 3; c++) {
+    int piv = c;
+    for (int r = c + 1; r < 3; r++)
+      if (fabs(a[r][c]) > fabs(a[piv][c]))
+        piv = r;
+    if (piv != c)
+      for (int k = 0; k < 6; k++) {
+        double t = a[c][k];
+        a[c][k] = a[piv][k];
+        a[piv][k] = t;
+      }
+    double d = a[c][c];
+    for (int k = 0; k < 6; k++)
+      a[c][k] /= d;
+    for (int r = 0; r < 3; r++)
+      if (r != c) {
+        double f = a[r][c];
+        for (int k = 0; k < 6; k++)
+          a[r][k] -= f * a[c][k];
+      }
+  }
+  inv6[0] = a[0][3];
+  inv6[1] = 0.5 * (a[0][4] + a[1][3]);
+  inv6[2] = 0.5 * (a[0][5] + a[2][3]);
+  inv6[3] = a[1][4];
+  inv6[4] = 0.5 * (a[1][5] + a[2][4]);
+  inv6[5] = a[2][5];
+}
+
+/* Preconditioner of the GPU solver for one pixel (no counterpart in the
+ * reference): a pixel whose Q/U are masked (pol hits < 4, doubled) keeps
+ * 1/B00 on I only; a block that is not positive on its diagonal or whose
+ * determinant is below 1e-10 B00 B11 B22 (one distinct angle) falls back to
+ * the inverse diagonal; otherwise the inverse block. */
+static void or_block_precond(const double *b6, int masked, double *m6) {
+  for (int k = 0; k < 6; k++)
+    m6[k] = 0.0;
+  if (masked) {
+    m6[0] = b6[0] != 0.0 ? 1.0 / b6[0] : 0.0;
+    return;
+  }
+  double det = b6[0] * (b6[3] * b6[5] - b6[4] * b6[4]) -
+               b6[1] * (b6[1] * b6[5] - b6[4] * b6[2]) +
+               b6[2] * (b6[1] * b6[4] - b6[3] * b6[2]);
+  double scale = b6[0] * b6[3] * b6[5];
+  if (!(b6[0] > 0.0 && b6[3] > 0.0 && b6[5] > 0.0) || !(det > 1e-10 * scale)) {
+    m6[0] = b6[0] > 0.0 ? 1.0 / b6[0] : 0.0;
+    m6[3] = b6[3] > 0.0 ? 1.0 / b6[3] : 0.0;
+    m6[5] = b6[5] > 0.0 ? 1.0 / b6[5] : 0.0;
+    return;
+  }
+  or_inv3(b6, m6);
+}
+
+/* z = M r in block layout. */
+static void or_apply_M(long npix, int kind, const double *Mdiag,
+                       const double *M6, const double *r, double *z) {
+  if (kind == 0) {
+    for (long i = 0; i < 3 * npix; i++)
+      z[i] = Mdiag[i] * r[i];
+    return;
+  }
+  for (long p = 0; p < npix; p++) {
+    const double *m = M6 + 6 * p;
+    double r0 = r[p], r1 = r[npix + p], r2 = r[2 * npix + p];
+    z[p] = m[0] * r0 + m[1] * r1 + m[2] * r2;
+    z[npix + p] = m[1] * r0 + m[3] * r1 + m[4] * r2;
+    z[2 * npix + p] = m[2] * r0 + m[4] * r1 + m[5] * r2;
+  }
+}
+
+static void or_mask_qu(const or_problem *P, double *v) {
+  long npix = P->npix;
+  for (long p = 0; p < npix; p++)
+    if (P->hits[npix + p] < 4) {
+      v[npix + p] = 0.0;
+      v[2 * npix + p] = 0.0;
+    }
+}
+
+/* PCG of the GPU solver, for kind 0 (the reference's diagonal Mp,
+ * 1351-1353) or kind 1 (per-pixel 3x3 block-Jacobi from or_exact_blocks),
+ * optionally with the two-level A-DEF2 preconditioner of Tang, Nabben, Vuik
+ * and Erlangga (J. Sci. Comput. 39, 2009) with the single coarse vector
+ * Z = e_I (1 on every I component): E = Z^T A Z,
+ *   z = M r + Z E^-1 (Z^T r - (A Z)^T M r),
+ * and the starting guess corrected by Z E^-1 Z^T (b - A x0).  The loop, the
+ * masks (1316-1330, 1455-1459, 1559-1563), the residual refresh every
+ * `refresh` iterations and the stop rule are those of or_problem_solve
+ * (1415-1761).  Returns the iteration count, or -1 when the reference's
+ * diagonal Mp has a zero I element (1344-1348). */
+int or_problem_solve_pcg(or_problem *P, const double *x0, int maxiter,
+                         double stop, int refresh, int kind, int deflate,
+                         double *S_out, double *crit_out, double *crit2_out,
+                         double *b_out, double *M6_out) {
+  long npix = P->npix, n3 = 3 * npix;
+  double *b = (double *)calloc(n3, sizeof(double));
+  or_rhs(P, b);
+  double *S = (double *)calloc(n3, sizeof(double));
+  if (x0)
+    memcpy(S, x0, sizeof(double) * n3);
+  double *q = (double *)calloc(n3, sizeof(double));
+  or_setup_pass(P, S, q); /* q = A x0 (unmasked), hits, Mp */
+  or_mask_qu(P, q);
+  or_mask_qu(P, S);
+  or_mask_qu(P, b);
+  for (long ii = 0; ii < npix; ii++)
+    if (P->Mp[ii] == 0) {
+      free(b);
+      free(S);
+      free(q);
+      return -1;
+    }
+  double *Md = (double *)malloc(sizeof(double) * n3);
+  double *M6 = NULL;
+  for (long ii = 0; ii < n3; ii++)
+    Md[ii] = P->Mp[ii];
+  or_mask_qu(P, Md);
+  for (long ii = 0; ii < n3; ii++)
+    Md[ii] = Md[ii] != 0.0 ? 1.0 / Md[ii] : 0.0;
+  if (kind == 1) {
+    double *B6 = (double *)malloc(sizeof(double) * 6 * npix);
+    or_exact_blocks(P, B6);
+    M6 = (double *)malloc(sizeof(double) * 6 * npix);
+    for (long p = 0; p < npix; p++)
+      or_block_precond(B6 + 6 * p, P->hits[npix + p] < 4, M6 + 6 * p);
+    free(B6);
+    if (M6_out)
+      memcpy(M6_out, M6, sizeof(double) * 6 * npix);
+  }
+  double *AZ = NULL, einv = 0.0;
+  if (deflate) {
+    double *Z = (double *)calloc(n3, sizeof(double));
+    AZ = (double *)malloc(sizeof(double) * n3);
+    for (long p = 0; p < npix; p++)
+      Z[p] = 1.0;
+    or_apply_A(P, Z, AZ);
+    or_mask_qu(P, AZ);
+    double e = 0.0, g = 0.0;
+    for (long p = 0; p < npix; p++)
+      e += AZ[p];
+    einv = e != 0.0 ? 1.0 / e : 0.0;
+    for (long p = 0; p < npix; p++)
+      g += b[p] - q[p];
+    double y0 = einv * g;
+    for (long p = 0; p < npix; p++)
+      S[p] += y0;
+    for (long ii = 0; ii < n3; ii++)
+      q[ii] += y0 * AZ[ii];
+    free(Z);
+  }
+  double *r = (double *)malloc(sizeof(double) * n3);
+  double *z = (double *)malloc(sizeof(double) * n3);
+  double *d = (double *)malloc(sizeof(double) * n3);
+  for (long ii = 0; ii < n3; ii++)
+    r[ii] = b[ii] - q[ii];
+  /* z = M r + c e_I, delta = r.z; returns c */
+  double delta_n, var_n = 0.0, var00 = 0.0;
+#define OR_PRECOND_STEP(DELTA)                                     \
+  do {                                                             \
+    or_apply_M(npix, kind, Md, M6, r, z);                          \
+    double rz = 0.0, zr_I = 0.0, azz = 0.0;                        \
+    for (long ii = 0; ii < n3; ii++)                               \
+      rz += r[ii] * z[ii];                                         \
+    double c = 0.0;                                                \
+    if (deflate) {                                                 \
+      for (long p = 0; p < npix; p++)                              \
+        zr_I += r[p];                                              \
+      for (long ii = 0; ii < n3; ii++)                             \
+        azz += AZ[ii] * z[ii];                                     \
+      c = einv * (zr_I - azz);                                     \
+      for (long p = 0; p < npix; p++)                              \
+        z[p] += c;                                                 \
+    }                                                              \
+    DELTA = rz + c * zr_I;                                         \
+  } while (0)
+  OR_PRECOND_STEP(delta_n);
+  for (long ii = 0; ii < n3; ii++) {
+    d[ii] = z[ii];
+    var_n += r[ii] * r[ii];
+    var00 += b[ii] * b[ii];
+  }
+  double delta0 = delta_n;
+  int iter = 0;
+  while ((iter < maxiter && var_n / var00 > stop) || iter == 0) {
+    or_apply_A(P, d, q);
+    or_mask_qu(P, q);
+    double rtq = 0.0;
+    for (long ii = 0; ii < n3; ii++)
+      rtq += q[ii] * d[ii];
+    double alpha = delta_n / rtq;
+    for (long ii = 0; ii < n3; ii++)
+      S[ii] += alpha * d[ii];
+    if ((iter % refresh) == 0) {
+      or_apply_A(P, S, r);
+      or_mask_qu(P, r);
+      for (long ii = 0; ii < n3; ii++)
+        r[ii] = b[ii] - r[ii];
+    } else {
+      for (long ii = 0; ii < n3; ii++)
+        r[ii] -= alpha * q[ii];
+    }
+    double delta_o = delta_n;
+    OR_PRECOND_STEP(delta_n);
+    var_n = 0.0;
+    for (long ii = 0; ii < n3; ii++)
+      var_n += r[ii] * r[ii];
+    double beta = delta_n / delta_o;
+    for (long ii = 0; ii < n3; ii++)
+      d[ii] = z[ii] + beta * d[ii];
+    if (crit_out)
+      crit_out[iter] = var_n / var00;
+    if (crit2_out)
+      crit2_out[iter] = delta_n / delta0;
+    iter++;
+  }
+#undef OR_PRECOND_STEP
+  if (S_out)
+    memcpy(S_out, S, sizeof(double) * n3);
+  if (b_out)
+    memcpy(b_out, b, sizeof(double) * n3);
+  free(b);
+  free(S);
+  free(q);
+  free(r);
+  free(z);
+  free(d);
+  free(Md);
+  free(M6);
+  free(AZ);
   return iter;
 }
 
