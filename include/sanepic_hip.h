@@ -118,6 +118,19 @@ int sanepic_set_pointing_range(sanepic_ctx *ctx, int64_t t0, int64_t n,
 int sanepic_set_tod_range(sanepic_ctx *ctx, int64_t idet, int64_t t0,
                           int64_t n, const double *tod, int32_t queued);
 
+/* Orbital dipole timeline of (idet, iseg): ns samples of the file
+ * <data dir>/orbdip_<bolo> (read at sanepic.cc:702-708 when -o is set).  It
+ * is taken out of the data before b = P^T N^-1 d and chi2, the reference's
+ * fdatas0 -= r2c(orbdip) (712-720, 1055-1067), here subtracted from the
+ * NaN-filled TOD (the transform is linear).  Optional; once any block has a
+ * dipole, every block needs one before sanepic_setup. */
+int sanepic_set_dipole(sanepic_ctx *ctx, int64_t idet, int64_t iseg,
+                       const double *odip);
+/* The same for a per-detector-length context: detector idet's samples
+ * [t0, t0 + n) of its whole segments. */
+int sanepic_set_dipole_range(sanepic_ctx *ctx, int64_t idet, int64_t t0,
+                             int64_t n, const double *odip);
+
 /* Pinned (page-locked) host memory for input buffers: copies from it run at
  * the full host-link rate.  No counterpart in the reference, which reads its
  * inputs with fread into ordinary buffers (sanepic.cc:680-692, 776-790). */

@@ -40,6 +40,7 @@ ABI_FUNCTIONS = [
     "sanepic_learn_eval", "sanepic_learn_grad", "sanepic_learn_stats", "sanepic_host_alloc", "sanepic_host_free", "sanepic_set_pointing_async",
     "sanepic_set_tod_async", "sanepic_autotune", "sanepic_set_fft_split", "sanepic_create_uneven",
     "sanepic_get_seglen", "sanepic_set_pointing_range", "sanepic_set_tod_range",
+    "sanepic_set_dipole", "sanepic_set_dipole_range",
 ]
 
 
@@ -163,6 +164,8 @@ def lib():
             "sanepic_get_seglen": [P, i64, i64p, i64p],
             "sanepic_set_pointing_range": [P, i64, i64, d_p, d_p, d_p, ctypes.c_int32],
             "sanepic_set_tod_range": [P, i64, i64, i64, d_p, ctypes.c_int32],
+            "sanepic_set_dipole": [P, i64, i64, d_p],
+            "sanepic_set_dipole_range": [P, i64, i64, i64, d_p],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -362,6 +365,10 @@ class Context:
         self._chk(lib().sanepic_set_tod_range(self.h, idet, t0, tod.size, _dp(tod), 1 if queued else 0),
                   "set_tod_range")
 
+    def set_dipole_range(self, idet, t0, odip):
+        odip = _f64(odip)
+        self._chk(lib().sanepic_set_dipole_range(self.h, idet, t0, odip.size, _dp(odip)), "set_dipole_range")
+
     def load_uneven(self, cfg, ds, queued=False):
         """Feeds a synth.make_uneven_dataset dataset (dict) or a PinnedInputs of one."""
         get = (lambda k: ds[k]) if isinstance(ds, dict) else (lambda k: getattr(ds, k))
@@ -407,6 +414,11 @@ class Context:
         assert self.ns is None or tod.size == self.ns
         self._chk(lib().sanepic_set_tod(self.h, idet, iseg, _dp(tod)), "set_tod")
 
+    def set_dipole(self, idet, iseg, odip):
+        odip = _f64(odip)
+        assert odip.size == self.ns
+        self._chk(lib().sanepic_set_dipole(self.h, idet, iseg, _dp(odip)), "set_dipole")
+
     def load(self, cfg, ds):
         """Feeds a sanepic_amd.synth dataset (all detectors and segments)."""
         ns = self.ns
@@ -419,6 +431,10 @@ class Context:
         for k in range(self.ndet):
             for s in range(self.nseg):
                 self.set_tod(k, s, ds["tod"][k, s * ns:(s + 1) * ns])
+        if ds.get("odip") is not None:
+            for k in range(self.ndet):
+                for s in range(self.nseg):
+                    self.set_dipole(k, s, ds["odip"][k, s * ns:(s + 1) * ns])
 
     def load_async(self, cfg, pin):
         """Queues the uploads of a PinnedInputs (returns before they finish)."""

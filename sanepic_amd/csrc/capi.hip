@@ -583,6 +583,7 @@ int sanepic_create(const sanepic_config *cfg, sanepic_ctx **out) {
   c->det_ok.assign(c->ndet, 0);
   c->pt_ok.assign(c->nseg, 0);
   c->tod_ok.assign((size_t)c->ndet * c->nseg, 0);
+  c->dip_ok.assign((size_t)c->ndet * c->nseg, 0);
   int rc = dalloc(c, c->ra, c->nseg * ns);
   if (!rc) rc = dalloc(c, c->dec, c->nseg * ns);
   if (!rc) rc = dalloc(c, c->psi, c->nseg * ns);
@@ -729,6 +730,20 @@ int sanepic_set_tod(sanepic_ctx *c, int64_t idet, int64_t iseg, const double *to
   HIPCHK(c, hipMemcpy(c->tod + (idet * c->nseg + iseg) * c->ns, tod, (size_t)c->ns * sizeof(double),
                       hipMemcpyHostToDevice));
   c->tod_ok[idet * c->nseg + iseg] = 1;
+  return SANEPIC_OK;
+}
+
+int sanepic_set_dipole(sanepic_ctx *c, int64_t idet, int64_t iseg, const double *odip) {
+  if (!c) return SANEPIC_EINVAL;
+  if (!c->sub.empty()) return fail(c, SANEPIC_EINVAL, "use sanepic_set_dipole_range for per-detector lengths");
+  if (idet < 0 || idet >= c->ndet || iseg < 0 || iseg >= c->nseg)
+    return fail(c, SANEPIC_EINVAL, "dipole index out of range");
+  if (c->setup_done) return fail(c, SANEPIC_EINVAL, "TOD already consumed by sanepic_setup");
+  if (!odip) return fail(c, SANEPIC_EINVAL, "null dipole buffer");
+  ACHK(c, dalloc(c, c->odip, c->ntot));
+  HIPCHK(c, hipMemcpy(c->odip + (idet * c->nseg + iseg) * c->ns, odip, (size_t)c->ns * sizeof(double),
+                      hipMemcpyHostToDevice));
+  c->dip_ok[idet * c->nseg + iseg] = 1;
   return SANEPIC_OK;
 }
 
@@ -1303,6 +1318,16 @@ int setup_pointing(sanepic_ctx *c) {
     c->h_slot2pix.clear();
     dfree_later(c, c->mpos, npix);
   }
+  // the FP64 gather map (a group always gathers through one, through the
+  // parent's pixel map) and the pre-gather streams, which apply_A would
+  // otherwise allocate on first use: a solve that starts in graph mode
+  // captures its first FP64 application, and no allocation may fall inside
+  // a captured iteration
+  if (c->pix_nested || c->pin_owner) ACHK(c, dalloc(c, c->map64, 3 * npix));
+  if (c->pregather) {
+    ACHK(c, dalloc(c, c->px64, ntot));
+    if (c->cfg.precision == SANEPIC_MIXED) ACHK(c, dalloc(c, c->px32, ntot));
+  }
   trace_mark("setup: pair sums, vectors (queued)");
   return 0;
 }
@@ -1311,6 +1336,10 @@ int setup_data(sanepic_ctx *c) {
   const int64_t npix = c->npix, ntot = c->ntot;
   // b = P^T N^-1 d  (1069-1125); the data spectrum is kept for chi2
   LCHK(c, launch_tod_nanfill(c->tod, ntot, c->ns, c->st));
+  // the orbital dipole leaves the data spectrum (fdatas0 -= fodipole,
+  // 1055-1067): r2c is linear, so it is subtracted from the NaN-filled TOD
+  // (the reference fills NaNs in the TOD only, 784-790, not in the dipole)
+  if (c->odip) LCHK(c, launch_axpy(c->tod, c->odip, -1.0, ntot, c->st));
   {
     OpArgs<double> a = op_args<double>(c);
     ACHK(c, run_filter<double>(c, a, SRC_TOD, ROW_FWD | ROW_STORE_D | ROW_FILTER | ROW_INV, nullptr));
@@ -1322,6 +1351,7 @@ int setup_data(sanepic_ctx *c) {
   HIPCHK(c, hipStreamSynchronize(c->st));
   trace_mark("setup: P^T N^-1 d");
   dfree(c, c->tod, ntot);
+  dfree(c, c->odip, ntot);
   return 0;
 }
 
@@ -1335,6 +1365,9 @@ int sanepic_setup(sanepic_ctx *c) {
   trace_mark("setup: enter");
   for (size_t i = 0; i < c->tod_ok.size(); i++)
     if (!c->tod_ok[i]) return fail(c, SANEPIC_EINVAL, "TOD block " + std::to_string(i) + " not set");
+  if (c->odip)  // a dipole for one block needs one for every block, as the reference reads them all
+    for (size_t i = 0; i < c->dip_ok.size(); i++)
+      if (!c->dip_ok[i]) return fail(c, SANEPIC_EINVAL, "dipole block " + std::to_string(i) + " not set");
   // the work that needs no TOD first, while the TOD upload may still run
   ACHK(c, setup_pointing(c));
   ACHK(c, sync_tod_uploads(c));

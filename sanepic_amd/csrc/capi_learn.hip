@@ -228,17 +228,19 @@ namespace sctx {
 // V = sqrt(W) / N.  At DC the solver's weight (1e-6 of bin 1, 537)
 // regularises the segment offsets and is no noise model: n's mean gets the
 // first bin's variance, V_0 = W_0 / (N sqrt(W_1)).
-int noise_rhs(sanepic_ctx *c, uint64_t seed, double *q) {
+int noise_rhs(sanepic_ctx *c, uint64_t seed, double *q, int mask) {
   if (!c->sub.empty()) {
+    // each group unmasked, the Q/U mask once on the union's counts (a pixel
+    // seen once by one group may be seen often enough over all groups)
     LCHK(c, launch_fill(q, 0.0, 3 * c->npix, c->st));
     for (size_t k = 0; k < c->sub.size(); k++) {
       sanepic_ctx *s = c->sub[k];
       ACHK(c, dalloc(s, s->q, 3 * s->npix));
-      const int rc = noise_rhs(s, seed + 0x9e3779b97f4a7c15ULL * (k + 1), s->q);
+      const int rc = noise_rhs(s, seed + 0x9e3779b97f4a7c15ULL * (k + 1), s->q, 0);
       if (rc) return fail(c, rc, s->err);
       LCHK(c, launch_group_scatter_add(q, c->gmap[k], s->q, s->npix, 3, c->st));
     }
-    LCHK(c, cg_launch_mask(q, c->cnt, c->npix, c->st));
+    if (mask) LCHK(c, cg_launch_mask(q, c->cnt, c->npix, c->st));
     return 0;
   }
   const int64_t N = c->ns;
@@ -260,7 +262,7 @@ int noise_rhs(sanepic_ctx *c, uint64_t seed, double *q) {
   a.wt = c->wt_sqrt;
   a.seed = seed;
   ACHK(c, run_filter<double>(c, a, SRC_RNG, ROW_FWD | ROW_FILTER | ROW_INV, nullptr));
-  ACHK(c, run_scatter<double>(c, c->z, q, 1));
+  ACHK(c, run_scatter<double>(c, c->z, q, mask));
   return 0;
 }
 

@@ -134,10 +134,32 @@ int group_pixelize(sanepic_ctx *c, int64_t *npix_out, int64_t *ndef_out) {
 
 int group_setup(sanepic_ctx *c) {
   trace_mark("group setup: enter");
+  // every sample of every detector's whole segments uploaded (overlapping or
+  // repeated ranges do not count twice)
   for (int64_t d = 0; d < c->ndet; d++) {
     const sanepic_ctx *s = c->sub[(size_t)c->det_group[(size_t)d]];
-    if (c->tod_set[(size_t)d] < s->nseg * s->ns)
-      return fail(c, SANEPIC_EINVAL, "TOD of detector " + std::to_string(d) + " not fully set");
+    std::vector<std::pair<int64_t, int64_t>> iv = c->tod_set[(size_t)d];
+    std::sort(iv.begin(), iv.end());
+    int64_t covered = 0;
+    for (auto &r : iv)
+      if (r.first <= covered) covered = std::max(covered, r.second);
+    if (covered < s->nseg * s->ns)
+      return fail(c, SANEPIC_EINVAL, "TOD of detector " + std::to_string(d) + " not set from sample " +
+                                         std::to_string(covered));
+  }
+  // a dipole for one detector needs one for every detector's whole segments
+  bool any_dip = false;
+  for (auto &v : c->dip_set) any_dip = any_dip || !v.empty();
+  for (int64_t d = 0; any_dip && d < c->ndet; d++) {
+    const sanepic_ctx *s = c->sub[(size_t)c->det_group[(size_t)d]];
+    std::vector<std::pair<int64_t, int64_t>> iv = c->dip_set[(size_t)d];
+    std::sort(iv.begin(), iv.end());
+    int64_t covered = 0;
+    for (auto &r : iv)
+      if (r.first <= covered) covered = std::max(covered, r.second);
+    if (covered < s->nseg * s->ns)
+      return fail(c, SANEPIC_EINVAL, "dipole of detector " + std::to_string(d) + " not set from sample " +
+                                         std::to_string(covered));
   }
   // every group's TOD-free set-up first, while the TOD upload may still run
   for (sanepic_ctx *s : c->sub) {
@@ -278,7 +300,8 @@ int sanepic_create_uneven(const sanepic_uneven_config *cfg, sanepic_ctx **out) {
   }
   c->det_group.assign((size_t)cfg->ndet, 0);
   c->det_local.assign((size_t)cfg->ndet, 0);
-  c->tod_set.assign((size_t)cfg->ndet, 0);
+  c->tod_set.assign((size_t)cfg->ndet, {});
+  c->dip_set.assign((size_t)cfg->ndet, {});
   c->ntot = 0;
   for (size_t g = 0; g < subs.size(); g++) {
     sanepic_ctx *s = subs[g];
@@ -355,7 +378,25 @@ int sanepic_set_tod_range(sanepic_ctx *c, int64_t idet, int64_t t0, int64_t n, c
   } else {
     HIPCHK(c, hipMemcpy(dst, tod, (size_t)n * sizeof(double), hipMemcpyHostToDevice));
   }
-  c->tod_set[(size_t)idet] += n;
+  c->tod_set[(size_t)idet].emplace_back(t0, t0 + n);
+  return SANEPIC_OK;
+}
+
+int sanepic_set_dipole_range(sanepic_ctx *c, int64_t idet, int64_t t0, int64_t n, const double *odip) {
+  if (!c) return SANEPIC_EINVAL;
+  ACHK(c, need_groups(c));
+  if (idet < 0 || idet >= c->ndet) return fail(c, SANEPIC_EINVAL, "detector index out of range");
+  if (c->setup_done) return fail(c, SANEPIC_EINVAL, "TOD already consumed by sanepic_setup");
+  sanepic_ctx *s = c->sub[(size_t)c->det_group[(size_t)idet]];
+  const int64_t len = s->nseg * s->ns;
+  if (t0 < 0 || n < 0 || t0 + n > len) return fail(c, SANEPIC_EINVAL, "dipole range beyond the detector's segments");
+  if (n == 0) return SANEPIC_OK;
+  if (!odip) return fail(c, SANEPIC_EINVAL, "null dipole buffer");
+  const int rc = dalloc(s, s->odip, s->ntot);
+  if (rc) return fail(c, rc, s->err);
+  HIPCHK(c, hipMemcpy(s->odip + (int64_t)c->det_local[(size_t)idet] * len + t0, odip, (size_t)n * sizeof(double),
+                      hipMemcpyHostToDevice));
+  c->dip_set[(size_t)idet].emplace_back(t0, t0 + n);
   return SANEPIC_OK;
 }
 

@@ -39,6 +39,11 @@ struct sanepic_ctx {
   int64_t bytes = 0;
   // device
   double *ra = nullptr, *dec = nullptr, *psi = nullptr, *tod = nullptr;
+  // orbital dipole timelines [ndet][nseg][ns], subtracted from the
+  // NaN-filled TOD before its FFT (set by sanepic_set_dipole; released with
+  // the TOD), and which blocks were set
+  double *odip = nullptr;
+  std::vector<char> dip_ok;
   DetConst *d_det = nullptr;
   int32_t *pix = nullptr, *cnt = nullptr;
   double2 *cs = nullptr, *csfp = nullptr, *cs_sorted = nullptr, *csfp_sorted = nullptr;
@@ -117,7 +122,8 @@ struct sanepic_ctx {
   const int32_t *gsrc = nullptr;
   int64_t tl_len = 0;                          // boresight timeline samples
   std::vector<std::pair<int64_t, int64_t>> tl_set;  // pointing ranges uploaded
-  std::vector<int64_t> tod_set;                // samples uploaded per detector
+  std::vector<std::vector<std::pair<int64_t, int64_t>>> tod_set;  // uploaded [t0, t1) per detector
+  std::vector<std::vector<std::pair<int64_t, int64_t>>> dip_set;  // dipole [t0, t1) per detector
   bool borrowed = false;                       // stream and pointing belong to a parent
   // HIP graphs of one plain PCG iteration: [0] normal, [1] residual refresh
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
@@ -356,7 +362,7 @@ int group_apply_A(sanepic_ctx *c, const double *x, double *q, int mask, bool chi
 int group_pixelize(sanepic_ctx *c, int64_t *npix, int64_t *n_deferred);
 int group_setup(sanepic_ctx *c);
 // q = P^T N^-1 n (masked) for noise n with the detectors' spectra drawn from seed
-int noise_rhs(sanepic_ctx *c, uint64_t seed, double *q);
+int noise_rhs(sanepic_ctx *c, uint64_t seed, double *q, int mask = 1);
 void set_create_error(const std::string &m);  // sanepic_last_error(NULL) of this thread
 // Flexible PCG with the attached network (capi_learn.hip): start after x0 and
 // the deflation set-up (q = A x0), and one iteration.

@@ -29,31 +29,36 @@ namespace {
 constexpr double kD2PI = 6.2831853071795864769252867665590057683943387987502;
 
 void usage(const char *name) {
-  std::cerr << "USAGE: " << name << ": Map-making (MI355X)\n\n"
-            << name << " [-F <data dir>] [-f <used samples (file name)>]\n"
-            << "-B <bolo ext>          bolo field extension\n"
-            << "-C <bolo name>         Add a single bolometer (can be repeated; see also -D)\n"
-            << "-d <Det number>        Number of detectors to include\n"
-            << "-D <detector list>     List of detectors to include (see also -C)\n"
-            << "-e <output file str>   id string in output file\n"
-            << "-F <data dir>          source data directory\n"
-            << "-Z <pointing dir>      source pointing directory + prefixe\n"
-            << "-f <sample list file>  file including list of 1st and last samples\n"
-            << "-k <noise prefixe>     noise power spectrum file prefixe\n"
-            << "-u Noise Nb samp       Number of samples of the noise power spectra\n"
-            << "-X <det location file> File providing pointing info for each detector wrt boresight\n"
-            << "-O <output dir>        output directory\n"
-            << "-P <pointing ext>      pointing field extension\n"
-            << "-p <polar>             set this keyword if polarization is included\n"
-            << "-l <data seg length>   Lenght of data segments. Must be the same for all segments\n"
-            << "-n <nb of segments>    Number of data segments\n"
-            << "-N <Nside>             Healpix nside\n"
-            << "-j <Delt iter>         Number of iterations after which the maps are recalculated\n"
-            << "-I <init map>          start from a compressed map (the _A_ output)\n"
-            << "-i <write iter maps>   write intermediate maps every -j iterations\n"
-            << "-h <HWP on>            Set if Half Wave Plate. Default is 0\n"
-            << "-w <Omega HWP>         HWP angular frequency per sample\n"
-            << "Accepted and ignored as in the reference: -H -J -A -m -G -L -g -z -M -o -R -r -c -b -K -E\n";
+  std::cerr
+      << "usage: " << name << " [options]   (GLS I/Q/U map-making on one MI355X)\n\n"
+      << "Inputs\n"
+      << "  -F dir      directory holding the detector timelines data_<detector><-B>.bi\n"
+      << "  -B ext      suffix of the timeline file names\n"
+      << "  -Z prefix   path prefix of the boresight files <prefix>{ra,dec,psi}<-P>.bi\n"
+      << "  -P ext      suffix of the boresight file names\n"
+      << "  -f file     segment list: int64 (first, last) sample pairs\n"
+      << "  -n count    number of segments in that list\n"
+      << "  -l length   samples per segment (the same for every segment)\n"
+      << "  -k prefix   path prefix of the inverse noise spectra <prefix><detector>\n"
+      << "  -u count    values read from each inverse noise spectrum\n"
+      << "  -X file     detector table: num pnum lel xel angle tel name (degrees)\n"
+      << "  -C name     use this detector (repeat for more)\n"
+      << "  -D file     use the detectors listed in this file, one per line\n"
+      << "  -d count    number of detectors to use\n"
+      << "  -o 0|1      subtract the orbital dipole read from <-F>/orbdip_<detector>\n"
+      << "Map and solver\n"
+      << "  -N nside    HEALPix resolution of the maps (RING ordering)\n"
+      << "  -p 1        solve for I, Q and U (required)\n"
+      << "  -h 0|1      half-wave plate in the optical path\n"
+      << "  -w omega    half-wave plate angle step per sample (radians)\n"
+      << "  -I file     start from this compressed [I|Q|U] map (an earlier _A_ output)\n"
+      << "Outputs\n"
+      << "  -O prefix   path prefix of every output map\n"
+      << "  -e tag      tag inserted in the output file names\n"
+      << "  -i 0|1      also write the I, Q, U maps of intermediate iterations\n"
+      << "  -j step     iterations between intermediate maps (default 10)\n"
+      << "Read for compatibility and without effect on the maps: -H -J -A -m -G -L -g -z -M -c\n"
+      << "Rejected unless 0 (not available): -r -R -b (-K), -E\n";
   exit(1);
 }
 
@@ -175,7 +180,8 @@ double now() {
 int main(int argc, char *argv[]) {
   const int rank = mpi_rank();
   int iterw = 10;
-  bool polar = false, hwp = false, initMapOn = false, writeIterOn = true, pfile = false;
+  bool polar = false, hwp = false, initMapOn = false, writeIterOn = true, pfile = false, odip = false;
+  int recalib = 0, galbp = 0;
   long ndet = 0, nsegtot = 0, ns = 0, nssp = 0, nside = 0;
   double omega = 0.0;
   std::string outdir, bext, pext, termin_out, noisepre, filesegnum, dirfile, dirfilep, initMap, filedetinfo;
@@ -213,9 +219,14 @@ int main(int argc, char *argv[]) {
           exit(0);
         }
         break;
-      case 'H': case 'J': case 'A': case 'm': case 'G': case 'L': case 'g': case 'z': case 'M':
-      case 'o': case 'R': case 'r': case 'c': case 'b': case 'K':
-        break;  // parsed and unused on the map-making path (see the reference's main)
+      case 'o': odip = atoi(optarg) != 0; break;
+      case 'r': recalib = atoi(optarg); break;
+      case 'b': galbp = atoi(optarg); break;
+      // -R acts only inside the -r loop (2254-2270); -K names the -b file;
+      // the rest are read by the reference and never used
+      case 'R': case 'K':
+      case 'H': case 'J': case 'A': case 'm': case 'G': case 'L': case 'g': case 'z': case 'M': case 'c':
+        break;
       default:
         std::cerr << "Option '" << (char)opt << "' not valid. Exiting.\n\n";
         usage(argv[0]);
@@ -231,6 +242,11 @@ int main(int argc, char *argv[]) {
   bolonames.resize(ndet);
   if (!polar) {
     fprintf(stderr, "sanepic: only -p 1 is supported (the reference reads psi unconditionally)\n");
+    exit(1);
+  }
+  if (recalib || galbp) {
+    fprintf(stderr, "sanepic: %s is not supported by this build\n",
+            recalib ? "-r (recalibration loop)" : "-b (GalBP leakage component)");
     exit(1);
   }
   const double t0 = now();
@@ -295,6 +311,13 @@ int main(int argc, char *argv[]) {
       tio += now() - tb;
       rc = sanepic_set_tod(ctx, i, s, tod.data());
       if (rc) die_lib(ctx, rc, "sanepic_set_tod");
+      if (odip) {  // 702-720: <data dir>/orbdip_<bolo>, no extension
+        const double tc = now();
+        if (!read_vect(tod.data(), dirfile + "/orbdip_" + bolonames[i], seg[2 * s], seg[2 * s + 1])) exit(1);
+        tio += now() - tc;
+        rc = sanepic_set_dipole(ctx, i, s, tod.data());
+        if (rc) die_lib(ctx, rc, "sanepic_set_dipole");
+      }
     }
   }
   printf("DATA SUCCESSFULLY READ\n");

@@ -109,6 +109,17 @@ def ang2pix_ring(nside, theta, phi):
     return out
 
 
+def orbital_dipole(ra, dec, start=0, fsamp=19.0, amp_k=2.725 * 1.0e-4):
+    """Orbital dipole seen along (ra, dec): T_CMB v/c (v = 30 km/s) times the
+    cosine between the line of sight and the orbital velocity, which lies in
+    the ecliptic plane a quarter turn ahead of the anti-solar direction the
+    scan's precession axis follows."""
+    t = np.arange(start, start + ra.size, dtype=np.float64) / fsamp
+    lam = 2.0 * np.pi * t / YEAR_S + 0.5 * np.pi
+    cd = np.cos(dec)
+    return amp_k * (np.cos(lam) * cd * np.cos(ra) + np.sin(lam) * cd * np.sin(ra))
+
+
 def make_sky(nside, seed=1, sigma_uK=(100.0, 10.0, 10.0)):
     rng = np.random.default_rng(seed)
     n = 12 * nside * nside
@@ -171,8 +182,10 @@ class Config:
 
     def __init__(self, nside=64, ndet=1, nseg=8, ns=2 ** 19, fsamp=19.0, sigma=50e-6,
                  fknee=0.1, alpha=1.0, det_angles_deg=None, offsets_deg=None,
-                 sky_seed=1, noise_seed0=1000, start=0, spin_min=20.0, prec_min=192.0):
+                 sky_seed=1, noise_seed0=1000, start=0, spin_min=20.0, prec_min=192.0,
+                 dipole=False):
         self.nside, self.ndet, self.nseg, self.ns = nside, ndet, nseg, ns
+        self.dipole = dipole  # TODs carry the orbital dipole, given as ds["odip"]
         self.spin_min, self.prec_min = spin_min, prec_min
         self.fsamp, self.sigma, self.fknee, self.alpha = fsamp, sigma, fknee, alpha
         if det_angles_deg is None:
@@ -212,19 +225,24 @@ def _tod_worker(args):
         n = noise_segment(ns, cfg.fsamp, cfg.sigma, cfg.fknee, cfg.alpha,
                           (cfg.noise_seed0 + idet, iseg))
         tod[idet, sl] = sig + n
+        if _SHARED["odip"] is not None:
+            tod[idet, sl] += _SHARED["odip"][sl]
     return idet
 
 
 def make_dataset(cfg, workers=None, log=None):
     """Returns dict(ra, dec, psi [nsamp], tod [ndet, nsamp], ispf [ndet, ns],
-    sky [3, 12 nside^2]).  TOD generation runs in forked workers writing into
-    one shared buffer, deterministic for any worker count."""
+    sky [3, 12 nside^2], odip [ndet, nsamp] or None).  TOD generation runs in
+    forked workers writing into one shared buffer, deterministic for any
+    worker count.  With cfg.dipole every TOD carries the orbital dipole of
+    the boresight direction, which odip holds for the -o input files."""
     ra, dec, psi = scan_pointing(cfg.nsamp, cfg.fsamp, spin_min=cfg.spin_min,
                                  prec_min=cfg.prec_min, start=cfg.start)
     sky = make_sky(cfg.nside, cfg.sky_seed)
     pix = ang2pix_ring(cfg.nside, 0.5 * np.pi - dec, ra)
+    od = orbital_dipole(ra, dec, cfg.start, cfg.fsamp) if getattr(cfg, "dipole", False) else None
     buf = _alloc_shared(cfg.ndet * cfg.nsamp * 8)
-    _SHARED.update(cfg=cfg, tod_buf=buf, pix=pix, psi=psi, sky=sky)
+    _SHARED.update(cfg=cfg, tod_buf=buf, pix=pix, psi=psi, sky=sky, odip=od)
     chunk = max(1, min(cfg.nseg, 8))
     jobs = [(d, s, min(s + chunk, cfg.nseg)) for d in range(cfg.ndet)
             for s in range(0, cfg.nseg, chunk)]
@@ -242,7 +260,8 @@ def make_dataset(cfg, workers=None, log=None):
     ispf = np.stack([ispf_spectrum(cfg.ns, cfg.fsamp, cfg.sigma, cfg.fknee, cfg.alpha)
                      for _ in range(cfg.ndet)])
     _SHARED.clear()
-    return dict(ra=ra, dec=dec, psi=psi, tod=tod, ispf=ispf, sky=sky, pix=pix)
+    odip = None if od is None else np.broadcast_to(od, (cfg.ndet, cfg.nsamp))
+    return dict(ra=ra, dec=dec, psi=psi, tod=tod, ispf=ispf, sky=sky, pix=pix, odip=odip)
 
 
 class UnevenConfig:
@@ -370,6 +389,9 @@ def write_tree(base, cfg, ds, suffix="output"):
         ds["ispf"][k].astype("<f8").tofile(os.path.join(base, "bolo_info", pre + name))
         np.ascontiguousarray(ds["tod"][k]).astype("<f8").tofile(
             os.path.join(base, "tod_list", f"data_{name}{suffix}.bi"))
+        if ds.get("odip") is not None:  # read with -o from <-F>/orbdip_<bolo> (702-708)
+            np.ascontiguousarray(ds["odip"][k]).astype("<f8").tofile(
+                os.path.join(base, "tod_list", f"orbdip_{name}"))
     for key in ("ra", "dec", "psi"):
         ds[key].astype("<f8").tofile(os.path.join(base, "pointings_list", f"pointings_{key}.bi"))
     seg = os.path.join(base, "segments", f"segment_list_{cfg.nseg}_{cfg.ns}.bi")
@@ -391,7 +413,7 @@ def write_tree(base, cfg, ds, suffix="output"):
     )
 
 
-def sanepic_argv(args, exe="sanepic", hwp=0, omega=1.123456789123, pol=1, write_iter=0):
+def sanepic_argv(args, exe="sanepic", hwp=0, omega=1.123456789123, pol=1, write_iter=0, odip=0):
     """The flag list of run_sanepic.sh:55-73 for the given tree."""
     argv = [exe, "-F", args["tod_dir"], "-B", args["suffix"], "-f", args["segfile"],
             "-d", str(args["ndet"])]
@@ -401,4 +423,6 @@ def sanepic_argv(args, exe="sanepic", hwp=0, omega=1.123456789123, pol=1, write_
              "-u", str(args["ns"]), "-O", args["out"], "-p", str(pol), "-l", str(args["ns"]),
              "-n", str(args["nseg"]), "-N", str(args["nside"]), "-X", args["bolofile"],
              "-Z", args["pointing"], "-i", str(write_iter)]
+    if odip:
+        argv += ["-o", str(odip)]
     return argv
