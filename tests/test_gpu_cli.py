@@ -16,10 +16,10 @@ from tests.helpers import oracle_problem, rel
 pytestmark = pytest.mark.gpu
 
 
-def _case(tmp_path, nside=8, ndet=2, nseg=4, ns=2048):
+def _case(tmp_path, nside=8, ndet=2, nseg=4, ns=2048, dipole=False):
     tmin = nseg * ns / 19.0 / 60.0
     cfg = synth.Config(nside=nside, ndet=ndet, nseg=nseg, ns=ns, fknee=0.2,
-                       spin_min=tmin / 40, prec_min=tmin / 5)
+                       spin_min=tmin / 40, prec_min=tmin / 5, dipole=dipole)
     ds = synth.make_dataset(cfg, workers=1)
     args = synth.write_tree(str(tmp_path), cfg, ds)
     return cfg, ds, args
@@ -88,6 +88,92 @@ def test_warm_start_from_A_file(tmp_path):
     # where the cold one stopped, not from the start
     assert warm[0] < 1e-6 * cold[0]
     assert warm[0] < 10 * cold[-1]
+
+
+def _crit(stdout):
+    return np.array([float(m) for m in re.findall(r"^iter = \d+, crit = (\S+),", stdout, re.M)])
+
+
+def test_warm_start_matches_oracle(tmp_path):
+    """-I x0 (1209-1227): the executable's PCG started from a compressed map
+    (here the oracle's 4th iterate: the restart builds new directions, so the
+    history differs from the cold run's) follows the oracle's PCG started
+    from the same map, and its final map solves the oracle's system as well
+    as the oracle's own warm-started solve does."""
+    cfg, ds, args = _case(tmp_path)
+    P = oracle_problem(cfg, ds)
+    P1 = oracle_problem(cfg, ds, nthreads=1)
+    x0 = P.solve(stop=0.0, maxiter=4)["S"]
+    fn = os.path.join(str(tmp_path), "x0.bi")
+    x0.astype("<f8").tofile(fn)
+    r = _run(synth.sanepic_argv(args, exe=sanepic_amd.EXE_PATH) + ["-I", fn])
+    got = _crit(r.stdout)
+    K = 6
+    ref = P.solve(x0=x0, stop=0.0, maxiter=K)
+    spread = rel(P1.solve(x0=x0, stop=0.0, maxiter=K)["crit"], ref["crit"])
+    e = rel(got[:K], ref["crit"])
+    print(f"  warm start: crit history gpu-oracle {e:.2e}, oracle spread {spread:.2e}", flush=True)
+    assert e < max(100 * spread, 1e-8)
+    full = P.solve(x0=x0, stop=1e-15, maxiter=2000)
+    npix = P.npix
+    hits = full["hits"]
+
+    def resid(x):
+        ax = P.apply_A(x)
+        ax[npix:2 * npix][hits[npix:] < 4] = 0.0
+        ax[2 * npix:][hits[npix:] < 4] = 0.0
+        return np.linalg.norm(full["b"] - ax) / np.linalg.norm(full["b"])
+
+    A = np.fromfile(f"{args['out']}_A_{args['ext']}_1", dtype="<f8")
+    rg, ro = resid(A), resid(full["S"])
+    print(f"  final maps: oracle residual of the executable's {rg:.2e}, of the oracle's {ro:.2e} "
+          f"({got.size} and {full['niter']} iterations)", flush=True)
+    assert rg <= 2.0 * ro + 1e-9
+
+
+def test_intermediate_maps_match_oracle_iterates(tmp_path):
+    """-i 1 -j 3 (1711-1751): after the update of iteration k with k % 3 == 0
+    the executable writes that iterate's I, Q, U maps; they equal the
+    oracle's k+1-th iterate."""
+    cfg, ds, args = _case(tmp_path)
+    argv = synth.sanepic_argv(args, exe=sanepic_amd.EXE_PATH, write_iter=1) + ["-j", "3"]
+    _run(argv)
+    P = oracle_problem(cfg, ds)
+    P1 = oracle_problem(cfg, ds, nthreads=1)
+    _, _, _, ind = P.pixels()
+    npix = P.npix
+    O, E = args["out"], args["ext"]
+    for k in (0, 3, 6):
+        ref = P.solve(stop=0.0, maxiter=k + 1)["S"]
+        spread = rel(P1.solve(stop=0.0, maxiter=k + 1)["S"], ref)
+        for c, comp in enumerate("IQU"):
+            got = np.fromfile(f"{O}_{comp}{E}_{k}", dtype="<f8")
+            want = oracle.expand_map(cfg.nside, ind, ref[c * npix:(c + 1) * npix])
+            e = rel(got, want)
+            assert e < max(100 * spread, 1e-10), (k, comp, e, spread)
+    assert not os.path.exists(f"{O}_I{E}_1") and not os.path.exists(f"{O}_I{E}_2")
+
+
+def test_drop_in_dipole_matches_oracle(tmp_path):
+    """-o 1: the orbital dipole files <-F>/orbdip_<bolo> come off the data
+    (700-720, 1055-1067); the executable's PCG follows the oracle's on the
+    same inputs and its map solves the oracle's system."""
+    cfg, ds, args = _case(tmp_path, dipole=True)
+    r = _run(synth.sanepic_argv(args, exe=sanepic_amd.EXE_PATH, odip=1))
+    got = _crit(r.stdout)
+    P = oracle_problem(cfg, ds)
+    ref = P.solve(stop=0.0, maxiter=5)
+    assert rel(got[:5], ref["crit"]) < 1e-8
+    A = np.fromfile(f"{args['out']}_A_{args['ext']}_1", dtype="<f8")
+    npix = P.npix
+    hits = ref["hits"]
+    ax = P.apply_A(A)
+    ax[npix:2 * npix][hits[npix:] < 4] = 0.0
+    ax[2 * npix:][hits[npix:] < 4] = 0.0
+    assert np.linalg.norm(ref["b"] - ax) / np.linalg.norm(ref["b"]) < 1e-6
+    # without -o the dipole stays in the data: a different system
+    r0 = _run(synth.sanepic_argv(args, exe=sanepic_amd.EXE_PATH))
+    assert rel(_crit(r0.stdout)[:5], ref["crit"]) > 1e-6
 
 
 def test_run_script_and_mpirun_compat(tmp_path):

@@ -138,6 +138,50 @@ def test_untrained_network_is_block_jacobi(problem):
     assert abs(len(hl) - len(hb)) <= 1
 
 
+@pytest.mark.parametrize("kind", ["nan", "garbage"])
+def test_bad_network_falls_back_to_block_jacobi(problem, kind):
+    """A network whose parameters are NaN, or random weights far from any
+    trained scale, cannot derail the flexible PCG: NaN or non-descent
+    directions are replaced by block-Jacobi (rejected), a kept network step
+    that lets |r| grow sends the next application to block-Jacobi
+    (fallback), and the solve reaches the same map as block-Jacobi.  Both the
+    default growth factor and the literal rule (fallback whenever a network
+    step fails to reduce |r|, growth 1) are run."""
+    ctx, un = problem
+    keep = un.params()
+    hb = _solve(ctx, sanepic_amd.PRECOND_BLOCK3)
+    xb = ctx.map()
+    ab = xb @ ctx.apply(xb)
+    rng = np.random.default_rng(12)
+    bad = (np.full(keep.size, np.nan, np.float32) if kind == "nan"
+           else (3.0 * rng.standard_normal(keep.size)).astype(np.float32))
+    try:
+        un.set_params(bad)
+        for growth in (0.0, 1.0):
+            ctx.solve_begin(sanepic_amd.PRECOND_LEARNED, 10, None, True, fallback_growth=growth)
+            h = []
+            while len(h) < 600:
+                h.append(ctx.step())
+                if h[-1]["crit"] <= 1e-12:
+                    break
+            st = ctx.learn_stats()
+            res = ctx.true_residual()
+            d = ctx.map() - xb
+            dA = (d @ ctx.apply(d)) / ab
+            print(f"  {kind} network, growth {growth or 2}: {len(h)} its (block-Jacobi {len(hb)}), {st}, "
+                  f"true residual {res:.2e}, A-norm difference {dA:.1e}", flush=True)
+            assert st["rejected"] + st["fallback"] > 0
+            assert h[-1]["crit"] <= 1e-12 and res <= 1.5e-6
+            assert dA <= 1e-6
+            if kind == "nan":  # every network direction (one per step and the
+                # first) is replaced: block-Jacobi's iterates
+                assert st["rejected"] == len(h) + 1 and st["learned"] == 0
+                assert abs(len(h) - len(hb)) <= 1
+    finally:
+        un.set_params(keep)
+    assert np.array_equal(un.params(), keep)
+
+
 def test_harvest_pairs_are_exact(problem):
     ctx, un = problem
     npix = ctx.npix

@@ -2,7 +2,9 @@
 
 At map nside 128 a 64-nside network sees the residual's features averaged
 over 2 x 2 face pixels and its output is added to every pixel of the block
-(a learned coarse-scale correction to block-Jacobi).  Checkers: the oracle's
+(a learned coarse-scale correction to block-Jacobi); at map nside 512 the
+same network pools 8 x 8 face pixels, the factor of the 16-detector
+workload's network.  Checkers: the oracle's
 features pooled with oracle.unet.pool (itself pinned against loops in
 test_oracle_unet.py); the residual loss evaluated in numpy from the
 network's own forward output, upsampled with oracle.unet.coarse_pos; and the
@@ -19,11 +21,12 @@ from tests.helpers import make_case, rel
 
 pytestmark = pytest.mark.gpu
 
-S, SN = 128, 64
+SN = 64
 
 
-@pytest.fixture(scope="module")
-def problem():
+@pytest.fixture(scope="module", params=[128, 512], ids=lambda s: f"map{s}_net64")
+def problem(request):
+    S = request.param
     print(f"coarse network: map nside {S}, network nside {SN}, 4 det x 4 x 2^16", flush=True)
     cfg, ds = make_case(nside=S, ndet=4, nseg=4, ns=2 ** 16, fknee=0.2)
     ctx = sanepic_amd.Context(cfg.nside, cfg.ndet, cfg.nseg, cfg.ns)
@@ -32,7 +35,7 @@ def problem():
     ctx.setup()
     un = sanepic_amd.UNet(SN, seed=1)
     ctx.learn_attach(un)
-    yield ctx, un
+    yield ctx, un, S
     ctx.learn_attach(None)
     ctx.close()
     un.close()
@@ -50,7 +53,7 @@ def _mask(v, cnt, npix):
 
 
 def test_network_finer_than_map_is_rejected(problem):
-    ctx, _ = problem
+    ctx, _, S = problem
     fine = sanepic_amd.UNet(2 * S)
     with pytest.raises(sanepic_amd.SanepicError):
         ctx.learn_attach(fine)
@@ -58,7 +61,7 @@ def test_network_finer_than_map_is_rejected(problem):
 
 
 def test_pooled_features(problem):
-    ctx, un = problem
+    ctx, un, S = problem
     npix = ctx.npix
     pos = ctx.face_index()
     r = np.random.default_rng(4).standard_normal(3 * npix)
@@ -79,7 +82,7 @@ def _trained_a_little(ctx):
 
 
 def test_residual_loss_and_gradient_through_pooling(problem):
-    ctx, un = problem
+    ctx, un, S = problem
     _trained_a_little(ctx)  # a nonzero head, so the output matters
     npix = ctx.npix
     cnt = _cnt(ctx)
@@ -113,7 +116,7 @@ def test_residual_loss_and_gradient_through_pooling(problem):
 
 
 def test_coarse_learned_solve_converges(problem):
-    ctx, un = problem
+    ctx, un, S = problem
     _trained_a_little(ctx)
     ctx.solve_begin(sanepic_amd.PRECOND_BLOCK3, 10, None, True)
     nb = 0

@@ -141,6 +141,21 @@ def test_uneven_rejects_uniform_uploads(case):
     ctx.close()
     with pytest.raises(sanepic_amd.SanepicError):
         sanepic_amd.Context.uneven(32, [8192, 7 * 8192], NSAMP * 3)
+    # the first half of detector 0 uploaded twice leaves its second half
+    # unset, although as many samples arrived as the detector has
+    ctx = sanepic_amd.Context.uneven(cfg.nside, cfg.seglen, cfg.nsamp)
+    for k in range(ctx.ndet):
+        ctx.set_detector(k, 0.0, 0.0, 0.0, ds["ispf"][k])
+    ctx.set_pointing_range(0, ds["ra"], ds["dec"], ds["psi"])
+    for k in range(ctx.ndet):
+        t = ds["tod"][k]
+        h = t.size // 2
+        ctx.set_tod_range(k, 0, t[:h])
+        ctx.set_tod_range(k, 0 if k == 0 else h, t[:h] if k == 0 else t[h:])
+    ctx.pixelize()
+    with pytest.raises(sanepic_amd.SanepicError, match="not set from sample"):
+        ctx.setup()
+    ctx.close()
 
 
 @pytest.mark.parametrize("env", ["SANEPIC_NESTED_MAP", "SANEPIC_ITEM_ORDER"])
