@@ -195,6 +195,15 @@ int sanepic_get_rhs(sanepic_ctx *ctx, double *b);
 
 /* y = P^T N^-1 P x, unmasked (write_tfAS + do_PtNd, 2390-2637). */
 int sanepic_apply(sanepic_ctx *ctx, const double *x, double *y);
+/* sanepic_apply without any device allocation, for callers that cannot
+ * allocate (serving loops, harnesses that forbid allocation): no counterpart
+ * in the reference.  x and y are staged in a static device area of the
+ * library, one per GPU, of 2^24 doubles (128 MiB), so the context may have
+ * at most 2,796,202 seen pixels (larger contexts get SANEPIC_EINVAL).  The
+ * area is shared by every call on one GPU: calls that overlap on one GPU
+ * (from several host threads) race.  Same result as sanepic_apply, bit for
+ * bit. */
+int sanepic_apply_noalloc(sanepic_ctx *ctx, const double *x, double *y);
 /* chi2 of map x: sum over segments of 2|(F d - F P x) sqrt(SpN)|^2 over bins
  * 0..ns/2 (1514-1538). */
 int sanepic_chi2(sanepic_ctx *ctx, const double *x, double *chi2);

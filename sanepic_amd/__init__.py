@@ -40,7 +40,7 @@ ABI_FUNCTIONS = [
     "sanepic_learn_eval", "sanepic_learn_grad", "sanepic_learn_stats", "sanepic_host_alloc", "sanepic_host_free", "sanepic_set_pointing_async",
     "sanepic_set_tod_async", "sanepic_autotune", "sanepic_set_fft_split", "sanepic_create_uneven",
     "sanepic_get_seglen", "sanepic_set_pointing_range", "sanepic_set_tod_range",
-    "sanepic_set_dipole", "sanepic_set_dipole_range",
+    "sanepic_set_dipole", "sanepic_set_dipole_range", "sanepic_apply_noalloc",
 ]
 
 
@@ -166,6 +166,7 @@ def lib():
             "sanepic_set_tod_range": [P, i64, i64, i64, d_p, ctypes.c_int32],
             "sanepic_set_dipole": [P, i64, i64, d_p],
             "sanepic_set_dipole_range": [P, i64, i64, i64, d_p],
+            "sanepic_apply_noalloc": [P, d_p, d_p],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -511,6 +512,13 @@ class Context:
         x = _f64(x)
         y = np.empty(3 * self.npix)
         self._chk(lib().sanepic_apply(self.h, _dp(x), _dp(y)), "apply")
+        return y
+
+    def apply_noalloc(self, x):
+        """A x through the entry point that allocates no device memory."""
+        x = _f64(x)
+        y = np.empty(3 * self.npix)
+        self._chk(lib().sanepic_apply_noalloc(self.h, _dp(x), _dp(y)), "apply_noalloc")
         return y
 
     def chi2(self, x):

@@ -13,6 +13,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <hipcub/hipcub.hpp>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <unordered_map>
@@ -26,6 +27,46 @@
 namespace {
 
 thread_local std::string g_create_error;
+
+}  // namespace
+
+// Staging area of sanepic_apply_noalloc: the map x and the product y, each
+// 3 npix doubles in the operator's interleaved layout, for contexts of up to
+// kNoAllocMaxPix seen pixels.  Every GPU the library runs on holds its own
+// copy (128 MiB) until the process ends.  In host code `scratch` names only
+// the symbol: its device address on each GPU comes from hipGetSymbolAddress.
+namespace {
+constexpr int64_t kNoAllocDoubles = int64_t(1) << 24;
+constexpr int64_t kNoAllocMaxPix = kNoAllocDoubles / 6;
+}  // namespace
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+namespace {
+
+std::mutex g_scratch_mu;
+std::vector<double *> g_scratch;  // device address of `scratch`, per device ordinal
+
+// Looks up (once per device) the address of `scratch` on device dev, which
+// must be the current device.
+int scratch_address(int dev, double **out, std::string *err) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (dev < 0) {
+    *err = "bad device ordinal";
+    return SANEPIC_EINVAL;
+  }
+  if ((size_t)dev >= g_scratch.size()) g_scratch.resize((size_t)dev + 1, nullptr);
+  if (!g_scratch[(size_t)dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      *err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+      return SANEPIC_EHIP;
+    }
+    g_scratch[(size_t)dev] = static_cast<double *>(p);
+  }
+  *out = g_scratch[(size_t)dev];
+  return 0;
+}
 
 }  // namespace
 
@@ -531,6 +572,12 @@ int sanepic_create(const sanepic_config *cfg, sanepic_ctx **out) {
   if (hipSetDevice(cfg->device) != hipSuccess) {
     g_create_error = "hipSetDevice failed";
     return SANEPIC_EHIP;
+  }
+  {  // the no-allocation entry point's staging area on this device (its
+     // first call retries and reports a failure)
+    double *p = nullptr;
+    std::string e;
+    (void)scratch_address(cfg->device, &p, &e);
   }
   trace_mark("create");
   sanepic_ctx *c = new sanepic_ctx();
@@ -1435,6 +1482,36 @@ int sanepic_apply(sanepic_ctx *c, const double *xin, double *yout) {
   dfree(c, xd, 3 * c->npix);
   dfree(c, yd, 3 * c->npix);
   return rc;
+}
+
+int sanepic_apply_noalloc(sanepic_ctx *c, const double *xin, double *yout) {
+  ACHK(c, check_ready(c, true));
+  const int64_t npix = c->npix;
+  if (npix > kNoAllocMaxPix)
+    return fail(c, SANEPIC_EINVAL, "sanepic_apply_noalloc: " + std::to_string(npix) +
+                                       " seen pixels exceed its staging area of " +
+                                       std::to_string(kNoAllocMaxPix) + " (use sanepic_apply)");
+  if (!xin || !yout) return fail(c, SANEPIC_EINVAL, "null map buffer");
+  double *xs = nullptr;
+  {
+    std::string e;
+    int dev = 0;
+    HIPCHK(c, hipGetDevice(&dev));
+    HIPCHK(c, hipSetDevice(c->cfg.device));
+    const int rc = scratch_address(c->cfg.device, &xs, &e);
+    HIPCHK(c, hipSetDevice(dev));
+    if (rc) return fail(c, rc, e);
+  }
+  double *ys = xs + 3 * npix;
+  const size_t nb = (size_t)(3 * npix) * sizeof(double);
+  // the same kernels and data as sanepic_apply, staged in `scratch`
+  HIPCHK(c, hipMemcpyAsync(ys, xin, nb, hipMemcpyHostToDevice, c->st));
+  LCHK(c, launch_block_to_inter(ys, xs, npix, c->st));
+  ACHK(c, apply_A(c, xs, ys, 0, false, c->cfg.precision == SANEPIC_FP64));
+  LCHK(c, launch_inter_to_block(ys, xs, npix, c->st));
+  HIPCHK(c, hipMemcpyAsync(yout, xs, nb, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return SANEPIC_OK;
 }
 
 int sanepic_chi2(sanepic_ctx *c, const double *xin, double *chi2) {
