@@ -6,28 +6,34 @@ scan, white + 1/f noise (fknee 0.1 Hz, index 1).  Synthetic inputs are
 generated in host RAM first (untimed) and copied into pinned host buffers,
 the form a caller hands to the library.
 
-Three complete solves are timed, each from inputs in pinned host memory to
-a true relative residual |b - A x| / |b| <= 1e-6 (H2D, pointing, ang2pix,
-sort, b, preconditioner, CG; file I/O excluded):
-  baseline   FP64 operator, plain 3x3 block-Jacobi PCG
-  headline   FP32 operator with FP64 CG scalars and FP64 residual refresh,
-             flexible PCG with the learned U-Net preconditioner (trained
-             offline by sanepic_amd.train_precond on disjoint seeds) and the
-             monopole deflation; block-Jacobi mixed PCG if no trained
-             weights exist for the workload
-  second     the headline's pipeline with block-Jacobi instead of the
-             network ("block_jacobi_same_precision"), or with the network
-             when the headline is not the learned solver ("learned")
+Complete solves are timed, each from inputs in pinned host memory to a true
+relative residual |b - A x| / |b| <= 1e-6 (H2D, pointing, ang2pix, sort, b,
+preconditioner, CG; file I/O excluded):
+  headline   FP64 operator, plain 3x3 block-Jacobi PCG (the reference's
+             arithmetic); after its time to solution it iterates on, untimed,
+             to |r|^2/|b|^2 <= 1e-15 for the reference map of the error figures
+  learned    FP32 operator with FP64 CG scalars, reductions, 3x3 inverses and
+             FP64 residual refresh; flexible PCG with the learned U-Net
+             preconditioner (trained offline by sanepic_amd.train_precond on
+             disjoint seeds) and the monopole deflation
+  block_jacobi_same_precision   the learned pipeline with block-Jacobi
+  learned_literal_fallback      the learned pipeline whose network steps
+             fall back to block-Jacobi whenever |r| does not decrease
 Reported:
-  value            TOD samples per second per CG iteration of the headline
-                   solve (W warm-up iterations, then exactly K timed)
-  time_to_1e-6_s   the headline solve's time to solution; the baseline's and
-                   the ratio are in "baseline"
-  roofline         the headline operator's dominant kernel: algorithmic bytes
-                   per launch over its average launch duration (HIP events on
-                   the library stream)
+  value            TOD samples per second per CG iteration of the FP64 solve
+                   (W warm-up iterations, then exactly K timed)
+  time_to_1e-6_s   the FP64 solve's time to solution; each other pipeline has
+                   its own, its ratio to the FP64 one, and its I/Q/U map error
+                   against the FP64 reference map (relative RMS over seen
+                   pixels, Q/U over unmasked ones, I monopole projected out)
+  roofline         the FP64 operator's dominant kernel: algorithmic bytes per
+                   launch over its average launch duration (HIP events on the
+                   library stream); traffic from the hardware byte counters in
+                   profiles/
   cpu_baseline     the CPU oracle (an FP64 restatement of the reference's
-                   path) on a bounded sample of the same workload
+                   path) on bounded samples of the same workload: throughput
+                   per CG iteration, and a complete block-Jacobi solve to 1e-6
+                   beside the GPU's solve of the same sample
 """
 import argparse
 import json
@@ -83,9 +89,15 @@ def calibration_copy(nbytes=1 << 30):
     log(f"calibration copy of {nbytes} bytes done")
 
 
-def cpu_baseline(wl, budget_s=20.0):
-    """The oracle on a bounded sample: one detector, a few segments of the
-    same length, same NSIDE; timed over full A-applications."""
+def cpu_baseline(wl, budget_s=20.0, settle=5.0):
+    """The oracle on bounded samples of the same workload (same NSIDE and
+    segment length), on every host core:
+    * throughput: one detector, a few segments, timed over full
+      A-applications (the work of one CG iteration);
+    * time to solution: two detectors (one alone leaves Q and U of most
+      pixels undetermined), 4 segments, block-Jacobi PCG from inputs in host
+      memory to a true residual of 1e-6, beside the GPU's FP64 solve of the
+      same inputs in this run."""
     import oracle
     from sanepic_amd import synth
     ncores = os.cpu_count() or 1
@@ -107,17 +119,87 @@ def cpu_baseline(wl, budget_s=20.0):
             break
     dt = (time.perf_counter() - t0) / n
     nsamp = nseg * cfg.ns
-    return {"value": nsamp / dt, "unit": "samples/s per CG iteration (one A-application)",
-            "cores": ncores, "kind": "port",
-            "sample": f"1 detector x {nseg} segments of {cfg.ns} samples, NSIDE {cfg.nside}, "
-                      f"{n} A-applications timed ({dt * 1e3:.1f} ms each)"}
+    del P, ds
+    out = {"value": nsamp / dt, "unit": "samples/s per CG iteration (one A-application)",
+           "cores": ncores, "kind": "port",
+           "sample": f"1 detector x {nseg} segments of {cfg.ns} samples, NSIDE {cfg.nside}, "
+                     f"{n} A-applications timed ({dt * 1e3:.1f} ms each)"}
+    # time to solution, CPU and GPU, on one sample
+    import sanepic_amd
+    cfg2 = synth.Config(nside=wl["nside"], ndet=2, nseg=nseg, ns=ns, det_angles_deg=[0.0, 112.5])
+    ds2 = synth.make_dataset(cfg2, workers=min(8, ncores))
+    t0 = time.perf_counter()
+    P = oracle.Problem(cfg2.nside, 2, nseg, ns, nthreads=ncores)
+    for k in range(2):
+        P.set_detector(k, 0.0, 0.0, cfg2.det_angles_deg[k], ds2["ispf"][k])
+    P.pixelize(ds2["ra"], ds2["dec"], ds2["psi"])
+    P.load_tod(ds2["tod"])
+    sol = P.solve_pcg(kind=1, deflate=False, stop=1e-12, maxiter=600)
+    npix = P.npix
+    ax = P.apply_A(sol["S"])
+    m = sol["hits"][npix:] < 4
+    ax[npix:2 * npix][m] = 0.0
+    ax[2 * npix:][m] = 0.0
+    res = float(np.linalg.norm(sol["b"] - ax) / np.linalg.norm(sol["b"]))
+    cpu_s = time.perf_counter() - t0
+    del P
+    pin = sanepic_amd.PinnedInputs(cfg2, ds2)
+    time.sleep(settle)
+    g = run_solve(cfg2, pin, "fp64", "block", False, None, 0, 0, 600, profile=False)
+    pin.close()
+    out["time_to_1e-6"] = {
+        "sample": f"2 detectors x {nseg} segments of {ns} samples, NSIDE {cfg2.nside}, FP64 block-Jacobi PCG",
+        "cpu_s": cpu_s, "cpu_iterations": sol["niter"], "cpu_true_residual": res, "cpu_cores": ncores,
+        "gpu_s": g["time_to_solution_s"], "gpu_iterations": g["iterations"],
+        "gpu_true_residual": g["final_true_residual"],
+        "gpu_speedup": (cpu_s / g["time_to_solution_s"]) if g["time_to_solution_s"] else None}
+    return out
+
+
+def read_traffic(precision, kernel):
+    """HBM bytes per launch of a kernel class, measured with the hardware
+    byte counters (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, each scaled
+    by a device copy of known size) and recorded in profiles/*.md as lines
+    'traffic <precision> <kernel class>: <bytes> bytes per launch'."""
+    import glob
+    import re
+    pat = re.compile(r"^traffic (\S+) (.+): (\d+) bytes per launch", re.M)
+    for fn in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        for prec, k, v in pat.findall(open(fn).read()):
+            if prec == precision and k.strip() == kernel:
+                return int(v), os.path.relpath(fn, ROOT)
+    return None, None
+
+
+def map_errors(x, ref, cnt):
+    """Relative RMS error of each component of map x (block layout) against
+    ref over the seen pixels; Q and U over the pixels they are solved for
+    (count >= 2), I with the monopole projected out (the DC bin's 1e-6 weight
+    leaves it nearly free, sanepic.cc:537)."""
+    npix = cnt.size
+    ok = cnt >= 2
+    out = {}
+    for c, name in enumerate("IQU"):
+        a, b = x[c * npix:(c + 1) * npix], ref[c * npix:(c + 1) * npix]
+        if c == 0:
+            a, b = a - a.mean(), b - b.mean()
+        else:
+            a, b = a[ok], b[ok]
+        out[name] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    return out
 
 
 def run_solve(cfg, pin, precision, precond, deflate, weights, steps, warmup, maxiter, profile, net_nside=None,
-              refresh_fp32=False, refresh=10):
+              refresh_fp32=False, refresh=10, fallback_growth=0.0, keep_map=False, reference=False,
+              track=None):
     """One timed pipeline: create -> uploads -> pixelize -> setup -> solve to a
     true residual <= 1e-6.  Returns timings, per-iteration time and kernel
-    times of `steps` iterations measured after `warmup`."""
+    times of `steps` iterations measured after `warmup`; with keep_map the
+    map at the time of solution, with reference also the map after iterating
+    on (untimed) to |r|^2/|b|^2 <= 1e-15.  With track = (reference map,
+    counts), an untimed second solve reads the map back after every
+    iteration and records the first iteration whose I, Q and U are all
+    within 1e-6 relative RMS of the reference (map_errors)."""
     import sanepic_amd
     prec = sanepic_amd.FP64 if precision == "fp64" else sanepic_amd.MIXED
     pc = {"block": sanepic_amd.PRECOND_BLOCK3, "diag": sanepic_amd.PRECOND_DIAG,
@@ -140,7 +222,7 @@ def run_solve(cfg, pin, precision, precond, deflate, weights, steps, warmup, max
     t_setup = time.perf_counter()
     if un is not None:
         ctx.learn_attach(un)
-    ctx.solve_begin(pc, refresh, None, deflate, refresh_fp32=refresh_fp32)
+    ctx.solve_begin(pc, refresh, None, deflate, fallback_growth=fallback_growth, refresh_fp32=refresh_fp32)
     t_begin = time.perf_counter()
     # time to solution: iterate until crit <= 1e-12, then confirm the true
     # residual with an explicit A x
@@ -157,11 +239,28 @@ def run_solve(cfg, pin, precision, precond, deflate, weights, steps, warmup, max
     if true_res is None:
         true_res = ctx.true_residual()
     lstats = ctx.learn_stats() if un is not None else None
+    x_sol = ctx.map() if keep_map or reference else None
+    x_ref, ref_iters, ref_res = None, None, None
+    if reference:  # untimed: on to the reference's own stop rule (1037, 1415)
+        n, crit = 0, 1.0
+        while n < 400 and crit > 1e-15:
+            n += 1
+            crit = ctx.step()["crit"]
+        x_ref, ref_iters, ref_res = ctx.map(), len(hist) + n, ctx.true_residual()
+    map_its, map_err = None, None
+    if track is not None:
+        ctx.solve_begin(pc, refresh, None, deflate, fallback_growth=fallback_growth, refresh_fp32=refresh_fp32)
+        for k in range(1, min(maxiter, 200) + 1):
+            ctx.step()
+            map_err = map_errors(ctx.map(), track[0], track[1])
+            if max(map_err.values()) <= 1e-6:
+                map_its = k
+                break
     # per-iteration time: a fresh start, `warmup` iterations, then exactly
     # `steps` iterations with the kernel timers on
     kt, ms_per_step = None, None
     if profile:
-        ctx.solve_begin(pc, refresh, None, deflate, refresh_fp32=refresh_fp32)
+        ctx.solve_begin(pc, refresh, None, deflate, fallback_growth=fallback_growth, refresh_fp32=refresh_fp32)
         for _ in range(warmup):
             ctx.step()
         ctx.set_profiling(True)
@@ -180,6 +279,11 @@ def run_solve(cfg, pin, precision, precond, deflate, weights, steps, warmup, max
                device_bytes=ctx.device_bytes(), ms_per_step=ms_per_step, kernel_times=kt)
     if un is not None:
         out["learned_stats"] = lstats
+    out["map"] = x_sol
+    out["map_error_iterations"], out["map_error_last"] = map_its, map_err
+    if reference:
+        out.update(ref_map=x_ref, ref_iterations=ref_iters, ref_true_residual=ref_res,
+                   cnt=ctx.get_setup()[0][:npix] // 2)
     log(f"[{precision} {precond}{' +defl' if deflate else ''}] upload {out['upload_s']:.2f} s, "
         f"pixelize {out['pixelize_s']:.2f} s, setup {out['setup_s']:.2f} s, begin {out['begin_s']:.2f} s; "
         f"{its} iterations, true residual {true_res:.3e}, time to solution "
@@ -198,22 +302,22 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="c1")
-    ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
-    ap.add_argument("--precond", default="learned", choices=["learned", "block", "diag"])
-    ap.add_argument("--no-second", action="store_true",
-                    help="skip the second pipeline (mixed block-Jacobi beside a learned headline, the "
-                         "learned solver beside any other)")
-    ap.add_argument("--no-deflate", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="only the FP64 solve")
+    ap.add_argument("--no-literal", action="store_true",
+                    help="skip the learned pipeline with the literal fallback rule (growth 1)")
+    ap.add_argument("--no-deflate", action="store_true", help="mixed pipelines without the monopole deflation")
     ap.add_argument("--mixed-refresh", type=int, default=20,
                     help="FP64 explicit residual every this many iterations in the mixed-precision pipelines "
-                         "(the FP64 baseline keeps the reference's 10)")
-    ap.add_argument("--no-baseline", action="store_true", help="skip the FP64 block-Jacobi solve")
+                         "(the FP64 solve keeps the reference's 10)")
     ap.add_argument("--maxiter", type=int, default=600)
     ap.add_argument("--settle", type=float, default=5.0,
                     help="idle seconds before each pipeline after the first, outside the timed regions: a "
                          "context created right after another released ~100 GB of device memory has stalled "
                          "for up to 3 s inside its set-up, so every pipeline starts from an idle device")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-map-track", action="store_true",
+                    help="skip the untimed re-solves that find the iteration where the maps reach 1e-6 of the "
+                         "FP64 reference")
     ap.add_argument("--weights", default=None, help="trained network (default: the workload's committed one)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run a 1 GiB device copy first (counter calibration)")
@@ -240,55 +344,59 @@ def main():
     log(f"inputs generated in {time.perf_counter() - tg:.1f} s "
         f"({cfg.ndet} det, {ntot} samples, NSIDE {cfg.nside}), in pinned host memory")
 
-    precond, weights, record = args.precond, None, None
+    weights, record = None, None
     path = args.weights or train_precond.weights_path(args.workload)
     if os.path.exists(path):
         weights, record = train_precond.load_weights(path)
-    if precond == "learned" and weights is None:
-        log(f"no trained network at {path}: block-Jacobi instead")
-        precond = "block"
     deflate = not args.no_deflate
+    rf = args.mixed_refresh
 
-    base = None
-    if not args.no_baseline:
-        base = run_solve(cfg, pin, "fp64", "block", False, None, args.steps, args.warmup,
-                         args.maxiter, profile=False)
-    rf = args.mixed_refresh if args.precision == "mixed" else 10
-    if base is not None:
-        time.sleep(args.settle)
-    head = run_solve(cfg, pin, args.precision, precond, deflate, weights, args.steps, args.warmup,
-                     args.maxiter, profile=True, net_nside=(record or {}).get("net_nside"), refresh=rf)
-    learned = second_block = None
-    if not args.no_second:
-        time.sleep(args.settle)
-        if precond != "learned" and weights is not None:
-            # the same pipeline with the trained network (reported beside the headline)
-            learned = run_solve(cfg, pin, args.precision, "learned", deflate, weights, 2, 1, args.maxiter,
-                                profile=False, net_nside=record.get("net_nside"), refresh=rf)
-        elif precond == "learned":
-            # the same pipeline with block-Jacobi: what the network buys
-            second_block = run_solve(cfg, pin, args.precision, "block", deflate, None, 2, 1, args.maxiter,
-                                     profile=False, refresh=rf)
+    # the headline: FP64 block-Jacobi, the reference's arithmetic
+    head = run_solve(cfg, pin, "fp64", "block", False, None, args.steps, args.warmup, args.maxiter,
+                     profile=True, reference=True)
+    ref_map, cnt = head["ref_map"], head["cnt"]
+    runs = {}
+    if not args.no_learned:
+        plan = [("block_jacobi_same_precision", "block", 0.0)]
+        if weights is not None:
+            plan.insert(0, ("learned", "learned", 0.0))
+            if not args.no_literal:
+                plan.append(("learned_literal_fallback", "learned", 1.0))
+        else:
+            log(f"no trained network at {path}: no learned pipeline")
+        for key, pc, growth in plan:
+            time.sleep(args.settle)
+            runs[key] = run_solve(cfg, pin, "mixed", pc, deflate, weights if pc == "learned" else None, 2, 1,
+                                  args.maxiter, profile=False, net_nside=(record or {}).get("net_nside"),
+                                  refresh=rf, fallback_growth=growth, keep_map=True,
+                                  track=None if args.no_map_track else (ref_map, cnt))
     pin.close()
 
     kt = head["kernel_times"]
     ms_per_step = head["ms_per_step"]
     per = [(kt[k][0], kt[k][1], k) for k in range(4) if kt[k][1] > 0]
-    tot_ms, cnt, dom = max(per)
+    tot_ms, cnt_launch, dom = max(per)
     # per operator application: with per-detector lengths one application
     # launches each kernel class once per detector group
     n_ops = max(kt[4][1], 1)
     avg_ms = tot_ms / n_ops
-    cbs = class_bytes_per_sample(args.precision)
+    cbs = class_bytes_per_sample("fp64")
     alg = ntot * cbs[dom]  # algorithmic bytes of the dominant kernel class per operator application
     achieved = alg / (avg_ms * 1e-3)
     for k in range(4):
         if kt[k][1]:
             log(f"  {CLASS_NAMES[k]}: {kt[k][0] / n_ops:.3f} ms per operator ({kt[k][1]} launches)")
     apply_ms = kt[4][0] / max(kt[4][1], 1)
-    s = 8 if args.precision == "fp64" else 4
     lens = cfg.seglen if uneven else [cfg.ns] * cfg.ndet
-    b_alg = ntot * sum(cbs) + sum(L // 2 + 1 for L in lens) * s + 12 * 3 * head["npix"] * 8
+    npix = head["npix"]
+
+    def b_alg(s):
+        # whole-operator algorithmic bytes: the per-(detector, sample) stream
+        # read once (int32 pixel + cos 2psi + sin 2psi), the per-detector
+        # noise weights, ~12 map-vector passes of the CG in FP64
+        return ntot * (4 + 2 * s) + sum(L // 2 + 1 for L in lens) * s + 12 * 3 * npix * 8
+
+    traffic, traffic_src = read_traffic("fp64", CLASS_NAMES[dom])
     tts = head["time_to_solution_s"]
     out = {
         "metric": "time-to-1e-6 residual (s) + TOD samples/s per CG iter, NSIDE512 16det, 1 GPU",
@@ -301,16 +409,15 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32 operator, f64 CG scalars",
+        "dtype": "f64",
         "data": "synthetic (spin/precession scan, Gaussian I/Q/U sky, white + 1/f noise)",
         "config": {"workload": (f"{cfg.ndet} det, {ntot} samples, NSIDE {cfg.nside}, per-detector "
-                                f"segment lengths m 2^l (m in 1,3,5), {precond} PCG" if uneven else
+                                f"segment lengths m 2^l (m in 1,3,5), FP64 block-Jacobi PCG" if uneven else
                                 f"{cfg.ndet} det x {cfg.nseg * cfg.ns} samples, NSIDE {cfg.nside}, "
-                                f"segments of {cfg.ns}, {precond} PCG")
-                               + (" + monopole deflation" if deflate else ""),
+                                f"segments of {cfg.ns}, FP64 block-Jacobi PCG"),
                    "nside": cfg.nside, "ndet": cfg.ndet, "samples": ntot,
-                   "segment": sorted(set(lens)) if uneven else cfg.ns, "npix": head["npix"], "precond": precond,
-                   "deflate": deflate, "precision": args.precision, "fp64_refresh_every": rf},
+                   "segment": sorted(set(lens)) if uneven else cfg.ns, "npix": npix, "precond": "block",
+                   "deflate": False, "precision": "fp64", "residual_refresh_every": 10},
         "time_to_1e-6_s": tts,
         "iterations_to_1e-6": head["iterations"] if head["converged"] else None,
         "final_true_residual": head["final_true_residual"],
@@ -318,42 +425,50 @@ def main():
         "phases_s": {k: head[k] for k in ("upload_s", "pixelize_s", "setup_s", "begin_s")},
         "idle_before_each_pipeline_s": args.settle,
         "operator_ms": apply_ms,
-        "operator_frac_hbm": b_alg / (apply_ms * 1e-3) / HBM_PEAK,
+        "operator_alg_bytes": b_alg(8),
+        "operator_frac_hbm": b_alg(8) / (apply_ms * 1e-3) / HBM_PEAK,
         "device_bytes": head["device_bytes"],
         "roofline": {"bound": "hbm", "kernel": CLASS_NAMES[dom], "achieved": achieved / 1e9,
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
-                     "traffic": None, "bytes_per_operator": alg, "ms_per_operator": avg_ms,
-                     "launches_per_operator": cnt // n_ops},
+                     "traffic": traffic, "traffic_source": traffic_src, "bytes_per_launch": alg,
+                     "ms_per_launch": avg_ms, "launches_per_operator": cnt_launch // n_ops},
+        "reference_map": {"solver": "the FP64 solve continued to |r|^2/|b|^2 <= 1e-15",
+                          "iterations": head["ref_iterations"], "true_residual": head["ref_true_residual"]},
+        "map_error_at_1e-6": map_errors(head["map"], ref_map, cnt),
     }
-    lrun = head if "learned_stats" in head else learned
-    if lrun is not None:
-        out["learned"] = {"solver": f"{args.precision} operator, flexible PCG with the trained network"
-                                    + (" + monopole deflation" if deflate else ""),
-                          "time_to_1e-6_s": lrun["time_to_solution_s"],
-                          "iterations_to_1e-6": lrun["iterations"],
-                          "ms_per_iteration_in_solve": lrun["solve_s"] * 1e3 / max(lrun["iterations"], 1),
-                          "stats": lrun["learned_stats"],
-                          "training": {k: record.get(k) for k in (
-                              "net_nside", "pairs", "steps", "rounds", "round_pairs", "round_steps", "loss", "noise_gain",
-                              "harvest_seeds", "on_policy_seeds", "holdout_block_jacobi", "holdout_trained",
-                              "validation_block_jacobi_iterations", "validation_iterations", "kept_phase",
-                              "total_seconds")}}
-    if second_block is not None:
-        out["block_jacobi_same_precision"] = {
-            "solver": f"{args.precision} operator, block-Jacobi PCG" + (" + monopole deflation" if deflate else ""),
-            "time_to_1e-6_s": second_block["time_to_solution_s"],
-            "iterations_to_1e-6": second_block["iterations"],
-            "ms_per_iteration_in_solve": second_block["solve_s"] * 1e3 / max(second_block["iterations"], 1)}
-    if base is not None:
-        out["baseline"] = {"solver": "FP64 operator, plain block-Jacobi PCG",
-                           "time_to_1e-6_s": base["time_to_solution_s"],
-                           "iterations_to_1e-6": base["iterations"],
-                           "setup_s": base["upload_s"] + base["pixelize_s"] + base["setup_s"] + base["begin_s"],
-                           "speedup_time_to_solution": (base["time_to_solution_s"] / tts
-                                                        if tts and base["time_to_solution_s"] else None)}
+    for key, r in runs.items():
+        sub = {"solver": ("mixed operator, " + ("flexible PCG with the trained network" if r["precond"] == "learned"
+                                                else "block-Jacobi PCG")
+                          + (" + monopole deflation" if deflate else "")
+                          + (", fallback whenever |r| does not decrease" if key == "learned_literal_fallback"
+                             else "")),
+               "time_to_1e-6_s": r["time_to_solution_s"],
+               "iterations_to_1e-6": r["iterations"] if r["converged"] else None,
+               "final_true_residual": r["final_true_residual"],
+               "ms_per_iteration_in_solve": r["solve_s"] * 1e3 / max(r["iterations"], 1),
+               "setup_s": r["upload_s"] + r["pixelize_s"] + r["setup_s"] + r["begin_s"],
+               "speedup_vs_fp64_time_to_solution": (tts / r["time_to_solution_s"]
+                                                    if tts and r["time_to_solution_s"] else None),
+               "map_error_vs_fp64_reference": map_errors(r["map"], ref_map, cnt),
+               "fp64_refresh_every": rf}
+        if r["map_error_iterations"] is not None or r["map_error_last"] is not None:
+            k = r["map_error_iterations"]
+            sub["maps_within_1e-6_of_fp64_reference"] = {
+                "iterations": k, "last_error": r["map_error_last"],
+                # setup + k iterations at the solve's own time per iteration
+                "time_s_estimate": (sub["setup_s"] + k * sub["ms_per_iteration_in_solve"] * 1e-3) if k else None}
+        if "learned_stats" in r:
+            sub["stats"] = r["learned_stats"]
+        if key == "learned":
+            sub["training"] = {k: record.get(k) for k in (
+                "net_nside", "pairs", "steps", "rounds", "round_pairs", "round_steps", "loss", "noise_gain",
+                "harvest_seeds", "on_policy_seeds", "holdout_block_jacobi", "holdout_trained",
+                "validation_block_jacobi_iterations", "validation_iterations", "kept_phase", "total_seconds")}
+        out[key] = sub
     if not args.no_cpu:
         try:
-            out["cpu_baseline"] = cpu_baseline(wl)
+            time.sleep(args.settle)
+            out["cpu_baseline"] = cpu_baseline(wl, settle=args.settle)
         except Exception as e:  # the GPU result stands on its own
             out["cpu_baseline"] = {"error": str(e)}
     print(json.dumps(out), flush=True)

@@ -68,13 +68,22 @@ def test_drop_in_outputs_match_oracle(tmp_path):
     A = rd(f"{O}_A_{E}_1", 3 * npix)
     for k, c in enumerate("IQU"):
         assert np.array_equal(rd(f"{O}_{c}_{E}_1"), full(A[k * npix:(k + 1) * npix]))
-    # the written map solves the oracle's system to the reference's tolerance
+    # the written map solves the oracle's system as well as the oracle's own
+    # map does (this case stops at the reference's 2000-iteration cap, with
+    # both true residuals near 1e-6)
     b = ref["b"]
-    ax = P.apply_A(A)
     m = hits[npix:] < 4
-    ax[npix:2 * npix][m] = 0
-    ax[2 * npix:][m] = 0
-    assert np.linalg.norm(b - ax) / np.linalg.norm(b) < 1e-6
+
+    def resid(x):
+        ax = P.apply_A(x)
+        ax[npix:2 * npix][m] = 0
+        ax[2 * npix:][m] = 0
+        return np.linalg.norm(b - ax) / np.linalg.norm(b)
+
+    rg, ro = resid(A), resid(ref["S"])
+    print(f"  final maps: oracle residual of the executable's {rg:.2e}, of the oracle's {ro:.2e} "
+          f"({len(crit)} and {ref['niter']} iterations)", flush=True)
+    assert rg <= 2.0 * ro + 1e-9
 
 
 def test_warm_start_from_A_file(tmp_path):
