@@ -14,8 +14,9 @@
  * own HIP stream.  Maps crossing the ABI are compressed vectors over the seen
  * pixels in SANEPIC's block layout [I | Q | U] (length 3*npix), exactly the
  * layout of the reference's `_A_` output file and `-I` warm-start input.
- * Pixel numbers are HEALPix RING.  Sample indices inside a segment run over
- * [0, ns).
+ * With the leakage component (sanepic_set_leakage) maps are [I | Q | U | L],
+ * 4 npix.  Pixel numbers are HEALPix RING.  Sample indices inside a segment
+ * run over [0, ns).
  */
 #ifndef SANEPIC_HIP_H
 #define SANEPIC_HIP_H
@@ -118,6 +119,18 @@ int sanepic_set_pointing_range(sanepic_ctx *ctx, int64_t t0, int64_t n,
 int sanepic_set_tod_range(sanepic_ctx *ctx, int64_t idet, int64_t t0,
                           int64_t n, const double *tod, int32_t queued);
 
+/* The GalBP leakage component (-b 1, sanepic.cc:551-575): every map gets a
+ * fourth block, [I | Q | U | L], and detector idet sees L times
+ * relfgal[idet] (deproject 2869-2870, do_PtNd 2606-2607).  relfgal[ndet]
+ * holds the coefficients already normalised to zero mean and unit variance
+ * over the detectors (561-570); the host reads them from the -K file, which
+ * the reference's ReadVECT (3061-3085) never actually loads.  The diagonal
+ * preconditioner's L entry is Mp_I relfgal[0]^2 (2565-2566).  Call before
+ * sanepic_pixelize; uniform contexts and the reference's diagonal
+ * preconditioner only.  Every map crossing the ABI then has 4 npix entries
+ * (mp of sanepic_get_setup too; sumcross keeps 3 npix). */
+int sanepic_set_leakage(sanepic_ctx *ctx, const double *relfgal);
+
 /* Orbital dipole timeline of (idet, iseg): ns samples of the file
  * <data dir>/orbdip_<bolo> (read at sanepic.cc:702-708 when -o is set).  It
  * is taken out of the data before b = P^T N^-1 d and chi2, the reference's
@@ -181,7 +194,8 @@ int sanepic_setup(sanepic_ctx *ctx);
 
 /* Products of sanepic_setup, as the reference writes them at iter == 1
  * (1616-1707): hits[2 npix] (I then pol, doubled as in the reference),
- * mp[3 npix] the reference's diagonal Mp (before inversion), sumcross[3 npix]
+ * mp[3 npix] (4 npix with the leakage component) the reference's diagonal Mp
+ * (before inversion), sumcross[3 npix]
  * (sum cos 2psi, sum sin 2psi, 0) -- all unmasked.  Any pointer may be NULL. */
 int sanepic_get_setup(sanepic_ctx *ctx, int64_t *hits, double *mp,
                       double *sumcross);

@@ -59,6 +59,8 @@ def lib():
             "or_expand_map": (None, [_L, _lp, _dp, _dp]),
             "or_exact_blocks": (None, [_V, _dp]),
             "or_problem_set_dipole": (_I, [_V, _dp]),
+            "or_problem_set_leakage": (None, [_V, _dp]),
+            "or_problem_ncomp": (_L, [_V]),
             "or_problem_solve_pcg": (_I, [_V, _dp, _I, _D, _I, _I, _I, _dp, _dp, _dp, _dp, _dp]),
             "or_version": (_I, []),
         }
@@ -173,6 +175,18 @@ class Problem:
         return lib().or_problem_npix(self.h)
 
     @property
+    def ncomp(self):
+        """Map components: 3 (I, Q, U), or 4 with the leakage component."""
+        return lib().or_problem_ncomp(self.h)
+
+    def set_leakage(self, relfgal):
+        """The GalBP leakage component with per-detector coefficients
+        (normalised as the reference does, sanepic.cc:561-570)."""
+        relfgal = _f64(relfgal)
+        assert relfgal.size == self.ndet
+        lib().or_problem_set_leakage(self.h, _d(relfgal))
+
+    @property
     def nbad(self):
         return lib().or_problem_nbad(self.h)
 
@@ -198,12 +212,12 @@ class Problem:
 
     def apply_A(self, x):
         x = _f64(x)
-        y = np.empty(3 * self.npix)
+        y = np.empty(self.ncomp * self.npix)
         lib().or_apply_A(self.h, _d(x), _d(y))
         return y
 
     def rhs(self):
-        b = np.empty(3 * self.npix)
+        b = np.empty(self.ncomp * self.npix)
         lib().or_rhs(self.h, _d(b))
         return b
 
@@ -217,7 +231,7 @@ class Problem:
         return lib().or_chi2(self.h, _d(S))
 
     def setup_pass(self, x0=None):
-        n3 = 3 * self.npix
+        n3 = self.ncomp * self.npix
         x0 = np.zeros(n3) if x0 is None else _f64(x0)
         ax = np.empty(n3)
         lib().or_setup_pass(self.h, _d(x0), _d(ax))
@@ -228,7 +242,7 @@ class Problem:
         return ax, hits, mp, sc
 
     def solve(self, x0=None, maxiter=2000, stop=1e-15, refresh=10):
-        n3 = 3 * self.npix
+        n3 = self.ncomp * self.npix
         S = np.empty(n3)
         Mpt = np.empty(n3)
         b = np.empty(n3)
@@ -256,7 +270,7 @@ class Problem:
     def solve_pcg(self, kind=1, deflate=False, x0=None, maxiter=2000, stop=1e-15, refresh=10):
         """The GPU solver's PCG: kind 0 = the reference's diagonal, 1 = 3x3
         block-Jacobi; deflate = A-DEF2 with the I monopole as coarse vector."""
-        n3 = 3 * self.npix
+        n3 = self.ncomp * self.npix
         S = np.empty(n3)
         b = np.empty(n3)
         M6 = np.zeros(6 * self.npix)

@@ -470,6 +470,9 @@ typedef struct or_problem {
   double omega;
   int nthreads;
   double *offx, *offy, *psidet; /* radians, per detector */
+  int galbp;                    /* 1: a 4th map component (-b, 551-575) */
+  double *relfgal;              /* [ndet] its coefficient per detector */
+  int nc;                       /* map components: 3, or 4 with galbp */
   double *spn;                  /* [ndet][ns] */
   long *datatopix;              /* [ndet][nseg][ns] full-sky RING */
   double *psip, *psipFP;        /* [ndet][nseg][ns] */
@@ -504,8 +507,23 @@ or_problem *or_problem_new(long nside, long ndet, long nseg, long ns,
   P->psidet = (double *)calloc(ndet, sizeof(double));
   P->spn = (double *)calloc((size_t)ndet * ns, sizeof(double));
   P->plan = or_rfft_new(ns);
+  P->nc = 3;
+  P->relfgal = (double *)calloc(ndet, sizeof(double));
   return P;
 }
+
+/* The GalBP leakage component (-b 1): maps get a 4th block [I|Q|U|L] and
+ * each detector sees L times its coefficient relfGal[idet] (deproject
+ * 2869-2870, do_PtNd 2606-2607).  rel: [ndet], already normalised to zero
+ * mean and unit variance over the detectors as at 561-570.  Call before
+ * any operator use. */
+void or_problem_set_leakage(or_problem *P, const double *rel) {
+  P->galbp = 1;
+  P->nc = 4;
+  memcpy(P->relfgal, rel, sizeof(double) * P->ndet);
+}
+
+long or_problem_ncomp(const or_problem *P) { return P->nc; }
 
 void or_problem_free(or_problem *P) {
   if (!P)
@@ -513,6 +531,7 @@ void or_problem_free(or_problem *P) {
   free(P->offx);
   free(P->offy);
   free(P->psidet);
+  free(P->relfgal);
   free(P->spn);
   free(P->datatopix);
   free(P->psip);
@@ -723,7 +742,8 @@ int or_problem_set_dipole(or_problem *P, const double *odip) {
 /* P: Ps[ii] = S[m] + cos2psi S[npix+m] + sin2psi S[2npix+m]  (2845-2873,
  * polar branch with polard = 1, polara = 0, no GalBP). */
 void or_deproject(const double *S, const long *indpix, const long *datatopix,
-                  const double *psip, long ns, long npix, double *Ps) {
+                  const double *psip, long ns, long npix, double *Ps,
+                  int galbp, double rfgal) {
   for (long ii = 0; ii < ns; ii++) {
     long mpos = indpix[datatopix[ii]];
     if (mpos < 0)
@@ -731,6 +751,8 @@ void or_deproject(const double *S, const long *indpix, const long *datatopix,
     Ps[ii] = 0.0;
     Ps[ii] = S[mpos] + 1.0 * (S[npix + mpos] * cos(2.0 * (psip[ii] + 0.0)) +
                               S[2 * npix + mpos] * sin(2.0 * (psip[ii] + 0.0)));
+    if (galbp)
+      Ps[ii] += S[3 * npix + mpos] * rfgal;
   }
 }
 
@@ -751,6 +773,8 @@ static void or_scatter(const or_problem *P, const double *Nd, long idet,
         PNd[mpos] += Nd[ii];
         PNd[npix + mpos] += 1.0 * cos(2.0 * (psiarr[ppos] + 0.0)) * Nd[ii];
         PNd[2 * npix + mpos] += 1.0 * sin(2.0 * (psiarr[ppos] + 0.0)) * Nd[ii];
+        if (P->galbp)
+          PNd[3 * npix + mpos] += P->relfgal[idet] * Nd[ii];
       }
     }
   }
@@ -814,7 +838,7 @@ static void or_diag_block(const or_problem *P, long idet, long iseg,
 static void or_pass(or_problem *P, const double *S, int from_data,
                     double *acc, const double *psiarr_scatter,
                     double *sumCross) {
-  long ns = P->ns, nh = ns / 2 + 1, npix = P->npix, ntot3 = 3 * npix;
+  long ns = P->ns, nh = ns / 2 + 1, npix = P->npix, ntot3 = P->nc * npix;
   int nt = P->nthreads;
   double *priv = (double *)calloc((size_t)nt * ntot3, sizeof(double));
   long nblk = P->ndet * P->nseg;
@@ -841,7 +865,8 @@ static void or_pass(or_problem *P, const double *S, int from_data,
                sizeof(cplx) * nh);
       } else {
         or_deproject(S, P->indpix, P->datatopix + IDX(P, idet, iseg, 0),
-                     P->psip + IDX(P, idet, iseg, 0), ns, npix, Ps);
+                     P->psip + IDX(P, idet, iseg, 0), ns, npix, Ps, P->galbp,
+                     P->relfgal[idet]);
         or_rfft_r2c(P->plan, Ps, fdata, work);
       }
       if (sumCross == NULL) {
@@ -872,13 +897,13 @@ static void or_pass(or_problem *P, const double *S, int from_data,
 
 /* y = P^T N^-1 P S (unmasked), S and y compressed block layout [I|Q|U]. */
 void or_apply_A(or_problem *P, const double *S, double *y) {
-  memset(y, 0, sizeof(double) * 3 * P->npix);
+  memset(y, 0, sizeof(double) * P->nc * P->npix);
   or_pass(P, S, 0, y, P->psip, NULL);
 }
 
 /* b = P^T N^-1 d (1069-1125). */
 void or_rhs(or_problem *P, double *b) {
-  memset(b, 0, sizeof(double) * 3 * P->npix);
+  memset(b, 0, sizeof(double) * P->nc * P->npix);
   or_pass(P, NULL, 1, b, P->psip, NULL);
 }
 
@@ -937,7 +962,8 @@ double or_chi2(or_problem *P, const double *S) {
       const double *Nk = P->spn + idet * ns;
       const cplx *D = P->fdatas0 + (idet * P->nseg + iseg) * nh;
       or_deproject(S, P->indpix, P->datatopix + IDX(P, idet, iseg, 0),
-                   P->psip + IDX(P, idet, iseg, 0), ns, npix, Ps);
+                   P->psip + IDX(P, idet, iseg, 0), ns, npix, Ps, P->galbp,
+                   P->relfgal[idet]);
       or_rfft_r2c(P->plan, Ps, fdata, work);
       double c = 0.0;
       for (long ii = 0; ii < nh; ii++) {
@@ -964,20 +990,20 @@ double or_chi2(or_problem *P, const double *S) {
  * from both do_PtNd calls), hits (twice), and A x0.  Writes the unmasked
  * products into the problem and Ax0. */
 void or_setup_pass(or_problem *P, const double *x0, double *Ax0) {
-  long npix = P->npix, ntot = P->ndet * P->nseg * P->ns;
+  long npix = P->npix, ntot = P->ndet * P->nseg * P->ns, nc = P->nc;
   free(P->hits);
   free(P->Mp);
   free(P->sumCross);
   P->hits = (long *)calloc(2 * npix, sizeof(long));
-  P->Mp = (double *)calloc(3 * npix, sizeof(double));
-  P->sumCross = (double *)calloc(3 * npix, sizeof(double));
+  P->Mp = (double *)calloc(nc * npix, sizeof(double));
+  P->sumCross = (double *)calloc(nc * npix, sizeof(double));
   /* crosslink sums (first do_PtNd call, 1258: sumCross != NULL) */
   or_pass(P, NULL, 0, P->sumCross, P->psipFP, P->sumCross);
   /* A x0 (second call, 1263) */
-  double *Sx = (double *)calloc(3 * npix, sizeof(double));
+  double *Sx = (double *)calloc(nc * npix, sizeof(double));
   if (x0)
-    memcpy(Sx, x0, sizeof(double) * 3 * npix);
-  memset(Ax0, 0, sizeof(double) * 3 * npix);
+    memcpy(Sx, x0, sizeof(double) * nc * npix);
+  memset(Ax0, 0, sizeof(double) * nc * npix);
   or_pass(P, Sx, 0, Ax0, P->psip, NULL);
   free(Sx);
   /* Mp: both calls accumulate the I pair sums; Q = U = I/2 afterwards */
@@ -987,6 +1013,10 @@ void or_setup_pass(or_problem *P, const double *x0, double *Ax0) {
     P->Mp[i] = 2.0 * MpI[i];
     P->Mp[npix + i] = P->Mp[i] / 2.0;
     P->Mp[2 * npix + i] = P->Mp[i] / 2.0;
+    /* the leakage entry scales I by the FIRST detector's coefficient
+     * squared (2565-2566) */
+    if (P->galbp)
+      P->Mp[3 * npix + i] = P->Mp[i] * P->relfgal[0] * P->relfgal[0];
   }
   free(MpI);
   /* hits: both calls, polardet = 1 > 0.5 so pol hits count too (2613-2626) */
@@ -1005,9 +1035,9 @@ void or_problem_get_setup(const or_problem *P, long *hits, double *Mp,
   if (hits)
     memcpy(hits, P->hits, sizeof(long) * 2 * npix);
   if (Mp)
-    memcpy(Mp, P->Mp, sizeof(double) * 3 * npix);
+    memcpy(Mp, P->Mp, sizeof(double) * P->nc * npix);
   if (sumCross)
-    memcpy(sumCross, P->sumCross, sizeof(double) * 3 * npix);
+    memcpy(sumCross, P->sumCross, sizeof(double) * P->nc * npix);
 }
 
 /* The reference's solve (1069-1761) with its masks, diagonal preconditioner,
@@ -1020,7 +1050,7 @@ int or_problem_solve(or_problem *P, const double *x0, int maxiter, double stop,
                      int refresh, double *S_out, double *Mptot_out,
                      double *crit_out, double *crit2_out, double *chi2_out,
                      double *b_out) {
-  long npix = P->npix, n3 = 3 * npix;
+  long npix = P->npix, n3 = P->nc * npix;
   double *b = (double *)calloc(n3, sizeof(double));
   or_rhs(P, b);
   double *S = (double *)calloc(n3, sizeof(double));
@@ -1272,10 +1302,10 @@ static void or_block_precond(const double *b6, int masked, double *m6) {
 }
 
 /* z = M r in block layout. */
-static void or_apply_M(long npix, int kind, const double *Mdiag,
+static void or_apply_M(long npix, long n, int kind, const double *Mdiag,
                        const double *M6, const double *r, double *z) {
   if (kind == 0) {
-    for (long i = 0; i < 3 * npix; i++)
+    for (long i = 0; i < n; i++)
       z[i] = Mdiag[i] * r[i];
     return;
   }
@@ -1312,7 +1342,9 @@ int or_problem_solve_pcg(or_problem *P, const double *x0, int maxiter,
                          double stop, int refresh, int kind, int deflate,
                          double *S_out, double *crit_out, double *crit2_out,
                          double *b_out, double *M6_out) {
-  long npix = P->npix, n3 = 3 * npix;
+  long npix = P->npix, n3 = P->nc * npix;
+  if (kind == 1 && P->galbp)
+    return -2; /* no 3x3 blocks for the 4-component map */
   double *b = (double *)calloc(n3, sizeof(double));
   or_rhs(P, b);
   double *S = (double *)calloc(n3, sizeof(double));
@@ -1377,7 +1409,7 @@ int or_problem_solve_pcg(or_problem *P, const double *x0, int maxiter,
   double delta_n, var_n = 0.0, var00 = 0.0;
 #define OR_PRECOND_STEP(DELTA)                                     \
   do {                                                             \
-    or_apply_M(npix, kind, Md, M6, r, z);                          \
+    or_apply_M(npix, n3, kind, Md, M6, r, z);                          \
     double rz = 0.0, zr_I = 0.0, azz = 0.0;                        \
     for (long ii = 0; ii < n3; ii++)                               \
       rz += r[ii] * z[ii];                                         \

@@ -40,7 +40,7 @@ ABI_FUNCTIONS = [
     "sanepic_learn_eval", "sanepic_learn_grad", "sanepic_learn_stats", "sanepic_host_alloc", "sanepic_host_free", "sanepic_set_pointing_async",
     "sanepic_set_tod_async", "sanepic_autotune", "sanepic_set_fft_split", "sanepic_create_uneven",
     "sanepic_get_seglen", "sanepic_set_pointing_range", "sanepic_set_tod_range",
-    "sanepic_set_dipole", "sanepic_set_dipole_range", "sanepic_apply_noalloc",
+    "sanepic_set_dipole", "sanepic_set_dipole_range", "sanepic_apply_noalloc", "sanepic_set_leakage",
 ]
 
 
@@ -167,6 +167,7 @@ def lib():
             "sanepic_set_dipole": [P, i64, i64, d_p],
             "sanepic_set_dipole_range": [P, i64, i64, i64, d_p],
             "sanepic_apply_noalloc": [P, d_p, d_p],
+            "sanepic_set_leakage": [P, d_p],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -330,6 +331,7 @@ class Context:
             raise SanepicError(f"sanepic_create: {lib().sanepic_last_error(None).decode()}")
         self.h = h
         self.npix = None
+        self.nc = 3
 
     @classmethod
     def uneven(cls, nside, seglen, nsamp, hwp=0, omega=0.0, precision=FP64, device=0):
@@ -345,6 +347,7 @@ class Context:
             raise SanepicError(f"sanepic_create_uneven: {lib().sanepic_last_error(None).decode()}")
         obj.h = h
         obj.nside, obj.ndet, obj.nseg, obj.ns, obj.npix = nside, sl.size, None, None, None
+        obj.nc = 3
         obj.nsamp = nsamp
         return obj
 
@@ -414,6 +417,13 @@ class Context:
         tod = _f64(tod)
         assert self.ns is None or tod.size == self.ns
         self._chk(lib().sanepic_set_tod(self.h, idet, iseg, _dp(tod)), "set_tod")
+
+    def set_leakage(self, relfgal):
+        """GalBP leakage component: maps become [I|Q|U|L] (4 npix)."""
+        relfgal = _f64(relfgal)
+        assert relfgal.size == self.ndet
+        self._chk(lib().sanepic_set_leakage(self.h, _dp(relfgal)), "set_leakage")
+        self.nc = 4
 
     def set_dipole(self, idet, iseg, odip):
         odip = _f64(odip)
@@ -493,7 +503,7 @@ class Context:
     def get_setup(self):
         n = self.npix
         hits = np.empty(2 * n, dtype=np.int64)
-        mp = np.empty(3 * n)
+        mp = np.empty(self.nc * n)
         sc = np.empty(3 * n)
         self._chk(lib().sanepic_get_setup(self.h, _ip(hits), _dp(mp), _dp(sc)), "get_setup")
         return hits, mp, sc
@@ -504,20 +514,20 @@ class Context:
         return out.reshape(self.npix, 6)
 
     def rhs(self):
-        b = np.empty(3 * self.npix)
+        b = np.empty(self.nc * self.npix)
         self._chk(lib().sanepic_get_rhs(self.h, _dp(b)), "get_rhs")
         return b
 
     def apply(self, x):
         x = _f64(x)
-        y = np.empty(3 * self.npix)
+        y = np.empty(self.nc * self.npix)
         self._chk(lib().sanepic_apply(self.h, _dp(x), _dp(y)), "apply")
         return y
 
     def apply_noalloc(self, x):
         """A x through the entry point that allocates no device memory."""
         x = _f64(x)
-        y = np.empty(3 * self.npix)
+        y = np.empty(self.nc * self.npix)
         self._chk(lib().sanepic_apply_noalloc(self.h, _dp(x), _dp(y)), "apply_noalloc")
         return y
 
@@ -555,12 +565,12 @@ class Context:
         return self.map(), hist
 
     def map(self):
-        x = np.empty(3 * self.npix)
+        x = np.empty(self.nc * self.npix)
         self._chk(lib().sanepic_get_map(self.h, _dp(x)), "get_map")
         return x
 
     def nvar(self):
-        x = np.empty(3 * self.npix)
+        x = np.empty(self.nc * self.npix)
         self._chk(lib().sanepic_get_nvar(self.h, _dp(x)), "get_nvar")
         return x
 

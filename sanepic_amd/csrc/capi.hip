@@ -188,6 +188,9 @@ OpArgs<double> op_args<double>(sanepic_ctx *c) {
   a.g = c->g;
   a.pix = c->pix;
   a.cs = c->cs;
+  a.nc = c->nc;
+  a.relf = c->d_relf64;
+  a.spd = c->nseg * c->ns;
   a.tod = c->tod;
   a.buf = c->buf;
   a.dspec = c->dspec;
@@ -211,6 +214,9 @@ OpArgs<float> op_args<float>(sanepic_ctx *c) {
   a.pix = c->pix;
   a.cs = nullptr;
   a.ang = c->ang32;
+  a.nc = c->nc;
+  a.relf = c->d_relf32;
+  a.spd = c->nseg * c->ns;
   a.tod = c->tod;
   a.buf = c->buf32;
   a.dspec = nullptr;
@@ -279,10 +285,15 @@ int run_scatter(sanepic_ctx *c, const T *z, double *q, int mask) {
   ScatterArgs<T> s{};
   s.it = c->items;
   s.perm = c->perm;
-  if constexpr (sizeof(T) == 8)
+  if constexpr (sizeof(T) == 8) {
     s.cs_sorted = c->cs_sorted;
-  else
+    s.relf = c->d_relf64;
+  } else {
     s.ang_sorted = c->ang_sorted32;
+    s.relf = c->d_relf32;
+  }
+  s.nc = c->nc;
+  s.spd = c->nseg * c->ns;
   s.z = z;
   s.sorted = (c->zsorted && c->rank) ? 1 : 0;
   // measured: contiguous item ranges per XCD make P^T slower (RING items
@@ -312,8 +323,8 @@ int apply_A(sanepic_ctx *c, const double *x, double *q, int mask, bool chi2, boo
     OpArgs<double> a = op_args<double>(c);
     a.map = x;
     if (c->pix_nested || c->gsrc) {
-      ACHK(c, dalloc(c, c->map64, 3 * c->npix));
-      LCHK(c, launch_map_slots(x, c->gsrc, c->pix_nested ? c->mpos : nullptr, c->map64, c->npix, c->st));
+      ACHK(c, dalloc(c, c->map64, c->nc * c->npix));
+      LCHK(c, launch_map_slots(x, c->gsrc, c->pix_nested ? c->mpos : nullptr, c->map64, c->npix, c->st, c->nc));
       a.map = c->map64;
     }
     int src = SRC_GATHER;
@@ -328,9 +339,9 @@ int apply_A(sanepic_ctx *c, const double *x, double *q, int mask, bool chi2, boo
     ACHK(c, run_scatter<double>(c, c->z, q, mask));
   } else {
     if (c->pix_nested || c->gsrc)
-      LCHK(c, launch_map_slots(x, c->gsrc, c->pix_nested ? c->mpos : nullptr, c->map32, c->npix, c->st));
+      LCHK(c, launch_map_slots(x, c->gsrc, c->pix_nested ? c->mpos : nullptr, c->map32, c->npix, c->st, c->nc));
     else
-      LCHK(c, launch_d2f(x, c->map32, 3 * c->npix, c->st));
+      LCHK(c, launch_d2f(x, c->map32, c->nc * c->npix, c->st));
     OpArgs<float> a = op_args<float>(c);
     a.map = c->map32;
     int src = SRC_GATHER;
@@ -356,17 +367,17 @@ int check_ready(sanepic_ctx *c, bool need_setup) {
 }
 
 int to_inter(sanepic_ctx *c, const double *host_blk, double *dev_inter) {
-  ACHK(c, dalloc(c, c->tmpblk, 3 * c->npix));
-  HIPCHK(c, hipMemcpyAsync(c->tmpblk, host_blk, 3 * c->npix * sizeof(double), hipMemcpyHostToDevice,
+  ACHK(c, dalloc(c, c->tmpblk, c->nc * c->npix));
+  HIPCHK(c, hipMemcpyAsync(c->tmpblk, host_blk, c->nc * c->npix * sizeof(double), hipMemcpyHostToDevice,
                            c->st));
-  LCHK(c, launch_block_to_inter(c->tmpblk, dev_inter, c->npix, c->st));
+  LCHK(c, launch_block_to_inter(c->tmpblk, dev_inter, c->npix, c->st, c->nc));
   return 0;
 }
 
 int from_inter(sanepic_ctx *c, const double *dev_inter, double *host_blk) {
-  ACHK(c, dalloc(c, c->tmpblk, 3 * c->npix));
-  LCHK(c, launch_inter_to_block(dev_inter, c->tmpblk, c->npix, c->st));
-  HIPCHK(c, hipMemcpyAsync(host_blk, c->tmpblk, 3 * c->npix * sizeof(double), hipMemcpyDeviceToHost,
+  ACHK(c, dalloc(c, c->tmpblk, c->nc * c->npix));
+  LCHK(c, launch_inter_to_block(dev_inter, c->tmpblk, c->npix, c->st, c->nc));
+  HIPCHK(c, hipMemcpyAsync(host_blk, c->tmpblk, c->nc * c->npix * sizeof(double), hipMemcpyDeviceToHost,
                            c->st));
   HIPCHK(c, hipStreamSynchronize(c->st));
   return 0;
@@ -383,6 +394,8 @@ int d2h(sanepic_ctx *c, void *dst, const void *src, size_t bytes) {
 CGArgs cg_args(sanepic_ctx *c) {
   CGArgs a{};
   a.npix = c->npix;
+  a.nc = c->nc;
+  a.ML = c->ML;
   a.x = c->x;
   a.r = c->r;
   a.d = c->d;
@@ -777,6 +790,26 @@ int sanepic_set_tod(sanepic_ctx *c, int64_t idet, int64_t iseg, const double *to
   HIPCHK(c, hipMemcpy(c->tod + (idet * c->nseg + iseg) * c->ns, tod, (size_t)c->ns * sizeof(double),
                       hipMemcpyHostToDevice));
   c->tod_ok[idet * c->nseg + iseg] = 1;
+  return SANEPIC_OK;
+}
+
+int sanepic_set_leakage(sanepic_ctx *c, const double *relfgal) {
+  if (!c) return SANEPIC_EINVAL;
+  if (!c->sub.empty())
+    return fail(c, SANEPIC_EINVAL, "the leakage component needs one segment length for every detector");
+  if (c->pixelized) return fail(c, SANEPIC_EINVAL, "sanepic_set_leakage belongs before sanepic_pixelize");
+  if (!relfgal) return fail(c, SANEPIC_EINVAL, "null leakage coefficients");
+  for (int64_t d = 0; d < c->ndet; d++)
+    if (!std::isfinite(relfgal[d]))
+      return fail(c, SANEPIC_EDATA, "leakage coefficient of detector " + std::to_string(d) + " is not finite");
+  c->relf.assign(relfgal, relfgal + c->ndet);
+  std::vector<float> r32(c->relf.begin(), c->relf.end());
+  ACHK(c, dalloc(c, c->d_relf64, c->ndet));
+  ACHK(c, dalloc(c, c->d_relf32, c->ndet));
+  HIPCHK(c, hipMemcpy(c->d_relf64, c->relf.data(), c->ndet * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->d_relf32, r32.data(), c->ndet * sizeof(float), hipMemcpyHostToDevice));
+  c->nc = 4;
+  c->pregather = true;  // the four-component gather runs in the pre-gather pass only
   return SANEPIC_OK;
 }
 
@@ -1230,12 +1263,12 @@ int sanepic_autotune(sanepic_ctx *c, const char *cache, int32_t *log2_n1, double
   if (c->cfg.precision == SANEPIC_MIXED) {
     ACHK(c, dalloc(c, c->buf32, c->g.npairs * N));
     ACHK(c, dalloc(c, c->z32, c->ntot));
-    ACHK(c, dalloc(c, c->map32, 3 * c->npix));
+    ACHK(c, dalloc(c, c->map32, c->nc * c->npix));
   }
   double *xm = nullptr, *qm = nullptr;
-  ACHK(c, dalloc(c, xm, 3 * c->npix));
-  ACHK(c, dalloc(c, qm, 3 * c->npix));
-  LCHK(c, cg_launch_monopole(xm, c->npix, c->st));
+  ACHK(c, dalloc(c, xm, c->nc * c->npix));
+  ACHK(c, dalloc(c, qm, c->nc * c->npix));
+  LCHK(c, cg_launch_monopole(xm, c->npix, c->st, c->nc));
   hipEvent_t ea, eb;
   HIPCHK(c, hipEventCreate(&ea));
   HIPCHK(c, hipEventCreate(&eb));
@@ -1265,8 +1298,8 @@ int sanepic_autotune(sanepic_ctx *c, const char *cache, int32_t *log2_n1, double
   c->prof = prof;
   (void)hipEventDestroy(ea);
   (void)hipEventDestroy(eb);
-  dfree(c, xm, 3 * c->npix);
-  dfree(c, qm, 3 * c->npix);
+  dfree(c, xm, c->nc * c->npix);
+  dfree(c, qm, c->nc * c->npix);
   if (rc) return rc;
   ACHK(c, set_split(c, best));
   ACHK(c, build_tables(c));
@@ -1308,8 +1341,8 @@ int setup_pointing(sanepic_ctx *c) {
   const int64_t nchi = c->single ? c->g.npairs : c->g.npairs * (N / 4096);
   ACHK(c, dalloc(c, c->chi2part, nchi + 1));
   c->nchi2 = nchi;
-  ACHK(c, dalloc(c, c->b_raw, 3 * npix));
-  ACHK(c, dalloc(c, c->b, 3 * npix));
+  ACHK(c, dalloc(c, c->b_raw, c->nc * npix));
+  ACHK(c, dalloc(c, c->b, c->nc * npix));
   // noise autocorrelation N_ = c2r(|SpN|) per detector (2698-2706)
   {
     double *dwabs = nullptr;
@@ -1348,17 +1381,18 @@ int setup_pointing(sanepic_ctx *c) {
   pa.partial = c->partial;
   LCHK(c, launch_pairsum(pa, c->st));
   // CG vectors
-  ACHK(c, dalloc(c, c->x, 3 * npix));
-  ACHK(c, dalloc(c, c->r, 3 * npix));
-  ACHK(c, dalloc(c, c->d, 3 * npix));
-  ACHK(c, dalloc(c, c->q, 3 * npix));
+  ACHK(c, dalloc(c, c->x, c->nc * npix));
+  ACHK(c, dalloc(c, c->r, c->nc * npix));
+  ACHK(c, dalloc(c, c->d, c->nc * npix));
+  ACHK(c, dalloc(c, c->q, c->nc * npix));
   ACHK(c, dalloc(c, c->M, 6 * npix));
-  ACHK(c, dalloc(c, c->nvar, 3 * npix));
+  ACHK(c, dalloc(c, c->nvar, c->nc * npix));
+  if (c->nc == 4) ACHK(c, dalloc(c, c->ML, npix));
   ACHK(c, dalloc(c, c->part, 8 * 512));
   if (c->cfg.precision == SANEPIC_MIXED) {
     if (!c->single) ACHK(c, dalloc(c, c->buf32, c->g.npairs * N));
     ACHK(c, dalloc(c, c->z32, ntot));
-    ACHK(c, dalloc(c, c->map32, 3 * npix));
+    ACHK(c, dalloc(c, c->map32, c->nc * npix));
   }
   if (c->nested_map && !c->pix_nested) ACHK(c, nest_gather_map(c));
   if (!c->nested_map) {  // pixels stay in ring order (get_pixels)
@@ -1370,7 +1404,7 @@ int setup_pointing(sanepic_ctx *c) {
   // otherwise allocate on first use: a solve that starts in graph mode
   // captures its first FP64 application, and no allocation may fall inside
   // a captured iteration
-  if (c->pix_nested || c->pin_owner) ACHK(c, dalloc(c, c->map64, 3 * npix));
+  if (c->pix_nested || c->pin_owner) ACHK(c, dalloc(c, c->map64, c->nc * npix));
   if (c->pregather) {
     ACHK(c, dalloc(c, c->px64, ntot));
     if (c->cfg.precision == SANEPIC_MIXED) ACHK(c, dalloc(c, c->px32, ntot));
@@ -1393,8 +1427,8 @@ int setup_data(sanepic_ctx *c) {
     ACHK(c, run_scatter<double>(c, c->z, c->b_raw, 0));
   }
   // masked right-hand side
-  HIPCHK(c, hipMemcpyAsync(c->b, c->b_raw, 3 * npix * sizeof(double), hipMemcpyDeviceToDevice, c->st));
-  LCHK(c, cg_launch_mask(c->b, c->cnt, npix, c->st));
+  HIPCHK(c, hipMemcpyAsync(c->b, c->b_raw, c->nc * npix * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+  LCHK(c, cg_launch_mask(c->b, c->cnt, npix, c->st, c->nc));
   HIPCHK(c, hipStreamSynchronize(c->st));
   trace_mark("setup: P^T N^-1 d");
   dfree(c, c->tod, ntot);
@@ -1442,6 +1476,7 @@ int sanepic_get_setup(sanepic_ctx *c, int64_t *hits, double *mp, double *sumcros
       mp[p] = mpI;
       mp[npix + p] = mpI / 2.0;
       mp[2 * npix + p] = mpI / 2.0;
+      if (c->nc == 4) mp[3 * npix + p] = mpI * c->relf[0] * c->relf[0];  // 2565-2566
     }
   }
   if (sumcross) {
@@ -1474,23 +1509,24 @@ int sanepic_get_rhs(sanepic_ctx *c, double *bout) {
 int sanepic_apply(sanepic_ctx *c, const double *xin, double *yout) {
   ACHK(c, check_ready(c, true));
   double *xd = nullptr, *yd = nullptr;
-  ACHK(c, dalloc(c, xd, 3 * c->npix));
-  ACHK(c, dalloc(c, yd, 3 * c->npix));
+  ACHK(c, dalloc(c, xd, c->nc * c->npix));
+  ACHK(c, dalloc(c, yd, c->nc * c->npix));
   int rc = to_inter(c, xin, xd);
   if (!rc) rc = apply_A(c, xd, yd, 0, false, c->cfg.precision == SANEPIC_FP64);
   if (!rc) rc = from_inter(c, yd, yout);
-  dfree(c, xd, 3 * c->npix);
-  dfree(c, yd, 3 * c->npix);
+  dfree(c, xd, c->nc * c->npix);
+  dfree(c, yd, c->nc * c->npix);
   return rc;
 }
 
 int sanepic_apply_noalloc(sanepic_ctx *c, const double *xin, double *yout) {
   ACHK(c, check_ready(c, true));
   const int64_t npix = c->npix;
-  if (npix > kNoAllocMaxPix)
+  const int64_t maxpix = c->nc == 3 ? kNoAllocMaxPix : kNoAllocDoubles / (2 * c->nc);
+  if (npix > maxpix)
     return fail(c, SANEPIC_EINVAL, "sanepic_apply_noalloc: " + std::to_string(npix) +
-                                       " seen pixels exceed its staging area of " +
-                                       std::to_string(kNoAllocMaxPix) + " (use sanepic_apply)");
+                                       " seen pixels exceed its staging area of " + std::to_string(maxpix) +
+                                       " (use sanepic_apply)");
   if (!xin || !yout) return fail(c, SANEPIC_EINVAL, "null map buffer");
   double *xs = nullptr;
   {
@@ -1502,13 +1538,13 @@ int sanepic_apply_noalloc(sanepic_ctx *c, const double *xin, double *yout) {
     HIPCHK(c, hipSetDevice(dev));
     if (rc) return fail(c, rc, e);
   }
-  double *ys = xs + 3 * npix;
-  const size_t nb = (size_t)(3 * npix) * sizeof(double);
+  double *ys = xs + c->nc * npix;
+  const size_t nb = (size_t)(c->nc * npix) * sizeof(double);
   // the same kernels and data as sanepic_apply, staged in `scratch`
   HIPCHK(c, hipMemcpyAsync(ys, xin, nb, hipMemcpyHostToDevice, c->st));
-  LCHK(c, launch_block_to_inter(ys, xs, npix, c->st));
+  LCHK(c, launch_block_to_inter(ys, xs, npix, c->st, c->nc));
   ACHK(c, apply_A(c, xs, ys, 0, false, c->cfg.precision == SANEPIC_FP64));
-  LCHK(c, launch_inter_to_block(ys, xs, npix, c->st));
+  LCHK(c, launch_inter_to_block(ys, xs, npix, c->st, c->nc));
   HIPCHK(c, hipMemcpyAsync(yout, xs, nb, hipMemcpyDeviceToHost, c->st));
   HIPCHK(c, hipStreamSynchronize(c->st));
   return SANEPIC_OK;
@@ -1517,16 +1553,16 @@ int sanepic_apply_noalloc(sanepic_ctx *c, const double *xin, double *yout) {
 int sanepic_chi2(sanepic_ctx *c, const double *xin, double *chi2) {
   ACHK(c, check_ready(c, true));
   double *xd = nullptr, *yd = nullptr;
-  ACHK(c, dalloc(c, xd, 3 * c->npix));
-  ACHK(c, dalloc(c, yd, 3 * c->npix));
+  ACHK(c, dalloc(c, xd, c->nc * c->npix));
+  ACHK(c, dalloc(c, yd, c->nc * c->npix));
   int rc = to_inter(c, xin, xd);
   if (!rc) rc = apply_A(c, xd, yd, 0, true, true);
   CGScalars h{};
   if (!rc) {
     rc = d2h(c, &h, c->sc, sizeof(h));
   }
-  dfree(c, xd, 3 * c->npix);
-  dfree(c, yd, 3 * c->npix);
+  dfree(c, xd, c->nc * c->npix);
+  dfree(c, yd, c->nc * c->npix);
   if (!rc) *chi2 = h.chi2;
   return rc;
 }
@@ -1554,6 +1590,8 @@ int sanepic_solve_begin(sanepic_ctx *c, const sanepic_solve_opts *opts, const do
       c->precond != SANEPIC_PRECOND_LEARNED)
     return fail(c, SANEPIC_EINVAL, "unknown preconditioner");
   const bool learned = c->precond == SANEPIC_PRECOND_LEARNED;
+  if (c->nc == 4 && c->precond != SANEPIC_PRECOND_DIAG)
+    return fail(c, SANEPIC_EINVAL, "the leakage component is solved with the reference's diagonal preconditioner only");
   drop_graphs(c);
   c->use_graph = opts && opts->graph && !learned;
   if (learned && !c->un) return fail(c, SANEPIC_EINVAL, "no network attached (sanepic_learn_attach)");
@@ -1562,22 +1600,23 @@ int sanepic_solve_begin(sanepic_ctx *c, const sanepic_solve_opts *opts, const do
   HIPCHK(c, hipMemsetAsync(c->zero_flag, 0, sizeof(int), c->st));
   LCHK(c, cg_launch_precond(c->pairs6, c->cnt, npix, c->precond == SANEPIC_PRECOND_DIAG ? SANEPIC_PRECOND_DIAG_K
                                                                                         : SANEPIC_PRECOND_BLOCK3_K,
-                            c->M, c->nvar, c->zero_flag, c->st));
+                            c->M, c->nvar, c->zero_flag, c->st, c->nc, c->relf.empty() ? 0.0 : c->relf[0],
+                            c->ML));
   int zf = 0;
   ACHK(c, d2h(c, &zf, c->zero_flag, sizeof(int)));
   if (zf) return fail(c, SANEPIC_EDATA, "ERROR: Mp has elements = 0 (sanepic.cc:1344-1348)");
-  HIPCHK(c, hipMemsetAsync(c->q, 0, 3 * npix * sizeof(double), c->st));
+  HIPCHK(c, hipMemsetAsync(c->q, 0, c->nc * npix * sizeof(double), c->st));
   if (x0) {
     ACHK(c, to_inter(c, x0, c->x));
     ACHK(c, apply_A(c, c->x, c->q, 1, false, true));
-    LCHK(c, cg_launch_mask(c->x, c->cnt, npix, c->st));
+    LCHK(c, cg_launch_mask(c->x, c->cnt, npix, c->st, c->nc));
   } else {
-    HIPCHK(c, hipMemsetAsync(c->x, 0, 3 * npix * sizeof(double), c->st));
+    HIPCHK(c, hipMemsetAsync(c->x, 0, c->nc * npix * sizeof(double), c->st));
   }
   if (c->deflate) {
     // AZ = A e_I (masked) once; r, d serve as scratch before cg_init
-    ACHK(c, dalloc(c, c->az, 3 * npix));
-    LCHK(c, cg_launch_monopole(c->d, npix, c->st));
+    ACHK(c, dalloc(c, c->az, c->nc * npix));
+    LCHK(c, cg_launch_monopole(c->d, npix, c->st, c->nc));
     ACHK(c, apply_A(c, c->d, c->az, 1, false, true));
     CGArgs a = cg_args(c);
     LCHK(c, cg_launch_coarse_setup(a, c->st));

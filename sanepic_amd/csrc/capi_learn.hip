@@ -449,6 +449,7 @@ int sanepic_unet_backward(sanepic_unet *u, const float *x, const float *dy, floa
 
 int sanepic_learn_attach(sanepic_ctx *c, sanepic_unet *u) {
   ACHK(c, check_ready(c, true));
+  if (u && c->nc != 3) return fail(c, SANEPIC_EINVAL, "the learned preconditioner serves I, Q, U maps only");
   if (!u) {  // detach
     if (c->un) {
       HIPCHK(c, hipStreamSynchronize(c->st));
@@ -565,6 +566,7 @@ int sanepic_learn_features(sanepic_ctx *c, const double *r, float *feat, double 
 
 int sanepic_learn_harvest(sanepic_ctx *c, const sanepic_harvest_opts *o, int64_t *added, int32_t *iters) {
   ACHK(c, check_ready(c, true));
+  if (c->nc != 3) return fail(c, SANEPIC_EINVAL, "training pairs are for I, Q, U maps only");
   if (!o || o->maxiter <= 0 || o->capacity <= 0) return fail(c, SANEPIC_EINVAL, "bad harvest options");
   if (o->policy != SANEPIC_HARVEST_BLOCK3 && o->policy != SANEPIC_HARVEST_LEARNED)
     return fail(c, SANEPIC_EINVAL, "unknown harvest policy");

@@ -1,4 +1,5 @@
-// cg.hip -- preconditioned CG on interleaved [npix][3] FP64 vectors.
+// cg.hip -- preconditioned CG on interleaved [npix][nc] FP64 vectors (nc = 3:
+// I, Q, U; nc = 4 with the leakage component L).
 //
 // Mirrors the master-side algebra of src/sanepic.cc (1351-1385 start,
 // 1455-1480 alpha and x update, 1557-1576 residual, 1578-1605 delta, beta and
@@ -36,11 +37,14 @@ __device__ __forceinline__ void block_sums(double (&v)[NV], double (&sh)[NV][RBS
   }
 }
 
-// z = M r for one pixel (M symmetric, stored m00 m01 m02 m11 m12 m22).
-__device__ __forceinline__ void apply_m(const double *M, const double *r, double *z) {
+// z = M r for one pixel (M symmetric on I, Q, U, stored m00 m01 m02 m11 m12
+// m22; the leakage component, if any, has the diagonal entry ML[p]).
+__device__ __forceinline__ void apply_m(const CGArgs &a, int64_t p, const double *r, double *z) {
+  const double *M = a.M + 6 * p;
   z[0] = M[0] * r[0] + M[1] * r[1] + M[2] * r[2];
   z[1] = M[1] * r[0] + M[3] * r[1] + M[4] * r[2];
   z[2] = M[2] * r[0] + M[4] * r[1] + M[5] * r[2];
+  z[3] = a.nc == 4 ? a.ML[p] * r[3] : 0.0;
 }
 
 constexpr int NP = 8;  // partial slots per block
@@ -52,29 +56,27 @@ __device__ __forceinline__ void store_part(double *part, const double (&acc)[5])
   }
 }
 
-__global__ __launch_bounds__(RBS) void k_mask(double *v, const int32_t *cnt, int64_t npix) {
+__global__ __launch_bounds__(RBS) void k_mask(double *v, const int32_t *cnt, int64_t npix, int nc) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p < npix && cnt[p] < 2) {
-    v[3 * p + 1] = 0.0;
-    v[3 * p + 2] = 0.0;
+    v[nc * p + 1] = 0.0;
+    v[nc * p + 2] = 0.0;
   }
 }
 
 // Per-pixel residual terms shared by init / update / refresh:
 // acc[0] += r.Mr, acc[1] += r.r, acc[3] += r_I (Z^T r), acc[4] += AZ.Mr
-__device__ __forceinline__ void resid_terms(const CGArgs &a, int64_t p, const double (&r)[3],
+__device__ __forceinline__ void resid_terms(const CGArgs &a, int64_t p, const double (&r)[4],
                                             double (&acc)[5]) {
-  double z[3];
-  apply_m(a.M + 6 * p, r, z);
-#pragma unroll
-  for (int c = 0; c < 3; c++) {
+  double z[4];
+  apply_m(a, p, r, z);
+  for (int c = 0; c < a.nc; c++) {
     acc[0] += r[c] * z[c];
     acc[1] += r[c] * r[c];
   }
   if (a.az) {
     acc[3] += r[0];
-#pragma unroll
-    for (int c = 0; c < 3; c++) acc[4] += a.az[3 * p + c] * z[c];
+    for (int c = 0; c < a.nc; c++) acc[4] += a.az[a.nc * p + c] * z[c];
   }
 }
 
@@ -83,12 +85,11 @@ __global__ __launch_bounds__(RBS) void k_init(CGArgs a) {
   __shared__ double sh[5][RBS / 64];
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   for (int64_t p = (int64_t)blockIdx.x * RBS + threadIdx.x; p < a.npix; p += (int64_t)RGRID * RBS) {
-    double r[3];
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-      r[c] = a.b[3 * p + c] - a.q[3 * p + c];
-      a.r[3 * p + c] = r[c];
-      acc[2] += a.b[3 * p + c] * a.b[3 * p + c];
+    double r[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int c = 0; c < a.nc; c++) {
+      r[c] = a.b[a.nc * p + c] - a.q[a.nc * p + c];
+      a.r[a.nc * p + c] = r[c];
+      acc[2] += a.b[a.nc * p + c] * a.b[a.nc * p + c];
     }
     resid_terms(a, p, r, acc);
   }
@@ -102,9 +103,9 @@ __global__ __launch_bounds__(RBS) void k_dot(CGArgs a, const double *u, const do
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   if (isum) {  // sum of the I components of v - u
     for (int64_t p = (int64_t)blockIdx.x * RBS + threadIdx.x; p < a.npix; p += (int64_t)RGRID * RBS)
-      acc[3] += v[3 * p] - (u ? u[3 * p] : 0.0);
+      acc[3] += v[a.nc * p] - (u ? u[a.nc * p] : 0.0);
   } else {
-    for (int64_t i = (int64_t)blockIdx.x * RBS + threadIdx.x; i < 3 * a.npix; i += (int64_t)RGRID * RBS)
+    for (int64_t i = (int64_t)blockIdx.x * RBS + threadIdx.x; i < a.nc * a.npix; i += (int64_t)RGRID * RBS)
       acc[0] += u[i] * v[i];
   }
   block_sums<5>(acc, sh);
@@ -164,14 +165,12 @@ __global__ __launch_bounds__(RBS) void k_update(CGArgs a, int refresh) {
   const double alpha = a.sc->alpha;
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   for (int64_t p = (int64_t)blockIdx.x * RBS + threadIdx.x; p < a.npix; p += (int64_t)RGRID * RBS) {
-#pragma unroll
-    for (int c = 0; c < 3; c++) a.x[3 * p + c] += alpha * a.d[3 * p + c];
+    for (int c = 0; c < a.nc; c++) a.x[a.nc * p + c] += alpha * a.d[a.nc * p + c];
     if (!refresh) {
-      double r[3];
-#pragma unroll
-      for (int c = 0; c < 3; c++) {
-        r[c] = a.r[3 * p + c] - alpha * a.q[3 * p + c];
-        a.r[3 * p + c] = r[c];
+      double r[4] = {0.0, 0.0, 0.0, 0.0};
+      for (int c = 0; c < a.nc; c++) {
+        r[c] = a.r[a.nc * p + c] - alpha * a.q[a.nc * p + c];
+        a.r[a.nc * p + c] = r[c];
       }
       resid_terms(a, p, r, acc);
     }
@@ -185,11 +184,10 @@ __global__ __launch_bounds__(RBS) void k_refresh(CGArgs a) {
   __shared__ double sh[5][RBS / 64];
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   for (int64_t p = (int64_t)blockIdx.x * RBS + threadIdx.x; p < a.npix; p += (int64_t)RGRID * RBS) {
-    double r[3];
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-      r[c] = a.b[3 * p + c] - a.q[3 * p + c];
-      a.r[3 * p + c] = r[c];
+    double r[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int c = 0; c < a.nc; c++) {
+      r[c] = a.b[a.nc * p + c] - a.q[a.nc * p + c];
+      a.r[a.nc * p + c] = r[c];
     }
     resid_terms(a, p, r, acc);
   }
@@ -201,13 +199,11 @@ __global__ __launch_bounds__(RBS) void k_refresh(CGArgs a) {
 __global__ __launch_bounds__(RBS) void k_dir(CGArgs a, int first) {
   const double beta = a.sc->beta, c0 = a.sc->coarse;
   for (int64_t p = (int64_t)blockIdx.x * RBS + threadIdx.x; p < a.npix; p += (int64_t)RGRID * RBS) {
-    double r[3], z[3];
-#pragma unroll
-    for (int c = 0; c < 3; c++) r[c] = a.r[3 * p + c];
-    apply_m(a.M + 6 * p, r, z);
+    double r[4] = {0.0, 0.0, 0.0, 0.0}, z[4];
+    for (int c = 0; c < a.nc; c++) r[c] = a.r[a.nc * p + c];
+    apply_m(a, p, r, z);
     z[0] += c0;
-#pragma unroll
-    for (int c = 0; c < 3; c++) a.d[3 * p + c] = first ? z[c] : z[c] + beta * a.d[3 * p + c];
+    for (int c = 0; c < a.nc; c++) a.d[a.nc * p + c] = first ? z[c] : z[c] + beta * a.d[a.nc * p + c];
   }
 }
 
@@ -215,7 +211,7 @@ __global__ __launch_bounds__(RBS) void k_dir(CGArgs a, int first) {
 __global__ __launch_bounds__(RBS) void k_resid(CGArgs a) {
   __shared__ double sh[5][RBS / 64];
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  for (int64_t i = (int64_t)blockIdx.x * RBS + threadIdx.x; i < 3 * a.npix; i += (int64_t)RGRID * RBS) {
+  for (int64_t i = (int64_t)blockIdx.x * RBS + threadIdx.x; i < a.nc * a.npix; i += (int64_t)RGRID * RBS) {
     const double e = a.b[i] - a.q[i];
     acc[0] += e * e;
     acc[1] += a.b[i] * a.b[i];
@@ -228,19 +224,17 @@ __global__ __launch_bounds__(RBS) void k_resid(CGArgs a) {
 __global__ __launch_bounds__(RBS) void k_coarse_start(CGArgs a) {
   const double y = a.sc->tmp[2];
   for (int64_t p = (int64_t)blockIdx.x * RBS + threadIdx.x; p < a.npix; p += (int64_t)RGRID * RBS) {
-    a.x[3 * p] += y;
-#pragma unroll
-    for (int c = 0; c < 3; c++) a.q[3 * p + c] += y * a.az[3 * p + c];
+    a.x[a.nc * p] += y;
+    for (int c = 0; c < a.nc; c++) a.q[a.nc * p + c] += y * a.az[a.nc * p + c];
   }
 }
 
-// e_I: 1 on every I component, 0 on Q and U (the monopole coarse vector)
-__global__ __launch_bounds__(RBS) void k_monopole(double *v, int64_t npix) {
+// e_I: 1 on every I component, 0 on the others (the monopole coarse vector)
+__global__ __launch_bounds__(RBS) void k_monopole(double *v, int64_t npix, int nc) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p < npix) {
-    v[3 * p] = 1.0;
-    v[3 * p + 1] = 0.0;
-    v[3 * p + 2] = 0.0;
+    v[nc * p] = 1.0;
+    for (int c = 1; c < nc; c++) v[nc * p + c] = 0.0;
   }
 }
 
@@ -256,9 +250,9 @@ inline int ok() { return hipGetLastError() == hipSuccess ? 0 : -1; }
 
 }  // namespace
 
-int cg_launch_mask(double *v, const int32_t *cnt, int64_t npix, hipStream_t s) {
+int cg_launch_mask(double *v, const int32_t *cnt, int64_t npix, hipStream_t s, int nc) {
   if (npix > 0)
-    hipLaunchKernelGGL(k_mask, dim3((unsigned)((npix + RBS - 1) / RBS)), dim3(RBS), 0, s, v, cnt, npix);
+    hipLaunchKernelGGL(k_mask, dim3((unsigned)((npix + RBS - 1) / RBS)), dim3(RBS), 0, s, v, cnt, npix, nc);
   return ok();
 }
 
@@ -298,9 +292,9 @@ int cg_launch_resid(const CGArgs &a, hipStream_t s) {
   return ok();
 }
 
-int cg_launch_monopole(double *v, int64_t npix, hipStream_t s) {
+int cg_launch_monopole(double *v, int64_t npix, hipStream_t s, int nc) {
   if (npix > 0)
-    hipLaunchKernelGGL(k_monopole, dim3((unsigned)((npix + RBS - 1) / RBS)), dim3(RBS), 0, s, v, npix);
+    hipLaunchKernelGGL(k_monopole, dim3((unsigned)((npix + RBS - 1) / RBS)), dim3(RBS), 0, s, v, npix, nc);
   return ok();
 }
 
@@ -331,7 +325,8 @@ namespace {
 // inverted, masked diagonal (1/Mp with Mp = 2S for I and S for Q, U; 0 where
 // masked), 1351-1353; the DIAG preconditioner is exactly that diagonal.
 __global__ __launch_bounds__(RBS) void k_precond(const double *p6, const int32_t *cnt, int64_t npix,
-                                                 int kind, double *M, double *nvar, int *zero) {
+                                                 int kind, double *M, double *nvar, int *zero, int nc,
+                                                 double rel0, double *ML) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= npix) return;
   const double *b = p6 + 8 * p;
@@ -341,9 +336,15 @@ __global__ __launch_bounds__(RBS) void k_precond(const double *p6, const int32_t
   const double mpI = 2.0 * S, mpQ = mpI / 2.0;
   const double nI = mpI != 0.0 ? 1.0 / mpI : 0.0;
   const double nQ = (!masked && mpQ != 0.0) ? 1.0 / mpQ : 0.0;
-  nvar[3 * p] = nI;
-  nvar[3 * p + 1] = nQ;
-  nvar[3 * p + 2] = nQ;
+  nvar[nc * p] = nI;
+  nvar[nc * p + 1] = nQ;
+  nvar[nc * p + 2] = nQ;
+  if (nc == 4) {  // leakage: Mp_L = Mp_I relfGal[0]^2, unmasked (2565-2566)
+    const double mpL = mpI * rel0 * rel0;
+    const double nL = mpL != 0.0 ? 1.0 / mpL : 0.0;
+    nvar[nc * p + 3] = nL;
+    ML[p] = nL;
+  }
   double *m = M + 6 * p;
   if (kind == SANEPIC_PRECOND_DIAG_K) {
     m[0] = nI; m[1] = 0.0; m[2] = 0.0; m[3] = nQ; m[4] = 0.0; m[5] = nQ;
@@ -377,9 +378,9 @@ __global__ __launch_bounds__(RBS) void k_precond(const double *p6, const int32_t
 }  // namespace
 
 int cg_launch_precond(const double *p6, const int32_t *cnt, int64_t npix, int kind, double *M,
-                      double *nvar, int *zero, hipStream_t s) {
+                      double *nvar, int *zero, hipStream_t s, int nc, double rel0, double *ML) {
   if (npix > 0)
     hipLaunchKernelGGL(k_precond, dim3((unsigned)((npix + RBS - 1) / RBS)), dim3(RBS), 0, s, p6, cnt,
-                       npix, kind, M, nvar, zero);
+                       npix, kind, M, nvar, zero, nc, rel0, ML);
   return ok();
 }

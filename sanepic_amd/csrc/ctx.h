@@ -48,6 +48,13 @@ struct sanepic_ctx {
   int32_t *pix = nullptr, *cnt = nullptr;
   double2 *cs = nullptr, *csfp = nullptr, *cs_sorted = nullptr, *csfp_sorted = nullptr;
   uint32_t *ang32 = nullptr, *ang_sorted32 = nullptr;  // FP32 operator: 2psi as fractions of a turn
+  // GalBP leakage component (-b): a 4th map component L that detector d sees
+  // times relf[d]; maps are [npix][nc] on the device
+  int nc = 3;
+  std::vector<double> relf;
+  double *d_relf64 = nullptr;
+  float *d_relf32 = nullptr;
+  double *ML = nullptr;              // [npix] diagonal preconditioner entry of L
   int64_t *off = nullptr;
   uint32_t *perm = nullptr;
   int32_t *it_pix = nullptr, *it_len = nullptr, *it_slot = nullptr, *sp_pix = nullptr,

@@ -91,7 +91,10 @@ struct OpArgs {
   const int32_t *pix;   // [ntot] compressed pixel
   const V *cs;          // [ntot] cos 2psi, sin 2psi (FP64 operator)
   const uint32_t *ang;  // [ntot] 2psi as a fraction of a turn (FP32 operator, turn.h)
-  const T *map;         // [npix][3] interleaved I, Q, U
+  const T *map;         // [npix][nc] interleaved I, Q, U (, L)
+  int nc;               // 3, or 4 with the leakage component
+  const T *relf;        // [ndet] leakage coefficient per detector (nc == 4)
+  int64_t spd;          // samples per detector (the detector of sample g is g / spd)
   const double *tod;    // [ntot] (SRC_TOD)
   const T *px;          // [ntot] (SRC_PX)
   V *buf;               // [npairs][N] four-step work buffer
@@ -148,8 +151,11 @@ struct ScatterArgs {
   double *q;              // [npix][3] output (pixel p at qidx[p] when qidx is set)
   const int32_t *qidx;    // output row of each pixel, or null (a detector group into its parent's map)
   int accumulate;         // add into q instead of writing it
-  double *partial;        // [nslots][3]
+  double *partial;        // [nslots][nc]
   int mask;               // apply the Q/U mask
+  int nc;                 // 3, or 4 with the leakage component (q and partial strides)
+  const T *relf;          // [ndet] leakage coefficient per detector (nc == 4)
+  int64_t spd;            // samples per detector: the detector of sample g is g / spd
 };
 
 template <typename T> int launch_scatter(const ScatterArgs<T> &a, hipStream_t s);
@@ -212,12 +218,12 @@ int launch_item_count(const int32_t *order, const int32_t *cnt, int64_t npix, in
 int launch_item_fill(const int32_t *order, const int32_t *cnt, const int64_t *off, const int32_t *kofs,
                      const int32_t *sofs, int64_t npix, int chunk, int32_t *it_pix, int64_t *it_beg,
                      int32_t *it_len, int32_t *it_slot, hipStream_t s);
-// Gather map in slot order: map[3 slot[p] + k] = x[3 src[p] + k] (interleaved x;
-// src reads a detector group's pixels from its parent's map; null = identity).
+// Gather map in slot order: map[nc slot[p] + k] = x[nc src[p] + k] (interleaved
+// x; src reads a detector group's pixels from its parent's map; null = identity).
 int launch_map_slots(const double *x, const int32_t *src, const int32_t *slot, double *map, int64_t npix,
-                     hipStream_t s);
+                     hipStream_t s, int nc = 3);
 int launch_map_slots(const double *x, const int32_t *src, const int32_t *slot, float *map, int64_t npix,
-                     hipStream_t s);
+                     hipStream_t s, int nc = 3);
 int launch_invperm(const uint32_t *perm, uint32_t *rank, int64_t n, hipStream_t s);  // rank[perm[j]] = j
 // dst[idx[i]] += val[i] for distinct idx within the launch
 int launch_add_at_i32(int32_t *dst, const int32_t *idx, const int32_t *val, int64_t n, hipStream_t s);
@@ -233,15 +239,17 @@ struct CGScalars {
 
 struct CGArgs {
   int64_t npix;
+  int nc;                 // components per pixel: 3 (I, Q, U) or 4 (+ leakage L)
   double *x, *r, *d, *q, *b;
-  const double *M;        // [npix][6] symmetric preconditioner blocks
+  const double *M;        // [npix][6] symmetric preconditioner blocks of I, Q, U
+  const double *ML;       // [npix] the leakage component's diagonal entry (nc == 4)
   const int32_t *cnt;     // mask source
   CGScalars *sc;
   double *part;           // [512][8] partials
   const double *az;       // A e_I for the monopole deflation, or null
 };
 
-int cg_launch_mask(double *v, const int32_t *cnt, int64_t npix, hipStream_t s);
+int cg_launch_mask(double *v, const int32_t *cnt, int64_t npix, hipStream_t s, int nc = 3);
 int cg_launch_init(const CGArgs &a, hipStream_t s);      // r = b - q; d = M r; delta, var, var00
 int cg_launch_alpha(const CGArgs &a, hipStream_t s);     // rtq = q.d ; alpha = delta_n / rtq
 int cg_launch_update(const CGArgs &a, int refresh, hipStream_t s);  // x += alpha d; r -= alpha q (or nothing)
@@ -250,19 +258,22 @@ int cg_launch_dir(const CGArgs &a, hipStream_t s);       // d = M r + beta d
 int cg_launch_resid(const CGArgs &a, hipStream_t s);     // tmp[0] = |b - q|^2 (q = A x masked), tmp[1] = |b|^2
 // Monopole deflation (two-level A-DEF2 with Z = e_I): coarse vector, the
 // scalar 1/(Z^T A Z) from AZ, and the starting correction x += y0 Z.
-int cg_launch_monopole(double *v, int64_t npix, hipStream_t s);
+int cg_launch_monopole(double *v, int64_t npix, hipStream_t s, int nc = 3);
 int cg_launch_coarse_setup(const CGArgs &a, hipStream_t s);
 int cg_launch_coarse_start(const CGArgs &a, hipStream_t s);
 enum { SANEPIC_PRECOND_DIAG_K = 0, SANEPIC_PRECOND_BLOCK3_K = 1 };
+// nvar: [npix][nc]; with nc == 4 the leakage entry 1/(Mp_I rel0^2) also
+// goes to ML[npix] (the diagonal preconditioner's fourth entry)
 int cg_launch_precond(const double *p6, const int32_t *cnt, int64_t npix, int kind, double *M,
-                      double *nvar, int *zero, hipStream_t s);
+                      double *nvar, int *zero, hipStream_t s, int nc = 3, double rel0 = 0.0,
+                      double *ML = nullptr);
 int launch_set_values(int32_t *dst, const int64_t *idx, const int32_t *val, int64_t n, hipStream_t s);
 int launch_reduce_chi2(const double *part, int64_t n, CGScalars *sc, hipStream_t s);
 
 // Layout conversion between the ABI's block layout and the device's
 // interleaved layout.
-int launch_block_to_inter(const double *blk, double *inter, int64_t npix, hipStream_t s);
-int launch_inter_to_block(const double *inter, double *blk, int64_t npix, hipStream_t s);
+int launch_block_to_inter(const double *blk, double *inter, int64_t npix, hipStream_t s, int nc = 3);
+int launch_inter_to_block(const double *inter, double *blk, int64_t npix, hipStream_t s, int nc = 3);
 int launch_d2f(const double *a, float *b, int64_t n, hipStream_t s);
 
 // Detector groups (contexts with per-detector segment lengths): add a group's

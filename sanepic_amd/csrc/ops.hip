@@ -42,14 +42,18 @@ __device__ __forceinline__ void pair_info(const PairGeom &g, int64_t pr, int &de
 
 // P: Ps = S[m] + polard (S[npix+m] cos 2psi + S[2npix+m] sin 2psi), polard = 1
 // (deproject, 2865-2868); the map is interleaved [npix][3] on the device.
-template <typename T>
+// LEAK: the map is [npix][4] and Ps += S[3 npix + m] rfGal (2869-2870), the
+// coefficient of the sample's detector g / spd.
+template <typename T, bool LEAK = false>
 __device__ __forceinline__ T gather_one(const OpArgs<T> &a, int64_t gi) {
   const int p = a.pix[gi];
   VT<T> c;
   if constexpr (sizeof(T) == 4) c = cs_of_turn(a.ang[gi]);
   else c = a.cs[gi];
-  const T *m = a.map + 3 * (int64_t)p;
-  return m[0] + (m[1] * c.x + m[2] * c.y);
+  const T *m = a.map + (LEAK ? 4 : 3) * (int64_t)p;
+  T v = m[0] + (m[1] * c.x + m[2] * c.y);
+  if constexpr (LEAK) v += m[3] * a.relf[gi / a.spd];
+  return v;
 }
 
 template <typename T, int SRC>
@@ -119,10 +123,10 @@ __device__ __forceinline__ int64_t zslot(const OpArgs<T> &a, int64_t g) {
   return a.zpos ? (int64_t)a.zpos[g] : g;
 }
 
-template <typename T>
+template <typename T, bool LEAK>
 __global__ __launch_bounds__(256) void k_gather_px(OpArgs<T> a, T *px, int64_t ntot) {
   const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (g < ntot) px[g] = gather_one<T>(a, g);
+  if (g < ntot) px[g] = gather_one<T, LEAK>(a, g);
 }
 
 // ---------------------------------------------------------------- column fwd
@@ -603,7 +607,13 @@ int launch_single(const OpArgs<T> &a, int src, hipStream_t s, int64_t *nblocks) 
 template <typename T>
 int launch_gather_px(const OpArgs<T> &a, T *px, int64_t ntot, hipStream_t s) {
   if (ntot <= 0) return 0;
-  hipLaunchKernelGGL(k_gather_px<T>, dim3((unsigned)((ntot + 255) / 256)), dim3(256), 0, s, a, px, ntot);
+  // the leakage component's maps are gathered only here (the context
+  // pre-gathers P x whenever it has one)
+  if (a.nc == 4)
+    hipLaunchKernelGGL((k_gather_px<T, true>), dim3((unsigned)((ntot + 255) / 256)), dim3(256), 0, s, a, px, ntot);
+  else
+    hipLaunchKernelGGL((k_gather_px<T, false>), dim3((unsigned)((ntot + 255) / 256)), dim3(256), 0, s, a, px,
+                       ntot);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -55,31 +55,34 @@ __device__ __forceinline__ VT<T> sorted_cs(const ScatterArgs<T> &a, int64_t j) {
 // short items (per-detector-length groups hold few detectors) fewer lanes
 // keep several entries per lane in flight.  The summation order depends on L
 // only, which the host derives from the data, so results are deterministic.
-template <typename T, int L>
+// LEAK: a 4th sum, the leakage component, of z times the coefficient of the
+// entry's detector (do_PtNd 2606-2607); the detector of sample g is g / spd.
+template <typename T, int L, bool LEAK>
 __global__ __launch_bounds__(256) void k_scatter(ScatterArgs<T> a) {
   using V = VT<T>;
+  constexpr int NC = LEAK ? 4 : 3;
   const int grp = threadIdx.x / L, lane = threadIdx.x % L;
   const int64_t item = logical_block(a.xcd) * (256 / L) + grp;
   if (item >= a.it.n) return;
   const int p = a.it.pix[item];
   const int64_t e0 = a.it.beg[item];
   const int n = a.it.len[item];
-  T s0 = 0, s1 = 0, s2 = 0;
+  T s0 = 0, s1 = 0, s2 = 0, s3 = 0;
   int k = lane;
   // four entries per lane in flight (their loads first), summed in entry order
   for (; k + 3 * L < n; k += 4 * L) {
     const int64_t j = e0 + k;
     T zv[4];
     V c[4];
-    if (a.sorted) {
+    uint32_t pp[4];
+    if (a.sorted && !LEAK) {
 #pragma unroll
       for (int u = 0; u < 4; u++) zv[u] = a.z[j + L * u];
     } else {
-      uint32_t pp[4];
 #pragma unroll
       for (int u = 0; u < 4; u++) pp[u] = a.perm[j + L * u];
 #pragma unroll
-      for (int u = 0; u < 4; u++) zv[u] = a.z[pp[u]];
+      for (int u = 0; u < 4; u++) zv[u] = a.sorted ? a.z[j + L * u] : a.z[pp[u]];
     }
 #pragma unroll
     for (int u = 0; u < 4; u++) c[u] = sorted_cs(a, j + L * u);
@@ -88,19 +91,23 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterArgs<T> a) {
       s0 += zv[u];
       s1 += zv[u] * c[u].x;
       s2 += zv[u] * c[u].y;
+      if constexpr (LEAK) s3 += zv[u] * a.relf[(int64_t)pp[u] / a.spd];
     }
   }
   for (; k < n; k += L) {
     const int64_t j = e0 + k;
-    const T zv = a.sorted ? a.z[j] : a.z[a.perm[j]];
+    const uint32_t pj = (a.sorted && !LEAK) ? 0u : a.perm[j];
+    const T zv = a.sorted ? a.z[j] : a.z[pj];
     const V c = sorted_cs(a, j);
     s0 += zv;
     s1 += zv * c.x;
     s2 += zv * c.y;
+    if constexpr (LEAK) s3 += zv * a.relf[(int64_t)pj / a.spd];
   }
   s0 = red_lanes<L>(s0);
   s1 = red_lanes<L>(s1);
   s2 = red_lanes<L>(s2);
+  if constexpr (LEAK) s3 = red_lanes<L>(s3);
   if (lane == 0) {
     const int slot = a.it.slot[item];
     if (slot < 0) {
@@ -109,21 +116,24 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterArgs<T> a) {
         q1 = 0.0;
         q2 = 0.0;
       }
-      double *q = a.q + 3 * (int64_t)(a.qidx ? a.qidx[p] : p);
+      double *q = a.q + NC * (int64_t)(a.qidx ? a.qidx[p] : p);
       if (a.accumulate) {
         q[0] += q0;
         q[1] += q1;
         q[2] += q2;
+        if constexpr (LEAK) q[3] += (double)s3;
       } else {
         q[0] = q0;
         q[1] = q1;
         q[2] = q2;
+        if constexpr (LEAK) q[3] = s3;
       }
     } else {
-      double *pp = a.partial + 3 * (int64_t)slot;
+      double *pp = a.partial + NC * (int64_t)slot;
       pp[0] = s0;
       pp[1] = s1;
       pp[2] = s2;
+      if constexpr (LEAK) pp[3] = s3;
     }
   }
 }
@@ -143,7 +153,7 @@ __global__ __launch_bounds__(256) void k_fix(Items it, const double *partial, do
 #pragma unroll
     for (int v = 0; v < NV; v++) acc[v] += partial[(int64_t)(f + k) * NV + v];
   }
-  if constexpr (NV == 3) {
+  if constexpr (NV == 3 || NV == 4) {  // I, Q, U (+ leakage): the Q/U mask
     if (mask && cnt[p] < 2) {
       acc[1] = 0.0;
       acc[2] = 0.0;
@@ -415,14 +425,16 @@ __global__ void k_copy_i32(const int32_t *src, int32_t *dst, int64_t n) {
 
 // map[slot[p]] = x[src[p]] per pixel p (3 values; a null index is the identity)
 template <typename T>
-__global__ void k_map_slots(const double *x, const int32_t *src, const int32_t *slot, T *map, int64_t npix) {
+__global__ void k_map_slots(const double *x, const int32_t *src, const int32_t *slot, T *map, int64_t npix,
+                            int nc) {
   const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (p >= npix) return;
-  const double *xs = x + 3 * (int64_t)(src ? src[p] : p);
-  T *m = map + 3 * (int64_t)(slot ? slot[p] : p);
+  const double *xs = x + nc * (int64_t)(src ? src[p] : p);
+  T *m = map + nc * (int64_t)(slot ? slot[p] : p);
   m[0] = (T)xs[0];
   m[1] = (T)xs[1];
   m[2] = (T)xs[2];
+  if (nc == 4) m[3] = (T)xs[3];
 }
 
 __global__ void k_abs(const double *src, double *dst, int64_t n) {
@@ -518,21 +530,15 @@ __global__ void k_iota(uint32_t *v, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) v[i] = (uint32_t)i;
 }
-__global__ void k_b2i(const double *blk, double *inter, int64_t npix) {
+__global__ void k_b2i(const double *blk, double *inter, int64_t npix, int nc) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < npix) {
-    inter[3 * i] = blk[i];
-    inter[3 * i + 1] = blk[npix + i];
-    inter[3 * i + 2] = blk[2 * npix + i];
-  }
+  if (i < npix)
+    for (int c = 0; c < nc; c++) inter[nc * i + c] = blk[c * npix + i];
 }
-__global__ void k_i2b(const double *inter, double *blk, int64_t npix) {
+__global__ void k_i2b(const double *inter, double *blk, int64_t npix, int nc) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < npix) {
-    blk[i] = inter[3 * i];
-    blk[npix + i] = inter[3 * i + 1];
-    blk[2 * npix + i] = inter[3 * i + 2];
-  }
+  if (i < npix)
+    for (int c = 0; c < nc; c++) blk[c * npix + i] = inter[nc * i + c];
 }
 __global__ void k_d2f(const double *a, float *b, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -547,16 +553,23 @@ inline int ok() { return hipGetLastError() == hipSuccess ? 0 : -1; }
 template <typename T>
 int launch_scatter(const ScatterArgs<T> &a, hipStream_t s) {
   if (a.it.n > 0) {
-    if (a.lanes == 4)
-      hipLaunchKernelGGL((k_scatter<T, 4>), dim3(nblk(a.it.n, 64)), dim3(256), 0, s, a);
+    if (a.nc == 4)  // leakage component: 16 lanes per item
+      hipLaunchKernelGGL((k_scatter<T, 16, true>), dim3(nblk(a.it.n, 16)), dim3(256), 0, s, a);
+    else if (a.lanes == 4)
+      hipLaunchKernelGGL((k_scatter<T, 4, false>), dim3(nblk(a.it.n, 64)), dim3(256), 0, s, a);
     else if (a.lanes == 8)
-      hipLaunchKernelGGL((k_scatter<T, 8>), dim3(nblk(a.it.n, 32)), dim3(256), 0, s, a);
+      hipLaunchKernelGGL((k_scatter<T, 8, false>), dim3(nblk(a.it.n, 32)), dim3(256), 0, s, a);
     else
-      hipLaunchKernelGGL((k_scatter<T, 16>), dim3(nblk(a.it.n, 16)), dim3(256), 0, s, a);
+      hipLaunchKernelGGL((k_scatter<T, 16, false>), dim3(nblk(a.it.n, 16)), dim3(256), 0, s, a);
   }
-  if (a.it.nsplit > 0)
-    hipLaunchKernelGGL(k_fix<3>, dim3(nblk(a.it.nsplit)), dim3(256), 0, s, a.it, a.partial, a.q,
-                       a.cnt, a.mask, a.qidx, a.accumulate);
+  if (a.it.nsplit > 0) {
+    if (a.nc == 4)
+      hipLaunchKernelGGL(k_fix<4>, dim3(nblk(a.it.nsplit)), dim3(256), 0, s, a.it, a.partial, a.q,
+                         a.cnt, a.mask, a.qidx, a.accumulate);
+    else
+      hipLaunchKernelGGL(k_fix<3>, dim3(nblk(a.it.nsplit)), dim3(256), 0, s, a.it, a.partial, a.q,
+                         a.cnt, a.mask, a.qidx, a.accumulate);
+  }
   return ok();
 }
 template int launch_scatter<double>(const ScatterArgs<double> &, hipStream_t);
@@ -651,14 +664,16 @@ int launch_copy_i32(const int32_t *src, int32_t *dst, int64_t n, hipStream_t s) 
 }
 
 int launch_map_slots(const double *x, const int32_t *src, const int32_t *slot, double *map, int64_t npix,
-                     hipStream_t s) {
-  if (npix > 0) hipLaunchKernelGGL(k_map_slots<double>, dim3(nblk(npix)), dim3(256), 0, s, x, src, slot, map, npix);
+                     hipStream_t s, int nc) {
+  if (npix > 0)
+    hipLaunchKernelGGL(k_map_slots<double>, dim3(nblk(npix)), dim3(256), 0, s, x, src, slot, map, npix, nc);
   return ok();
 }
 
 int launch_map_slots(const double *x, const int32_t *src, const int32_t *slot, float *map, int64_t npix,
-                     hipStream_t s) {
-  if (npix > 0) hipLaunchKernelGGL(k_map_slots<float>, dim3(nblk(npix)), dim3(256), 0, s, x, src, slot, map, npix);
+                     hipStream_t s, int nc) {
+  if (npix > 0)
+    hipLaunchKernelGGL(k_map_slots<float>, dim3(nblk(npix)), dim3(256), 0, s, x, src, slot, map, npix, nc);
   return ok();
 }
 
@@ -686,12 +701,12 @@ int launch_iota_u32(uint32_t *v, int64_t n, hipStream_t s) {
   hipLaunchKernelGGL(k_iota, dim3(nblk(n)), dim3(256), 0, s, v, n);
   return ok();
 }
-int launch_block_to_inter(const double *blk, double *inter, int64_t npix, hipStream_t s) {
-  hipLaunchKernelGGL(k_b2i, dim3(nblk(npix)), dim3(256), 0, s, blk, inter, npix);
+int launch_block_to_inter(const double *blk, double *inter, int64_t npix, hipStream_t s, int nc) {
+  hipLaunchKernelGGL(k_b2i, dim3(nblk(npix)), dim3(256), 0, s, blk, inter, npix, nc);
   return ok();
 }
-int launch_inter_to_block(const double *inter, double *blk, int64_t npix, hipStream_t s) {
-  hipLaunchKernelGGL(k_i2b, dim3(nblk(npix)), dim3(256), 0, s, inter, blk, npix);
+int launch_inter_to_block(const double *inter, double *blk, int64_t npix, hipStream_t s, int nc) {
+  hipLaunchKernelGGL(k_i2b, dim3(nblk(npix)), dim3(256), 0, s, inter, blk, npix, nc);
   return ok();
 }
 int launch_d2f(const double *a, float *b, int64_t n, hipStream_t s) {

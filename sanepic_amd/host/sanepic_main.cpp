@@ -46,6 +46,8 @@ void usage(const char *name) {
       << "  -D file     use the detectors listed in this file, one per line\n"
       << "  -d count    number of detectors to use\n"
       << "  -o 0|1      subtract the orbital dipole read from <-F>/orbdip_<detector>\n"
+      << "  -b 0|1      also solve for a leakage map L that each detector sees times its coefficient\n"
+      << "  -K file     the leakage coefficients, one float64 per detector (normalised on reading)\n"
       << "Map and solver\n"
       << "  -N nside    HEALPix resolution of the maps (RING ordering)\n"
       << "  -p 1        solve for I, Q and U (required)\n"
@@ -57,8 +59,8 @@ void usage(const char *name) {
       << "  -e tag      tag inserted in the output file names\n"
       << "  -i 0|1      also write the I, Q, U maps of intermediate iterations\n"
       << "  -j step     iterations between intermediate maps (default 10)\n"
-      << "Read for compatibility and without effect on the maps: -H -J -A -m -G -L -g -z -M -c\n"
-      << "Rejected unless 0 (not available): -r -R -b (-K), -E\n";
+      << "Read for compatibility and without effect on the maps: -H -J -A -m -G -L -g -z -M -c -R\n"
+      << "Rejected unless 0 (not available): -r, -E\n";
   exit(1);
 }
 
@@ -184,7 +186,8 @@ int main(int argc, char *argv[]) {
   int recalib = 0, galbp = 0;
   long ndet = 0, nsegtot = 0, ns = 0, nssp = 0, nside = 0;
   double omega = 0.0;
-  std::string outdir, bext, pext, termin_out, noisepre, filesegnum, dirfile, dirfilep, initMap, filedetinfo;
+  std::string outdir, bext, pext, termin_out, noisepre, filesegnum, dirfile, dirfilep, initMap, filedetinfo,
+      relfgal_file;
   std::list<std::string> bolos;
   int opt;
   while ((opt = getopt(argc, argv,
@@ -224,7 +227,8 @@ int main(int argc, char *argv[]) {
       case 'b': galbp = atoi(optarg); break;
       // -R acts only inside the -r loop (2254-2270); -K names the -b file;
       // the rest are read by the reference and never used
-      case 'R': case 'K':
+      case 'K': relfgal_file = optarg; break;
+      case 'R':
       case 'H': case 'J': case 'A': case 'm': case 'G': case 'L': case 'g': case 'z': case 'M': case 'c':
         break;
       default:
@@ -244,9 +248,8 @@ int main(int argc, char *argv[]) {
     fprintf(stderr, "sanepic: only -p 1 is supported (the reference reads psi unconditionally)\n");
     exit(1);
   }
-  if (recalib || galbp) {
-    fprintf(stderr, "sanepic: %s is not supported by this build\n",
-            recalib ? "-r (recalibration loop)" : "-b (GalBP leakage component)");
+  if (recalib) {
+    fprintf(stderr, "sanepic: -r (recalibration loop) is not supported by this build\n");
     exit(1);
   }
   const double t0 = now();
@@ -279,6 +282,25 @@ int main(int argc, char *argv[]) {
   sanepic_ctx *ctx = nullptr;
   int rc = sanepic_create(&cfg, &ctx);
   if (rc) die_lib(nullptr, rc, "sanepic_create");
+  const int nc = galbp ? 4 : 3;
+  if (galbp) {
+    // leakage coefficients: ndet doubles from the -K file, normalised to zero
+    // mean and unit (population) variance over the detectors (551-575).  The
+    // reference's ReadVECT (3061-3085) reads into a buffer of its own and
+    // leaves relfGal at 0, so its maps turn NaN; here the file is read.
+    std::vector<double> relf(ndet);
+    if (relfgal_file.empty() || !read_vect(relf.data(), relfgal_file, 0, ndet - 1)) {
+      fprintf(stderr, "sanepic: -b 1 needs -K <file of %ld leakage coefficients>\n", ndet);
+      exit(1);
+    }
+    double ttmean = 0.0, ttsig = 0.0;
+    for (long i = 0; i < ndet; i++) ttmean += relf[i] / double(ndet);
+    for (long i = 0; i < ndet; i++) ttsig += (relf[i] - ttmean) * (relf[i] - ttmean) / double(ndet);
+    for (long i = 0; i < ndet; i++) relf[i] = (relf[i] - ttmean) / sqrt(ttsig);
+    for (long i = 0; i < ndet; i++) printf("!!!!!!!!!!!!!!!!!!!!!!!!!!!!! relfGal[%ld] = %10.15g\n", i, relf[i]);
+    rc = sanepic_set_leakage(ctx, relf.data());
+    if (rc) die_lib(ctx, rc, "sanepic_set_leakage");
+  }
   // noise spectra (521-538)
   std::vector<double> spec(nssp);
   for (long i = 0; i < ndet; i++) {
@@ -324,7 +346,7 @@ int main(int argc, char *argv[]) {
   int64_t npix = 0, ndef = 0;
   rc = sanepic_pixelize(ctx, &npix, &ndef);
   if (rc) die_lib(ctx, rc, "sanepic_pixelize");
-  const long npixtot = 3 * npix;
+  const long npixtot = nc * npix;
   printf("TOTAL NUMBER OF DETECTORS: %d\n", (int)ndet);
   printf("TOTAL NUMBER OF DATA SEGMENTS: %d\n", (int)nsegtot);
   printf("NUMBER OF SAMPLE PER DATA SEGMENT: %d\n", (int)ns);
@@ -415,6 +437,8 @@ int main(int argc, char *argv[]) {
       write_reduced_map(fmt("%s_SIN2P_%s", outdir, termin_out), nside, sumc.data() + npix, indpix);
       write_reduced_map(fmt("%s_Q_NVAR%s", outdir, termin_out), nside, aux.data() + npix, indpix);
       write_reduced_map(fmt("%s_U_NVAR%s", outdir, termin_out), nside, aux.data() + 2 * npix, indpix);
+      if (galbp)  // 1693-1697
+        write_reduced_map(fmt("%s_Leak_NVAR%s", outdir, termin_out), nside, aux.data() + 3 * npix, indpix);
       for (long i = 0; i < npix; i++) tmp[i] = double(hits[i + npix]);
       write_reduced_map(fmt("%s_POL_HITS%s", outdir, termin_out), nside, tmp.data(), indpix);
     }
@@ -423,6 +447,8 @@ int main(int argc, char *argv[]) {
       write_reduced_map(fmt("%s_I%s_%d", outdir, termin_out, iter), nside, S.data(), indpix);
       write_reduced_map(fmt("%s_Q%s_%d", outdir, termin_out, iter), nside, S.data() + npix, indpix);
       write_reduced_map(fmt("%s_U%s_%d", outdir, termin_out, iter), nside, S.data() + 2 * npix, indpix);
+      if (galbp)  // 1738-1742
+        write_reduced_map(fmt("%s_Leak%s_%d", outdir, termin_out, iter), nside, S.data() + 3 * npix, indpix);
     }
     iter++;
   }
@@ -434,6 +460,8 @@ int main(int argc, char *argv[]) {
   write_reduced_map(fmt("%s_I_%s_%d", outdir, termin_out, 1), nside, S.data(), indpix);
   write_reduced_map(fmt("%s_Q_%s_%d", outdir, termin_out, 1), nside, S.data() + npix, indpix);
   write_reduced_map(fmt("%s_U_%s_%d", outdir, termin_out, 1), nside, S.data() + 2 * npix, indpix);
+  if (galbp)  // 1803-1806
+    write_reduced_map(fmt("%s_Leak_%s_%d", outdir, termin_out, 1), nside, S.data() + 3 * npix, indpix);
   printf("Map-making Ends \n");
   fprintf(stderr,
           "sanepic: %d iterations, set-up %.3f s (file I/O %.3f s), CG loop %.3f s (%.3f ms/iter), "

@@ -413,7 +413,8 @@ def write_tree(base, cfg, ds, suffix="output"):
     )
 
 
-def sanepic_argv(args, exe="sanepic", hwp=0, omega=1.123456789123, pol=1, write_iter=0, odip=0):
+def sanepic_argv(args, exe="sanepic", hwp=0, omega=1.123456789123, pol=1, write_iter=0, odip=0,
+                 leakage_file=None):
     """The flag list of run_sanepic.sh:55-73 for the given tree."""
     argv = [exe, "-F", args["tod_dir"], "-B", args["suffix"], "-f", args["segfile"],
             "-d", str(args["ndet"])]
@@ -425,4 +426,6 @@ def sanepic_argv(args, exe="sanepic", hwp=0, omega=1.123456789123, pol=1, write_
              "-Z", args["pointing"], "-i", str(write_iter)]
     if odip:
         argv += ["-o", str(odip)]
+    if leakage_file:  # -b 1 with the coefficients file (551-575)
+        argv += ["-b", "1", "-K", leakage_file]
     return argv

@@ -97,11 +97,9 @@ def test_cli_errors_before_device(tmp_path):
 
 @pytest.mark.skipif(not os.path.exists(EXE), reason="drop-in executable not built")
 def test_cli_unsupported_flags_fail(tmp_path):
-    """-r (recalibration) and -b (GalBP component) change the reference's
-    maps; this build refuses them rather than ignore them.  -R and -K act
-    only with -r and -b, and the reference's unused flags are accepted."""
+    """-r (recalibration) changes the reference's maps; this build refuses it
+    rather than ignore it.  -R acts only with -r."""
     cfg, ds, args = _tree(tmp_path)
     argv = synth.sanepic_argv(args)[1:]
-    for flag in ("-r", "-b"):
-        r = _run(argv + [flag, "1"])
-        assert r.returncode == 1 and "not supported" in r.stderr, flag
+    r = _run(argv + ["-r", "1"])
+    assert r.returncode == 1 and "not supported" in r.stderr
