@@ -301,7 +301,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c1")
+    ap.add_argument("--workload", default="nside512_16det")
     ap.add_argument("--no-learned", action="store_true", help="only the FP64 solve")
     ap.add_argument("--no-literal", action="store_true",
                     help="skip the learned pipeline with the literal fallback rule (growth 1)")

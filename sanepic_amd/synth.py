@@ -151,9 +151,9 @@ def noise_segment(ns, fsamp, sigma, fknee, alpha, seed):
 # Named workloads: nside, detectors, samples per detector, segment length.
 WORKLOADS = {
     # 16 detectors at k pi/8, NSIDE 512, 2^26 samples each in segments of 2^20
-    "c1": dict(nside=512, ndet=16, nsamp=2 ** 26, ns=2 ** 20),
+    "nside512_16det": dict(nside=512, ndet=16, nsamp=2 ** 26, ns=2 ** 20),
     # 1 detector, NSIDE 64, 2^22 samples in 8 segments of 2^19
-    "c0": dict(nside=64, ndet=1, nsamp=2 ** 22, ns=2 ** 19),
+    "nside64_1det": dict(nside=64, ndet=1, nsamp=2 ** 22, ns=2 ** 19),
     "small": dict(nside=64, ndet=4, nsamp=2 ** 21, ns=2 ** 18),
 }
 
@@ -164,7 +164,7 @@ WORKLOADS = {
 # (1.07e9 samples in all: 2^26 per detector would need ~390 GB of resident
 # per-sample state, more than one GPU's 288 GB).
 UNEVEN_WORKLOADS = {
-    "c3": dict(nside=1024, ndet=64, nsamp=2 ** 24, seed=7),
+    "nside1024_64det_uneven": dict(nside=1024, ndet=64, nsamp=2 ** 24, seed=7),
 }
 
 

@@ -1,6 +1,6 @@
 """Train the learned preconditioner for one scan strategy.
 
-    python -m sanepic_amd.train_precond --workload c1 --pairs 2000 --steps 10000
+    python -m sanepic_amd.train_precond --workload nside512_16det --pairs 2000 --steps 10000
 
 Builds the named synthetic workload (the generator bench.py uses: same
 pointing, detectors and noise spectra), then on the GPU:
@@ -68,7 +68,7 @@ def log(*a):
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
-    ap.add_argument("--workload", default="c1", choices=sorted(synth.WORKLOADS) + sorted(synth.UNEVEN_WORKLOADS))
+    ap.add_argument("--workload", default="nside512_16det", choices=sorted(synth.WORKLOADS) + sorted(synth.UNEVEN_WORKLOADS))
     ap.add_argument("--pairs", type=int, default=1500, help="block-Jacobi pairs")
     ap.add_argument("--holdout", type=int, default=100)
     ap.add_argument("--steps", type=int, default=6000)

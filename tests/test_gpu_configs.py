@@ -66,16 +66,16 @@ def _check_setup(ctx, P, npix):
 # --------------------------------------------------------------------------
 
 @pytest.fixture(scope="module")
-def c0():
-    cfg = synth.workload_config("c0")
-    print(f"C0 full size: {cfg.ndet} det x {cfg.nseg} x {cfg.ns}, NSIDE {cfg.nside}", flush=True)
+def one_det():
+    cfg = synth.workload_config("nside64_1det")
+    print(f"full single-detector case: {cfg.ndet} det x {cfg.nseg} x {cfg.ns}, NSIDE {cfg.nside}", flush=True)
     ds = synth.make_dataset(cfg, workers=NT)
     P = oracle_problem(cfg, ds, nthreads=NT)
     return cfg, ds, P
 
 
-def test_c0_full_size_operator(c0):
-    cfg, ds, P = c0
+def test_one_detector_full_size_operator(one_det):
+    cfg, ds, P = one_det
     ctx, npix, ndef = _gpu(cfg, ds)
     print(f"  npix {npix}, deferred samples {ndef}", flush=True)
     _check_setup(ctx, P, npix)
@@ -89,14 +89,14 @@ def test_c0_full_size_operator(c0):
     ctx.close()
 
 
-def test_c0_full_size_iterates(c0):
+def test_one_detector_full_size_iterates(one_det):
     """First 8 iterates of the reference's diagonal PCG and of block-Jacobi
     with and without the monopole deflation, against the oracle's, within
     the oracle's own 1-vs-NT-thread reordering spread (block-Jacobi: at least
     1e-6, since half of this workload's 3x3 blocks are within rounding of
     singular and their inverses amplify the GPU's different summation order;
     an indexing or sign error moves the iterates by O(1))."""
-    cfg, ds, P = c0
+    cfg, ds, P = one_det
     P1 = oracle_problem(cfg, ds, nthreads=1)
     ctx, npix, _ = _gpu(cfg, ds)
     K = 8
@@ -119,7 +119,7 @@ def test_c0_full_size_iterates(c0):
     ctx.close()
 
 
-def test_c0_full_size_block_jacobi_solve(c0):
+def test_one_detector_full_size_block_jacobi_solve(one_det):
     """Block-Jacobi (+ deflation) PCG, 150 iterations.  With one detector
     whose angle is the spin phase, a pixel is crossed at nearly one angle:
     more than half of the pixels' 3x3 blocks have a reciprocal condition
@@ -129,7 +129,7 @@ def test_c0_full_size_block_jacobi_solve(c0):
     150-iteration map solves the oracle's system as well as the oracle's
     own 150-iteration map does, both residuals measured with the oracle's
     operator."""
-    cfg, ds, P = c0
+    cfg, ds, P = one_det
     ctx, npix, _ = _gpu(cfg, ds)
     K = 150
     h, S = _run(ctx, K, precond=sanepic_amd.PRECOND_BLOCK3, deflate=True)
@@ -191,14 +191,14 @@ def test_nside512_segments_2e20_mixed(n512):
     ctx.close()
 
 
-def test_c1_pixels_of_one_detector():
+def test_pixels_of_one_detector_at_nside512():
     """One detector of the 16-detector workload (angle 112.5 deg): its full
     2^26-sample pointing at NSIDE 512 in 64 segments of 2^20, every RING
     pixel bit-exact and the seen-pixel compaction identical."""
     nside, nseg, ns = 512, 64, 2 ** 20
     ra, dec, psi = synth.scan_pointing(nseg * ns)
     ispf = synth.ispf_spectrum(ns, 19.0, 50e-6, 0.1, 1.0)
-    print("C1 pixels: 1 det x 64 x 2^20 at NSIDE 512", flush=True)
+    print("one detector's pixels: 1 det x 64 x 2^20 at NSIDE 512", flush=True)
     ctx = sanepic_amd.Context(nside, 1, nseg, ns)
     ctx.set_detector(0, 0.0, 0.0, 112.5, ispf)
     for s in range(nseg):
