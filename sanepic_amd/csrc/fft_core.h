@@ -186,18 +186,36 @@ __device__ __forceinline__ V tw4096(const V *__restrict__ tw, int j) {
 }
 
 // One radix-16 Stockham pass with l already applied (l, m compile-time).
-template <typename V, int DIR, int L, int LCUR, int B>
-__device__ __forceinline__ void pass16(V (&v)[16], int t, V *__restrict__ lds,
+// E is the element of the exchange array: the complex value itself, or its
+// scalar type, in which case the real parts and then the imaginary parts go
+// through LDS in two rounds (half the LDS per block).
+template <typename V, int DIR, int L, int LCUR, int B, typename E>
+__device__ __forceinline__ void pass16(V (&v)[16], int t, E *__restrict__ lds,
                                        const V *__restrict__ tw) {
   constexpr int TS = L / 16;
   constexpr int M = L / (16 * LCUR);
-  __syncthreads();
-#pragma unroll
-  for (int i = 0; i < 16; i++) lds[lpad(t + TS * i) * B] = v[i];
-  __syncthreads();
   const int k = t / M, c = t % M;
+  if constexpr (sizeof(E) == sizeof(V)) {
+    __syncthreads();
 #pragma unroll
-  for (int r = 0; r < 16; r++) v[r] = lds[lpad((k * 16 + r) * M + c) * B];
+    for (int i = 0; i < 16; i++) lds[lpad(t + TS * i) * B] = v[i];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 16; r++) v[r] = lds[lpad((k * 16 + r) * M + c) * B];
+  } else {
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 16; i++) lds[lpad(t + TS * i) * B] = v[i].x;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 16; r++) v[r].x = lds[lpad((k * 16 + r) * M + c) * B];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 16; i++) lds[lpad(t + TS * i) * B] = v[i].y;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 16; r++) v[r].y = lds[lpad((k * 16 + r) * M + c) * B];
+  }
   // w_{16 l}^{r k} = w_4096^{r k M (4096/L)}
   const int bi = k * M * (4096 / L);
   V w1 = tw4096<V, DIR>(tw, bi), w2 = tw4096<V, DIR>(tw, 2 * bi);
@@ -221,24 +239,24 @@ __device__ __forceinline__ void pass16(V (&v)[16], int t, V *__restrict__ lds,
   dft16<V, DIR>(v);
 }
 
-template <typename V, int DIR, int L, int LCUR, int B>
-__device__ __forceinline__ void passes16(V (&v)[16], int t, V *lds, const V *tw) {
+template <typename V, int DIR, int L, int LCUR, int B, typename E>
+__device__ __forceinline__ void passes16(V (&v)[16], int t, E *lds, const V *tw) {
   if constexpr (LCUR < L) {
-    pass16<V, DIR, L, LCUR, B>(v, t, lds, tw);
-    passes16<V, DIR, L, LCUR * 16, B>(v, t, lds, tw);
+    pass16<V, DIR, L, LCUR, B, E>(v, t, lds, tw);
+    passes16<V, DIR, L, LCUR * 16, B, E>(v, t, lds, tw);
   }
 }
 
 // Full transform of one sequence held by TS = L/16 threads.
 // lds points at this sequence's first element (interleaved stride B).
-template <typename V, int DIR, int LOG2L, int B>
-__device__ __forceinline__ void fft_reg(V (&v)[16], int t, V *lds, const V *tw) {
+template <typename V, int DIR, int LOG2L, int B, typename E>
+__device__ __forceinline__ void fft_reg(V (&v)[16], int t, E *lds, const V *tw) {
   constexpr int L = 1 << LOG2L;
   static_assert(LOG2L >= 4 && LOG2L <= 12, "16 <= L <= 4096");
   constexpr int REM = LOG2L & 3;
   if constexpr (REM == 0) {
     dft16<V, DIR>(v);  // first radix-16 pass straight from registers
-    passes16<V, DIR, L, 16, B>(v, t, lds, tw);
+    passes16<V, DIR, L, 16, B, E>(v, t, lds, tw);
   } else {
     constexpr int R0 = 1 << REM, NB = 16 / R0;
 #pragma unroll
@@ -256,9 +274,20 @@ __device__ __forceinline__ void fft_reg(V (&v)[16], int t, V *lds, const V *tw) 
         for (int s = 0; s < 8; s++) v[j + NB * s] = a[s];
       }
     }
-    passes16<V, DIR, L, R0, B>(v, t, lds, tw);
+    passes16<V, DIR, L, R0, B, E>(v, t, lds, tw);
   }
 }
+
+// Exchange element: FP64 sequences go through LDS as two rounds of doubles
+// (a 256-point column block of 16 sequences then holds 35 KB instead of
+// 70 KB, so four blocks fit a CU's 160 KB instead of two).
+#ifndef SANEPIC_LDS_SPLIT64
+#define SANEPIC_LDS_SPLIT64 1
+#endif
+template <typename V, bool SPLIT> struct LdsElem { using type = V; };
+template <typename V> struct LdsElem<V, true> { using type = decltype(V::x); };
+template <typename V>
+using LdsE = typename LdsElem<V, sizeof(V) == 16 && SANEPIC_LDS_SPLIT64>::type;
 
 template <int LOG2L, int B>
 struct LdsSize {

@@ -134,7 +134,7 @@ template <typename T, int LOG2N2, int SRC>
 __device__ __forceinline__ void col_fwd_body(const OpArgs<T> &a) {
   using V = VT<T>;
   constexpr int N2 = 1 << LOG2N2, TS = N2 / 16, B = 256 / TS;
-  __shared__ V lds[LdsSize<LOG2N2, B>::value];
+  __shared__ LdsE<V> lds[LdsSize<LOG2N2, B>::value];
   const int tid = threadIdx.x, b = tid % B, t = tid / B;
   const int64_t N1 = a.g.N1;
   const int64_t ncb = N1 / B;
@@ -217,7 +217,7 @@ template <typename T, int LOG2P, int M, int SRC>
 __global__ __launch_bounds__(256) void k_col_fwd_m(OpArgs<T> a) {
   using V = VT<T>;
   constexpr int P = 1 << LOG2P, TS = P / 16, B = 256 / TS;
-  __shared__ V lds[LdsSize<LOG2P, B>::value];
+  __shared__ LdsE<V> lds[LdsSize<LOG2P, B>::value];
   const int tid = threadIdx.x, b = tid % B, t = tid / B;
   const int64_t N1 = a.g.N1;
   const int64_t ncb = N1 / B;
@@ -276,7 +276,7 @@ template <typename T, int LOG2P, int M>
 __global__ __launch_bounds__(256) void k_col_inv_m(OpArgs<T> a) {
   using V = VT<T>;
   constexpr int P = 1 << LOG2P, TS = P / 16, B = 256 / TS;
-  __shared__ V lds[LdsSize<LOG2P, B>::value];
+  __shared__ LdsE<V> lds[LdsSize<LOG2P, B>::value];
   const int tid = threadIdx.x, b = tid % B, t = tid / B;
   const int64_t N1 = a.g.N1;
   const int64_t ncb = N1 / B;
@@ -322,7 +322,7 @@ template <typename T, int LOG2N1>
 __global__ __launch_bounds__(256) void k_row(OpArgs<T> a) {
   using V = VT<T>;
   constexpr int N1 = 1 << LOG2N1, TS = N1 / 16, B = 256 / TS;
-  __shared__ V lds[LdsSize<LOG2N1, B>::value];
+  __shared__ LdsE<V> lds[LdsSize<LOG2N1, B>::value];
   __shared__ double red[4];
   const int tid = threadIdx.x, b = tid % B, t = tid / B;
   const int64_t N = a.g.N, N2 = a.g.N2;
@@ -391,7 +391,7 @@ template <typename T, int LOG2N2>
 __global__ __launch_bounds__(256) void k_col_inv(OpArgs<T> a) {
   using V = VT<T>;
   constexpr int N2 = 1 << LOG2N2, TS = N2 / 16, B = 256 / TS;
-  __shared__ V lds[LdsSize<LOG2N2, B>::value];
+  __shared__ LdsE<V> lds[LdsSize<LOG2N2, B>::value];
   const int tid = threadIdx.x, b = tid % B, t = tid / B;
   const int64_t N1 = a.g.N1;
   const int64_t ncb = N1 / B;
@@ -419,7 +419,7 @@ template <typename T, int LOG2N, int SRC>
 __global__ __launch_bounds__(256) void k_single(OpArgs<T> a) {
   using V = VT<T>;
   constexpr int N = 1 << LOG2N, TS = N / 16, B = 256 / TS;
-  __shared__ V lds[LdsSize<LOG2N, B>::value];
+  __shared__ LdsE<V> lds[LdsSize<LOG2N, B>::value];
   __shared__ double red[4];
   const int tid = threadIdx.x, b = tid % B, t = tid / B;
   const int64_t pr = (int64_t)blockIdx.x * B + b;
