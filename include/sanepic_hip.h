@@ -250,6 +250,9 @@ typedef struct sanepic_solve_opts {
                        `refresh` iterations uses the FP32 operator too (no
                        chi2 then; the true residual reachable is ~1e-7 of
                        |b|, the FP32 operator's own error).  0: FP64 refresh. */
+  double var00;     /* > 0: the denominator of crit in place of |b|^2 (the
+                       recalibration loop keeps its first loop's |b|^2,
+                       sanepic.cc:1377-1385); 0: |b|^2 of this solve. */
 } sanepic_solve_opts;
 
 /* Start the PCG of sanepic.cc:1316-1410 from x0 (compressed 3 npix, NULL for
@@ -269,6 +272,26 @@ typedef struct sanepic_iter_info {
 
 /* One iteration of the loop body 1417-1760. */
 int sanepic_solve_step(sanepic_ctx *ctx, sanepic_iter_info *info);
+
+/* Relative calibration (the reference's -r loop, sanepic.cc:1041-1067,
+ * 2114-2316).  sanepic_set_recalib(ctx, 1) before sanepic_setup keeps the
+ * NaN-filled TOD and the dipole on the device (one segment length for every
+ * detector).  sanepic_set_gains sets relCalib g[idet nseg + iseg] of every
+ * block: the following solves see the data d / g - dip with N^-1 SpN g^2
+ * (do_PtNd 2543-2545), i.e. b = P^T g N^-1 (d - g dip) and
+ * A = P^T g N^-1 g P; the preconditioner stays the one of unit gains, as the
+ * reference builds Mp in its first loop only (1352), and a new
+ * sanepic_solve_begin is needed.  sanepic_recal_sums returns, for map x
+ * (compressed) and every block b, with C = r2c(P x) + r2c(dip), D = r2c(d):
+ *   sums[3b]   = sum_{k <= ns/2} SpN_k Re(D_k conj C_k)   (zeta, 2207-2215)
+ *   sums[3b+1] = sum_{k <= ns/2} SpN_k |C_k|^2            (gamma, 2217-2221)
+ *   sums[3b+2] = sum_{k <= ns/2} SpN_k |D_k|^2
+ * from which the gain minimising the reference's chi2 (3206-3255) is
+ * zeta / gamma per block, or sum zeta / sum gamma over a detector's blocks
+ * (-R 1, 2255-2270), and chi2(g) = 2 sum_b (dd - 2 g zeta + g^2 gamma). */
+int sanepic_set_recalib(sanepic_ctx *ctx, int32_t on);
+int sanepic_set_gains(sanepic_ctx *ctx, const double *g);
+int sanepic_recal_sums(sanepic_ctx *ctx, const double *x, double *sums);
 
 /* Current iterate (compressed 3 npix). */
 int sanepic_get_map(sanepic_ctx *ctx, double *x);

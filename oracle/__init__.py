@@ -61,7 +61,10 @@ def lib():
             "or_problem_set_dipole": (_I, [_V, _dp]),
             "or_problem_set_leakage": (None, [_V, _dp]),
             "or_problem_ncomp": (_L, [_V]),
-            "or_problem_solve_pcg": (_I, [_V, _dp, _I, _D, _I, _I, _I, _dp, _dp, _dp, _dp, _dp]),
+            "or_problem_solve_pcg": (_I, [_V, _dp, _I, _D, _I, _I, _I, _D, _dp, _dp, _dp, _dp, _dp]),
+            "or_problem_set_gains": (_I, [_V, _dp]),
+            "or_recal_sums": (None, [_V, _dp, _dp]),
+            "or_recal_chi2": (_D, [_V, _dp, _dp]),
             "or_version": (_I, []),
         }
         for name, (res, args) in sig.items():
@@ -210,6 +213,25 @@ class Problem:
         assert odip.size == self.ndet * self.nseg * self.ns
         assert lib().or_problem_set_dipole(self.h, _d(odip)) == 0
 
+    def set_gains(self, g):
+        """Relative calibration [ndet, nseg] of every (det, seg) block (None: 1)."""
+        if g is None:
+            assert lib().or_problem_set_gains(self.h, None) == 0
+            return
+        g = _f64(g)
+        assert g.size == self.ndet * self.nseg
+        assert lib().or_problem_set_gains(self.h, _d(g)) == 0
+
+    def recal_sums(self, S):
+        """[ndet, nseg, 3]: zeta, gamma and the data term of the gain fit of map S."""
+        out = np.empty(3 * self.ndet * self.nseg)
+        lib().or_recal_sums(self.h, _d(_f64(S)), _d(out))
+        return out.reshape(self.ndet, self.nseg, 3)
+
+    def recal_chi2(self, S, g=None):
+        """The reference's MPI_compute_chi2 of map S with gains g [ndet, nseg]."""
+        return lib().or_recal_chi2(self.h, _d(_f64(S)), None if g is None else _d(_f64(g)))
+
     def apply_A(self, x):
         x = _f64(x)
         y = np.empty(self.ncomp * self.npix)
@@ -267,9 +289,10 @@ class Problem:
         lib().or_exact_blocks(self.h, _d(out))
         return out.reshape(self.npix, 6)
 
-    def solve_pcg(self, kind=1, deflate=False, x0=None, maxiter=2000, stop=1e-15, refresh=10):
+    def solve_pcg(self, kind=1, deflate=False, x0=None, maxiter=2000, stop=1e-15, refresh=10, var00=0.0):
         """The GPU solver's PCG: kind 0 = the reference's diagonal, 1 = 3x3
-        block-Jacobi; deflate = A-DEF2 with the I monopole as coarse vector."""
+        block-Jacobi; deflate = A-DEF2 with the I monopole as coarse vector;
+        var00 > 0 replaces |b|^2 in crit (the recalibration loop's rule)."""
         n3 = self.ncomp * self.npix
         S = np.empty(n3)
         b = np.empty(n3)
@@ -278,8 +301,8 @@ class Problem:
         crit2 = np.zeros(maxiter + 1)
         x0p = None if x0 is None else _d(_f64(x0))
         it = lib().or_problem_solve_pcg(self.h, x0p, maxiter, stop, refresh, int(kind),
-                                        1 if deflate else 0, _d(S), _d(crit), _d(crit2), _d(b),
-                                        _d(M6))
+                                        1 if deflate else 0, float(var00), _d(S), _d(crit), _d(crit2),
+                                        _d(b), _d(M6))
         if it < 0:
             raise RuntimeError("oracle: Mp has a zero I element")
         hits = np.empty(2 * self.npix, dtype=np.int64)

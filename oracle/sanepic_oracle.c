@@ -22,6 +22,9 @@
  *   or_exact_blocks       exact 3x3 diagonal blocks of A  (no counterpart)
  *   or_problem_solve_pcg  block-Jacobi / A-DEF2 PCG       (no counterpart)
  *   or_chi2               chi2 of a map                   1514-1538
+ *   or_problem_set_gains  relCalib of the recalibration   1041-1067, 2543-2545
+ *   or_recal_sums         gain fit of one loop            2185-2228
+ *   or_recal_chi2         MPI_compute_chi2                3206-3255
  *
  * FFT: the reference calls FFTW3 (version unpinned in the reference; its
  * Make.cm2 names fftw 3.3.10), which is absent from this image.  FFTW's
@@ -479,7 +482,9 @@ typedef struct or_problem {
   long *indpix;                 /* 12 nside^2 */
   long npix;
   long n_bad_pixel;             /* samples that took the previous pixel */
-  cplx *fdatas0;                /* [ndet][nseg][ns/2+1] */
+  cplx *fraw;                   /* [ndet][nseg][ns/2+1] r2c of the NaN-filled TOD */
+  cplx *fdatas0;                /* [ndet][nseg][ns/2+1] fraw / relCalib - fodipole */
+  double *gain;                 /* [ndet][nseg] relCalib, NULL: all 1 */
   cplx *fodipole;               /* [ndet][nseg][ns/2+1] or NULL */
   double *b;                    /* PNdtot, 3 npix */
   double chi20;
@@ -490,6 +495,10 @@ typedef struct or_problem {
 
 #define IDX(P, idet, iseg, ii) \
   ((ii) + (P)->ns * (iseg) + (P)->ns * (P)->nseg * (idet))
+
+static inline double or_gain(const or_problem *P, long idet, long iseg) {
+  return P->gain ? P->gain[idet * P->nseg + iseg] : 1.0;
+}
 
 or_problem *or_problem_new(long nside, long ndet, long nseg, long ns,
                            int polar, int hwp, double omega, int nthreads) {
@@ -538,6 +547,8 @@ void or_problem_free(or_problem *P) {
   free(P->psipFP);
   free(P->indpix);
   free(P->fdatas0);
+  free(P->fraw);
+  free(P->gain);
   free(P->fodipole);
   free(P->b);
   free(P->hits);
@@ -693,6 +704,7 @@ void or_problem_get_pixels(const or_problem *P, long *datatopix, double *psip,
 int or_problem_load_tod(or_problem *P, const double *tod) {
   long ns = P->ns, nh = ns / 2 + 1;
   P->fdatas0 = (cplx *)malloc(sizeof(cplx) * nh * P->ndet * P->nseg);
+  P->fraw = (cplx *)malloc(sizeof(cplx) * nh * P->ndet * P->nseg);
 #pragma omp parallel num_threads(P->nthreads)
   {
     double *data = (double *)malloc(sizeof(double) * ns);
@@ -702,8 +714,10 @@ int or_problem_load_tod(or_problem *P, const double *tod) {
       long idet = blk / P->nseg, iseg = blk % P->nseg;
       memcpy(data, tod + IDX(P, idet, iseg, 0), sizeof(double) * ns);
       or_nan_fill(data, ns);
-      or_rfft_r2c(P->plan, data, P->fdatas0 + (idet * P->nseg + iseg) * nh,
+      or_rfft_r2c(P->plan, data, P->fraw + (idet * P->nseg + iseg) * nh,
                   work);
+      memcpy(P->fdatas0 + (idet * P->nseg + iseg) * nh,
+             P->fraw + (idet * P->nseg + iseg) * nh, sizeof(cplx) * nh);
     }
     free(data);
     free(work);
@@ -728,10 +742,12 @@ int or_problem_set_dipole(or_problem *P, const double *odip) {
       long idet = blk / P->nseg, iseg = blk % P->nseg;
       cplx *F = P->fodipole + (idet * P->nseg + iseg) * nh;
       cplx *D = P->fdatas0 + (idet * P->nseg + iseg) * nh;
+      const cplx *R = P->fraw + (idet * P->nseg + iseg) * nh;
+      double g = P->gain ? P->gain[idet * P->nseg + iseg] : 1.0;
       or_rfft_r2c(P->plan, odip + IDX(P, idet, iseg, 0), F, work);
       for (long ii = 0; ii < nh; ii++) {
-        D[ii].re -= F[ii].re;
-        D[ii].im -= F[ii].im;
+        D[ii].re = R[ii].re / g - F[ii].re;
+        D[ii].im = R[ii].im / g - F[ii].im;
       }
     }
     free(work);
@@ -858,6 +874,7 @@ static void or_pass(or_problem *P, const double *S, int from_data,
     for (long blk = 0; blk < nblk; blk++) {
       long iseg = blk / P->ndet, idet = blk % P->ndet;
       const double *Nk = P->spn + idet * ns;
+      const double g = or_gain(P, idet, iseg);
       if (sumCross != NULL) {
         /* crosslink pass: the spectra are not used */
       } else if (from_data) {
@@ -871,7 +888,7 @@ static void or_pass(or_problem *P, const double *S, int from_data,
       }
       if (sumCross == NULL) {
         for (long ii = 0; ii < nh; ii++) {
-          double nk = Nk[ii] * 1.0 * 1.0; /* relCalib = 1 (2543-2545) */
+          double nk = Nk[ii] * g * g; /* SpN relCalib^2 (2543-2545) */
           fdata[ii].re = 0 + fdata[ii].re * nk;
           fdata[ii].im = 0 + fdata[ii].im * nk;
         }
@@ -944,8 +961,8 @@ void or_diag_corr(or_problem *P, double *MpI) {
   free(priv);
 }
 
-/* chi2 = sum over blocks of 2 |(fdatas0 - r2c(P S)) sqrt(SpN)|^2 over bins
- * 0..ns/2 (1514-1538; relCalib = 1). */
+/* chi2 = sum over blocks of 2 |(fdatas0 - r2c(P S)) sqrt(SpN) relCalib|^2
+ * over bins 0..ns/2 (1514-1538). */
 double or_chi2(or_problem *P, const double *S) {
   long ns = P->ns, nh = ns / 2 + 1, npix = P->npix;
   long nblk = P->ndet * P->nseg;
@@ -961,13 +978,14 @@ double or_chi2(or_problem *P, const double *S) {
       long iseg = blk / P->ndet, idet = blk % P->ndet;
       const double *Nk = P->spn + idet * ns;
       const cplx *D = P->fdatas0 + (idet * P->nseg + iseg) * nh;
+      const double g = or_gain(P, idet, iseg);
       or_deproject(S, P->indpix, P->datatopix + IDX(P, idet, iseg, 0),
                    P->psip + IDX(P, idet, iseg, 0), ns, npix, Ps, P->galbp,
                    P->relfgal[idet]);
       or_rfft_r2c(P->plan, Ps, fdata, work);
       double c = 0.0;
       for (long ii = 0; ii < nh; ii++) {
-        double sq = sqrt(Nk[ii]) * 1.0;
+        double sq = sqrt(Nk[ii]) * g;
         double re = (D[ii].re - fdata[ii].re) * sq;
         double im = (D[ii].im - fdata[ii].im) * sq;
         c += (re * re + im * im) / 1.0 * 2.0;
@@ -979,6 +997,138 @@ double or_chi2(or_problem *P, const double *S) {
     free(work);
   }
   /* serial sum in the reference's block order (segments outer) */
+  double chi2 = 0.0;
+  for (long blk = 0; blk < nblk; blk++)
+    chi2 += part[blk];
+  free(part);
+  return chi2;
+}
+
+/* Relative calibration relCalib of every (det, seg) block, g: [ndet][nseg]
+ * (NULL: all 1).  The data spectrum of the next solve is fraw / g - fodipole
+ * (1044-1067, with the block index of 2128-2132: the reference's division at
+ * 1050-1052 indexes relCalib[idet + iseg + iseg_min ndet] instead), and
+ * N^-1 of the block is SpN g^2 in the operator and b (do_PtNd 2543-2545) and
+ * in chi2 (1524-1530).  The preconditioner stays the one of unit gains, as
+ * the reference builds Mp in its first loop only (1178-1199, 1352). */
+int or_problem_set_gains(or_problem *P, const double *g) {
+  long ns = P->ns, nh = ns / 2 + 1, nblk = P->ndet * P->nseg;
+  if (!P->fraw)
+    return -1;
+  if (!g) {
+    free(P->gain);
+    P->gain = NULL;
+  } else {
+    if (!P->gain)
+      P->gain = (double *)malloc(sizeof(double) * nblk);
+    memcpy(P->gain, g, sizeof(double) * nblk);
+  }
+  for (long blk = 0; blk < nblk; blk++) {
+    const double gg = P->gain ? P->gain[blk] : 1.0;
+    const cplx *R = P->fraw + blk * nh;
+    const cplx *F = P->fodipole ? P->fodipole + blk * nh : NULL;
+    cplx *D = P->fdatas0 + blk * nh;
+    for (long ii = 0; ii < nh; ii++) {
+      D[ii].re = R[ii].re / gg;
+      D[ii].im = R[ii].im / gg;
+      if (F) {
+        D[ii].re -= F[ii].re;
+        D[ii].im -= F[ii].im;
+      }
+    }
+  }
+  return 0;
+}
+
+/* The gain fit after a solve (2185-2228): for every (det, seg) block, with
+ * C = r2c(P S) + fodipole (the model with the dipole added back, 2116-2122)
+ * and D = fraw (the data with the dipole and relCalib put back, 2116-2133),
+ *   out[3 blk + 0] = sum_{ii <= ns/2} SpN (Re D Re C + Im D Im C)   (zeta)
+ *   out[3 blk + 1] = sum_{ii <= ns/2} SpN |C|^2                     (gamma)
+ *   out[3 blk + 2] = sum_{ii <= ns/2} SpN |D|^2
+ * blk = idet nseg + iseg.  The gain that minimises the reference's chi2
+ * (3206-3255) for that block is out[0] / out[1]; the reference's own ratio
+ * sPNd / cPNPc carries a factor 2 in sPNd and 1/2 in cPNPc (2214, 2218-2220),
+ * 4 times that minimiser, and scales the map by absrecal, which is read
+ * from zeta_all / gamma_all (2282-2291) that only -R sets.  These sums are
+ * the parts of that fit with both factors equal and absrecal = 1. */
+void or_recal_sums(or_problem *P, const double *S, double *out) {
+  long ns = P->ns, nh = ns / 2 + 1, npix = P->npix;
+  long nblk = P->ndet * P->nseg;
+#pragma omp parallel num_threads(P->nthreads)
+  {
+    double *Ps = (double *)malloc(sizeof(double) * ns);
+    cplx *C = (cplx *)malloc(sizeof(cplx) * nh);
+    cplx *work = (cplx *)malloc(sizeof(cplx) * 2 * ns);
+#pragma omp for schedule(static)
+    for (long blk = 0; blk < nblk; blk++) {
+      long idet = blk / P->nseg, iseg = blk % P->nseg;
+      const double *Nk = P->spn + idet * ns;
+      const cplx *D = P->fraw + blk * nh;
+      const cplx *F = P->fodipole ? P->fodipole + blk * nh : NULL;
+      or_deproject(S, P->indpix, P->datatopix + IDX(P, idet, iseg, 0),
+                   P->psip + IDX(P, idet, iseg, 0), ns, npix, Ps, P->galbp,
+                   P->relfgal[idet]);
+      or_rfft_r2c(P->plan, Ps, C, work);
+      double zeta = 0.0, gamma = 0.0, dd = 0.0;
+      for (long ii = 0; ii < nh; ii++) {
+        double cr = C[ii].re, ci = C[ii].im;
+        if (F) {
+          cr += F[ii].re;
+          ci += F[ii].im;
+        }
+        zeta += (D[ii].re * cr + D[ii].im * ci) * Nk[ii];
+        gamma += (cr * cr + ci * ci) * Nk[ii];
+        dd += (D[ii].re * D[ii].re + D[ii].im * D[ii].im) * Nk[ii];
+      }
+      out[3 * blk + 0] = zeta;
+      out[3 * blk + 1] = gamma;
+      out[3 * blk + 2] = dd;
+    }
+    free(Ps);
+    free(C);
+    free(work);
+  }
+}
+
+/* MPI_compute_chi2 (3206-3255): sum over blocks of
+ * 2 |(fraw - g (r2c(P S) + fodipole)) sqrt(SpN)|^2 over bins 0..ns/2, with
+ * the gains g: [ndet][nseg] (NULL: 1).  Blocks are summed in the reference's
+ * order (segments outer, 3219). */
+double or_recal_chi2(or_problem *P, const double *S, const double *g) {
+  long ns = P->ns, nh = ns / 2 + 1, npix = P->npix;
+  long nblk = P->ndet * P->nseg;
+  double *part = (double *)calloc(nblk, sizeof(double));
+#pragma omp parallel num_threads(P->nthreads)
+  {
+    double *Ps = (double *)malloc(sizeof(double) * ns);
+    cplx *C = (cplx *)malloc(sizeof(cplx) * nh);
+    cplx *work = (cplx *)malloc(sizeof(cplx) * 2 * ns);
+#pragma omp for schedule(static)
+    for (long blk = 0; blk < nblk; blk++) {
+      long iseg = blk / P->ndet, idet = blk % P->ndet;
+      long b = idet * P->nseg + iseg;
+      const double *Nk = P->spn + idet * ns;
+      const cplx *D = P->fraw + b * nh;
+      const cplx *F = P->fodipole ? P->fodipole + b * nh : NULL;
+      const double gg = g ? g[b] : 1.0;
+      or_deproject(S, P->indpix, P->datatopix + IDX(P, idet, iseg, 0),
+                   P->psip + IDX(P, idet, iseg, 0), ns, npix, Ps, P->galbp,
+                   P->relfgal[idet]);
+      or_rfft_r2c(P->plan, Ps, C, work);
+      double c = 0.0;
+      for (long ii = 0; ii < nh; ii++) {
+        double sq = sqrt(Nk[ii]);
+        double re = (D[ii].re - gg * (C[ii].re + (F ? F[ii].re : 0.0))) * sq;
+        double im = (D[ii].im - gg * (C[ii].im + (F ? F[ii].im : 0.0))) * sq;
+        c += (re * re + im * im) / 1.0 * 2.0;
+      }
+      part[blk] = c;
+    }
+    free(Ps);
+    free(C);
+    free(work);
+  }
   double chi2 = 0.0;
   for (long blk = 0; blk < nblk; blk++)
     chi2 += part[blk];
@@ -1336,12 +1486,13 @@ static void or_mask_qu(const or_problem *P, double *v) {
  * and the starting guess corrected by Z E^-1 Z^T (b - A x0).  The loop, the
  * masks (1316-1330, 1455-1459, 1559-1563), the residual refresh every
  * `refresh` iterations and the stop rule are those of or_problem_solve
- * (1415-1761).  Returns the iteration count, or -1 when the reference's
- * diagonal Mp has a zero I element (1344-1348). */
+ * (1415-1761); var00_in > 0 replaces |b|^2 in crit.  Returns the iteration
+ * count, or -1 when the reference's diagonal Mp has a zero I element
+ * (1344-1348). */
 int or_problem_solve_pcg(or_problem *P, const double *x0, int maxiter,
                          double stop, int refresh, int kind, int deflate,
-                         double *S_out, double *crit_out, double *crit2_out,
-                         double *b_out, double *M6_out) {
+                         double var00_in, double *S_out, double *crit_out,
+                         double *crit2_out, double *b_out, double *M6_out) {
   long npix = P->npix, n3 = P->nc * npix;
   if (kind == 1 && P->galbp)
     return -2; /* no 3x3 blocks for the 4-component map */
@@ -1431,6 +1582,8 @@ int or_problem_solve_pcg(or_problem *P, const double *x0, int maxiter,
     var_n += r[ii] * r[ii];
     var00 += b[ii] * b[ii];
   }
+  if (var00_in > 0.0) /* the recalibration loop keeps rloop 1's |b|^2 (1377-1385) */
+    var00 = var00_in;
   double delta0 = delta_n;
   int iter = 0;
   while ((iter < maxiter && var_n / var00 > stop) || iter == 0) {

@@ -41,6 +41,7 @@ ABI_FUNCTIONS = [
     "sanepic_set_tod_async", "sanepic_autotune", "sanepic_set_fft_split", "sanepic_create_uneven",
     "sanepic_get_seglen", "sanepic_set_pointing_range", "sanepic_set_tod_range",
     "sanepic_set_dipole", "sanepic_set_dipole_range", "sanepic_apply_noalloc", "sanepic_set_leakage",
+    "sanepic_set_recalib", "sanepic_set_gains", "sanepic_recal_sums",
 ]
 
 
@@ -62,7 +63,8 @@ class SolveOpts(ctypes.Structure):
     _fields_ = [("precond", ctypes.c_int32), ("refresh", ctypes.c_int32),
                 ("deflate", ctypes.c_int32), ("fallback_growth", ctypes.c_double),
                 ("graph", ctypes.c_int32), ("fcg_window", ctypes.c_int32),
-                ("learned_iters", ctypes.c_int32), ("refresh_fp32", ctypes.c_int32)]
+                ("learned_iters", ctypes.c_int32), ("refresh_fp32", ctypes.c_int32),
+                ("var00", ctypes.c_double)]
 
 
 class IterInfo(ctypes.Structure):
@@ -168,6 +170,9 @@ def lib():
             "sanepic_set_dipole_range": [P, i64, i64, i64, d_p],
             "sanepic_apply_noalloc": [P, d_p, d_p],
             "sanepic_set_leakage": [P, d_p],
+            "sanepic_set_recalib": [P, ctypes.c_int32],
+            "sanepic_set_gains": [P, d_p],
+            "sanepic_recal_sums": [P, d_p, d_p],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -425,6 +430,24 @@ class Context:
         self._chk(lib().sanepic_set_leakage(self.h, _dp(relfgal)), "set_leakage")
         self.nc = 4
 
+    def set_recalib(self, on=True):
+        """Keep the TOD and dipole resident for the relative calibration loop
+        (before setup; one segment length for every detector)."""
+        self._chk(lib().sanepic_set_recalib(self.h, 1 if on else 0), "set_recalib")
+
+    def set_gains(self, g):
+        """relCalib [ndet, nseg] of every block; rebuilds b (a new solve_begin follows)."""
+        g = _f64(g)
+        assert g.size == self.ndet * self.nseg
+        self._chk(lib().sanepic_set_gains(self.h, _dp(g)), "set_gains")
+
+    def recal_sums(self, x):
+        """[ndet, nseg, 3]: zeta, gamma and the data term of the gain fit of map x."""
+        x = _f64(x)
+        out = np.empty(3 * self.ndet * self.nseg)
+        self._chk(lib().sanepic_recal_sums(self.h, _dp(x), _dp(out)), "recal_sums")
+        return out.reshape(self.ndet, self.nseg, 3)
+
     def set_dipole(self, idet, iseg, odip):
         odip = _f64(odip)
         assert odip.size == self.ns
@@ -538,9 +561,9 @@ class Context:
         return out.value
 
     def solve_begin(self, precond=PRECOND_DIAG, refresh=10, x0=None, deflate=False, fallback_growth=0.0,
-                    graph=False, fcg_window=0, learned_iters=0, refresh_fp32=False):
+                    graph=False, fcg_window=0, learned_iters=0, refresh_fp32=False, var00=0.0):
         opts = SolveOpts(precond, refresh, 1 if deflate else 0, fallback_growth, 1 if graph else 0,
-                         fcg_window, learned_iters, 1 if refresh_fp32 else 0)
+                         fcg_window, learned_iters, 1 if refresh_fp32 else 0, float(var00))
         xp = None if x0 is None else _dp(_f64(x0))
         self._x0_keep = None if x0 is None else _f64(x0)
         self._chk(lib().sanepic_solve_begin(self.h, ctypes.byref(opts), xp), "solve_begin")

@@ -204,6 +204,8 @@ OpArgs<double> op_args<double>(sanepic_ctx *c) {
   a.chi2part = c->chi2part;
   a.xcd = c->xcd_blocks ? 1 : 0;
   a.zpos = c->zsorted ? c->rank : nullptr;
+  a.gain = c->recal ? c->gain64 : nullptr;
+  a.blk = c->ns;
   return a;
 }
 
@@ -233,6 +235,8 @@ OpArgs<float> op_args<float>(sanepic_ctx *c) {
   // that writes another buffer must clear it
   a.zpos = c->zsorted ? c->rank : nullptr;
   a.waves = c->col_waves;
+  a.gain = c->recal ? c->gain32 : nullptr;
+  a.blk = c->ns;
   return a;
 }
 
@@ -1417,6 +1421,12 @@ int setup_data(sanepic_ctx *c) {
   const int64_t npix = c->npix, ntot = c->ntot;
   // b = P^T N^-1 d  (1069-1125); the data spectrum is kept for chi2
   LCHK(c, launch_tod_nanfill(c->tod, ntot, c->ns, c->st));
+  if (c->recal) {  // the TOD and the dipole stay for the gain fits
+    ACHK(c, recal_rhs(c));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    trace_mark("setup: P^T N^-1 d");
+    return 0;
+  }
   // the orbital dipole leaves the data spectrum (fdatas0 -= fodipole,
   // 1055-1067): r2c is linear, so it is subtracted from the NaN-filled TOD
   // (the reference fills NaNs in the TOD only, 784-790, not in the dipole)
@@ -1627,6 +1637,13 @@ int sanepic_solve_begin(sanepic_ctx *c, const sanepic_solve_opts *opts, const do
   } else {
     CGArgs a = cg_args(c);
     LCHK(c, cg_launch_init(a, c->st));
+  }
+  const double var00 = opts ? opts->var00 : 0.0;
+  if (var00 > 0.0) {  // the recalibration loop's crit denominator (1377-1385)
+    double *dst = learned ? &c->fs->var00 : &c->sc->var00;
+    HIPCHK(c, hipMemcpyAsync(dst, &var00, sizeof(double), hipMemcpyHostToDevice, c->st));
+  } else if (var00 < 0.0 || !std::isfinite(var00)) {
+    return fail(c, SANEPIC_EINVAL, "var00 must be >= 0");
   }
   HIPCHK(c, hipStreamSynchronize(c->st));
   c->iter = 0;

@@ -55,6 +55,17 @@ struct sanepic_ctx {
   double *d_relf64 = nullptr;
   float *d_relf32 = nullptr;
   double *ML = nullptr;              // [npix] diagonal preconditioner entry of L
+  // Relative calibration (-r): the NaN-filled TOD and the dipole stay on the
+  // device after set-up, and gain64 / gain32 ([ndet][nseg], the reference's
+  // relCalib) scale P x and z of each block (N^-1 relCalib^2, do_PtNd
+  // 2543-2545); b is rebuilt from the TOD whenever the gains change.
+  bool recal = false;
+  std::vector<double> h_gain;
+  std::vector<float> h_gain32;
+  double *gain64 = nullptr;
+  float *gain32 = nullptr;
+  std::vector<double> recal_dd;      // per block: the data term of the gain fit (cached)
+  double *rc_map = nullptr, *rc_part = nullptr;
   int64_t *off = nullptr;
   uint32_t *perm = nullptr;
   int32_t *it_pix = nullptr, *it_len = nullptr, *it_slot = nullptr, *sp_pix = nullptr,
@@ -351,6 +362,8 @@ int fetch_host_order(sanepic_ctx *c);
 // while the TOD upload may still run; then b = P^T N^-1 d from the TOD.
 int setup_pointing(sanepic_ctx *c);
 int setup_data(sanepic_ctx *c);
+// b = P^T relCalib N^-1 (d - relCalib dip) from the resident TOD (capi_recal.hip)
+int recal_rhs(sanepic_ctx *c);
 template <typename T> OpArgs<T> op_args(sanepic_ctx *c);
 template <> OpArgs<double> op_args<double>(sanepic_ctx *c);
 template <> OpArgs<float> op_args<float>(sanepic_ctx *c);

@@ -111,6 +111,9 @@ struct OpArgs {
   uint64_t seed;        // SRC_RNG stream
   int xcd;              // XCD-contiguous block numbering (four-step passes)
   int waves;            // occupancy target of the FP32 gather column pass (0: compiler's choice)
+  const T *gain;        // [ndet][nseg] relative calibration of each segment, or null: it
+                        // multiplies P x in the pre-gather pass and z in the inverse pass
+  int64_t blk;          // samples per (detector, segment) block (the block of sample g is g / blk)
 };
 
 template <typename T> int launch_col_fwd(const OpArgs<T> &a, int src, hipStream_t s);

@@ -123,10 +123,24 @@ __device__ __forceinline__ int64_t zslot(const OpArgs<T> &a, int64_t g) {
   return a.zpos ? (int64_t)a.zpos[g] : g;
 }
 
-template <typename T, bool LEAK>
+// Relative calibration of the (detector, segment) block holding sample g
+// (1 when the context has none): the filtered z of the block is scaled by
+// it once more on the way out, N^-1 relCalib^2 of do_PtNd (2543-2545).
+template <typename T>
+__device__ __forceinline__ T seg_gain(const OpArgs<T> &a, int64_t g) {
+  return a.gain ? a.gain[g / a.blk] : (T)1;
+}
+
+// GAIN: P x of the block times its relative calibration (the model the
+// recalibrated data are compared with, write_tfAS on S, 1252).
+template <typename T, bool LEAK, bool GAIN>
 __global__ __launch_bounds__(256) void k_gather_px(OpArgs<T> a, T *px, int64_t ntot) {
   const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (g < ntot) px[g] = gather_one<T, LEAK>(a, g);
+  if (g < ntot) {
+    T v = gather_one<T, LEAK>(a, g);
+    if constexpr (GAIN) v *= a.gain[g / a.blk];
+    px[g] = v;
+  }
 }
 
 // ---------------------------------------------------------------- column fwd
@@ -286,6 +300,7 @@ __global__ __launch_bounds__(256) void k_col_inv_m(OpArgs<T> a) {
   pair_info(a.g, pr, det, ga, gb);
   const int64_t n1 = cb * B + b;
   V *io = a.buf + pr * a.g.N;
+  const T sa = seg_gain(a, ga), sb = gb >= 0 ? seg_gain(a, gb) : (T)1;
   // Radix-M DFT of rows vv + P q, written back in place as sub-column j's
   // input at row P j + vv: each thread reads and writes the same rows of
   // its own column, so no barrier is needed before the sub-FFTs read them.
@@ -311,8 +326,8 @@ __global__ __launch_bounds__(256) void k_col_inv_m(OpArgs<T> a) {
 #pragma unroll
     for (int s = 0; s < 16; s++) {
       const int64_t idx = n1 + N1 * (int64_t)(j + M * (t + TS * s));
-      a.z[zslot(a, ga + idx)] = v[s].x;
-      if (gb >= 0) a.z[zslot(a, gb + idx)] = v[s].y;
+      a.z[zslot(a, ga + idx)] = v[s].x * sa;
+      if (gb >= 0) a.z[zslot(a, gb + idx)] = v[s].y * sb;
     }
   }
 }
@@ -401,6 +416,7 @@ __global__ __launch_bounds__(256) void k_col_inv(OpArgs<T> a) {
   pair_info(a.g, pr, det, ga, gb);
   const int64_t n1 = cb * B + b;
   const V *in = a.buf + pr * a.g.N;
+  const T sa = seg_gain(a, ga), sb = gb >= 0 ? seg_gain(a, gb) : (T)1;
   V v[16];
 #pragma unroll
   for (int i = 0; i < 16; i++) v[i] = in[(int64_t)(t + TS * i) * N1 + n1];
@@ -408,8 +424,8 @@ __global__ __launch_bounds__(256) void k_col_inv(OpArgs<T> a) {
 #pragma unroll
   for (int s = 0; s < 16; s++) {
     const int64_t idx = n1 + N1 * (int64_t)(t + TS * s);
-    a.z[zslot(a, ga + idx)] = v[s].x;
-    if (gb >= 0) a.z[zslot(a, gb + idx)] = v[s].y;
+    a.z[zslot(a, ga + idx)] = v[s].x * sa;
+    if (gb >= 0) a.z[zslot(a, gb + idx)] = v[s].y * sb;
   }
 }
 
@@ -475,11 +491,12 @@ __global__ __launch_bounds__(256) void k_single(OpArgs<T> a) {
   if (mode & ROW_INV) {
     fft_reg<V, +1, LOG2N, B>(v, t, lds + b, a.tw4096);
     if (valid) {
+      const T sa = seg_gain(a, ga), sb = gb >= 0 ? seg_gain(a, gb) : (T)1;
 #pragma unroll
       for (int s = 0; s < 16; s++) {
         const int64_t idx = t + TS * s;
-        a.z[zslot(a, ga + idx)] = v[s].x;
-        if (gb >= 0) a.z[zslot(a, gb + idx)] = v[s].y;
+        a.z[zslot(a, ga + idx)] = v[s].x * sa;
+        if (gb >= 0) a.z[zslot(a, gb + idx)] = v[s].y * sb;
       }
     }
   }
@@ -609,11 +626,15 @@ int launch_gather_px(const OpArgs<T> &a, T *px, int64_t ntot, hipStream_t s) {
   if (ntot <= 0) return 0;
   // the leakage component's maps are gathered only here (the context
   // pre-gathers P x whenever it has one)
-  if (a.nc == 4)
-    hipLaunchKernelGGL((k_gather_px<T, true>), dim3((unsigned)((ntot + 255) / 256)), dim3(256), 0, s, a, px, ntot);
+  const dim3 grid((unsigned)((ntot + 255) / 256));
+  if (a.nc == 4 && a.gain)
+    hipLaunchKernelGGL((k_gather_px<T, true, true>), grid, dim3(256), 0, s, a, px, ntot);
+  else if (a.nc == 4)
+    hipLaunchKernelGGL((k_gather_px<T, true, false>), grid, dim3(256), 0, s, a, px, ntot);
+  else if (a.gain)
+    hipLaunchKernelGGL((k_gather_px<T, false, true>), grid, dim3(256), 0, s, a, px, ntot);
   else
-    hipLaunchKernelGGL((k_gather_px<T, false>), dim3((unsigned)((ntot + 255) / 256)), dim3(256), 0, s, a, px,
-                       ntot);
+    hipLaunchKernelGGL((k_gather_px<T, false, false>), grid, dim3(256), 0, s, a, px, ntot);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

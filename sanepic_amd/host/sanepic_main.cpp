@@ -54,13 +54,17 @@ void usage(const char *name) {
       << "  -h 0|1      half-wave plate in the optical path\n"
       << "  -w omega    half-wave plate angle step per sample (radians)\n"
       << "  -I file     start from this compressed [I|Q|U] map (an earlier _A_ output)\n"
+      << "  -r 0|1      alternate map solves with fits of a gain per detector and segment, until a\n"
+      << "              solve starts converged (maps _<pass>, gains relCalib.dat<pass> in the working\n"
+      << "              directory; SANEPIC_MAX_RLOOP caps the passes)\n"
+      << "  -R 0|1      with -r: one gain per detector instead\n"
       << "Outputs\n"
       << "  -O prefix   path prefix of every output map\n"
       << "  -e tag      tag inserted in the output file names\n"
       << "  -i 0|1      also write the I, Q, U maps of intermediate iterations\n"
       << "  -j step     iterations between intermediate maps (default 10)\n"
-      << "Read for compatibility and without effect on the maps: -H -J -A -m -G -L -g -z -M -c -R\n"
-      << "Rejected unless 0 (not available): -r, -E\n";
+      << "Read for compatibility and without effect on the maps: -H -J -A -m -G -L -g -z -M -c\n"
+      << "Rejected unless 0 (not available): -E\n";
   exit(1);
 }
 
@@ -183,7 +187,7 @@ int main(int argc, char *argv[]) {
   const int rank = mpi_rank();
   int iterw = 10;
   bool polar = false, hwp = false, initMapOn = false, writeIterOn = true, pfile = false, odip = false;
-  int recalib = 0, galbp = 0;
+  int recalib = 0, recalibGlobal = 0, galbp = 0;
   long ndet = 0, nsegtot = 0, ns = 0, nssp = 0, nside = 0;
   double omega = 0.0;
   std::string outdir, bext, pext, termin_out, noisepre, filesegnum, dirfile, dirfilep, initMap, filedetinfo,
@@ -225,10 +229,9 @@ int main(int argc, char *argv[]) {
       case 'o': odip = atoi(optarg) != 0; break;
       case 'r': recalib = atoi(optarg); break;
       case 'b': galbp = atoi(optarg); break;
-      // -R acts only inside the -r loop (2254-2270); -K names the -b file;
-      // the rest are read by the reference and never used
+      // -K names the -b file; the rest are read by the reference and never used
       case 'K': relfgal_file = optarg; break;
-      case 'R':
+      case 'R': recalibGlobal = atoi(optarg); break;
       case 'H': case 'J': case 'A': case 'm': case 'G': case 'L': case 'g': case 'z': case 'M': case 'c':
         break;
       default:
@@ -248,8 +251,8 @@ int main(int argc, char *argv[]) {
     fprintf(stderr, "sanepic: only -p 1 is supported (the reference reads psi unconditionally)\n");
     exit(1);
   }
-  if (recalib) {
-    fprintf(stderr, "sanepic: -r (recalibration loop) is not supported by this build\n");
+  if (recalib && galbp) {  // 1122-1123, 1237-1239, 1567-1569 (forceLeakzero, 3262-3275)
+    fprintf(stderr, "sanepic: -r with -b (leakage forced to zero at high latitude) is not supported by this build\n");
     exit(1);
   }
   const double t0 = now();
@@ -356,6 +359,10 @@ int main(int argc, char *argv[]) {
   printf("Using input noise \n");
   std::vector<int64_t> indpix(12 * nside * nside);
   sanepic_get_indpix(ctx, indpix.data());
+  if (recalib) {  // the TOD and the dipole stay on the device for the gain fits
+    rc = sanepic_set_recalib(ctx, 1);
+    if (rc) die_lib(ctx, rc, "sanepic_set_recalib");
+  }
   rc = sanepic_setup(ctx);
   if (rc) die_lib(ctx, rc, "sanepic_setup");
   printf("PtNd Computed...\n");
@@ -404,69 +411,163 @@ int main(int argc, char *argv[]) {
     }
   }
   so.refresh = 10;
-  rc = sanepic_solve_begin(ctx, &so, initMapOn ? x0.data() : nullptr);
-  if (rc == SANEPIC_EDATA) {
-    printf("%s\n", sanepic_last_error(ctx));
-    exit(0);  // the reference exits 0 here (1344-1348)
-  }
-  if (rc) die_lib(ctx, rc, "sanepic_solve_begin");
-  const double tsetup = now() - t0;
-  // 1415-1761
+  // Q/U entries of pixels with fewer than 4 polarised hits are out of every
+  // [3 npix] vector (1316-1330): |b|^2 below is taken over the rest
+  std::vector<int64_t> hits(2 * npix);
+  sanepic_get_setup(ctx, hits.data(), nullptr, nullptr);
+  auto masked_norm2 = [&](const std::vector<double> &v) {
+    double n2 = 0.0;
+    for (long i = 0; i < npixtot; i++) {
+      const long p = i % npix, c = i / npix;
+      if ((c == 1 || c == 2) && hits[npix + p] < 4) continue;
+      n2 += v[i] * v[i];
+    }
+    return n2;
+  };
+  // The recalibration loop (1041-2316): each pass solves the map with the
+  // current relCalib, writes it with the pass number, and with -r fits the
+  // gains of every (det, seg) block to it.  The loop ends when a pass starts
+  // from a map that already solves its system to 1e-15 (var0 / var00, 1042),
+  // after 499 passes, or at SANEPIC_MAX_RLOOP passes.
+  const long nblk = ndet * nsegtot;
+  std::vector<double> gains(nblk, 1.0);  // [det][seg]
+  int rloop_cap = 500;
+  if (const char *e = getenv("SANEPIC_MAX_RLOOP")) rloop_cap = std::min(500, atoi(e) + 1);
   const double StopCriterium = 1e-15;
-  int iter = 0;
-  double crit = 1.0;
-  std::vector<double> S(npixtot), aux(npixtot);
-  const double tl0 = now();
-  while ((iter < 2000 && crit > StopCriterium) || iter == 0) {
-    sanepic_iter_info info{};
-    rc = sanepic_solve_step(ctx, &info);
-    if (rc) die_lib(ctx, rc, "sanepic_solve_step");
-    crit = info.crit;
-    printf("iter = %d, crit = %10.15g, crit2 = %10.15g, chi2 = %10.15g\n", iter, info.crit, info.crit2, info.chi2);
-    fflush(stdout);
-    if (iter == 1) {  // 1616-1707
-      std::vector<int64_t> hits(2 * npix);
-      std::vector<double> tmp(npix), sumc(npixtot);
-      sanepic_get_nvar(ctx, aux.data());
-      write_reduced_map(fmt("%s_NVAR%s", outdir, termin_out), nside, aux.data(), indpix);
-      sanepic_get_setup(ctx, hits.data(), nullptr, nullptr);
-      sanepic_get_sumcross(ctx, sumc.data());
-      for (long i = 0; i < npix; i++) tmp[i] = double(hits[i]);
-      write_reduced_map(fmt("%s_HITS%s", outdir, termin_out), nside, tmp.data(), indpix);
-      write_reduced_map(fmt("%s_COS2P_%s", outdir, termin_out), nside, sumc.data(), indpix);
-      write_reduced_map(fmt("%s_SIN2P_%s", outdir, termin_out), nside, sumc.data() + npix, indpix);
-      write_reduced_map(fmt("%s_Q_NVAR%s", outdir, termin_out), nside, aux.data() + npix, indpix);
-      write_reduced_map(fmt("%s_U_NVAR%s", outdir, termin_out), nside, aux.data() + 2 * npix, indpix);
-      if (galbp)  // 1693-1697
-        write_reduced_map(fmt("%s_Leak_NVAR%s", outdir, termin_out), nside, aux.data() + 3 * npix, indpix);
-      for (long i = 0; i < npix; i++) tmp[i] = double(hits[i + npix]);
-      write_reduced_map(fmt("%s_POL_HITS%s", outdir, termin_out), nside, tmp.data(), indpix);
+  double var0 = 1.0, var00 = 1.0, var00_first = 0.0, tsetup = 0.0, tloop = 0.0;
+  int iter_total = 0;
+  std::vector<double> S(npixtot), aux(npixtot), bvec(npixtot);
+  int rloop = 1;
+  while (rloop && rloop < rloop_cap && var0 / var00 > StopCriterium) {
+    if (rloop > 1) {
+      rc = sanepic_set_gains(ctx, gains.data());
+      if (rc) die_lib(ctx, rc, "sanepic_set_gains");
     }
-    if (iterw && (iter % iterw) == 0 && writeIterOn) {  // 1711-1751
-      sanepic_get_map(ctx, S.data());
-      write_reduced_map(fmt("%s_I%s_%d", outdir, termin_out, iter), nside, S.data(), indpix);
-      write_reduced_map(fmt("%s_Q%s_%d", outdir, termin_out, iter), nside, S.data() + npix, indpix);
-      write_reduced_map(fmt("%s_U%s_%d", outdir, termin_out, iter), nside, S.data() + 2 * npix, indpix);
-      if (galbp)  // 1738-1742
-        write_reduced_map(fmt("%s_Leak%s_%d", outdir, termin_out, iter), nside, S.data() + 3 * npix, indpix);
+    // the first pass keeps its own |b|^2 as the denominator of crit (1377-1385)
+    so.var00 = rloop > 1 ? var00_first : 0.0;
+    const double *xstart = rloop > 1 ? S.data() : (initMapOn ? x0.data() : nullptr);
+    rc = sanepic_solve_begin(ctx, &so, xstart);
+    if (rc == SANEPIC_EDATA) {
+      printf("%s\n", sanepic_last_error(ctx));
+      exit(0);  // the reference exits 0 here (1344-1348)
     }
-    iter++;
+    if (rc) die_lib(ctx, rc, "sanepic_solve_begin");
+    // var0 = |b - A S|^2 of the starting map, var00 = |b|^2 (1360-1399)
+    sanepic_get_rhs(ctx, bvec.data());
+    const double bb = masked_norm2(bvec);
+    if (rloop == 1) var00 = var00_first = bb;
+    var0 = bb;
+    if (xstart) {
+      double rel = 0.0;
+      rc = sanepic_true_residual(ctx, &rel);
+      if (rc) die_lib(ctx, rc, "sanepic_true_residual");
+      var0 = rel * rel * bb;
+    }
+    printf("var0 = %lg\n", var0);
+    printf("var00 = %lg\n", var00);
+    if (rloop == 1) tsetup = now() - t0;
+    // 1415-1761
+    int iter = 0;
+    double crit = 1.0;
+    const double tl0 = now();
+    while ((iter < 2000 && crit > StopCriterium) || iter == 0) {
+      sanepic_iter_info info{};
+      rc = sanepic_solve_step(ctx, &info);
+      if (rc) die_lib(ctx, rc, "sanepic_solve_step");
+      crit = info.crit;
+      printf("iter = %d, crit = %10.15g, crit2 = %10.15g, chi2 = %10.15g\n", iter, info.crit, info.crit2, info.chi2);
+      fflush(stdout);
+      if (iter == 1 && rloop == 1) {  // 1616-1707
+        std::vector<double> tmp(npix), sumc(npixtot);
+        sanepic_get_nvar(ctx, aux.data());
+        write_reduced_map(fmt("%s_NVAR%s", outdir, termin_out), nside, aux.data(), indpix);
+        sanepic_get_sumcross(ctx, sumc.data());
+        for (long i = 0; i < npix; i++) tmp[i] = double(hits[i]);
+        write_reduced_map(fmt("%s_HITS%s", outdir, termin_out), nside, tmp.data(), indpix);
+        write_reduced_map(fmt("%s_COS2P_%s", outdir, termin_out), nside, sumc.data(), indpix);
+        write_reduced_map(fmt("%s_SIN2P_%s", outdir, termin_out), nside, sumc.data() + npix, indpix);
+        write_reduced_map(fmt("%s_Q_NVAR%s", outdir, termin_out), nside, aux.data() + npix, indpix);
+        write_reduced_map(fmt("%s_U_NVAR%s", outdir, termin_out), nside, aux.data() + 2 * npix, indpix);
+        if (galbp)  // 1693-1697
+          write_reduced_map(fmt("%s_Leak_NVAR%s", outdir, termin_out), nside, aux.data() + 3 * npix, indpix);
+        for (long i = 0; i < npix; i++) tmp[i] = double(hits[i + npix]);
+        write_reduced_map(fmt("%s_POL_HITS%s", outdir, termin_out), nside, tmp.data(), indpix);
+      }
+      if (iterw && (iter % iterw) == 0 && writeIterOn && rloop == 1) {  // 1711-1751
+        sanepic_get_map(ctx, S.data());
+        write_reduced_map(fmt("%s_I%s_%d", outdir, termin_out, iter), nside, S.data(), indpix);
+        write_reduced_map(fmt("%s_Q%s_%d", outdir, termin_out, iter), nside, S.data() + npix, indpix);
+        write_reduced_map(fmt("%s_U%s_%d", outdir, termin_out, iter), nside, S.data() + 2 * npix, indpix);
+        if (galbp)  // 1738-1742
+          write_reduced_map(fmt("%s_Leak%s_%d", outdir, termin_out, iter), nside, S.data() + 3 * npix, indpix);
+      }
+      iter++;
+    }
+    printf("\n");
+    tloop += now() - tl0;
+    iter_total += iter;
+    // 1773-1817
+    sanepic_get_map(ctx, S.data());
+    write_raw(fmt("%s_A_%s_%d", outdir, termin_out, rloop), S.data(), npixtot);
+    write_reduced_map(fmt("%s_I_%s_%d", outdir, termin_out, rloop), nside, S.data(), indpix);
+    write_reduced_map(fmt("%s_Q_%s_%d", outdir, termin_out, rloop), nside, S.data() + npix, indpix);
+    write_reduced_map(fmt("%s_U_%s_%d", outdir, termin_out, rloop), nside, S.data() + 2 * npix, indpix);
+    if (galbp)  // 1803-1806
+      write_reduced_map(fmt("%s_Leak_%s_%d", outdir, termin_out, rloop), nside, S.data() + 3 * npix, indpix);
+    printf("Map-making Ends \n");
+    printf("JUST BEFORE recalib \n");
+    if (recalib) {
+      // gain fit (2157-2316): per block zeta / gamma, the minimiser of
+      // MPI_compute_chi2; with -R 1 one gain per detector from the sums over
+      // its blocks.  The reference's own ratio carries factors 2 and 1/2 in
+      // zeta and gamma and a map scale absrecal read from values only -R
+      // sets; with those equal and absrecal = 1 its 50 refits of a pass are
+      // identical, so one is made.
+      std::vector<double> sums(3 * nblk), g2(nblk);
+      rc = sanepic_recal_sums(ctx, S.data(), sums.data());
+      if (rc) die_lib(ctx, rc, "sanepic_recal_sums");
+      auto chi2_of = [&](const std::vector<double> &g) {  // 3206-3255
+        double c2 = 0.0;
+        for (long is = 0; is < nsegtot; is++)
+          for (long id = 0; id < ndet; id++) {
+            const long b = id * nsegtot + is;
+            const double *sm = sums.data() + 3 * b;
+            c2 += 2.0 * (sm[2] - 2.0 * g[b] * sm[0] + g[b] * g[b] * sm[1]);
+          }
+        return c2;
+      };
+      printf("Before recal chi2 = %10.15g\n", chi2_of(gains));
+      for (long id = 0; id < ndet; id++) {
+        if (recalibGlobal) {
+          double za = 0.0, ga = 0.0;
+          for (long is = 0; is < nsegtot; is++) {
+            za += sums[3 * (id * nsegtot + is)];
+            ga += sums[3 * (id * nsegtot + is) + 1];
+          }
+          for (long is = 0; is < nsegtot; is++) g2[id * nsegtot + is] = za / ga;
+          printf("Coef calib det %ld is %10.15g\n", id, za / ga);
+        } else {
+          for (long is = 0; is < nsegtot; is++) {
+            const long b = id * nsegtot + is;
+            g2[b] = sums[3 * b] / sums[3 * b + 1];
+          }
+        }
+      }
+      printf("After recal chi2 = %10.15g\n", chi2_of(g2));
+      gains = g2;
+      // relCalib.dat<rloop> in the working directory: relCalib[idet + iseg ndet]
+      std::vector<double> rel(nblk);
+      for (long id = 0; id < ndet; id++)
+        for (long is = 0; is < nsegtot; is++) rel[id + is * ndet] = gains[id * nsegtot + is];
+      write_raw("relCalib.dat" + std::to_string(rloop), rel.data(), nblk);
+    }
+    rloop += 1;
+    if (!recalib) rloop = 0;
   }
-  printf("\n");
-  const double tloop = now() - tl0;
-  // 1773-1817
-  sanepic_get_map(ctx, S.data());
-  write_raw(fmt("%s_A_%s_%d", outdir, termin_out, 1), S.data(), npixtot);
-  write_reduced_map(fmt("%s_I_%s_%d", outdir, termin_out, 1), nside, S.data(), indpix);
-  write_reduced_map(fmt("%s_Q_%s_%d", outdir, termin_out, 1), nside, S.data() + npix, indpix);
-  write_reduced_map(fmt("%s_U_%s_%d", outdir, termin_out, 1), nside, S.data() + 2 * npix, indpix);
-  if (galbp)  // 1803-1806
-    write_reduced_map(fmt("%s_Leak_%s_%d", outdir, termin_out, 1), nside, S.data() + 3 * npix, indpix);
-  printf("Map-making Ends \n");
   fprintf(stderr,
           "sanepic: %d iterations, set-up %.3f s (file I/O %.3f s), CG loop %.3f s (%.3f ms/iter), "
           "%ld deferred pixel samples, %.2f GB on device\n",
-          iter, tsetup, tio, tloop, 1e3 * tloop / std::max(iter, 1), (long)ndef,
+          iter_total, tsetup, tio, tloop, 1e3 * tloop / std::max(iter_total, 1), (long)ndef,
           [&] { int64_t b = 0; sanepic_device_bytes(ctx, &b); return b / 1e9; }());
   sanepic_destroy(ctx);
   sanepic_unet_destroy(unet);

@@ -97,9 +97,10 @@ def test_cli_errors_before_device(tmp_path):
 
 @pytest.mark.skipif(not os.path.exists(EXE), reason="drop-in executable not built")
 def test_cli_unsupported_flags_fail(tmp_path):
-    """-r (recalibration) changes the reference's maps; this build refuses it
-    rather than ignore it.  -R acts only with -r."""
+    """-r with -b forces the leakage map to zero at high latitude inside the
+    loop (forceLeakzero, 3262-3275), which this build does not do: it refuses
+    the combination rather than ignore it, before touching the device."""
     cfg, ds, args = _tree(tmp_path)
     argv = synth.sanepic_argv(args)[1:]
-    r = _run(argv + ["-r", "1"])
+    r = _run(argv + ["-r", "1", "-b", "1", "-K", str(tmp_path / "k.bi")])
     assert r.returncode == 1 and "not supported" in r.stderr

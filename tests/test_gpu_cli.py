@@ -176,10 +176,22 @@ def test_drop_in_dipole_matches_oracle(tmp_path):
     A = np.fromfile(f"{args['out']}_A_{args['ext']}_1", dtype="<f8")
     npix = P.npix
     hits = ref["hits"]
-    ax = P.apply_A(A)
-    ax[npix:2 * npix][hits[npix:] < 4] = 0.0
-    ax[2 * npix:][hits[npix:] < 4] = 0.0
-    assert np.linalg.norm(ref["b"] - ax) / np.linalg.norm(ref["b"]) < 1e-6
+
+    def resid(x):
+        ax = P.apply_A(x)
+        ax[npix:2 * npix][hits[npix:] < 4] = 0.0
+        ax[2 * npix:][hits[npix:] < 4] = 0.0
+        return np.linalg.norm(ref["b"] - ax) / np.linalg.norm(ref["b"])
+
+    # in this case the residual refreshes every 10 iterations hold crit near
+    # 1e-15 from ~1400 iterations on while the true residual drifts up to
+    # ~1e-6, and whether a refresh lands below 1e-15 depends on rounding: the
+    # map solves the system as well as the oracle's after the same iterations
+    same = P.solve(stop=0.0, maxiter=len(got))
+    rg, ro = resid(A), resid(same["S"])
+    print(f"  final maps after {len(got)} iterations: residual of the executable's {rg:.2e}, "
+          f"of the oracle's {ro:.2e}", flush=True)
+    assert rg <= 2.0 * ro + 1e-9
     # without -o the dipole stays in the data: a different system
     r0 = _run(synth.sanepic_argv(args, exe=sanepic_amd.EXE_PATH))
     assert rel(_crit(r0.stdout)[:5], ref["crit"]) > 1e-6
