@@ -206,6 +206,7 @@ OpArgs<double> op_args<double>(sanepic_ctx *c) {
   a.zpos = c->zsorted ? c->rank : nullptr;
   a.gain = c->recal ? c->gain64 : nullptr;
   a.blk = c->ns;
+  a.waves = c->col_waves64;
   return a;
 }
 
@@ -315,14 +316,10 @@ int run_scatter(sanepic_ctx *c, const T *z, double *q, int mask) {
   return 0;
 }
 
-// q = P^T N^-1 P x (x, q interleaved, device).  chi2 -> sc->chi2 when asked.
-int apply_A(sanepic_ctx *c, const double *x, double *q, int mask, bool chi2, bool fp64) {
-  ProfScope whole(c, 4);  // groups: the parent times the whole operator (all groups on its stream)
-  if (!c->sub.empty()) return group_apply_A(c, x, q, mask, chi2, fp64);
+// z = N^-1 P x of this context in the precision of its operator (c->z or
+// c->z32), with the chi2 partials of its row blocks when asked (*nb blocks).
+int apply_filter(sanepic_ctx *c, const double *x, bool chi2, bool use64, int64_t *nb) {
   const int mode = ROW_FWD | ROW_FILTER | ROW_INV | (chi2 ? ROW_CHI2 : 0);
-  int64_t nb = 0;
-  const bool use64 = fp64 || c->cfg.precision == SANEPIC_FP64;
-  if (chi2 && !use64) return fail(c, SANEPIC_EINVAL, "chi2 needs the FP64 operator");
   if (use64) {
     OpArgs<double> a = op_args<double>(c);
     a.map = x;
@@ -339,26 +336,37 @@ int apply_A(sanepic_ctx *c, const double *x, double *q, int mask, bool chi2, boo
       a.px = c->px64;
       src = SRC_PX;
     }
-    ACHK(c, run_filter<double>(c, a, src, mode, &nb));
-    ACHK(c, run_scatter<double>(c, c->z, q, mask));
-  } else {
-    if (c->pix_nested || c->gsrc)
-      LCHK(c, launch_map_slots(x, c->gsrc, c->pix_nested ? c->mpos : nullptr, c->map32, c->npix, c->st, c->nc));
-    else
-      LCHK(c, launch_d2f(x, c->map32, c->nc * c->npix, c->st));
-    OpArgs<float> a = op_args<float>(c);
-    a.map = c->map32;
-    int src = SRC_GATHER;
-    if (c->pregather) {
-      ACHK(c, dalloc(c, c->px32, c->ntot));
-      ProfScope ps(c, 0);
-      LCHK(c, launch_gather_px<float>(a, c->px32, c->ntot, c->st));
-      a.px = c->px32;
-      src = SRC_PX;
-    }
-    ACHK(c, run_filter<float>(c, a, src, mode, &nb));
-    ACHK(c, run_scatter<float>(c, c->z32, q, mask));
+    return run_filter<double>(c, a, src, mode, nb);
   }
+  if (c->pix_nested || c->gsrc)
+    LCHK(c, launch_map_slots(x, c->gsrc, c->pix_nested ? c->mpos : nullptr, c->map32, c->npix, c->st, c->nc));
+  else
+    LCHK(c, launch_d2f(x, c->map32, c->nc * c->npix, c->st));
+  OpArgs<float> a = op_args<float>(c);
+  a.map = c->map32;
+  int src = SRC_GATHER;
+  if (c->pregather) {
+    ACHK(c, dalloc(c, c->px32, c->ntot));
+    ProfScope ps(c, 0);
+    LCHK(c, launch_gather_px<float>(a, c->px32, c->ntot, c->st));
+    a.px = c->px32;
+    src = SRC_PX;
+  }
+  return run_filter<float>(c, a, src, mode, nb);
+}
+
+// q = P^T N^-1 P x (x, q interleaved, device).  chi2 -> sc->chi2 when asked.
+int apply_A(sanepic_ctx *c, const double *x, double *q, int mask, bool chi2, bool fp64) {
+  ProfScope whole(c, 4);  // groups: the parent times the whole operator (all groups on its stream)
+  if (!c->sub.empty()) return group_apply_A(c, x, q, mask, chi2, fp64);
+  const bool use64 = fp64 || c->cfg.precision == SANEPIC_FP64;
+  if (chi2 && !use64) return fail(c, SANEPIC_EINVAL, "chi2 needs the FP64 operator");
+  int64_t nb = 0;
+  ACHK(c, apply_filter(c, x, chi2, use64, &nb));
+  if (use64)
+    ACHK(c, run_scatter<double>(c, c->z, q, mask));
+  else
+    ACHK(c, run_scatter<float>(c, c->z32, q, mask));
   if (chi2) LCHK(c, launch_reduce_chi2(c->chi2part, nb, c->sc, c->st));
   return 0;
 }
@@ -609,6 +617,7 @@ int sanepic_create(const sanepic_config *cfg, sanepic_ctx **out) {
   if (const char *e = getenv("SANEPIC_XCD_BLOCKS")) c->xcd_blocks = atoi(e) != 0;
   if (const char *e = getenv("SANEPIC_ZSORT")) c->zsorted = atoi(e) != 0;
   if (const char *e = getenv("SANEPIC_COL_WAVES")) c->col_waves = atoi(e);
+  if (const char *e = getenv("SANEPIC_COL_WAVES64")) c->col_waves64 = atoi(e);
   if (const char *e = getenv("SANEPIC_NESTED_MAP")) c->nested_map = atoi(e) != 0;
   if (const char *e = getenv("SANEPIC_ITEM_ORDER")) c->items_nested = atoi(e) != 0;
   if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
@@ -874,6 +883,12 @@ int nested_order(sanepic_ctx *c, const int32_t *dlist) {
 
 }  // namespace
 
+extern "C++" {
+namespace sctx {
+int pixel_nested_order(sanepic_ctx *c, const int32_t *dlist) { return nested_order(c, dlist); }
+}  // namespace sctx
+}  // extern "C++"
+
 int sanepic_pixelize(sanepic_ctx *c, int64_t *npix_out, int64_t *n_deferred) {
   if (!c) return SANEPIC_EINVAL;
   if (c->pixelized) return fail(c, SANEPIC_EINVAL, "sanepic_pixelize already ran");
@@ -1047,26 +1062,7 @@ int sanepic_pixelize(sanepic_ctx *c, int64_t *npix_out, int64_t *n_deferred) {
   HIPCHK(c, hipStreamSynchronize(c->st));
   dfree_later(c, counts, nsky);
   // stable sort of samples by compressed pixel -> perm
-  {
-    uint32_t *keys_out = nullptr;
-    uint32_t *vals_in = nullptr;
-    ACHK(c, dalloc(c, keys_out, ntot));
-    ACHK(c, dalloc(c, vals_in, ntot));
-    ACHK(c, dalloc(c, c->perm, ntot));
-    LCHK(c, launch_iota_u32(vals_in, ntot, c->st));
-    const int end_bit = std::max(1, ilog2(npix + 1));
-    size_t tbytes = 0;
-    HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(nullptr, tbytes, (const uint32_t *)c->pix, keys_out, vals_in, c->perm,
-                                                 (int)ntot, 0, end_bit, c->st));
-    char *tmp = nullptr;
-    ACHK(c, dalloc(c, tmp, (int64_t)tbytes));
-    HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(tmp, tbytes, (const uint32_t *)c->pix, keys_out, vals_in, c->perm, (int)ntot,
-                                                 0, end_bit, c->st));
-    HIPCHK(c, hipStreamSynchronize(c->st));
-    dfree_later(c, tmp, (int64_t)tbytes);
-    dfree_later(c, keys_out, ntot);
-    dfree_later(c, vals_in, ntot);
-  }
+  ACHK(c, sort_samples(c, (const uint32_t *)c->pix, ntot, npix));
   trace_mark("pixelize: sample sort");
   if (c->zsorted) {
     ACHK(c, dalloc(c, c->rank, ntot));
@@ -1086,95 +1082,9 @@ int sanepic_pixelize(sanepic_ctx *c, int64_t *npix_out, int64_t *n_deferred) {
     LCHK(c, launch_cs_to_turn(c->cs, c->ang32, ntot, c->st));
     LCHK(c, launch_gather_turn(c->perm, c->cs, c->ang_sorted32, ntot, c->st));
   }
-  // P^T work items: pixels, split into <= kChunk-entry chunks, built on the
-  // device (two prefix sums and a fill); the host only lists the rare split
-  // pixels.  Items in NESTED order when enabled: pixels reduced at the same
-  // time are neighbours on the sky, whose runs of z share cache lines (each
-  // pixel's sums are unchanged).
   trace_mark("pixelize: sorted angles");
-  const bool nest_items = c->items_nested && c->d_slot2pix != nullptr;
-  // item count in pixel order (it does not depend on the order); the rare
-  // split pixels then listed in item order with their partial slots
-  std::vector<int32_t> spix, sfirst, scount;
-  int64_t nit = 0;
-  int32_t nslots = 0;
-  for (int64_t p = 0; p < npix; p++) {
-    const int64_t n = c->h_cnt[(size_t)p];
-    if (n <= kChunk) {
-      nit++;
-    } else {
-      nit += (n + kChunk - 1) / kChunk;
-      spix.push_back((int32_t)p);
-    }
-  }
-  if (nest_items && !spix.empty()) {
-    ACHK(c, fetch_host_order(c));
-    std::vector<uint8_t> split((size_t)npix, 0);
-    for (int32_t p : spix) split[(size_t)p] = 1;
-    spix.clear();
-    for (int64_t q = 0; q < npix; q++) {
-      const int32_t p = c->h_slot2pix[(size_t)q];
-      if (split[(size_t)p]) spix.push_back(p);
-    }
-  }
-  for (int32_t p : spix) {
-    const int32_t k = (int32_t)((c->h_cnt[(size_t)p] + kChunk - 1) / kChunk);
-    sfirst.push_back(nslots);
-    scount.push_back(k);
-    nslots += k;
-  }
-  const int64_t nsp = (int64_t)spix.size();
-  trace_mark("pixelize: item counts (host)");
-  ACHK(c, dalloc(c, c->it_pix, nit));
-  ACHK(c, dalloc(c, c->it_beg, nit));
-  ACHK(c, dalloc(c, c->it_len, nit));
-  ACHK(c, dalloc(c, c->it_slot, nit));
-  ACHK(c, dalloc(c, c->sp_pix, nsp));
-  ACHK(c, dalloc(c, c->sp_first, nsp));
-  ACHK(c, dalloc(c, c->sp_count, nsp));
-  if (npix > 0) {
-    const int32_t *order = nest_items ? c->d_slot2pix : nullptr;
-    int32_t *kcnt = nullptr, *scnt = nullptr, *kofs = nullptr, *sofs = nullptr;
-    ACHK(c, dalloc(c, kcnt, npix));
-    ACHK(c, dalloc(c, scnt, npix));
-    ACHK(c, dalloc(c, kofs, npix));
-    ACHK(c, dalloc(c, sofs, npix));
-    LCHK(c, launch_item_count(order, c->cnt, npix, kChunk, kcnt, scnt, c->st));
-    size_t tb = 0;
-    HIPCHK(c, hipcub::DeviceScan::ExclusiveSum(nullptr, tb, kcnt, kofs, (int)npix, c->st));
-    char *tmp = nullptr;
-    ACHK(c, dalloc(c, tmp, (int64_t)tb));
-    HIPCHK(c, hipcub::DeviceScan::ExclusiveSum(tmp, tb, kcnt, kofs, (int)npix, c->st));
-    HIPCHK(c, hipcub::DeviceScan::ExclusiveSum(tmp, tb, scnt, sofs, (int)npix, c->st));
-    LCHK(c, launch_item_fill(order, c->cnt, c->off, kofs, sofs, npix, kChunk, c->it_pix, c->it_beg, c->it_len,
-                             c->it_slot, c->st));
-    dfree_later(c, tmp, (int64_t)tb);
-    dfree_later(c, kcnt, npix);
-    dfree_later(c, scnt, npix);
-    dfree_later(c, kofs, npix);
-    dfree_later(c, sofs, npix);
-  }
-  if (nsp) {
-    HIPCHK(c, hipMemcpy(c->sp_pix, spix.data(), nsp * 4, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->sp_first, sfirst.data(), nsp * 4, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->sp_count, scount.data(), nsp * 4, hipMemcpyHostToDevice));
-  }
+  ACHK(c, build_items(c, ntot, c->items_nested && c->d_slot2pix != nullptr));
   if (!c->pin_owner) dfree_later(c, c->d_slot2pix, npix);  // a group keeps it for fetch_host_order
-  c->items = Items{c->it_pix, c->it_beg, c->it_len, c->it_slot, nit, c->sp_pix, c->sp_first, c->sp_count, nsp};
-  c->nslots = nslots;
-  // lanes per P^T item: 16 keep four entries each in flight from 64 entries
-  // on; shorter items (a group of a few detectors at a fine nside) take 8
-  // (measured at ~30 entries per item: 16 lanes 7.74, 8 lanes 6.30, 4 lanes
-  // 6.76 ms per operator)
-  {
-    const double mean = nit > 0 ? (double)ntot / (double)nit : 0.0;
-    c->scatter_lanes = mean < 128.0 ? 8 : 16;
-    if (const char *e = getenv("SANEPIC_SCATTER_LANES")) {
-      const int l = atoi(e);
-      if (l == 4 || l == 8 || l == 16) c->scatter_lanes = l;
-    }
-  }
-  ACHK(c, dalloc(c, c->partial, 8 * std::max<int64_t>(1, nslots)));
   HIPCHK(c, hipStreamSynchronize(c->st));
   if (c->pin_owner) flush_frees(c);  // a group: its temporaries serve the next group
   trace_mark("pixelize: done");
@@ -1334,6 +1244,125 @@ int nest_gather_map(sanepic_ctx *c) {
 
 extern "C++" {
 namespace sctx {
+
+// Stable sort of ntot samples by key (compressed pixel, < npix) into c->perm:
+// sorted position -> sample index.
+int sort_samples(sanepic_ctx *c, const uint32_t *keys, int64_t ntot, int64_t npix) {
+  uint32_t *keys_out = nullptr;
+  uint32_t *vals_in = nullptr;
+  ACHK(c, dalloc(c, keys_out, ntot));
+  ACHK(c, dalloc(c, vals_in, ntot));
+  ACHK(c, dalloc(c, c->perm, ntot));
+  LCHK(c, launch_iota_u32(vals_in, ntot, c->st));
+  const int end_bit = std::max(1, ilog2(npix + 1));
+  size_t tbytes = 0;
+  HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(nullptr, tbytes, keys, keys_out, vals_in, c->perm,
+                                               (int)ntot, 0, end_bit, c->st));
+  char *tmp = nullptr;
+  ACHK(c, dalloc(c, tmp, (int64_t)tbytes));
+  HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(tmp, tbytes, keys, keys_out, vals_in, c->perm, (int)ntot,
+                                               0, end_bit, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  dfree_later(c, tmp, (int64_t)tbytes);
+  dfree_later(c, keys_out, ntot);
+  dfree_later(c, vals_in, ntot);
+  return 0;
+}
+
+// P^T work items over the c->npix pixels (counts c->cnt and c->h_cnt, sorted
+// offsets c->off) of ntot sorted samples: c->items, c->nslots, c->partial
+// and the lanes per item.  With nest_items the items follow c->d_slot2pix
+// (NESTED order).
+int build_items(sanepic_ctx *c, int64_t ntot, bool nest_items) {
+  const int64_t npix = c->npix;
+  // P^T work items: pixels, split into <= kChunk-entry chunks, built on the
+  // device (two prefix sums and a fill); the host only lists the rare split
+  // pixels.  Items in NESTED order when enabled: pixels reduced at the same
+  // time are neighbours on the sky, whose runs of z share cache lines (each
+  // pixel's sums are unchanged).
+  // item count in pixel order (it does not depend on the order); the rare
+  // split pixels then listed in item order with their partial slots
+  std::vector<int32_t> spix, sfirst, scount;
+  int64_t nit = 0;
+  int32_t nslots = 0;
+  for (int64_t p = 0; p < npix; p++) {
+    const int64_t n = c->h_cnt[(size_t)p];
+    if (n <= kChunk) {
+      nit++;
+    } else {
+      nit += (n + kChunk - 1) / kChunk;
+      spix.push_back((int32_t)p);
+    }
+  }
+  if (nest_items && !spix.empty()) {
+    ACHK(c, fetch_host_order(c));
+    std::vector<uint8_t> split((size_t)npix, 0);
+    for (int32_t p : spix) split[(size_t)p] = 1;
+    spix.clear();
+    for (int64_t q = 0; q < npix; q++) {
+      const int32_t p = c->h_slot2pix[(size_t)q];
+      if (split[(size_t)p]) spix.push_back(p);
+    }
+  }
+  for (int32_t p : spix) {
+    const int32_t k = (int32_t)((c->h_cnt[(size_t)p] + kChunk - 1) / kChunk);
+    sfirst.push_back(nslots);
+    scount.push_back(k);
+    nslots += k;
+  }
+  const int64_t nsp = (int64_t)spix.size();
+  trace_mark("pixelize: item counts (host)");
+  ACHK(c, dalloc(c, c->it_pix, nit));
+  ACHK(c, dalloc(c, c->it_beg, nit));
+  ACHK(c, dalloc(c, c->it_len, nit));
+  ACHK(c, dalloc(c, c->it_slot, nit));
+  ACHK(c, dalloc(c, c->sp_pix, nsp));
+  ACHK(c, dalloc(c, c->sp_first, nsp));
+  ACHK(c, dalloc(c, c->sp_count, nsp));
+  if (npix > 0) {
+    const int32_t *order = nest_items ? c->d_slot2pix : nullptr;
+    int32_t *kcnt = nullptr, *scnt = nullptr, *kofs = nullptr, *sofs = nullptr;
+    ACHK(c, dalloc(c, kcnt, npix));
+    ACHK(c, dalloc(c, scnt, npix));
+    ACHK(c, dalloc(c, kofs, npix));
+    ACHK(c, dalloc(c, sofs, npix));
+    LCHK(c, launch_item_count(order, c->cnt, npix, kChunk, kcnt, scnt, c->st));
+    size_t tb = 0;
+    HIPCHK(c, hipcub::DeviceScan::ExclusiveSum(nullptr, tb, kcnt, kofs, (int)npix, c->st));
+    char *tmp = nullptr;
+    ACHK(c, dalloc(c, tmp, (int64_t)tb));
+    HIPCHK(c, hipcub::DeviceScan::ExclusiveSum(tmp, tb, kcnt, kofs, (int)npix, c->st));
+    HIPCHK(c, hipcub::DeviceScan::ExclusiveSum(tmp, tb, scnt, sofs, (int)npix, c->st));
+    LCHK(c, launch_item_fill(order, c->cnt, c->off, kofs, sofs, npix, kChunk, c->it_pix, c->it_beg, c->it_len,
+                             c->it_slot, c->st));
+    dfree_later(c, tmp, (int64_t)tb);
+    dfree_later(c, kcnt, npix);
+    dfree_later(c, scnt, npix);
+    dfree_later(c, kofs, npix);
+    dfree_later(c, sofs, npix);
+  }
+  if (nsp) {
+    HIPCHK(c, hipMemcpy(c->sp_pix, spix.data(), nsp * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->sp_first, sfirst.data(), nsp * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->sp_count, scount.data(), nsp * 4, hipMemcpyHostToDevice));
+  }
+  c->items = Items{c->it_pix, c->it_beg, c->it_len, c->it_slot, nit, c->sp_pix, c->sp_first, c->sp_count, nsp};
+  c->nslots = nslots;
+  // lanes per P^T item: 16 keep four entries each in flight from 64 entries
+  // on; shorter items (a group of a few detectors at a fine nside) take 8
+  // (measured at ~30 entries per item: 16 lanes 7.74, 8 lanes 6.30, 4 lanes
+  // 6.76 ms per operator)
+  {
+    const double mean = nit > 0 ? (double)ntot / (double)nit : 0.0;
+    c->scatter_lanes = mean < 128.0 ? 8 : 16;
+    if (const char *e = getenv("SANEPIC_SCATTER_LANES")) {
+      const int l = atoi(e);
+      if (l == 4 || l == 8 || l == 16) c->scatter_lanes = l;
+    }
+  }
+  ACHK(c, dalloc(c, c->partial, 8 * std::max<int64_t>(1, nslots)));
+  return 0;
+}
 
 int setup_pointing(sanepic_ctx *c) {
   const int64_t npix = c->npix, ntot = c->ntot, N = c->ns;

@@ -29,9 +29,106 @@ using namespace sctx;
 
 namespace sctx {
 
+// P^T of the operator over the union of the groups' samples: group k's
+// samples take numbers gbase[k] + g, are keyed by their union pixel
+// (gmap[k] of the group pixel) and sorted together, so each pixel of q is
+// one item summing every detector that saw it, in one launch.  The groups'
+// own sorted lists stay for the set-up sums; their z become slices of the
+// union's.
+int build_union_pt(sanepic_ctx *c, const int32_t *dlist) {
+  const int64_t npix = c->npix, ng = (int64_t)c->sub.size();
+  c->gbase.assign((size_t)ng + 1, 0);
+  int64_t ntot = 0;
+  for (int64_t k = 0; k < ng; k++) {
+    c->gbase[(size_t)k] = ntot;
+    ntot += c->sub[(size_t)k]->ntot;
+  }
+  c->gbase[(size_t)ng] = ntot;
+  if (ntot > INT32_MAX) return fail(c, SANEPIC_EINVAL, "more than 2^31 samples in one context");
+  int32_t *keys = nullptr;
+  ACHK(c, dalloc(c, keys, ntot));
+  for (int64_t k = 0; k < ng; k++) {
+    sanepic_ctx *s = c->sub[(size_t)k];
+    LCHK(c, launch_copy_i32(s->pix, keys + c->gbase[(size_t)k], s->ntot, c->st));
+    LCHK(c, launch_compress_pix(keys + c->gbase[(size_t)k], s->ntot, c->gmap[(size_t)k], c->st));
+  }
+  ACHK(c, sort_samples(c, (const uint32_t *)keys, ntot, npix));
+  dfree_later(c, keys, ntot);
+  double2 *cs = nullptr;  // every group's (cos 2psi, sin 2psi) in union sample order
+  ACHK(c, dalloc(c, cs, ntot));
+  for (int64_t k = 0; k < ng; k++) {
+    sanepic_ctx *s = c->sub[(size_t)k];
+    HIPCHK(c, hipMemcpyAsync(cs + c->gbase[(size_t)k], s->cs, s->ntot * sizeof(double2), hipMemcpyDeviceToDevice,
+                             c->st));
+  }
+  ACHK(c, dalloc(c, c->cs_sorted, ntot));
+  LCHK(c, launch_gather_cs(c->perm, cs, c->cs_sorted, ntot, c->st));
+  if (c->cfg.precision == SANEPIC_MIXED) {
+    ACHK(c, dalloc(c, c->ang_sorted32, ntot));
+    LCHK(c, launch_gather_turn(c->perm, cs, c->ang_sorted32, ntot, c->st));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  dfree_later(c, cs, ntot);
+  {  // each pixel's run in the sorted list (64-bit prefix sum of the counts)
+    int64_t *wide = nullptr;
+    ACHK(c, dalloc(c, wide, npix + 1));
+    ACHK(c, dalloc(c, c->off, npix + 1));
+    LCHK(c, launch_widen(c->cnt, npix, wide, c->st));
+    size_t tb = 0;
+    HIPCHK(c, hipcub::DeviceScan::ExclusiveSum(nullptr, tb, wide, c->off, (int)(npix + 1), c->st));
+    char *tmp = nullptr;
+    ACHK(c, dalloc(c, tmp, (int64_t)tb));
+    HIPCHK(c, hipcub::DeviceScan::ExclusiveSum(tmp, tb, wide, c->off, (int)(npix + 1), c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    dfree_later(c, tmp, (int64_t)tb);
+    dfree_later(c, wide, npix + 1);
+  }
+  // items in NESTED pixel order, as a uniform context's (pixels reduced
+  // together are neighbours on the sky); the order is dropped afterwards
+  if (c->items_nested) ACHK(c, pixel_nested_order(c, dlist));
+  ACHK(c, build_items(c, ntot, c->d_slot2pix != nullptr));
+  dfree_later(c, c->mpos, npix);
+  dfree_later(c, c->d_slot2pix, npix);
+  c->h_slot2pix.clear();
+  ACHK(c, dalloc(c, c->z, ntot));
+  if (c->cfg.precision == SANEPIC_MIXED) ACHK(c, dalloc(c, c->z32, ntot));
+  for (int64_t k = 0; k < ng; k++) {
+    sanepic_ctx *s = c->sub[(size_t)k];
+    s->z = c->z + c->gbase[(size_t)k];  // not the group's allocation: it never frees it
+    if (c->z32) s->z32 = c->z32 + c->gbase[(size_t)k];
+  }
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  trace_mark("group pixelize: union P^T");
+  return 0;
+}
+
 int group_apply_A(sanepic_ctx *c, const double *x, double *q, int mask, bool chi2, bool fp64) {
-  LCHK(c, launch_fill(q, 0.0, 3 * c->npix, c->st));
   if (chi2) LCHK(c, launch_fill(&c->sc->chi2, 0.0, 1, c->st));
+  if (c->union_pt) {
+    // each group filters its detectors into its slice of the union's z
+    // (reading its pixels of x through gmap); one P^T over the union writes
+    // every pixel of q
+    const bool use64 = fp64 || c->cfg.precision == SANEPIC_FP64;
+    if (chi2 && !use64) return fail(c, SANEPIC_EINVAL, "chi2 needs the FP64 operator");
+    for (size_t k = 0; k < c->sub.size(); k++) {
+      sanepic_ctx *s = c->sub[k];
+      s->gsrc = c->gmap[k];
+      int64_t nb = 0;
+      const int rc = apply_filter(s, x, chi2, use64, &nb);
+      s->gsrc = nullptr;
+      if (rc) return fail(c, rc, s->err);
+      if (chi2) {
+        LCHK(c, launch_reduce_chi2(s->chi2part, nb, s->sc, c->st));
+        LCHK(c, launch_add_scalar(&c->sc->chi2, &s->sc->chi2, c->st));
+      }
+    }
+    if (use64)
+      ACHK(c, run_scatter<double>(c, c->z, q, mask));
+    else
+      ACHK(c, run_scatter<float>(c, c->z32, q, mask));
+    return 0;
+  }
+  LCHK(c, launch_fill(q, 0.0, 3 * c->npix, c->st));
   // each group reads its pixels of x through gmap into its gather map and its
   // P^T adds into q at the same rows (a group lists a pixel once; the groups
   // run in order on one stream)
@@ -116,6 +213,7 @@ int group_pixelize(sanepic_ctx *c, int64_t *npix_out, int64_t *ndef_out) {
   c->h_pixlist.assign(pinned, pinned + npix);
   c->h_cnt.assign(pinned + npix, pinned + 2 * npix);
   c->h_indpix.clear();  // built from h_pixlist when asked for (sanepic_get_indpix)
+  if (c->union_pt) ACHK(c, build_union_pt(c, dlist));
   dfree_later(c, counts, nsky);
   dfree_later(c, flag, nsky);
   dfree_later(c, pos, nsky);
@@ -281,6 +379,10 @@ int sanepic_create_uneven(const sanepic_uneven_config *cfg, sanepic_ctx **out) {
     cleanup();
     return rc;
   }
+  // one P^T over all groups' samples (SANEPIC_UNION_PT=0: one per group,
+  // adding into the union's q in turn)
+  c->union_pt = true;
+  if (const char *e = getenv("SANEPIC_UNION_PT")) c->union_pt = atoi(e) != 0;
   // the parent keeps only what a group context does not: the timeline
   // pointing (read by every group in place), the solver and the union map
   dfree(c, c->ra, 16);

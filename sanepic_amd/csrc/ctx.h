@@ -66,6 +66,10 @@ struct sanepic_ctx {
   float *gain32 = nullptr;
   std::vector<double> recal_dd;      // per block: the data term of the gain fit (cached)
   double *rc_map = nullptr, *rc_part = nullptr;
+  // Per-detector lengths: P^T runs once over the union of the groups' samples
+  // (gbase[k]: group k's first sample; its z is the slice of z / z32 there)
+  bool union_pt = false;
+  std::vector<int64_t> gbase;
   int64_t *off = nullptr;
   uint32_t *perm = nullptr;
   int32_t *it_pix = nullptr, *it_len = nullptr, *it_slot = nullptr, *sp_pix = nullptr,
@@ -115,6 +119,7 @@ struct sanepic_ctx {
   bool xcd_blocks = true;            // XCD-contiguous numbering of the four-step blocks
   bool zsorted = false;              // the filter output z is written in pixel-sorted order
   int col_waves = 3;                 // occupancy target of the FP32 gather column pass (0, 3, 4)
+  int col_waves64 = 0;               // the same for the FP64 gather column pass
   uint32_t *rank = nullptr;          // [ntot] sorted position of each sample (zsorted)
   double *px64 = nullptr;
   float *px32 = nullptr;
@@ -362,6 +367,13 @@ int fetch_host_order(sanepic_ctx *c);
 // while the TOD upload may still run; then b = P^T N^-1 d from the TOD.
 int setup_pointing(sanepic_ctx *c);
 int setup_data(sanepic_ctx *c);
+// Stable sort of samples by key (< npix) into c->perm, and the P^T work
+// items over c->npix pixels from c->cnt / c->h_cnt / c->off (capi.hip).
+int sort_samples(sanepic_ctx *c, const uint32_t *keys, int64_t ntot, int64_t npix);
+int build_items(sanepic_ctx *c, int64_t ntot, bool nest_items);
+// NESTED order of the c->npix pixels listed (RING) in dlist: c->mpos,
+// c->d_slot2pix, c->h_slot2pix (nothing when nside is not a power of two)
+int pixel_nested_order(sanepic_ctx *c, const int32_t *dlist);
 // b = P^T relCalib N^-1 (d - relCalib dip) from the resident TOD (capi_recal.hip)
 int recal_rhs(sanepic_ctx *c);
 template <typename T> OpArgs<T> op_args(sanepic_ctx *c);
@@ -371,6 +383,8 @@ template <typename T> int run_filter(sanepic_ctx *c, OpArgs<T> a, int src, int m
 template <typename T> int run_scatter(sanepic_ctx *c, const T *z, double *q, int mask);
 // q = A x (interleaved device vectors); chi2 -> sc->chi2 when asked
 int apply_A(sanepic_ctx *c, const double *x, double *q, int mask, bool chi2, bool fp64);
+// its first half: z = N^-1 P x (c->z or c->z32), chi2 partials of *nb row blocks
+int apply_filter(sanepic_ctx *c, const double *x, bool chi2, bool use64, int64_t *nb);
 int check_ready(sanepic_ctx *c, bool need_setup);
 int to_inter(sanepic_ctx *c, const double *host_blk, double *dev_inter);
 int from_inter(sanepic_ctx *c, const double *dev_inter, double *host_blk);

@@ -17,6 +17,7 @@
 //   column pass  : inverse DFT over k2 gives z[n1 + N1 n2]
 // The gather of P runs inside the first column pass and the output of the
 // last one is written in time order for the pixel-sorted P^T reduction.
+#include <algorithm>
 #include <type_traits>
 
 #include "fft_core.h"
@@ -144,15 +145,20 @@ __global__ __launch_bounds__(256) void k_gather_px(OpArgs<T> a, T *px, int64_t n
 }
 
 // ---------------------------------------------------------------- column fwd
+// Each pass is a device body over logical block lb with the block's LDS
+// passed in (kLds elements: every transform length's exchange fits), run by
+// a thin kernel.
+constexpr int kLds = 4352;
+
 template <typename T, int LOG2N2, int SRC>
-__device__ __forceinline__ void col_fwd_body(const OpArgs<T> &a) {
+__device__ __forceinline__ void col_fwd_body(const OpArgs<T> &a, int64_t lb, LdsE<VT<T>> *lds) {
   using V = VT<T>;
   constexpr int N2 = 1 << LOG2N2, TS = N2 / 16, B = 256 / TS;
-  __shared__ LdsE<V> lds[LdsSize<LOG2N2, B>::value];
+  static_assert(LdsSize<LOG2N2, B>::value <= kLds, "LDS");
   const int tid = threadIdx.x, b = tid % B, t = tid / B;
   const int64_t N1 = a.g.N1;
   const int64_t ncb = N1 / B;
-  const int64_t lb = logical_block(a.xcd), pr = lb / ncb, cb = lb % ncb;
+  const int64_t pr = lb / ncb, cb = lb % ncb;
   int det;
   int64_t ga, gb;
   pair_info(a.g, pr, det, ga, gb);
@@ -177,14 +183,16 @@ __device__ __forceinline__ void col_fwd_body(const OpArgs<T> &a) {
 
 template <typename T, int LOG2N2, int SRC>
 __global__ __launch_bounds__(256) void k_col_fwd(OpArgs<T> a) {
-  col_fwd_body<T, LOG2N2, SRC>(a);
+  __shared__ LdsE<VT<T>> lds[kLds];
+  col_fwd_body<T, LOG2N2, SRC>(a, logical_block(a.xcd), lds);
 }
 
 // The same pass held to WAVES waves per SIMD (fewer registers per lane, more
 // gathers in flight per CU) for the FP32 map gather.
 template <typename T, int LOG2N2, int SRC, int WAVES>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_col_fwd_w(OpArgs<T> a) {
-  col_fwd_body<T, LOG2N2, SRC>(a);
+  __shared__ LdsE<VT<T>> lds[kLds];
+  col_fwd_body<T, LOG2N2, SRC>(a, logical_block(a.xcd), lds);
 }
 
 // ------------------------------------------------- mixed-radix column passes
@@ -228,14 +236,14 @@ __device__ __forceinline__ void dft_m(const V (&y)[M], V (&Y)[M]) {
 }
 
 template <typename T, int LOG2P, int M, int SRC>
-__global__ __launch_bounds__(256) void k_col_fwd_m(OpArgs<T> a) {
+__device__ __forceinline__ void col_fwd_m_body(const OpArgs<T> &a, int64_t lb, LdsE<VT<T>> *lds) {
   using V = VT<T>;
   constexpr int P = 1 << LOG2P, TS = P / 16, B = 256 / TS;
-  __shared__ LdsE<V> lds[LdsSize<LOG2P, B>::value];
+  static_assert(LdsSize<LOG2P, B>::value <= kLds, "LDS");
   const int tid = threadIdx.x, b = tid % B, t = tid / B;
   const int64_t N1 = a.g.N1;
   const int64_t ncb = N1 / B;
-  const int64_t lb = logical_block(a.xcd), pr = lb / ncb, cb = lb % ncb;
+  const int64_t pr = lb / ncb, cb = lb % ncb;
   int det;
   int64_t ga, gb;
   pair_info(a.g, pr, det, ga, gb);
@@ -286,15 +294,21 @@ __global__ __launch_bounds__(256) void k_col_fwd_m(OpArgs<T> a) {
   }
 }
 
+template <typename T, int LOG2P, int M, int SRC>
+__global__ __launch_bounds__(256) void k_col_fwd_m(OpArgs<T> a) {
+  __shared__ LdsE<VT<T>> lds[kLds];
+  col_fwd_m_body<T, LOG2P, M, SRC>(a, logical_block(a.xcd), lds);
+}
+
 template <typename T, int LOG2P, int M>
-__global__ __launch_bounds__(256) void k_col_inv_m(OpArgs<T> a) {
+__device__ __forceinline__ void col_inv_m_body(const OpArgs<T> &a, int64_t lb, LdsE<VT<T>> *lds) {
   using V = VT<T>;
   constexpr int P = 1 << LOG2P, TS = P / 16, B = 256 / TS;
-  __shared__ LdsE<V> lds[LdsSize<LOG2P, B>::value];
+  static_assert(LdsSize<LOG2P, B>::value <= kLds, "LDS");
   const int tid = threadIdx.x, b = tid % B, t = tid / B;
   const int64_t N1 = a.g.N1;
   const int64_t ncb = N1 / B;
-  const int64_t lb = logical_block(a.xcd), pr = lb / ncb, cb = lb % ncb;
+  const int64_t pr = lb / ncb, cb = lb % ncb;
   int det;
   int64_t ga, gb;
   pair_info(a.g, pr, det, ga, gb);
@@ -332,17 +346,22 @@ __global__ __launch_bounds__(256) void k_col_inv_m(OpArgs<T> a) {
   }
 }
 
+template <typename T, int LOG2P, int M>
+__global__ __launch_bounds__(256) void k_col_inv_m(OpArgs<T> a) {
+  __shared__ LdsE<VT<T>> lds[kLds];
+  col_inv_m_body<T, LOG2P, M>(a, logical_block(a.xcd), lds);
+}
+
 // ---------------------------------------------------------------------- row
 template <typename T, int LOG2N1>
-__global__ __launch_bounds__(256) void k_row(OpArgs<T> a) {
+__device__ __forceinline__ void row_body(const OpArgs<T> &a, int64_t lb, LdsE<VT<T>> *lds, double *red, int mode) {
   using V = VT<T>;
   constexpr int N1 = 1 << LOG2N1, TS = N1 / 16, B = 256 / TS;
-  __shared__ LdsE<V> lds[LdsSize<LOG2N1, B>::value];
-  __shared__ double red[4];
+  static_assert(LdsSize<LOG2N1, B>::value <= kLds, "LDS");
   const int tid = threadIdx.x, b = tid % B, t = tid / B;
   const int64_t N = a.g.N, N2 = a.g.N2;
   const int64_t nrb = N2 / B;
-  const int64_t lb = logical_block(a.xcd), pr = lb / nrb, rb = lb % nrb;
+  const int64_t pr = lb / nrb, rb = lb % nrb;
   const int64_t k2 = rb * B + b;
   int det;
   int64_t ga, gb;
@@ -350,7 +369,6 @@ __global__ __launch_bounds__(256) void k_row(OpArgs<T> a) {
   const int64_t base = pr * N + k2 * N1;
   const int64_t sbase = (pr + a.g.pair0) * N + k2 * N1;
   const T *w = a.wt + (int64_t)det * N + k2 * N1;
-  const int mode = a.mode;
   V v[16];
   if (mode & ROW_SRC_WT) {
 #pragma unroll
@@ -401,16 +419,23 @@ __global__ __launch_bounds__(256) void k_row(OpArgs<T> a) {
   }
 }
 
+template <typename T, int LOG2N1>
+__global__ __launch_bounds__(256) void k_row(OpArgs<T> a) {
+  __shared__ LdsE<VT<T>> lds[kLds];
+  __shared__ double red[4];
+  row_body<T, LOG2N1>(a, logical_block(a.xcd), lds, red, a.mode);
+}
+
 // ---------------------------------------------------------------- column inv
 template <typename T, int LOG2N2>
-__global__ __launch_bounds__(256) void k_col_inv(OpArgs<T> a) {
+__device__ __forceinline__ void col_inv_body(const OpArgs<T> &a, int64_t lb, LdsE<VT<T>> *lds) {
   using V = VT<T>;
   constexpr int N2 = 1 << LOG2N2, TS = N2 / 16, B = 256 / TS;
-  __shared__ LdsE<V> lds[LdsSize<LOG2N2, B>::value];
+  static_assert(LdsSize<LOG2N2, B>::value <= kLds, "LDS");
   const int tid = threadIdx.x, b = tid % B, t = tid / B;
   const int64_t N1 = a.g.N1;
   const int64_t ncb = N1 / B;
-  const int64_t lb = logical_block(a.xcd), pr = lb / ncb, cb = lb % ncb;
+  const int64_t pr = lb / ncb, cb = lb % ncb;
   int det;
   int64_t ga, gb;
   pair_info(a.g, pr, det, ga, gb);
@@ -427,6 +452,12 @@ __global__ __launch_bounds__(256) void k_col_inv(OpArgs<T> a) {
     a.z[zslot(a, ga + idx)] = v[s].x * sa;
     if (gb >= 0) a.z[zslot(a, gb + idx)] = v[s].y * sb;
   }
+}
+
+template <typename T, int LOG2N2>
+__global__ __launch_bounds__(256) void k_col_inv(OpArgs<T> a) {
+  __shared__ LdsE<VT<T>> lds[kLds];
+  col_inv_body<T, LOG2N2>(a, logical_block(a.xcd), lds);
 }
 
 // -------------------------------------------------------------- single pass
@@ -555,12 +586,10 @@ int launch_col_fwd(const OpArgs<T> &a, int src, hipStream_t s) {
   const int64_t nb = a.g.npairs * (a.g.N / 4096);
 #define CF(L)                                                                      \
   case L:                                                                          \
-    if constexpr (sizeof(T) == 4) {                                                \
-      if (src == SRC_GATHER && a.waves == 3)                                       \
-        return launch1(k_col_fwd_w<T, L, SRC_GATHER, 3>, nb, s, a);                \
-      if (src == SRC_GATHER && a.waves == 4)                                       \
-        return launch1(k_col_fwd_w<T, L, SRC_GATHER, 4>, nb, s, a);                \
-    }                                                                              \
+    if (src == SRC_GATHER && a.waves == 3)                                         \
+      return launch1(k_col_fwd_w<T, L, SRC_GATHER, 3>, nb, s, a);                  \
+    if (src == SRC_GATHER && a.waves == 4)                                         \
+      return launch1(k_col_fwd_w<T, L, SRC_GATHER, 4>, nb, s, a);                  \
     if (src == SRC_GATHER) return launch1(k_col_fwd<T, L, SRC_GATHER>, nb, s, a);  \
     if (src == SRC_TOD) return launch1(k_col_fwd<T, L, SRC_TOD>, nb, s, a);        \
     if (src == SRC_PX) return launch1(k_col_fwd<T, L, SRC_PX>, nb, s, a);          \

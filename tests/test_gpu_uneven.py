@@ -178,3 +178,28 @@ def test_uneven_layout_options_are_bit_identical(case, env, monkeypatch):
     for a, b in zip(out[:2], out[2:]):
         assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and a[2] == b[2]
         assert np.array_equal(a[3], b[3])
+
+
+def test_union_pt_matches_per_group_pt(case, monkeypatch):
+    """P^T once over the union of the groups' samples (the default) against
+    one P^T per group adding into the union's q in turn: the same sums in
+    another order, so A x and chi2 agree to rounding; b (built per group at
+    set-up either way) is bit-identical; both match the oracle."""
+    cfg, ds, probs, ring = case
+    out = {}
+    for v in ("1", "0"):
+        monkeypatch.setenv("SANEPIC_UNION_PT", v)
+        for prec in (sanepic_amd.FP64, sanepic_amd.MIXED):
+            ctx = _ctx(cfg, ds, precision=prec)
+            x = np.random.default_rng(12).standard_normal(3 * ctx.npix)
+            out[(v, prec)] = (ctx.apply(x), ctx.rhs(), ctx.chi2(x) if prec == sanepic_amd.FP64 else 0.0)
+            ctx.close()
+    ref = _sum_groups(probs, ring, lambda P, xg: P.apply_A(xg), np.random.default_rng(12).standard_normal(3 * ring.size))
+    for prec, tol in ((sanepic_amd.FP64, 1e-13), (sanepic_amd.MIXED, 1e-6)):
+        u, g = out[("1", prec)], out[("0", prec)]
+        e = rel(u[0], g[0])
+        print(f"  precision {prec}: union vs per-group A x {e:.2e}", flush=True)
+        assert e < tol and np.array_equal(u[1], g[1])
+    u, g = out[("1", sanepic_amd.FP64)], out[("0", sanepic_amd.FP64)]
+    assert abs(u[2] - g[2]) <= 1e-13 * abs(g[2])
+    assert rel(u[0], ref) < 1e-11
