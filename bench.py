@@ -156,17 +156,18 @@ def cpu_baseline(wl, budget_s=20.0, settle=5.0):
     return out
 
 
-def read_traffic(precision, kernel):
-    """HBM bytes per launch of a kernel class, measured with the hardware
-    byte counters (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, each scaled
-    by a device copy of known size) and recorded in profiles/*.md as lines
-    'traffic <precision> <kernel class>: <bytes> bytes per launch'."""
+def read_traffic(workload, precision, kernel):
+    """HBM bytes per launch of a kernel class on a workload, measured with
+    the hardware byte counters (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE,
+    each scaled by a device copy of known size) and recorded in
+    profiles/*.md as lines
+    'traffic <workload> <precision> <kernel class>: <bytes> bytes per launch'."""
     import glob
     import re
-    pat = re.compile(r"^traffic (\S+) (.+): (\d+) bytes per launch", re.M)
+    pat = re.compile(r"^traffic (\S+) (\S+) (.+): (\d+) bytes per launch", re.M)
     for fn in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
-        for prec, k, v in pat.findall(open(fn).read()):
-            if prec == precision and k.strip() == kernel:
+        for wl, prec, k, v in pat.findall(open(fn).read()):
+            if wl == workload and prec == precision and k.strip() == kernel:
                 return int(v), os.path.relpath(fn, ROOT)
     return None, None
 
@@ -396,7 +397,7 @@ def main():
         # noise weights, ~12 map-vector passes of the CG in FP64
         return ntot * (4 + 2 * s) + sum(L // 2 + 1 for L in lens) * s + 12 * 3 * npix * 8
 
-    traffic, traffic_src = read_traffic("fp64", CLASS_NAMES[dom])
+    traffic, traffic_src = read_traffic(args.workload, "fp64", CLASS_NAMES[dom])
     tts = head["time_to_solution_s"]
     out = {
         "metric": "time-to-1e-6 residual (s) + TOD samples/s per CG iter, NSIDE512 16det, 1 GPU",

@@ -120,6 +120,7 @@ struct sanepic_ctx {
   bool zsorted = false;              // the filter output z is written in pixel-sorted order
   int col_waves = 3;                 // occupancy target of the FP32 gather column pass (0, 3, 4)
   int col_waves64 = 0;               // the same for the FP64 gather column pass
+  bool row_det = true;               // FP64 row pass in detector-major row order (weights reused from L2)
   uint32_t *rank = nullptr;          // [ntot] sorted position of each sample (zsorted)
   double *px64 = nullptr;
   float *px32 = nullptr;

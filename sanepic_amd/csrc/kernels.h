@@ -114,6 +114,7 @@ struct OpArgs {
   const T *gain;        // [ndet][nseg] relative calibration of each segment, or null: it
                         // multiplies P x in the pre-gather pass and z in the inverse pass
   int64_t blk;          // samples per (detector, segment) block (the block of sample g is g / blk)
+  int rowdet;           // row pass: row k2 of all of a detector's pairs back to back (whole detectors only)
 };
 
 template <typename T> int launch_col_fwd(const OpArgs<T> &a, int src, hipStream_t s);

@@ -361,7 +361,17 @@ __device__ __forceinline__ void row_body(const OpArgs<T> &a, int64_t lb, LdsE<VT
   const int tid = threadIdx.x, b = tid % B, t = tid / B;
   const int64_t N = a.g.N, N2 = a.g.N2;
   const int64_t nrb = N2 / B;
-  const int64_t pr = lb / nrb, rb = lb % nrb;
+  int64_t pr, rb;
+  if (a.rowdet) {
+    // consecutive blocks take the same rows of one detector's pairs, so the
+    // detector's weight row is read from L2 by all but the first
+    const int64_t ppd = a.g.ppd, per = ppd * nrb, r = lb % per;
+    rb = r / ppd;
+    pr = (lb / per) * ppd + r % ppd;
+  } else {
+    pr = lb / nrb;
+    rb = lb % nrb;
+  }
   const int64_t k2 = rb * B + b;
   int det;
   int64_t ga, gb;
