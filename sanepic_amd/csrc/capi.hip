@@ -328,7 +328,8 @@ int apply_filter(sanepic_ctx *c, const double *x, bool chi2, bool use64, int64_t
     a.map = x;
     if (c->pix_nested || c->gsrc) {
       ACHK(c, dalloc(c, c->map64, c->nc * c->npix));
-      LCHK(c, launch_map_slots(x, c->gsrc, c->pix_nested ? c->mpos : nullptr, c->map64, c->npix, c->st, c->nc));
+      if (!c->map_filled)
+        LCHK(c, launch_map_slots(x, c->gsrc, c->pix_nested ? c->mpos : nullptr, c->map64, c->npix, c->st, c->nc));
       a.map = c->map64;
     }
     int src = SRC_GATHER;
@@ -341,9 +342,10 @@ int apply_filter(sanepic_ctx *c, const double *x, bool chi2, bool use64, int64_t
     }
     return run_filter<double>(c, a, src, mode, nb);
   }
-  if (c->pix_nested || c->gsrc)
-    LCHK(c, launch_map_slots(x, c->gsrc, c->pix_nested ? c->mpos : nullptr, c->map32, c->npix, c->st, c->nc));
-  else
+  if (c->pix_nested || c->gsrc) {
+    if (!c->map_filled)
+      LCHK(c, launch_map_slots(x, c->gsrc, c->pix_nested ? c->mpos : nullptr, c->map32, c->npix, c->st, c->nc));
+  } else
     LCHK(c, launch_d2f(x, c->map32, c->nc * c->npix, c->st));
   OpArgs<float> a = op_args<float>(c);
   a.map = c->map32;

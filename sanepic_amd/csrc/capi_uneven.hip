@@ -27,6 +27,17 @@
 
 using namespace sctx;
 
+namespace {
+
+// slot[i] = (mpos ? mpos[i] : i) + off: a group's gather-map slots in the
+// union's buffer of all groups' maps
+__global__ void k_slot_offset(int32_t *slot, const int32_t *mpos, int64_t n, int64_t off) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) slot[i] = (int32_t)((mpos ? mpos[i] : i) + off);
+}
+
+}  // namespace
+
 namespace sctx {
 
 // P^T of the operator over the union of the groups' samples: group k's
@@ -92,10 +103,17 @@ int build_union_pt(sanepic_ctx *c, const int32_t *dlist) {
   c->h_slot2pix.clear();
   ACHK(c, dalloc(c, c->z, ntot));
   if (c->cfg.precision == SANEPIC_MIXED) ACHK(c, dalloc(c, c->z32, ntot));
-  for (int64_t k = 0; k < ng; k++) {
+  c->moff.assign((size_t)ng + 1, 0);
+  for (int64_t k = 0; k < ng; k++) c->moff[(size_t)k + 1] = c->moff[(size_t)k] + c->sub[(size_t)k]->npix;
+  const int64_t nmap = c->moff[(size_t)ng];
+  ACHK(c, dalloc(c, c->map_all64, 3 * nmap));
+  if (c->cfg.precision == SANEPIC_MIXED) ACHK(c, dalloc(c, c->map_all32, 3 * nmap));
+  for (int64_t k = 0; k < ng; k++) {  // slices, not the groups' allocations: they never free them
     sanepic_ctx *s = c->sub[(size_t)k];
-    s->z = c->z + c->gbase[(size_t)k];  // not the group's allocation: it never frees it
+    s->z = c->z + c->gbase[(size_t)k];
     if (c->z32) s->z32 = c->z32 + c->gbase[(size_t)k];
+    s->map64 = c->map_all64 + 3 * c->moff[(size_t)k];
+    if (c->map_all32) s->map32 = c->map_all32 + 3 * c->moff[(size_t)k];
   }
   HIPCHK(c, hipStreamSynchronize(c->st));
   trace_mark("group pixelize: union P^T");
@@ -110,12 +128,19 @@ int group_apply_A(sanepic_ctx *c, const double *x, double *q, int mask, bool chi
     // every pixel of q
     const bool use64 = fp64 || c->cfg.precision == SANEPIC_FP64;
     if (chi2 && !use64) return fail(c, SANEPIC_EINVAL, "chi2 needs the FP64 operator");
+    // every group's gather map in one launch
+    if (use64)
+      LCHK(c, launch_map_slots(x, c->map_src, c->map_slot, c->map_all64, c->moff.back(), c->st, 3));
+    else
+      LCHK(c, launch_map_slots(x, c->map_src, c->map_slot, c->map_all32, c->moff.back(), c->st, 3));
     for (size_t k = 0; k < c->sub.size(); k++) {
       sanepic_ctx *s = c->sub[k];
       s->gsrc = c->gmap[k];
+      s->map_filled = true;
       int64_t nb = 0;
       const int rc = apply_filter(s, x, chi2, use64, &nb);
       s->gsrc = nullptr;
+      s->map_filled = false;
       if (rc) return fail(c, rc, s->err);
       if (chi2) {
         LCHK(c, launch_reduce_chi2(s->chi2part, nb, s->sc, c->st));
@@ -289,6 +314,23 @@ int group_setup(sanepic_ctx *c) {
   }
   HIPCHK(c, hipMemcpyAsync(c->b, c->b_raw, 3 * npix * sizeof(double), hipMemcpyDeviceToDevice, c->st));
   LCHK(c, cg_launch_mask(c->b, c->cnt, npix, c->st));
+  if (c->union_pt) {  // the one-launch fill of every group's gather map: x rows and map slots
+    const int64_t nmap = c->moff.back();
+    ACHK(c, dalloc(c, c->map_src, nmap));
+    ACHK(c, dalloc(c, c->map_slot, nmap));
+    for (size_t k = 0; k < c->sub.size(); k++) {
+      sanepic_ctx *s = c->sub[k];
+      if (s->map64 != c->map_all64 + 3 * c->moff[k] ||
+          (c->map_all32 && s->map32 != c->map_all32 + 3 * c->moff[k]))
+        return fail(c, SANEPIC_EHIP, "group gather map is not a slice of the union's");
+      LCHK(c, launch_copy_i32(c->gmap[k], c->map_src + c->moff[k], s->npix, c->st));
+      if (s->npix > 0)
+        hipLaunchKernelGGL(k_slot_offset, dim3((unsigned)((s->npix + 255) / 256)), dim3(256), 0, c->st,
+                           c->map_slot + c->moff[k], s->pix_nested ? (const int32_t *)s->mpos : nullptr, s->npix,
+                           c->moff[k]);
+      HIPCHK(c, hipGetLastError());
+    }
+  }
   ACHK(c, dalloc(c, c->x, 3 * npix));
   ACHK(c, dalloc(c, c->r, 3 * npix));
   ACHK(c, dalloc(c, c->d, 3 * npix));

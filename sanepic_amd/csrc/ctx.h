@@ -70,6 +70,13 @@ struct sanepic_ctx {
   // (gbase[k]: group k's first sample; its z is the slice of z / z32 there)
   bool union_pt = false;
   std::vector<int64_t> gbase;
+  // ... and one gather-map fill: the groups' maps are slices of one buffer
+  // (group k's at moff[k] pixels), filled from x through map_src / map_slot
+  std::vector<int64_t> moff;
+  double *map_all64 = nullptr;
+  float *map_all32 = nullptr;
+  int32_t *map_src = nullptr, *map_slot = nullptr;
+  bool map_filled = false;           // a group: its map is already filled for this application
   int64_t *off = nullptr;
   uint32_t *perm = nullptr;
   int32_t *it_pix = nullptr, *it_len = nullptr, *it_slot = nullptr, *sp_pix = nullptr,
