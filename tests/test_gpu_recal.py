@@ -162,3 +162,38 @@ def test_drop_in_recalibration_loop(tmp_path, global_gain):
     v00 = [float(v) for v in re.findall(r"^var00 = (\S+)", r.stdout, re.M)]
     assert len(v0) == npass and len(set(v00)) == 1
     assert v0[0] == v00[0] and v0[1] < 1e-3 * v00[0]
+
+
+def test_recalibration_error_paths():
+    """Gains that are zero or not finite, fits without the resident TOD,
+    a negative crit denominator and per-detector lengths are refused with an
+    error, not a wrong map."""
+    cfg, ds = make_case(nside=8, ndet=2, nseg=2, ns=1024, dipole=True)
+    ctx = sanepic_amd.Context(cfg.nside, cfg.ndet, cfg.nseg, cfg.ns)
+    ctx.load(cfg, ds)
+    ctx.pixelize()
+    ctx.setup()
+    x = np.zeros(3 * ctx.npix)
+    with pytest.raises(sanepic_amd.SanepicError, match="relative calibration is off"):
+        ctx.recal_sums(x)
+    with pytest.raises(sanepic_amd.SanepicError, match="relative calibration is off"):
+        ctx.set_gains(np.ones((cfg.ndet, cfg.nseg)))
+    with pytest.raises(sanepic_amd.SanepicError, match="before sanepic_setup"):
+        ctx.set_recalib(True)
+    with pytest.raises(sanepic_amd.SanepicError, match="var00"):
+        ctx.solve_begin(sanepic_amd.PRECOND_DIAG, 10, var00=-1.0)
+    ctx.close()
+    cfg, ds, P, ctx = _pair(1024, nseg=2)
+    b0 = ctx.rhs()
+    for bad in (0.0, np.nan, np.inf):
+        g = np.ones((cfg.ndet, cfg.nseg))
+        g[1, 0] = bad
+        with pytest.raises(sanepic_amd.SanepicError, match="zero or not finite"):
+            ctx.set_gains(g)
+    assert np.array_equal(ctx.rhs(), b0)  # a refused call leaves b as it was
+    ctx.close()
+    u = synth.UnevenConfig(8, [1024, 2048], 4096, fknee=[0.2, 0.2], alpha=[1.0, 1.0])
+    cu = sanepic_amd.Context.uneven(u.nside, u.seglen, u.nsamp)
+    with pytest.raises(sanepic_amd.SanepicError, match="one segment length"):
+        cu.set_recalib(True)
+    cu.close()
