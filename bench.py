@@ -8,10 +8,13 @@ the form a caller hands to the library.
 
 Complete solves are timed, each from inputs in pinned host memory to a true
 relative residual |b - A x| / |b| <= 1e-6 (H2D, pointing, ang2pix, sort, b,
-preconditioner, CG; file I/O excluded):
+preconditioner, CG; file I/O excluded), after one small untimed solve that
+loads the library's code onto the device:
   headline   FP64 operator, plain 3x3 block-Jacobi PCG (the reference's
              arithmetic); after its time to solution it iterates on, untimed,
-             to |r|^2/|b|^2 <= 1e-15 for the reference map of the error figures
+             to |r|^2/|b|^2 <= 1e-24 (a true residual near 1e-13) for the
+             reference map of the error figures: at the reference's own stop
+             (1e-15, a true residual near 3e-8) Q/U still carry ~5e-6 of error
   learned    FP32 operator with FP64 CG scalars, reductions, 3x3 inverses and
              FP64 residual refresh; flexible PCG with the learned U-Net
              preconditioner (trained offline by sanepic_amd.train_precond on
@@ -172,6 +175,22 @@ def read_traffic(workload, precision, kernel):
     return None, None
 
 
+def warm_runtime():
+    """One small untimed solve, so that loading the library's code onto the
+    device (the first kernel launch of a process) is in no pipeline's time."""
+    import sanepic_amd
+    from sanepic_amd import synth
+    cfg = synth.Config(nside=8, ndet=1, nseg=2, ns=1024)
+    ds = synth.make_dataset(cfg, workers=1)
+    for prec in (sanepic_amd.FP64, sanepic_amd.MIXED):
+        with sanepic_amd.Context(cfg.nside, cfg.ndet, cfg.nseg, cfg.ns, precision=prec) as ctx:
+            ctx.load(cfg, ds)
+            ctx.pixelize()
+            ctx.setup()
+            ctx.solve(precond=sanepic_amd.PRECOND_BLOCK3, maxiter=3)
+    log("runtime warmed up (small untimed solve)")
+
+
 def map_errors(x, ref, cnt):
     """Relative RMS error of each component of map x (block layout) against
     ref over the seen pixels; Q and U over the pixels they are solved for
@@ -192,12 +211,13 @@ def map_errors(x, ref, cnt):
 
 def run_solve(cfg, pin, precision, precond, deflate, weights, steps, warmup, maxiter, profile, net_nside=None,
               refresh_fp32=False, refresh=10, fallback_growth=0.0, keep_map=False, reference=False,
-              track=None):
+              track=None, track_self=True):
     """One timed pipeline: create -> uploads -> pixelize -> setup -> solve to a
     true residual <= 1e-6.  Returns timings, per-iteration time and kernel
     times of `steps` iterations measured after `warmup`; with keep_map the
     map at the time of solution, with reference also the map after iterating
-    on (untimed) to |r|^2/|b|^2 <= 1e-15.  With track = (reference map,
+    on (untimed) to |r|^2/|b|^2 <= 1e-24 (and that map tracked as below).
+    With track = (reference map,
     counts), an untimed second solve reads the map back after every
     iteration and records the first iteration whose I, Q and U are all
     within 1e-6 relative RMS of the reference (map_errors)."""
@@ -242,12 +262,15 @@ def run_solve(cfg, pin, precision, precond, deflate, weights, steps, warmup, max
     lstats = ctx.learn_stats() if un is not None else None
     x_sol = ctx.map() if keep_map or reference else None
     x_ref, ref_iters, ref_res = None, None, None
-    if reference:  # untimed: on to the reference's own stop rule (1037, 1415)
+    if reference:  # untimed: on well past the reference's own stop rule (1e-15, 1037, 1415)
         n, crit = 0, 1.0
-        while n < 400 and crit > 1e-15:
+        while n < 300 and crit > 1e-24:
             n += 1
             crit = ctx.step()["crit"]
         x_ref, ref_iters, ref_res = ctx.map(), len(hist) + n, ctx.true_residual()
+        cnt_ref = ctx.get_setup()[0][:ctx.npix] // 2
+        if track is None and track_self:
+            track = (x_ref, cnt_ref)
     map_its, map_err = None, None
     if track is not None:
         ctx.solve_begin(pc, refresh, None, deflate, fallback_growth=fallback_growth, refresh_fp32=refresh_fp32)
@@ -283,8 +306,7 @@ def run_solve(cfg, pin, precision, precond, deflate, weights, steps, warmup, max
     out["map"] = x_sol
     out["map_error_iterations"], out["map_error_last"] = map_its, map_err
     if reference:
-        out.update(ref_map=x_ref, ref_iterations=ref_iters, ref_true_residual=ref_res,
-                   cnt=ctx.get_setup()[0][:npix] // 2)
+        out.update(ref_map=x_ref, ref_iterations=ref_iters, ref_true_residual=ref_res, cnt=cnt_ref)
     log(f"[{precision} {precond}{' +defl' if deflate else ''}] upload {out['upload_s']:.2f} s, "
         f"pixelize {out['pixelize_s']:.2f} s, setup {out['setup_s']:.2f} s, begin {out['begin_s']:.2f} s; "
         f"{its} iterations, true residual {true_res:.3e}, time to solution "
@@ -333,6 +355,7 @@ def main():
 
     if args.calib_copy:
         calibration_copy()
+    warm_runtime()
     uneven = args.workload in synth.UNEVEN_WORKLOADS
     wl = synth.UNEVEN_WORKLOADS[args.workload] if uneven else synth.WORKLOADS[args.workload]
     cfg = synth.workload_config(args.workload)
@@ -354,7 +377,7 @@ def main():
 
     # the headline: FP64 block-Jacobi, the reference's arithmetic
     head = run_solve(cfg, pin, "fp64", "block", False, None, args.steps, args.warmup, args.maxiter,
-                     profile=True, reference=True)
+                     profile=True, reference=True, track_self=not args.no_map_track)
     ref_map, cnt = head["ref_map"], head["cnt"]
     runs = {}
     if not args.no_learned:
@@ -433,10 +456,18 @@ def main():
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
                      "traffic": traffic, "traffic_source": traffic_src, "bytes_per_launch": alg,
                      "ms_per_launch": avg_ms, "launches_per_operator": cnt_launch // n_ops},
-        "reference_map": {"solver": "the FP64 solve continued to |r|^2/|b|^2 <= 1e-15",
+        "reference_map": {"solver": "the FP64 solve continued to |r|^2/|b|^2 <= 1e-24",
                           "iterations": head["ref_iterations"], "true_residual": head["ref_true_residual"]},
         "map_error_at_1e-6": map_errors(head["map"], ref_map, cnt),
     }
+    head_ms_it = head["solve_s"] * 1e3 / max(head["iterations"], 1)
+    head_setup = head["upload_s"] + head["pixelize_s"] + head["setup_s"] + head["begin_s"]
+    t64_maps = None
+    if head["map_error_iterations"] is not None or head["map_error_last"] is not None:
+        k = head["map_error_iterations"]
+        t64_maps = (head_setup + k * head_ms_it * 1e-3) if k else None
+        out["maps_within_1e-6_of_fp64_reference"] = {
+            "iterations": k, "last_error": head["map_error_last"], "time_s_estimate": t64_maps}
     for key, r in runs.items():
         sub = {"solver": ("mixed operator, " + ("flexible PCG with the trained network" if r["precond"] == "learned"
                                                 else "block-Jacobi PCG")
@@ -454,10 +485,12 @@ def main():
                "fp64_refresh_every": rf}
         if r["map_error_iterations"] is not None or r["map_error_last"] is not None:
             k = r["map_error_iterations"]
+            tm = (sub["setup_s"] + k * sub["ms_per_iteration_in_solve"] * 1e-3) if k else None
             sub["maps_within_1e-6_of_fp64_reference"] = {
                 "iterations": k, "last_error": r["map_error_last"],
                 # setup + k iterations at the solve's own time per iteration
-                "time_s_estimate": (sub["setup_s"] + k * sub["ms_per_iteration_in_solve"] * 1e-3) if k else None}
+                "time_s_estimate": tm,
+                "speedup_vs_fp64": (t64_maps / tm) if (t64_maps and tm) else None}
         if "learned_stats" in r:
             sub["stats"] = r["learned_stats"]
         if key == "learned":
