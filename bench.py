@@ -52,22 +52,30 @@ sys.path.insert(0, ROOT)
 HBM_PEAK = 8.0e12
 CLASS_NAMES = ["gather+column FFT", "row FFT + noise filter", "inverse column FFT",
                "pixel-sorted P^T reduction"]
+# the same classes when P^T runs over run records (the inverse pass sums runs)
+CLASS_NAMES_RUNS = ["gather+column FFT", "row FFT + noise filter", "inverse column FFT + run sums",
+                    "P^T over run records"]
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def class_bytes_per_sample(precision):
+def class_bytes_per_sample(precision, rps=0.0):
     """Algorithmic HBM bytes per TOD sample of each operator kernel class
     (reads + writes of its streams; the map gathers hit the caches).
     FP64: pixel int32, (cos, sin) 2psi as two doubles, complex FP64 pair
     buffer (8 bytes per sample), FP64 z.  FP32: pixel int32, 2psi as a 32-bit
-    fraction of a turn, complex FP32 pair buffer (4 bytes per sample), FP32 z."""
+    fraction of a turn, complex FP32 pair buffer (4 bytes per sample), FP32 z.
+    rps > 0: P^T over run records, rps records per sample; the inverse pass
+    reads the angles and one 8-byte word per 16-sample piece and writes three
+    sums per record, P^T reads a 4-byte index and the record."""
     fp64 = precision == "fp64"
     ang = 16 if fp64 else 4
     w = 8 if fp64 else 4
     z = 8 if fp64 else 4
+    if rps:
+        return [4 + ang + w, 2 * w, w + ang + 8 / 16 + 3 * z * rps, (4 + 3 * z) * rps]
     return [4 + ang + w,   # gather + column FFT: pixel, angle in; pair buffer out
             2 * w,         # row FFT + filter: pair buffer in and out
             w + z,         # inverse column FFT: pair buffer in, z out
@@ -300,7 +308,8 @@ def run_solve(cfg, pin, precision, precond, deflate, weights, steps, warmup, max
                begin_s=t_begin - t_setup, solve_s=t_end - t_begin, iterations=its, converged=converged,
                final_true_residual=true_res,
                time_to_solution_s=(t_end - t0) if converged else None,
-               device_bytes=ctx.device_bytes(), ms_per_step=ms_per_step, kernel_times=kt)
+               device_bytes=ctx.device_bytes(), ms_per_step=ms_per_step, kernel_times=kt,
+               run_records=ctx.run_records())
     if un is not None:
         out["learned_stats"] = lstats
     out["map"] = x_sol
@@ -404,12 +413,16 @@ def main():
     # launches each kernel class once per detector group
     n_ops = max(kt[4][1], 1)
     avg_ms = tot_ms / n_ops
-    cbs = class_bytes_per_sample("fp64")
+    nrec, runs_on = head["run_records"]
+    rps = nrec / ntot if runs_on else 0.0
+    names = CLASS_NAMES_RUNS if runs_on else CLASS_NAMES
+    cbs = class_bytes_per_sample("fp64", rps)
     alg = ntot * cbs[dom]  # algorithmic bytes of the dominant kernel class per operator application
     achieved = alg / (avg_ms * 1e-3)
     for k in range(4):
         if kt[k][1]:
-            log(f"  {CLASS_NAMES[k]}: {kt[k][0] / n_ops:.3f} ms per operator ({kt[k][1]} launches)")
+            log(f"  {names[k]}: {kt[k][0] / n_ops:.3f} ms per operator ({kt[k][1]} launches, "
+                f"{ntot * cbs[k] / (kt[k][0] / n_ops * 1e-3) / 1e12:.2f} TB/s of algorithmic bytes)")
     apply_ms = kt[4][0] / max(kt[4][1], 1)
     lens = cfg.seglen if uneven else [cfg.ns] * cfg.ndet
     npix = head["npix"]
@@ -420,7 +433,7 @@ def main():
         # noise weights, ~12 map-vector passes of the CG in FP64
         return ntot * (4 + 2 * s) + sum(L // 2 + 1 for L in lens) * s + 12 * 3 * npix * 8
 
-    traffic, traffic_src = read_traffic(args.workload, "fp64", CLASS_NAMES[dom])
+    traffic, traffic_src = read_traffic(args.workload, "fp64", names[dom])
     tts = head["time_to_solution_s"]
     out = {
         "metric": "time-to-1e-6 residual (s) + TOD samples/s per CG iter, NSIDE512 16det, 1 GPU",
@@ -452,7 +465,8 @@ def main():
         "operator_alg_bytes": b_alg(8),
         "operator_frac_hbm": b_alg(8) / (apply_ms * 1e-3) / HBM_PEAK,
         "device_bytes": head["device_bytes"],
-        "roofline": {"bound": "hbm", "kernel": CLASS_NAMES[dom], "achieved": achieved / 1e9,
+        "run_records": {"records": nrec, "per_sample": nrec / ntot, "used": runs_on},
+        "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": achieved / 1e9,
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
                      "traffic": traffic, "traffic_source": traffic_src, "bytes_per_launch": alg,
                      "ms_per_launch": avg_ms, "launches_per_operator": cnt_launch // n_ops},

@@ -314,6 +314,14 @@ int sanepic_kernel_time(sanepic_ctx *ctx, int32_t which, double *ms,
                         int64_t *count);
 /* Device bytes held by the context. */
 int sanepic_device_bytes(sanepic_ctx *ctx, int64_t *bytes);
+/* The run layout of P^T (no counterpart in the reference, whose do_PtNd
+ * scatters sample by sample, sanepic.cc:2584-2611): *nrec = runs of
+ * consecutive samples in one pixel (aligned 16-sample pieces split them),
+ * *active = 1 when the operator's inverse column pass sums each run into one
+ * record and P^T sums records; 0 when it writes z and P^T sums samples (the
+ * layout is off, or the current segment length / FFT split cannot use it).
+ * SANEPIC_RUNS=0 at creation turns the layout off. */
+int sanepic_run_records(sanepic_ctx *ctx, int64_t *nrec, int32_t *active);
 
 /* ------------------------------------------------------------------------
  * Learned preconditioner.  None of this has a counterpart in the reference:

@@ -31,6 +31,7 @@ ABI_FUNCTIONS = [
     "sanepic_solve_step", "sanepic_get_map", "sanepic_get_nvar", "sanepic_get_sumcross",
     "sanepic_true_residual", "sanepic_set_profiling", "sanepic_kernel_time",
     "sanepic_device_bytes",
+    "sanepic_run_records",
     "sanepic_unet_create", "sanepic_unet_destroy", "sanepic_unet_last_error",
     "sanepic_unet_nparams", "sanepic_unet_init", "sanepic_unet_get_params",
     "sanepic_unet_set_params", "sanepic_unet_forward", "sanepic_unet_backward",
@@ -134,6 +135,7 @@ def lib():
             "sanepic_set_profiling": [P, ctypes.c_int32],
             "sanepic_kernel_time": [P, ctypes.c_int32, d_p, i64p],
             "sanepic_device_bytes": [P, i64p],
+            "sanepic_run_records": [P, i64p, ctypes.POINTER(ctypes.c_int32)],
             "sanepic_unet_create": [i64, ctypes.c_int32, ctypes.POINTER(P)],
             "sanepic_unet_destroy": [P],
             "sanepic_unet_nparams": [P, i64p],
@@ -621,6 +623,13 @@ class Context:
         b = ctypes.c_int64()
         self._chk(lib().sanepic_device_bytes(self.h, ctypes.byref(b)), "device_bytes")
         return b.value
+
+    def run_records(self):
+        """(number of run records, whether the operator uses them)."""
+        n = ctypes.c_int64()
+        on = ctypes.c_int32()
+        self._chk(lib().sanepic_run_records(self.h, ctypes.byref(n), ctypes.byref(on)), "run_records")
+        return n.value, bool(on.value)
 
     # ------------------------------------------------ learned preconditioner
     def learn_attach(self, unet):

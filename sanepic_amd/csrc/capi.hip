@@ -289,10 +289,11 @@ int run_filter(sanepic_ctx *c, OpArgs<T> a, int src, int mode, int64_t *nchi) {
 }
 
 template <typename T>
-int run_scatter(sanepic_ctx *c, const T *z, double *q, int mask) {
+int run_scatter(sanepic_ctx *c, const T *z, double *q, int mask, const T *rec) {
   ScatterArgs<T> s{};
-  s.it = c->items;
-  s.perm = c->perm;
+  s.it = rec ? c->ritems.items : c->items;
+  s.perm = rec ? c->rperm : c->perm;
+  s.rec = rec;
   if constexpr (sizeof(T) == 8) {
     s.cs_sorted = c->cs_sorted;
     s.relf = c->d_relf64;
@@ -307,12 +308,12 @@ int run_scatter(sanepic_ctx *c, const T *z, double *q, int mask) {
   // measured: contiguous item ranges per XCD make P^T slower (RING items
   // 4.49 -> 5.16 ms, NESTED items 3.50 -> 4.43 ms)
   s.xcd = 0;
-  s.lanes = c->scatter_lanes;
+  s.lanes = rec ? c->ritems.lanes : c->scatter_lanes;
   s.cnt = c->cnt;
   s.q = q;
   s.qidx = c->gsrc;
   s.accumulate = c->gsrc ? 1 : 0;
-  s.partial = c->partial;
+  s.partial = rec ? c->ritems.partial : c->partial;
   s.mask = mask;
   ProfScope ps(c, 3);
   LCHK(c, launch_scatter<T>(s, c->st));
@@ -321,11 +322,16 @@ int run_scatter(sanepic_ctx *c, const T *z, double *q, int mask) {
 
 // z = N^-1 P x of this context in the precision of its operator (c->z or
 // c->z32), with the chi2 partials of its row blocks when asked (*nb blocks).
-int apply_filter(sanepic_ctx *c, const double *x, bool chi2, bool use64, int64_t *nb) {
+int apply_filter(sanepic_ctx *c, const double *x, bool chi2, bool use64, int64_t *nb, bool runs) {
   const int mode = ROW_FWD | ROW_FILTER | ROW_INV | (chi2 ? ROW_CHI2 : 0);
   if (use64) {
     OpArgs<double> a = op_args<double>(c);
     a.map = x;
+    if (runs) {
+      a.rec = c->rec64;
+      a.rmeta = c->rmeta;
+      a.recv = c->run_loads64;
+    }
     if (c->pix_nested || c->gsrc) {
       ACHK(c, dalloc(c, c->map64, c->nc * c->npix));
       if (!c->map_filled)
@@ -349,6 +355,11 @@ int apply_filter(sanepic_ctx *c, const double *x, bool chi2, bool use64, int64_t
     LCHK(c, launch_d2f(x, c->map32, c->nc * c->npix, c->st));
   OpArgs<float> a = op_args<float>(c);
   a.map = c->map32;
+  if (runs) {
+    a.rec = c->rec32;
+    a.rmeta = c->rmeta;
+    a.recv = c->run_loads32;
+  }
   int src = SRC_GATHER;
   if (c->pregather) {
     ACHK(c, dalloc(c, c->px32, c->ntot));
@@ -367,11 +378,14 @@ int apply_A(sanepic_ctx *c, const double *x, double *q, int mask, bool chi2, boo
   const bool use64 = fp64 || c->cfg.precision == SANEPIC_FP64;
   if (chi2 && !use64) return fail(c, SANEPIC_EINVAL, "chi2 needs the FP64 operator");
   int64_t nb = 0;
-  ACHK(c, apply_filter(c, x, chi2, use64, &nb));
+  // P^T over run records where the layout exists (the inverse pass writes
+  // them instead of z)
+  const bool runs = use_runs(c) && (use64 ? c->rec64 != nullptr : c->rec32 != nullptr);
+  ACHK(c, apply_filter(c, x, chi2, use64, &nb, runs));
   if (use64)
-    ACHK(c, run_scatter<double>(c, c->z, q, mask));
+    ACHK(c, run_scatter<double>(c, c->z, q, mask, runs ? c->rec64 : nullptr));
   else
-    ACHK(c, run_scatter<float>(c, c->z32, q, mask));
+    ACHK(c, run_scatter<float>(c, c->z32, q, mask, runs ? c->rec32 : nullptr));
   if (chi2) LCHK(c, launch_reduce_chi2(c->chi2part, nb, c->sc, c->st));
   return 0;
 }
@@ -428,8 +442,8 @@ CGArgs cg_args(sanepic_ctx *c) {
 
 template int run_filter<double>(sanepic_ctx *, OpArgs<double>, int, int, int64_t *);
 template int run_filter<float>(sanepic_ctx *, OpArgs<float>, int, int, int64_t *);
-template int run_scatter<double>(sanepic_ctx *, const double *, double *, int);
-template int run_scatter<float>(sanepic_ctx *, const float *, double *, int);
+template int run_scatter<double>(sanepic_ctx *, const double *, double *, int, const double *);
+template int run_scatter<float>(sanepic_ctx *, const float *, double *, int, const float *);
 
 // The groups of a per-detector-length context run one after another on the
 // host and share their parent's staging buffer (one allocation instead of one
@@ -621,6 +635,9 @@ int sanepic_create(const sanepic_config *cfg, sanepic_ctx **out) {
   if (const char *e = getenv("SANEPIC_PREGATHER")) c->pregather = atoi(e) != 0;
   if (const char *e = getenv("SANEPIC_XCD_BLOCKS")) c->xcd_blocks = atoi(e) != 0;
   if (const char *e = getenv("SANEPIC_ZSORT")) c->zsorted = atoi(e) != 0;
+  if (const char *e = getenv("SANEPIC_RUNS")) c->runs = atoi(e) != 0;
+  if (const char *e = getenv("SANEPIC_RUN_LOADS32")) c->run_loads32 = std::min(4, std::max(0, atoi(e)));
+  if (const char *e = getenv("SANEPIC_RUN_LOADS64")) c->run_loads64 = std::min(4, std::max(0, atoi(e)));
   if (const char *e = getenv("SANEPIC_COL_WAVES")) c->col_waves = atoi(e);
   if (const char *e = getenv("SANEPIC_COL_WAVES64")) c->col_waves64 = atoi(e);
   if (const char *e = getenv("SANEPIC_ROW_ORDER")) c->row_det = atoi(e) != 0;
@@ -1090,6 +1107,7 @@ int sanepic_pixelize(sanepic_ctx *c, int64_t *npix_out, int64_t *n_deferred) {
   }
   trace_mark("pixelize: sorted angles");
   ACHK(c, build_items(c, ntot, c->items_nested && c->d_slot2pix != nullptr));
+  ACHK(c, build_runs(c));
   if (!c->pin_owner) dfree_later(c, c->d_slot2pix, npix);  // a group keeps it for fetch_host_order
   HIPCHK(c, hipStreamSynchronize(c->st));
   if (c->pin_owner) flush_frees(c);  // a group: its temporaries serve the next group
@@ -1251,52 +1269,58 @@ int nest_gather_map(sanepic_ctx *c) {
 extern "C++" {
 namespace sctx {
 
-// Stable sort of ntot samples by key (compressed pixel, < npix) into c->perm:
-// sorted position -> sample index.
-int sort_samples(sanepic_ctx *c, const uint32_t *keys, int64_t ntot, int64_t npix) {
-  uint32_t *keys_out = nullptr;
+int sort_keys(sanepic_ctx *c, const uint32_t *keys, int64_t n, int64_t nkey, uint32_t *&perm,
+              uint32_t **keys_out) {
+  uint32_t *kout = nullptr;
   uint32_t *vals_in = nullptr;
-  ACHK(c, dalloc(c, keys_out, ntot));
-  ACHK(c, dalloc(c, vals_in, ntot));
-  ACHK(c, dalloc(c, c->perm, ntot));
-  LCHK(c, launch_iota_u32(vals_in, ntot, c->st));
-  const int end_bit = std::max(1, ilog2(npix + 1));
+  ACHK(c, dalloc(c, kout, n));
+  ACHK(c, dalloc(c, vals_in, n));
+  ACHK(c, dalloc(c, perm, n));
+  LCHK(c, launch_iota_u32(vals_in, n, c->st));
+  const int end_bit = std::max(1, ilog2(nkey + 1));
   size_t tbytes = 0;
-  HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(nullptr, tbytes, keys, keys_out, vals_in, c->perm,
-                                               (int)ntot, 0, end_bit, c->st));
+  HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(nullptr, tbytes, keys, kout, vals_in, perm, (int)n, 0, end_bit,
+                                               c->st));
   char *tmp = nullptr;
   ACHK(c, dalloc(c, tmp, (int64_t)tbytes));
-  HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(tmp, tbytes, keys, keys_out, vals_in, c->perm, (int)ntot,
-                                               0, end_bit, c->st));
+  HIPCHK(c, hipcub::DeviceRadixSort::SortPairs(tmp, tbytes, keys, kout, vals_in, perm, (int)n, 0, end_bit,
+                                               c->st));
   HIPCHK(c, hipStreamSynchronize(c->st));
   dfree_later(c, tmp, (int64_t)tbytes);
-  dfree_later(c, keys_out, ntot);
-  dfree_later(c, vals_in, ntot);
+  dfree_later(c, vals_in, n);
+  if (keys_out)
+    *keys_out = kout;
+  else
+    dfree_later(c, kout, n);
   return 0;
 }
 
-// P^T work items over the c->npix pixels (counts c->cnt and c->h_cnt, sorted
-// offsets c->off) of ntot sorted samples: c->items, c->nslots, c->partial
-// and the lanes per item.  With nest_items the items follow c->d_slot2pix
-// (NESTED order).
-int build_items(sanepic_ctx *c, int64_t ntot, bool nest_items) {
+// Stable sort of ntot samples by key (compressed pixel, < npix) into c->perm:
+// sorted position -> sample index.
+int sort_samples(sanepic_ctx *c, const uint32_t *keys, int64_t ntot, int64_t npix) {
+  return sort_keys(c, keys, ntot, npix, c->perm, nullptr);
+}
+
+// P^T work items over the c->npix pixels of n sorted entries (dcnt, hcnt
+// entries per pixel, offsets doff): pixels, split into <= kChunk-entry
+// chunks, built on the device (two prefix sums and a fill); the host only
+// lists the rare split pixels.  Items in NESTED order when enabled: pixels
+// reduced at the same time are neighbours on the sky, whose runs of z share
+// cache lines (each pixel's sums are unchanged).
+int make_items(sanepic_ctx *c, int64_t n, bool nest_items, const int32_t *dcnt, const int64_t *doff,
+               const std::vector<int32_t> &hcnt, ItemSet &out) {
   const int64_t npix = c->npix;
-  // P^T work items: pixels, split into <= kChunk-entry chunks, built on the
-  // device (two prefix sums and a fill); the host only lists the rare split
-  // pixels.  Items in NESTED order when enabled: pixels reduced at the same
-  // time are neighbours on the sky, whose runs of z share cache lines (each
-  // pixel's sums are unchanged).
   // item count in pixel order (it does not depend on the order); the rare
   // split pixels then listed in item order with their partial slots
   std::vector<int32_t> spix, sfirst, scount;
   int64_t nit = 0;
   int32_t nslots = 0;
   for (int64_t p = 0; p < npix; p++) {
-    const int64_t n = c->h_cnt[(size_t)p];
-    if (n <= kChunk) {
+    const int64_t k = hcnt[(size_t)p];
+    if (k <= kChunk) {
       nit++;
     } else {
-      nit += (n + kChunk - 1) / kChunk;
+      nit += (k + kChunk - 1) / kChunk;
       spix.push_back((int32_t)p);
     }
   }
@@ -1311,20 +1335,20 @@ int build_items(sanepic_ctx *c, int64_t ntot, bool nest_items) {
     }
   }
   for (int32_t p : spix) {
-    const int32_t k = (int32_t)((c->h_cnt[(size_t)p] + kChunk - 1) / kChunk);
+    const int32_t k = (int32_t)((hcnt[(size_t)p] + kChunk - 1) / kChunk);
     sfirst.push_back(nslots);
     scount.push_back(k);
     nslots += k;
   }
   const int64_t nsp = (int64_t)spix.size();
   trace_mark("pixelize: item counts (host)");
-  ACHK(c, dalloc(c, c->it_pix, nit));
-  ACHK(c, dalloc(c, c->it_beg, nit));
-  ACHK(c, dalloc(c, c->it_len, nit));
-  ACHK(c, dalloc(c, c->it_slot, nit));
-  ACHK(c, dalloc(c, c->sp_pix, nsp));
-  ACHK(c, dalloc(c, c->sp_first, nsp));
-  ACHK(c, dalloc(c, c->sp_count, nsp));
+  ACHK(c, dalloc(c, out.pix, nit));
+  ACHK(c, dalloc(c, out.beg, nit));
+  ACHK(c, dalloc(c, out.len, nit));
+  ACHK(c, dalloc(c, out.slot, nit));
+  ACHK(c, dalloc(c, out.sp_pix, nsp));
+  ACHK(c, dalloc(c, out.sp_first, nsp));
+  ACHK(c, dalloc(c, out.sp_count, nsp));
   if (npix > 0) {
     const int32_t *order = nest_items ? c->d_slot2pix : nullptr;
     int32_t *kcnt = nullptr, *scnt = nullptr, *kofs = nullptr, *sofs = nullptr;
@@ -1332,15 +1356,15 @@ int build_items(sanepic_ctx *c, int64_t ntot, bool nest_items) {
     ACHK(c, dalloc(c, scnt, npix));
     ACHK(c, dalloc(c, kofs, npix));
     ACHK(c, dalloc(c, sofs, npix));
-    LCHK(c, launch_item_count(order, c->cnt, npix, kChunk, kcnt, scnt, c->st));
+    LCHK(c, launch_item_count(order, dcnt, npix, kChunk, kcnt, scnt, c->st));
     size_t tb = 0;
     HIPCHK(c, hipcub::DeviceScan::ExclusiveSum(nullptr, tb, kcnt, kofs, (int)npix, c->st));
     char *tmp = nullptr;
     ACHK(c, dalloc(c, tmp, (int64_t)tb));
     HIPCHK(c, hipcub::DeviceScan::ExclusiveSum(tmp, tb, kcnt, kofs, (int)npix, c->st));
     HIPCHK(c, hipcub::DeviceScan::ExclusiveSum(tmp, tb, scnt, sofs, (int)npix, c->st));
-    LCHK(c, launch_item_fill(order, c->cnt, c->off, kofs, sofs, npix, kChunk, c->it_pix, c->it_beg, c->it_len,
-                             c->it_slot, c->st));
+    LCHK(c, launch_item_fill(order, dcnt, doff, kofs, sofs, npix, kChunk, out.pix, out.beg, out.len, out.slot,
+                             c->st));
     dfree_later(c, tmp, (int64_t)tb);
     dfree_later(c, kcnt, npix);
     dfree_later(c, scnt, npix);
@@ -1348,25 +1372,53 @@ int build_items(sanepic_ctx *c, int64_t ntot, bool nest_items) {
     dfree_later(c, sofs, npix);
   }
   if (nsp) {
-    HIPCHK(c, hipMemcpy(c->sp_pix, spix.data(), nsp * 4, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->sp_first, sfirst.data(), nsp * 4, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->sp_count, scount.data(), nsp * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(out.sp_pix, spix.data(), nsp * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(out.sp_first, sfirst.data(), nsp * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(out.sp_count, scount.data(), nsp * 4, hipMemcpyHostToDevice));
   }
-  c->items = Items{c->it_pix, c->it_beg, c->it_len, c->it_slot, nit, c->sp_pix, c->sp_first, c->sp_count, nsp};
-  c->nslots = nslots;
+  out.items = Items{out.pix, out.beg, out.len, out.slot, nit, out.sp_pix, out.sp_first, out.sp_count, nsp};
+  out.nslots = nslots;
   // lanes per P^T item: 16 keep four entries each in flight from 64 entries
   // on; shorter items (a group of a few detectors at a fine nside) take 8
   // (measured at ~30 entries per item: 16 lanes 7.74, 8 lanes 6.30, 4 lanes
   // 6.76 ms per operator)
   {
-    const double mean = nit > 0 ? (double)ntot / (double)nit : 0.0;
-    c->scatter_lanes = mean < 128.0 ? 8 : 16;
+    const double mean = nit > 0 ? (double)n / (double)nit : 0.0;
+    out.lanes = mean < 128.0 ? 8 : 16;
     if (const char *e = getenv("SANEPIC_SCATTER_LANES")) {
       const int l = atoi(e);
-      if (l == 4 || l == 8 || l == 16) c->scatter_lanes = l;
+      if (l == 4 || l == 8 || l == 16) out.lanes = l;
     }
   }
-  ACHK(c, dalloc(c, c->partial, 8 * std::max<int64_t>(1, nslots)));
+  ACHK(c, dalloc(c, out.partial, 8 * std::max<int64_t>(1, nslots)));
+  return 0;
+}
+
+// The items over the pixel-sorted samples (c->cnt, c->h_cnt, c->off):
+// c->items, c->nslots, c->partial and the lanes per item.  With nest_items
+// the items follow c->d_slot2pix (NESTED order).
+int build_items(sanepic_ctx *c, int64_t ntot, bool nest_items) {
+  ItemSet s;
+  s.pix = c->it_pix;
+  s.beg = c->it_beg;
+  s.len = c->it_len;
+  s.slot = c->it_slot;
+  s.sp_pix = c->sp_pix;
+  s.sp_first = c->sp_first;
+  s.sp_count = c->sp_count;
+  s.partial = c->partial;
+  ACHK(c, make_items(c, ntot, nest_items, c->cnt, c->off, c->h_cnt, s));
+  c->it_pix = s.pix;
+  c->it_beg = s.beg;
+  c->it_len = s.len;
+  c->it_slot = s.slot;
+  c->sp_pix = s.sp_pix;
+  c->sp_first = s.sp_first;
+  c->sp_count = s.sp_count;
+  c->partial = s.partial;
+  c->items = s.items;
+  c->nslots = s.nslots;
+  c->scatter_lanes = s.lanes;
   return 0;
 }
 

@@ -20,6 +20,17 @@
 constexpr int kChunk = 4096;  // max sorted entries per P^T item
 constexpr int kProfClasses = 5;
 
+// P^T work items over the pixels (Items) and the buffers they live in.
+struct ItemSet {
+  int32_t *pix = nullptr, *len = nullptr, *slot = nullptr, *sp_pix = nullptr, *sp_first = nullptr,
+          *sp_count = nullptr;
+  int64_t *beg = nullptr;
+  Items items{};
+  int64_t nslots = 0;
+  double *partial = nullptr;  // [nslots][8]
+  int lanes = 16;             // lanes per item
+};
+
 struct sanepic_ctx {
   sanepic_config cfg{};
   std::string err;
@@ -129,6 +140,17 @@ struct sanepic_ctx {
   int col_waves64 = 0;               // the same for the FP64 gather column pass
   bool row_det = true;               // FP64 row pass in detector-major row order (weights reused from L2)
   uint32_t *rank = nullptr;          // [ntot] sorted position of each sample (zsorted)
+  // P^T over runs of consecutive samples in one pixel: the inverse column
+  // pass writes one record per run (ops.hip run_record) and P^T sums the
+  // records of each pixel (items over records, sorted by pixel)
+  bool runs = true;                  // build the run layout at pixelisation (SANEPIC_RUNS)
+  int64_t nrec = 0;                  // records (0: no run layout)
+  uint64_t *rmeta = nullptr;         // [ntot / 16] first record << 16 | run-end bits of each piece
+  int run_loads32 = 0, run_loads64 = 0;  // load batching of the record pass (OpArgs::recv)
+  uint32_t *rperm = nullptr;         // [nrec] pixel-sorted entry -> record
+  double *rec64 = nullptr;           // [nrec][3] FP64 operator's records
+  float *rec32 = nullptr;            // [nrec][3] FP32 operator's records
+  ItemSet ritems;                    // P^T items over records
   double *px64 = nullptr;
   float *px32 = nullptr;
   bool deflate = false;
@@ -379,6 +401,20 @@ int setup_data(sanepic_ctx *c);
 // items over c->npix pixels from c->cnt / c->h_cnt / c->off (capi.hip).
 int sort_samples(sanepic_ctx *c, const uint32_t *keys, int64_t ntot, int64_t npix);
 int build_items(sanepic_ctx *c, int64_t ntot, bool nest_items);
+// The same over n sorted entries with per-pixel entry counts dcnt / hcnt and
+// offsets doff, into out (buffers already in out are reused).
+int make_items(sanepic_ctx *c, int64_t n, bool nest_items, const int32_t *dcnt, const int64_t *doff,
+               const std::vector<int32_t> &hcnt, ItemSet &out);
+// Stable sort of n keys (< nkey): perm[sorted position] = index; the sorted
+// keys too when keys_out is not null (the caller frees them)
+int sort_keys(sanepic_ctx *c, const uint32_t *keys, int64_t n, int64_t nkey, uint32_t *&perm,
+              uint32_t **keys_out);
+// Run layout of the pixel stream (capi_runs.hip): pieces, records, their
+// pixel-sorted order and items; called at pixelisation while c->pix holds
+// compressed pixels.  use_runs: whether an operator application writes
+// records (the layout exists and the inverse pass can write it).
+int build_runs(sanepic_ctx *c);
+bool use_runs(const sanepic_ctx *c);
 // NESTED order of the c->npix pixels listed (RING) in dlist: c->mpos,
 // c->d_slot2pix, c->h_slot2pix (nothing when nside is not a power of two)
 int pixel_nested_order(sanepic_ctx *c, const int32_t *dlist);
@@ -388,11 +424,13 @@ template <typename T> OpArgs<T> op_args(sanepic_ctx *c);
 template <> OpArgs<double> op_args<double>(sanepic_ctx *c);
 template <> OpArgs<float> op_args<float>(sanepic_ctx *c);
 template <typename T> int run_filter(sanepic_ctx *c, OpArgs<T> a, int src, int mode, int64_t *nchi);
-template <typename T> int run_scatter(sanepic_ctx *c, const T *z, double *q, int mask);
+// P^T of z (time order), or of run records when rec is set
+template <typename T> int run_scatter(sanepic_ctx *c, const T *z, double *q, int mask, const T *rec = nullptr);
 // q = A x (interleaved device vectors); chi2 -> sc->chi2 when asked
 int apply_A(sanepic_ctx *c, const double *x, double *q, int mask, bool chi2, bool fp64);
 // its first half: z = N^-1 P x (c->z or c->z32), chi2 partials of *nb row blocks
-int apply_filter(sanepic_ctx *c, const double *x, bool chi2, bool use64, int64_t *nb);
+// (with runs: the records of the run layout instead of z)
+int apply_filter(sanepic_ctx *c, const double *x, bool chi2, bool use64, int64_t *nb, bool runs = false);
 int check_ready(sanepic_ctx *c, bool need_setup);
 int to_inter(sanepic_ctx *c, const double *host_blk, double *dev_inter);
 int from_inter(sanepic_ctx *c, const double *dev_inter, double *host_blk);

@@ -115,6 +115,16 @@ struct OpArgs {
                         // multiplies P x in the pre-gather pass and z in the inverse pass
   int64_t blk;          // samples per (detector, segment) block (the block of sample g is g / blk)
   int rowdet;           // row pass: row k2 of all of a detector's pairs back to back (whole detectors only)
+  // Runs: instead of z, the inverse column pass writes one record per run of
+  // consecutive samples in one pixel, (sum z, sum z cos 2psi, sum z sin 2psi),
+  // and P^T sums records.  Samples are grouped in aligned pieces of 16; a run
+  // never crosses a piece.  rmeta[k] = first record << 16 | run-end bits of
+  // piece k: bit i marks sample 16k + i as the last of its run (bit 15 is
+  // always set), and the piece's runs are consecutive records in time order.
+  // Null rec: z is written.
+  T *rec;                 // [nrec][3]
+  const uint64_t *rmeta;  // [ntot / 16]
+  int recv;               // load batching / occupancy of the record pass (N2 = 256): 0 default
 };
 
 template <typename T> int launch_col_fwd(const OpArgs<T> &a, int src, hipStream_t s);
@@ -148,6 +158,8 @@ struct ScatterArgs {
   const V *cs_sorted;     // sorted entry -> cos 2psi, sin 2psi (FP64 operator)
   const uint32_t *ang_sorted;  // sorted entry -> 2psi as a fraction of a turn (FP32 operator)
   const T *z;             // [ntot] time-domain values (in sorted order when `sorted`)
+  const T *rec;           // or, when set, [nrec][3] run records: entries are runs and perm maps
+                          // a sorted entry to its record (no angles are read)
   int sorted;             // z is stored pixel-sorted: entry j at z[j]
   int xcd;                // XCD-contiguous item ranges (neighbouring pixels share an L2)
   int lanes;              // lanes per item: 4, 8 or 16 (fewer for short items; 0 means 16)

@@ -436,6 +436,70 @@ __global__ __launch_bounds__(256) void k_row(OpArgs<T> a) {
   row_body<T, LOG2N1>(a, logical_block(a.xcd), lds, red, a.mode);
 }
 
+// ------------------------------------------------------------- run records
+// Angle of sample g in the operator's form: a turn (FP32) or (cos, sin) (FP64).
+template <typename T> struct AngOf;
+template <> struct AngOf<float> {
+  using type = uint32_t;
+  __device__ static type load(const OpArgs<float> &a, int64_t g) { return a.ang[g]; }
+  __device__ static float2 cs(type v) { return cs_of_turn(v); }
+};
+template <> struct AngOf<double> {
+  using type = double2;
+  __device__ static type load(const OpArgs<double> &a, int64_t g) { return a.cs[g]; }
+  __device__ static double2 cs(type v) { return v; }
+};
+
+// Lane i of a 16-lane row reads lane i - OFF of the same row (DPP row_shr;
+// lanes below OFF read 0).
+template <int OFF>
+__device__ __forceinline__ float row_shr(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x110 + OFF, 0xF,
+                                                               0xF, true));
+}
+template <int OFF>
+__device__ __forceinline__ double row_shr(double v) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, 0x110 + OFF, 0xF, 0xF, true);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), 0x110 + OFF, 0xF, 0xF, true);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// One step of a segmented inclusive scan over the row (Hillis-Steele): lane
+// j passes its partial sum to lane j + OFF when both are in one run, i.e. no
+// run ends at j .. j + OFF - 1 (bits of m).  The sender zeroes what it passes
+// otherwise, so the receiver adds with the shifted operand in one instruction.
+template <int OFF, typename T>
+__device__ __forceinline__ void seg_step(T &x0, T &x1, T &x2, int i, uint32_t m) {
+  const bool pass = ((m >> i) & ((1u << OFF) - 1u)) == 0u;
+  x0 += row_shr<OFF>(pass ? x0 : (T)0);
+  x1 += row_shr<OFF>(pass ? x1 : (T)0);
+  x2 += row_shr<OFF>(pass ? x2 : (T)0);
+}
+
+// z of sample g, held by lane i = g % 16 of a 16-lane row that holds g's whole
+// piece (meta = the piece's rmeta): the run sums (z, z cos 2psi, z sin 2psi)
+// by a segmented scan (a fixed tree for a given run layout), written by the
+// lane that ends each run.
+template <typename T>
+__device__ __forceinline__ void run_record(const OpArgs<T> &a, uint64_t meta, int i, T zv,
+                                           typename AngOf<T>::type ang) {
+  const uint32_t m = (uint32_t)meta & 0xFFFFu;
+  const uint32_t below = m & ((1u << i) - 1u);
+  const VT<T> c = AngOf<T>::cs(ang);
+  T x0 = zv, x1 = zv * c.x, x2 = zv * c.y;
+  seg_step<1>(x0, x1, x2, i, m);
+  seg_step<2>(x0, x1, x2, i, m);
+  seg_step<4>(x0, x1, x2, i, m);
+  seg_step<8>(x0, x1, x2, i, m);
+  if ((m >> i) & 1u) {
+    T *r = a.rec + 3 * (int64_t)((uint32_t)(meta >> 16) + (uint32_t)__popc(below));
+    r[0] = x0;
+    r[1] = x1;
+    r[2] = x2;
+  }
+}
+
 // ---------------------------------------------------------------- column inv
 template <typename T, int LOG2N2>
 __device__ __forceinline__ void col_inv_body(const OpArgs<T> &a, int64_t lb, LdsE<VT<T>> *lds) {
@@ -468,6 +532,88 @@ template <typename T, int LOG2N2>
 __global__ __launch_bounds__(256) void k_col_inv(OpArgs<T> a) {
   __shared__ LdsE<VT<T>> lds[kLds];
   col_inv_body<T, LOG2N2>(a, logical_block(a.xcd), lds);
+}
+
+// The inverse column pass writing run records (N2 <= 256, so B >= 16: the 16
+// lanes of a row hold 16 consecutive columns of one row n2, i.e. one aligned
+// piece, sample g in lane g % 16).  After the FFT each thread parks its
+// outputs in the LDS (its own slots) so that they do not hold registers while
+// the records are formed.  Each output needs its angle and its piece's rmeta:
+// they are loaded in batches of SB rows, the next batch in flight while one
+// is summed; with PRE the first batch is issued with the column loads,
+// before the FFT.
+template <typename T, int LOG2N2, int SB, bool PRE>
+__device__ __forceinline__ void col_inv_rec_body(const OpArgs<T> &a, int64_t lb, LdsE<VT<T>> *lds) {
+  using V = VT<T>;
+  using A = typename AngOf<T>::type;
+  constexpr int N2 = 1 << LOG2N2, TS = N2 / 16, B = 256 / TS, NB = 16 / SB;
+  // both segments' outputs fit in the FFT's LDS in FP32; in FP64 one at a time
+  constexpr bool BOTH = 2 * 4096 * sizeof(T) <= kLds * sizeof(LdsE<V>);
+  static_assert(B >= 16 && 16 % SB == 0, "record pass layout");
+  static_assert(4096 * sizeof(T) <= kLds * sizeof(LdsE<V>), "LDS tile");
+  static_assert(LdsSize<LOG2N2, B>::value <= kLds, "LDS");
+  const int tid = threadIdx.x, b = tid % B, t = tid / B, i = tid & 15;
+  const int64_t N1 = a.g.N1;
+  const int64_t ncb = N1 / B;
+  const int64_t pr = lb / ncb, cb = lb % ncb;
+  int det;
+  int64_t ga, gb;
+  pair_info(a.g, pr, det, ga, gb);
+  const int64_t n1 = cb * B + b;
+  const int64_t step = N1 * (int64_t)TS;
+  const int nq = (gb >= 0 ? 2 : 1) * NB;  // batches: segment a's rows, then b's
+  A an[SB], bn[SB];
+  uint64_t am[SB], bm[SB];
+  auto fetch = [&](int q, A *pa, uint64_t *pm) {
+    const int64_t g0 = (q >= NB ? gb : ga) + n1 + N1 * (int64_t)t + step * (int64_t)((q % NB) * SB);
+#pragma unroll
+    for (int j = 0; j < SB; j++) {
+      const int64_t g = g0 + step * j;
+      pa[j] = AngOf<T>::load(a, g);
+      pm[j] = a.rmeta[g >> 4];
+    }
+  };
+  const V *in = a.buf + pr * a.g.N;
+  V v[16];
+#pragma unroll
+  for (int k = 0; k < 16; k++) v[k] = in[(int64_t)(t + TS * k) * N1 + n1];
+  if constexpr (PRE) fetch(0, an, am);
+  fft_reg<V, +1, LOG2N2, B>(v, t, lds + b, a.tw4096);
+  __syncthreads();  // the FFT's last reads of its LDS are done
+  T *tile = reinterpret_cast<T *>(lds);
+#pragma unroll
+  for (int s = 0; s < 16; s++) {
+    tile[s * 256 + tid] = v[s].x;
+    if constexpr (BOTH) tile[4096 + s * 256 + tid] = v[s].y;
+  }
+  if constexpr (!PRE) fetch(0, an, am);
+  const T sa = seg_gain(a, ga), sb = gb >= 0 ? seg_gain(a, gb) : (T)1;
+#pragma unroll 1
+  for (int q = 0; q < nq; q++) {
+    if (q + 1 < nq) fetch(q + 1, bn, bm);
+    const int h = q >= NB ? 1 : 0;
+    if constexpr (!BOTH) {
+      if (q == NB) {  // segment b: its outputs replace a's (own slots only)
+#pragma unroll
+        for (int s = 0; s < 16; s++) tile[s * 256 + tid] = v[s].y;
+      }
+    }
+    const T *z = tile + (BOTH && h ? 4096 : 0) + (q % NB) * SB * 256 + tid;
+    const T sg = h ? sb : sa;
+#pragma unroll
+    for (int j = 0; j < SB; j++) run_record(a, am[j], i, z[j * 256] * sg, an[j]);
+#pragma unroll
+    for (int j = 0; j < SB; j++) {
+      an[j] = bn[j];
+      am[j] = bm[j];
+    }
+  }
+}
+
+template <typename T, int LOG2N2, int SB, bool PRE, int WAVES>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_col_inv_rec(OpArgs<T> a) {
+  __shared__ LdsE<VT<T>> lds[kLds];
+  col_inv_rec_body<T, LOG2N2, SB, PRE>(a, logical_block(a.xcd), lds);
 }
 
 // -------------------------------------------------------------- single pass
@@ -631,6 +777,21 @@ int launch_col_inv(const OpArgs<T> &a, hipStream_t s) {
   if (a.g.M == 5) return launch_col_inv_m<T, 5>(a, s);
   if (a.g.M != 1) return -2;
   const int64_t nb = a.g.npairs * (a.g.N / 4096);
+  if (a.rec) {
+    if (a.g.log2N2 == 8) switch (a.recv) {  // load batching and occupancy of the record pass
+        case 1: return launch1(k_col_inv_rec<T, 8, 16, true, 2>, nb, s, a);
+        case 2: return launch1(k_col_inv_rec<T, 8, 8, false, 2>, nb, s, a);
+        case 3: return launch1(k_col_inv_rec<T, 8, 4, true, 4>, nb, s, a);
+        default: return launch1(k_col_inv_rec<T, 8, 8, true, 3>, nb, s, a);
+      }
+#define CR(L) \
+  case L: return launch1(k_col_inv_rec<T, L, 8, true, 3>, nb, s, a);
+    switch (a.g.log2N2) {
+      CR(4) CR(5) CR(6) CR(7)
+      default: return -2;  // the host writes records only where N2 <= 256
+    }
+#undef CR
+  }
 #define CI(L) \
   case L: return launch1(k_col_inv<T, L>, nb, s, a);
   switch (a.g.log2N2) {

@@ -57,7 +57,9 @@ __device__ __forceinline__ VT<T> sorted_cs(const ScatterArgs<T> &a, int64_t j) {
 // only, which the host derives from the data, so results are deterministic.
 // LEAK: a 4th sum, the leakage component, of z times the coefficient of the
 // entry's detector (do_PtNd 2606-2607); the detector of sample g is g / spd.
-template <typename T, int L, bool LEAK>
+// REC: entries are runs, whose records (sum z, sum z cos, sum z sin) the
+// inverse column pass wrote; perm maps a sorted entry to its record.
+template <typename T, int L, bool LEAK, bool REC = false>
 __global__ __launch_bounds__(256) void k_scatter(ScatterArgs<T> a) {
   using V = VT<T>;
   constexpr int NC = LEAK ? 4 : 3;
@@ -69,8 +71,36 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterArgs<T> a) {
   const int n = a.it.len[item];
   T s0 = 0, s1 = 0, s2 = 0, s3 = 0;
   int k = lane;
+  if constexpr (REC) {
+    for (; k + 3 * L < n; k += 4 * L) {
+      const int64_t j = e0 + k;
+      uint32_t pp[4];
+      T r0[4], r1[4], r2[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) pp[u] = a.perm[j + L * u];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const T *r = a.rec + 3 * (int64_t)pp[u];
+        r0[u] = r[0];
+        r1[u] = r[1];
+        r2[u] = r[2];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        s0 += r0[u];
+        s1 += r1[u];
+        s2 += r2[u];
+      }
+    }
+    for (; k < n; k += L) {
+      const T *r = a.rec + 3 * (int64_t)a.perm[e0 + k];
+      s0 += r[0];
+      s1 += r[1];
+      s2 += r[2];
+    }
+  }
   // four entries per lane in flight (their loads first), summed in entry order
-  for (; k + 3 * L < n; k += 4 * L) {
+  for (; !REC && k + 3 * L < n; k += 4 * L) {
     const int64_t j = e0 + k;
     T zv[4];
     V c[4];
@@ -94,7 +124,7 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterArgs<T> a) {
       if constexpr (LEAK) s3 += zv[u] * a.relf[(int64_t)pp[u] / a.spd];
     }
   }
-  for (; k < n; k += L) {
+  for (; !REC && k < n; k += L) {
     const int64_t j = e0 + k;
     const uint32_t pj = (a.sorted && !LEAK) ? 0u : a.perm[j];
     const T zv = a.sorted ? a.z[j] : a.z[pj];
@@ -552,9 +582,16 @@ inline int ok() { return hipGetLastError() == hipSuccess ? 0 : -1; }
 
 template <typename T>
 int launch_scatter(const ScatterArgs<T> &a, hipStream_t s) {
+  if (a.rec && a.nc == 4) return -2;  // run records hold three sums
   if (a.it.n > 0) {
     if (a.nc == 4)  // leakage component: 16 lanes per item
       hipLaunchKernelGGL((k_scatter<T, 16, true>), dim3(nblk(a.it.n, 16)), dim3(256), 0, s, a);
+    else if (a.rec && a.lanes == 4)
+      hipLaunchKernelGGL((k_scatter<T, 4, false, true>), dim3(nblk(a.it.n, 64)), dim3(256), 0, s, a);
+    else if (a.rec && a.lanes == 8)
+      hipLaunchKernelGGL((k_scatter<T, 8, false, true>), dim3(nblk(a.it.n, 32)), dim3(256), 0, s, a);
+    else if (a.rec)
+      hipLaunchKernelGGL((k_scatter<T, 16, false, true>), dim3(nblk(a.it.n, 16)), dim3(256), 0, s, a);
     else if (a.lanes == 4)
       hipLaunchKernelGGL((k_scatter<T, 4, false>), dim3(nblk(a.it.n, 64)), dim3(256), 0, s, a);
     else if (a.lanes == 8)

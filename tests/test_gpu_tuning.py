@@ -145,6 +145,8 @@ def test_layout_options_are_bit_identical(data, env, monkeypatch):
     pixels reported back are the same."""
     cfg, ds, P = data
     print("layout option", env, flush=True)
+    if env == "SANEPIC_ZSORT":  # sorted z replaces the run records: compare both on samples
+        monkeypatch.setenv("SANEPIC_RUNS", "0")
     out = []
     for v in ("0", "1"):
         monkeypatch.setenv(env, v)
@@ -158,3 +160,33 @@ def test_layout_options_are_bit_identical(data, env, monkeypatch):
         assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and a[2] == b[2]
         assert np.array_equal(a[3], b[3])
     assert rel(out[2][0], P.apply_A(np.random.default_rng(11).standard_normal(3 * out[2][0].size // 3))) < 1e-11
+
+
+@pytest.mark.parametrize("split", [9, 12])
+def test_run_records_match_samples(data, split, monkeypatch):
+    """P^T over run records (the inverse column pass sums each run of
+    consecutive samples in one pixel, P^T sums the runs) against P^T over
+    samples and the oracle: the same sums in another order, so A x agrees to
+    rounding; b, built from samples, is bit-identical."""
+    cfg, ds, P = data
+    x = np.random.default_rng(12).standard_normal(3 * P.npix)
+    want = P.apply_A(x)
+    for prec, tol in ((sanepic_amd.FP64, 1e-11), (sanepic_amd.MIXED, 2e-6)):
+        got = {}
+        for v in ("0", "1"):
+            monkeypatch.setenv("SANEPIC_RUNS", v)
+            ctx = _ctx(cfg, ds, split, precision=prec)
+            got[v] = (ctx.apply(x), ctx.rhs(), ctx.device_bytes())
+            nrec, used = ctx.run_records()
+            assert used == (v == "1") and (nrec > 0) == (v == "1")
+            if v == "1":  # every piece of 16 samples holds at least one run, at most 16
+                assert cfg.ndet * cfg.nseg * cfg.ns // 16 <= nrec <= cfg.ndet * cfg.nseg * cfg.ns
+            ctx.close()
+        e_runs, e_samples = rel(got["1"][0], want), rel(got["0"][0], want)
+        d = rel(got["1"][0], got["0"][0])
+        print(f"  log2 N1 {split} {prec}: runs {e_runs:.2e}, samples {e_samples:.2e}, apart {d:.2e}, "
+              f"records {(got['1'][2] - got['0'][2]) / 2**20:.1f} MiB", flush=True)
+        assert e_runs < tol and e_samples < tol
+        assert 0 < d < (1e-11 if prec == sanepic_amd.FP64 else 2e-6)
+        assert np.array_equal(got["1"][1], got["0"][1])
+        assert got["1"][2] > got["0"][2]  # the run layout was built
