@@ -224,7 +224,9 @@ int sanepic_chi2(sanepic_ctx *ctx, const double *x, double *chi2);
 
 typedef struct sanepic_solve_opts {
   int32_t precond;  /* SANEPIC_PRECOND_* */
-  int32_t refresh;  /* explicit residual every `refresh` iterations (10) */
+  int32_t refresh;  /* explicit residual every `refresh` iterations (10; the
+                       reference's PCG at iterations 0, 10, 20, ..., the
+                       learned flexible PCG at 10, 20, ...) */
   int32_t deflate;  /* 1: two-level preconditioner that solves the map's I
                        monopole exactly (A-DEF2 with the single coarse vector
                        Z = 1 on I).  The DC bin's 1e-6 weight (sanepic.cc:537)

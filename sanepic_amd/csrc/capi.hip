@@ -307,7 +307,7 @@ int run_scatter(sanepic_ctx *c, const T *z, double *q, int mask, const T *rec) {
   s.sorted = (c->zsorted && c->rank) ? 1 : 0;
   // measured: contiguous item ranges per XCD make P^T slower (RING items
   // 4.49 -> 5.16 ms, NESTED items 3.50 -> 4.43 ms)
-  s.xcd = 0;
+  s.xcd = c->pt_xcd ? 1 : 0;
   s.lanes = rec ? c->ritems.lanes : c->scatter_lanes;
   s.cnt = c->cnt;
   s.q = q;
@@ -516,6 +516,24 @@ void split_range(const PairGeom &g, int *lo, int *hi) {
   *hi = std::min(12, l - 4);
 }
 
+}  // namespace
+
+namespace sctx {
+
+// The FP32 operator's pixel-sorted angles, which P^T over samples reads: made
+// when the operator cannot use the run records (P^T over records reads none).
+int sorted_turns(sanepic_ctx *c) {
+  if (c->cfg.precision != SANEPIC_MIXED || c->ang_sorted32 || !c->perm || !c->cs || use_runs(c)) return 0;
+  ACHK(c, dalloc(c, c->ang_sorted32, c->ntot));
+  LCHK(c, launch_gather_turn(c->perm, c->cs, c->ang_sorted32, c->ntot, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return 0;
+}
+
+}  // namespace sctx
+
+namespace {
+
 // Four-step split N1 = 2^log2n1 (the weight tables follow the split).
 int set_split(sanepic_ctx *c, int log2n1) {
   PairGeom &g = c->g;
@@ -540,7 +558,7 @@ int set_split(sanepic_ctx *c, int log2n1) {
   }
   dfree(c, c->wt, c->ndet * c->ns);
   dfree(c, c->wt32, c->ndet * c->ns);
-  return 0;
+  return c->pixelized ? sorted_turns(c) : 0;  // a split without run records
 }
 
 double pixel_eps() {
@@ -636,6 +654,8 @@ int sanepic_create(const sanepic_config *cfg, sanepic_ctx **out) {
   if (const char *e = getenv("SANEPIC_XCD_BLOCKS")) c->xcd_blocks = atoi(e) != 0;
   if (const char *e = getenv("SANEPIC_ZSORT")) c->zsorted = atoi(e) != 0;
   if (const char *e = getenv("SANEPIC_RUNS")) c->runs = atoi(e) != 0;
+  if (const char *e = getenv("SANEPIC_PT_XCD")) c->pt_xcd = atoi(e) != 0;
+  if (const char *e = getenv("SANEPIC_MONO_FFT")) c->mono_fft = atoi(e) != 0;
   if (const char *e = getenv("SANEPIC_RUN_LOADS32")) c->run_loads32 = std::min(4, std::max(0, atoi(e)));
   if (const char *e = getenv("SANEPIC_RUN_LOADS64")) c->run_loads64 = std::min(4, std::max(0, atoi(e)));
   if (const char *e = getenv("SANEPIC_COL_WAVES")) c->col_waves = atoi(e);
@@ -1101,13 +1121,12 @@ int sanepic_pixelize(sanepic_ctx *c, int64_t *npix_out, int64_t *n_deferred) {
   }
   if (c->cfg.precision == SANEPIC_MIXED) {
     ACHK(c, dalloc(c, c->ang32, ntot));
-    ACHK(c, dalloc(c, c->ang_sorted32, ntot));
     LCHK(c, launch_cs_to_turn(c->cs, c->ang32, ntot, c->st));
-    LCHK(c, launch_gather_turn(c->perm, c->cs, c->ang_sorted32, ntot, c->st));
   }
   trace_mark("pixelize: sorted angles");
   ACHK(c, build_items(c, ntot, c->items_nested && c->d_slot2pix != nullptr));
   ACHK(c, build_runs(c));
+  ACHK(c, sorted_turns(c));
   if (!c->pin_owner) dfree_later(c, c->d_slot2pix, npix);  // a group keeps it for fetch_host_order
   HIPCHK(c, hipStreamSynchronize(c->st));
   if (c->pin_owner) flush_frees(c);  // a group: its temporaries serve the next group
@@ -1714,7 +1733,23 @@ int sanepic_solve_begin(sanepic_ctx *c, const sanepic_solve_opts *opts, const do
     // AZ = A e_I (masked) once; r, d serve as scratch before cg_init
     ACHK(c, dalloc(c, c->az, c->nc * npix));
     LCHK(c, cg_launch_monopole(c->d, npix, c->st, c->nc));
-    ACHK(c, apply_A(c, c->d, c->az, 1, false, true));
+    if (c->sub.empty() && !c->recal && c->nc == 3 && !c->mono_fft) {
+      // P e_I is 1 on every sample, and the unnormalised filter of a
+      // constant, c2r(SpN r2c(1)) = N SpN[0], is a constant per detector:
+      // A e_I is P^T of that, without the FFTs of an operator application
+      const int64_t nh = c->ns / 2 + 1;
+      std::vector<double> w((size_t)c->ndet);
+      for (int64_t d = 0; d < c->ndet; d++) w[(size_t)d] = (double)c->ns * c->spn[(size_t)(d * nh)];
+      double *dw = nullptr;
+      ACHK(c, dalloc(c, dw, c->ndet));
+      HIPCHK(c, hipMemcpyAsync(dw, w.data(), c->ndet * sizeof(double), hipMemcpyHostToDevice, c->st));
+      LCHK(c, cg_launch_det_const(c->z, dw, c->nseg * c->ns, c->ntot, c->st));
+      ACHK(c, run_scatter<double>(c, c->z, c->az, 1));
+      HIPCHK(c, hipStreamSynchronize(c->st));  // w is a host temporary
+      dfree(c, dw, c->ndet);
+    } else {
+      ACHK(c, apply_A(c, c->d, c->az, 1, false, true));
+    }
     CGArgs a = cg_args(c);
     LCHK(c, cg_launch_coarse_setup(a, c->st));
     LCHK(c, cg_launch_coarse_start(a, c->st));

@@ -11,7 +11,9 @@
 //   d = z - sum_{j = k-m+1..k} (z.A d_j / d_j.A d_j) d_j
 // (m = 1 is the Polak-Ribiere form), the directions and A d_j in a ring.
 // with the optional A-DEF2 monopole step folded into z as in the plain
-// solver, and the explicit residual b - A x every `refresh` iterations.
+// solver, and the explicit residual b - A x every `refresh` iterations
+// (after iterations refresh, 2 refresh, ...: the residual of the start is
+// exact already).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -309,7 +311,7 @@ int fpcg_step(sanepic_ctx *c, sanepic_iter_info *info, float *store, double *tai
   const bool fallback = c->last_z_learned && !(h.rr <= c->fgrowth * h.rr_prev);
   if (c->last_z_learned) c->n_learned++;
   if (fallback) c->n_fallback++;
-  const bool refresh = (c->iter % c->refresh) == 0;
+  const bool refresh = c->iter > 0 && (c->iter % c->refresh) == 0;
   if (refresh) {
     ACHK(c, apply_A(c, c->x, c->qx, 1, !c->refresh32, !c->refresh32));
     LCHK(c, f_launch_refresh(fa, c->st));

@@ -238,6 +238,11 @@ __global__ __launch_bounds__(RBS) void k_monopole(double *v, int64_t npix, int n
   }
 }
 
+__global__ __launch_bounds__(RBS) void k_det_const(double *z, const double *w, int64_t spd, int64_t ntot) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < ntot) z[g] = w[g / spd];
+}
+
 __global__ __launch_bounds__(RBS) void k_reduce_chi2(const double *part, int64_t n, CGScalars *sc) {
   __shared__ double sh[1][RBS / 64];
   double acc[1] = {0.0};
@@ -295,6 +300,12 @@ int cg_launch_resid(const CGArgs &a, hipStream_t s) {
 int cg_launch_monopole(double *v, int64_t npix, hipStream_t s, int nc) {
   if (npix > 0)
     hipLaunchKernelGGL(k_monopole, dim3((unsigned)((npix + RBS - 1) / RBS)), dim3(RBS), 0, s, v, npix, nc);
+  return ok();
+}
+
+int cg_launch_det_const(double *z, const double *w, int64_t spd, int64_t ntot, hipStream_t s) {
+  if (ntot > 0)
+    hipLaunchKernelGGL(k_det_const, dim3((unsigned)((ntot + RBS - 1) / RBS)), dim3(RBS), 0, s, z, w, spd, ntot);
   return ok();
 }
 

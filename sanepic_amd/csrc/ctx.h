@@ -151,6 +151,8 @@ struct sanepic_ctx {
   double *rec64 = nullptr;           // [nrec][3] FP64 operator's records
   float *rec32 = nullptr;            // [nrec][3] FP32 operator's records
   ItemSet ritems;                    // P^T items over records
+  bool pt_xcd = false;               // P^T items in XCD-contiguous ranges (SANEPIC_PT_XCD)
+  bool mono_fft = false;             // deflation: A e_I by a full operator application (SANEPIC_MONO_FFT)
   double *px64 = nullptr;
   float *px32 = nullptr;
   bool deflate = false;
@@ -415,6 +417,9 @@ int sort_keys(sanepic_ctx *c, const uint32_t *keys, int64_t n, int64_t nkey, uin
 // records (the layout exists and the inverse pass can write it).
 int build_runs(sanepic_ctx *c);
 bool use_runs(const sanepic_ctx *c);
+// The FP32 operator's pixel-sorted angles where it cannot use the run records
+// (after pixelisation and after every change of the FFT split)
+int sorted_turns(sanepic_ctx *c);
 // NESTED order of the c->npix pixels listed (RING) in dlist: c->mpos,
 // c->d_slot2pix, c->h_slot2pix (nothing when nside is not a power of two)
 int pixel_nested_order(sanepic_ctx *c, const int32_t *dlist);

@@ -275,6 +275,8 @@ int cg_launch_resid(const CGArgs &a, hipStream_t s);     // tmp[0] = |b - q|^2 (
 // Monopole deflation (two-level A-DEF2 with Z = e_I): coarse vector, the
 // scalar 1/(Z^T A Z) from AZ, and the starting correction x += y0 Z.
 int cg_launch_monopole(double *v, int64_t npix, hipStream_t s, int nc = 3);
+// z[g] = w[g / spd] for the ntot samples (a constant per detector)
+int cg_launch_det_const(double *z, const double *w, int64_t spd, int64_t ntot, hipStream_t s);
 int cg_launch_coarse_setup(const CGArgs &a, hipStream_t s);
 int cg_launch_coarse_start(const CGArgs &a, hipStream_t s);
 enum { SANEPIC_PRECOND_DIAG_K = 0, SANEPIC_PRECOND_BLOCK3_K = 1 };

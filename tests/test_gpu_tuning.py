@@ -190,3 +190,19 @@ def test_run_records_match_samples(data, split, monkeypatch):
         assert 0 < d < (1e-11 if prec == sanepic_amd.FP64 else 2e-6)
         assert np.array_equal(got["1"][1], got["0"][1])
         assert got["1"][2] > got["0"][2]  # the run layout was built
+
+
+def test_monopole_image_without_fft(data, monkeypatch):
+    """The deflated solver's A e_I taken as P^T of the constant N SpN[0] per
+    detector (P e_I = 1, and the filter of a constant keeps only its DC bin)
+    against a full FP64 operator application: the same iterates."""
+    cfg, ds, P = data
+    crit = {}
+    for v in ("0", "1"):
+        monkeypatch.setenv("SANEPIC_MONO_FFT", v)
+        ctx = _ctx(cfg, ds)
+        ctx.solve_begin(sanepic_amd.PRECOND_BLOCK3, 10, deflate=True)
+        crit[v] = [ctx.step()["crit"] for _ in range(12)]
+        ctx.close()
+    print("  crit", crit["0"][:3], crit["1"][:3], flush=True)
+    assert np.allclose(crit["0"], crit["1"], rtol=1e-9, atol=0)
