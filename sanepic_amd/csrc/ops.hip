@@ -71,6 +71,19 @@ __device__ __forceinline__ V twN(const V *__restrict__ lo, const V *__restrict__
   return DIR < 0 ? w : cconj(w);
 }
 
+// Angle of sample g in the operator's form: a turn (FP32) or (cos, sin) (FP64).
+template <typename T> struct AngOf;
+template <> struct AngOf<float> {
+  using type = uint32_t;
+  __device__ static type load(const OpArgs<float> &a, int64_t g) { return a.ang[g]; }
+  __device__ static float2 cs(type v) { return cs_of_turn(v); }
+};
+template <> struct AngOf<double> {
+  using type = double2;
+  __device__ static type load(const OpArgs<double> &a, int64_t g) { return a.cs[g]; }
+  __device__ static double2 cs(type v) { return v; }
+};
+
 // Outer four-step twiddles w_N^{DIR (j0 + dj s)}, s = 0, 1, ..., for one
 // thread's outputs.  A lookup per output (twN) spreads a wave's 64 lanes over
 // up to 64 lines of the 4096-entry table; a run reads w_N^{j0} and w_N^{dj}
@@ -163,13 +176,19 @@ __device__ __forceinline__ void col_fwd_body(const OpArgs<T> &a, int64_t lb, Lds
   int64_t ga, gb;
   pair_info(a.g, pr, det, ga, gb);
   const int64_t n1 = cb * B + b;
+  // The b segment's loads are unconditional (the last pair of an odd segment
+  // count reads segment a twice and drops the copy): a branch around them
+  // would split the unrolled loop into blocks, and the compiler would then
+  // wait for each sample's pixel and map loads before issuing the next's.
+  const bool hb = gb >= 0;
+  const int64_t gbb = hb ? gb : ga;
   V v[16];
 #pragma unroll
   for (int i = 0; i < 16; i++) {
     const int64_t s = n1 + N1 * (int64_t)(t + TS * i);
     const T xa = load_src<T, SRC>(a, ga + s);
-    const T xb = gb >= 0 ? load_src<T, SRC>(a, gb + s) : (T)0;
-    v[i] = mk<V>(xa, xb);
+    const T xb = load_src<T, SRC>(a, gbb + s);
+    v[i] = mk<V>(xa, hb ? xb : (T)0);
   }
   fft_reg<V, -1, LOG2N2, B>(v, t, lds + b, a.tw4096);
   V *out = a.buf + pr * a.g.N;
@@ -253,6 +272,8 @@ __device__ __forceinline__ void col_fwd_m_body(const OpArgs<T> &a, int64_t lb, L
   // reads back exactly the rows it wrote (rows P j + t + TS i for every j),
   // so the M x 16 values pass through its own buffer rows instead of
   // registers, with no barrier.
+  const bool hb = gb >= 0;  // unconditional b loads, as in col_fwd_body
+  const int64_t gbb = hb ? gb : ga;
 #pragma unroll 1
   for (int j = 0; j < M; j++) {
     V v[16];
@@ -260,8 +281,8 @@ __device__ __forceinline__ void col_fwd_m_body(const OpArgs<T> &a, int64_t lb, L
     for (int i = 0; i < 16; i++) {
       const int64_t s = n1 + N1 * (int64_t)(j + M * (t + TS * i));
       const T xa = load_src<T, SRC>(a, ga + s);
-      const T xb = gb >= 0 ? load_src<T, SRC>(a, gb + s) : (T)0;
-      v[i] = mk<V>(xa, xb);
+      const T xb = load_src<T, SRC>(a, gbb + s);
+      v[i] = mk<V>(xa, hb ? xb : (T)0);
     }
     if (j) __syncthreads();  // the previous sub-FFT's last LDS reads
     fft_reg<V, -1, LOG2P, B>(v, t, lds + b, a.tw4096);
@@ -387,6 +408,13 @@ __device__ __forceinline__ void row_body(const OpArgs<T> &a, int64_t lb, LdsE<VT
 #pragma unroll
     for (int i = 0; i < 16; i++) v[i] = a.buf[base + t + TS * i];
   }
+  // the weights are loaded with the row, so their latency hides under the
+  // forward FFT instead of stalling the filter
+  T wf[16];
+  if (mode & ROW_FILTER) {
+#pragma unroll
+    for (int s = 0; s < 16; s++) wf[s] = w[t + TS * s];
+  }
   double chi = 0.0;
   if (mode & ROW_FWD) {
     fft_reg<V, -1, LOG2N1, B>(v, t, lds + b, a.tw4096);
@@ -408,10 +436,7 @@ __device__ __forceinline__ void row_body(const OpArgs<T> &a, int64_t lb, LdsE<VT
     }
     if (mode & ROW_FILTER) {
 #pragma unroll
-      for (int s = 0; s < 16; s++) {
-        const T ww = w[t + TS * s];
-        v[s] = mk<V>(v[s].x * ww, v[s].y * ww);
-      }
+      for (int s = 0; s < 16; s++) v[s] = mk<V>(v[s].x * wf[s], v[s].y * wf[s]);
     }
   }
   if (mode & ROW_INV) {
@@ -437,19 +462,6 @@ __global__ __launch_bounds__(256) void k_row(OpArgs<T> a) {
 }
 
 // ------------------------------------------------------------- run records
-// Angle of sample g in the operator's form: a turn (FP32) or (cos, sin) (FP64).
-template <typename T> struct AngOf;
-template <> struct AngOf<float> {
-  using type = uint32_t;
-  __device__ static type load(const OpArgs<float> &a, int64_t g) { return a.ang[g]; }
-  __device__ static float2 cs(type v) { return cs_of_turn(v); }
-};
-template <> struct AngOf<double> {
-  using type = double2;
-  __device__ static type load(const OpArgs<double> &a, int64_t g) { return a.cs[g]; }
-  __device__ static double2 cs(type v) { return v; }
-};
-
 // Lane i of a 16-lane row reads lane i - OFF of the same row (DPP row_shr;
 // lanes below OFF read 0).
 template <int OFF>
@@ -637,15 +649,16 @@ __global__ __launch_bounds__(256) void k_single(OpArgs<T> a) {
 #pragma unroll
     for (int i = 0; i < 16; i++) v[i] = mk<V>(valid ? w[t + TS * i] : (T)0, (T)0);
   } else {
+    // unconditional loads (a thread past the last pair reads sample 0 and
+    // drops it), as in col_fwd_body
+    const bool ha = valid, hb = valid && gb >= 0;
+    const int64_t gaa = ha ? ga : 0, gbb = hb ? gb : gaa;
 #pragma unroll
     for (int i = 0; i < 16; i++) {
       const int64_t s = t + TS * i;
-      T xa = (T)0, xb = (T)0;
-      if (valid) {
-        xa = load_src<T, SRC>(a, ga + s);
-        if (gb >= 0) xb = load_src<T, SRC>(a, gb + s);
-      }
-      v[i] = mk<V>(xa, xb);
+      const T xa = load_src<T, SRC>(a, gaa + s);
+      const T xb = load_src<T, SRC>(a, gbb + s);
+      v[i] = mk<V>(ha ? xa : (T)0, hb ? xb : (T)0);
     }
   }
   double chi = 0.0;
