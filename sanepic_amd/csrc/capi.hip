@@ -188,6 +188,7 @@ OpArgs<double> op_args<double>(sanepic_ctx *c) {
   a.g = c->g;
   a.pix = c->pix;
   a.cs = c->cs;
+  a.ang64 = c->ang64;
   a.nc = c->nc;
   a.relf = c->d_relf64;
   a.spd = c->nseg * c->ns;
@@ -656,8 +657,9 @@ int sanepic_create(const sanepic_config *cfg, sanepic_ctx **out) {
   if (const char *e = getenv("SANEPIC_RUNS")) c->runs = atoi(e) != 0;
   if (const char *e = getenv("SANEPIC_PT_XCD")) c->pt_xcd = atoi(e) != 0;
   if (const char *e = getenv("SANEPIC_MONO_FFT")) c->mono_fft = atoi(e) != 0;
-  if (const char *e = getenv("SANEPIC_RUN_LOADS32")) c->run_loads32 = std::min(4, std::max(0, atoi(e)));
-  if (const char *e = getenv("SANEPIC_RUN_LOADS64")) c->run_loads64 = std::min(4, std::max(0, atoi(e)));
+  if (const char *e = getenv("SANEPIC_TURN64")) c->turn64 = atoi(e) != 0;
+  if (const char *e = getenv("SANEPIC_RUN_LOADS32")) c->run_loads32 = std::min(6, std::max(0, atoi(e)));
+  if (const char *e = getenv("SANEPIC_RUN_LOADS64")) c->run_loads64 = std::min(6, std::max(0, atoi(e)));
   if (const char *e = getenv("SANEPIC_COL_WAVES")) c->col_waves = atoi(e);
   if (const char *e = getenv("SANEPIC_COL_WAVES64")) c->col_waves64 = atoi(e);
   if (const char *e = getenv("SANEPIC_ROW_ORDER")) c->row_det = atoi(e) != 0;
@@ -1122,6 +1124,10 @@ int sanepic_pixelize(sanepic_ctx *c, int64_t *npix_out, int64_t *n_deferred) {
   if (c->cfg.precision == SANEPIC_MIXED) {
     ACHK(c, dalloc(c, c->ang32, ntot));
     LCHK(c, launch_cs_to_turn(c->cs, c->ang32, ntot, c->st));
+  }
+  if (c->turn64 && c->nc == 3) {
+    ACHK(c, dalloc(c, c->ang64, ntot));
+    LCHK(c, launch_cs_to_turn64(c->cs, c->ang64, ntot, c->st));
   }
   trace_mark("pixelize: sorted angles");
   ACHK(c, build_items(c, ntot, c->items_nested && c->d_slot2pix != nullptr));

@@ -57,6 +57,7 @@ enum : int {
   SRC_TOD = 1,     // the TOD (right-hand side)
   SRC_RNG = 2,     // white Gaussian samples from (seed, sample index) (FP64 only)
   SRC_PX = 3,      // P x already evaluated in time order (launch_gather_px)
+  SRC_GATHER_T64 = 4,  // as SRC_GATHER in FP64, the angle read as a 64-bit turn (ang64)
 };
 
 // Counter-based standard normal deviate: the SplitMix64 finalizer (Steele,
@@ -91,6 +92,7 @@ struct OpArgs {
   const int32_t *pix;   // [ntot] compressed pixel
   const V *cs;          // [ntot] cos 2psi, sin 2psi (FP64 operator)
   const uint32_t *ang;  // [ntot] 2psi as a fraction of a turn (FP32 operator, turn.h)
+  const uint64_t *ang64;  // [ntot] 2psi as a 64-bit turn (FP64 gather, SRC_GATHER_T64) or null
   const T *map;         // [npix][nc] interleaved I, Q, U (, L)
   int nc;               // 3, or 4 with the leakage component
   const T *relf;        // [ndet] leakage coefficient per detector (nc == 4)
@@ -202,6 +204,8 @@ int launch_gather_cs(const uint32_t *perm, const double2 *src, double2 *dst, int
 // dst[i] = the angle of src[perm[i]] (or of src[i]) as a fraction of a turn (turn.h)
 int launch_gather_turn(const uint32_t *perm, const double2 *src, uint32_t *dst, int64_t n, hipStream_t s);
 int launch_cs_to_turn(const double2 *src, uint32_t *dst, int64_t n, hipStream_t s);
+// the same as 64-bit turns (turn.h) for the FP64 gather
+int launch_cs_to_turn64(const double2 *src, uint64_t *dst, int64_t n, hipStream_t s);
 
 // TOD NaN fill (784-790) on the device, per segment.
 int launch_tod_nanfill(double *tod, int64_t ntot, int64_t ns, hipStream_t s);

@@ -319,6 +319,10 @@ __global__ void k_gather_turn(const uint32_t *perm, const double2 *src, uint32_t
     dst[i] = turn_of(v.x, v.y);
   }
 }
+__global__ void k_cs_to_turn64(const double2 *src, uint64_t *dst, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = turn64_of(src[i].x, src[i].y);
+}
 __global__ void k_cs_to_turn(const double2 *src, uint32_t *dst, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) dst[i] = turn_of(src[i].x, src[i].y);
@@ -639,6 +643,10 @@ int launch_gather_cs(const uint32_t *perm, const double2 *src, double2 *dst, int
 int launch_gather_turn(const uint32_t *perm, const double2 *src, uint32_t *dst, int64_t n,
                        hipStream_t s) {
   hipLaunchKernelGGL(k_gather_turn, dim3(nblk(n)), dim3(256), 0, s, perm, src, dst, n);
+  return ok();
+}
+int launch_cs_to_turn64(const double2 *src, uint64_t *dst, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_cs_to_turn64, dim3(nblk(n)), dim3(256), 0, s, src, dst, n);
   return ok();
 }
 int launch_cs_to_turn(const double2 *src, uint32_t *dst, int64_t n, hipStream_t s) {

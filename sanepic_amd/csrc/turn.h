@@ -40,3 +40,53 @@ __device__ __forceinline__ float2 cs_of_turn(uint32_t u) {
   if (u == kNanTurn) a = b = __int_as_float(0x7fc00000);
   return make_float2(a, b);
 }
+
+// The FP64 operator's angle as a 64-bit fraction of a turn (even multiples of
+// 2^-64 turn; the odd value kNanTurn64 marks a NaN angle): 8 bytes instead of
+// the 16 of an FP64 (cos, sin) pair.  atan2 / (2 pi) carries ~1e-16 of
+// relative error, so the decoded pair is within a few 1e-16 of the stored one.
+constexpr uint64_t kNanTurn64 = ~(uint64_t)0;
+
+__device__ __forceinline__ uint64_t turn64_of(double c, double s) {
+  const double t = atan2(s, c) * 0.15915494309189533577;  // t in [-1/2, 1/2]
+  if (!(t == t)) return kNanTurn64;
+  const int64_t k = (int64_t)llrint(t * 9223372036854775808.0 * 0.5);  // t 2^62, |k| <= 2^61
+  return (uint64_t)k << 2;
+}
+
+// (cos, sin) in FP64 of a 64-bit turn: the nearest quarter turn taken off in
+// integer arithmetic, x = f 2 pi / 2^64 with |x| <= pi/4, then the Taylor
+// series of sin to x^15 and of cos to x^16 (the next terms are below 5e-17
+// there) and the quarter-turn rotation.
+__device__ __forceinline__ double2 cs_of_turn64(uint64_t u) {
+  const uint64_t q = (u + ((uint64_t)1 << 61)) >> 62;
+  const int64_t f = (int64_t)(u - (q << 62));  // |f| <= 2^61
+  const double x = (double)f * 3.4061215800865545e-19;  // 2 pi / 2^64
+  // explicit fused multiply-adds: every kernel that decodes gets the same bits
+  const double x2 = x * x;
+  double ps = -1.0 / 1307674368000.0;
+  ps = fma(ps, x2, 1.0 / 6227020800.0);
+  ps = fma(ps, x2, -1.0 / 39916800);
+  ps = fma(ps, x2, 1.0 / 362880);
+  ps = fma(ps, x2, -1.0 / 5040);
+  ps = fma(ps, x2, 1.0 / 120);
+  ps = fma(ps, x2, -1.0 / 6);
+  const double s = fma(x * x2, ps, x);
+  double pc = 1.0 / 20922789888000.0;
+  pc = fma(pc, x2, -1.0 / 87178291200.0);
+  pc = fma(pc, x2, 1.0 / 479001600);
+  pc = fma(pc, x2, -1.0 / 3628800);
+  pc = fma(pc, x2, 1.0 / 40320);
+  pc = fma(pc, x2, -1.0 / 720);
+  pc = fma(pc, x2, 1.0 / 24);
+  pc = fma(pc, x2, -0.5);
+  const double c = fma(x2, pc, 1.0);
+  double a = (q & 1) ? -s : c;
+  double b = (q & 1) ? c : s;
+  if (q & 2) {
+    a = -a;
+    b = -b;
+  }
+  if (u == kNanTurn64) a = b = __longlong_as_double(0x7ff8000000000000LL);
+  return make_double2(a, b);
+}

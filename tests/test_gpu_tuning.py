@@ -206,3 +206,24 @@ def test_monopole_image_without_fft(data, monkeypatch):
         ctx.close()
     print("  crit", crit["0"][:3], crit["1"][:3], flush=True)
     assert np.allclose(crit["0"], crit["1"], rtol=1e-9, atol=0)
+
+
+def test_fp64_angles_as_turns(data, monkeypatch):
+    """The FP64 operator reading 2psi as 64-bit turns (decoded by integer
+    quarter-turn reduction and Taylor series) against reading the stored
+    (cos, sin) pairs: the decoded pairs are within a few 1e-16, so A x agrees
+    to rounding, and both match the oracle."""
+    cfg, ds, P = data
+    x = np.random.default_rng(13).standard_normal(3 * P.npix)
+    want = P.apply_A(x)
+    got = {}
+    for v in ("0", "1"):
+        monkeypatch.setenv("SANEPIC_TURN64", v)
+        ctx = _ctx(cfg, ds)
+        got[v] = (ctx.apply(x), ctx.chi2(x), ctx.device_bytes())
+        ctx.close()
+    d = rel(got["1"][0], got["0"][0])
+    print(f"  turns vs pairs {d:.2e}, vs oracle {rel(got['1'][0], want):.2e}", flush=True)
+    assert d < 1e-14 and rel(got["1"][0], want) < 1e-11
+    assert abs(got["1"][1] - got["0"][1]) <= 1e-12 * abs(got["0"][1])
+    assert got["1"][2] > got["0"][2]  # the turns were made
