@@ -324,6 +324,12 @@ int sanepic_device_bytes(sanepic_ctx *ctx, int64_t *bytes);
  * layout is off, or the current segment length / FFT split cannot use it).
  * SANEPIC_RUNS=0 at creation turns the layout off. */
 int sanepic_run_records(sanepic_ctx *ctx, int64_t *nrec, int32_t *active);
+/* Bytes per sample of the FP64 operator's polarisation angle in its gather
+ * and record passes (no counterpart in the reference, which recomputes
+ * cos/sin 2psi inside its loops): 8 when 2psi is stored as a 64-bit fraction
+ * of a turn (the default), 16 for a (cos 2psi, sin 2psi) pair
+ * (SANEPIC_TURN64=0 at creation). */
+int sanepic_fp64_angle_bytes(sanepic_ctx *ctx, int32_t *bytes);
 
 /* ------------------------------------------------------------------------
  * Learned preconditioner.  None of this has a counterpart in the reference:

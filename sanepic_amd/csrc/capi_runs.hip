@@ -161,3 +161,10 @@ int sanepic_run_records(sanepic_ctx *c, int64_t *nrec, int32_t *active) {
   *active = use_runs(c) ? 1 : 0;
   return SANEPIC_OK;
 }
+
+int sanepic_fp64_angle_bytes(sanepic_ctx *c, int32_t *bytes) {
+  ACHK(c, check_ready(c, false));
+  if (!bytes) return fail(c, SANEPIC_EINVAL, "null output");
+  *bytes = c->ang64 ? 8 : 16;
+  return SANEPIC_OK;
+}

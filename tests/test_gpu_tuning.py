@@ -221,6 +221,7 @@ def test_fp64_angles_as_turns(data, monkeypatch):
         monkeypatch.setenv("SANEPIC_TURN64", v)
         ctx = _ctx(cfg, ds)
         got[v] = (ctx.apply(x), ctx.chi2(x), ctx.device_bytes())
+        assert ctx.fp64_angle_bytes() == (8 if v == "1" else 16)
         ctx.close()
     d = rel(got["1"][0], got["0"][0])
     print(f"  turns vs pairs {d:.2e}, vs oracle {rel(got['1'][0], want):.2e}", flush=True)
