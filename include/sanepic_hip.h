@@ -328,7 +328,9 @@ int sanepic_run_records(sanepic_ctx *ctx, int64_t *nrec, int32_t *active);
  * and record passes (no counterpart in the reference, which recomputes
  * cos/sin 2psi inside its loops): 8 when 2psi is stored as a 64-bit fraction
  * of a turn (the default), 16 for a (cos 2psi, sin 2psi) pair
- * (SANEPIC_TURN64=0 at creation). */
+ * (SANEPIC_TURN64=0 at creation, and segment lengths that the four-step
+ * power-of-two passes do not take).  With per-detector lengths: the
+ * sample-weighted mean over the detector groups, rounded. */
 int sanepic_fp64_angle_bytes(sanepic_ctx *ctx, int32_t *bytes);
 
 /* ------------------------------------------------------------------------

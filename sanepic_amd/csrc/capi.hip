@@ -1125,7 +1125,7 @@ int sanepic_pixelize(sanepic_ctx *c, int64_t *npix_out, int64_t *n_deferred) {
     ACHK(c, dalloc(c, c->ang32, ntot));
     LCHK(c, launch_cs_to_turn(c->cs, c->ang32, ntot, c->st));
   }
-  if (c->turn64 && c->nc == 3) {
+  if (c->turn64 && c->nc == 3 && c->g.M == 1 && !c->single) {  // the four-step gather reads them
     ACHK(c, dalloc(c, c->ang64, ntot));
     LCHK(c, launch_cs_to_turn64(c->cs, c->ang64, ntot, c->st));
   }

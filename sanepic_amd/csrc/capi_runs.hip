@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
+#include <cmath>
 #include <vector>
 
 #include "ctx.h"
@@ -165,6 +166,16 @@ int sanepic_run_records(sanepic_ctx *c, int64_t *nrec, int32_t *active) {
 int sanepic_fp64_angle_bytes(sanepic_ctx *c, int32_t *bytes) {
   ACHK(c, check_ready(c, false));
   if (!bytes) return fail(c, SANEPIC_EINVAL, "null output");
-  *bytes = c->ang64 ? 8 : 16;
+  if (c->sub.empty()) {
+    *bytes = c->ang64 ? 8 : 16;
+    return SANEPIC_OK;
+  }
+  // per-detector lengths: the sample-weighted mean over the detector groups
+  double n = 0.0, w = 0.0;
+  for (const sanepic_ctx *s : c->sub) {
+    n += (double)s->ntot;
+    w += (double)s->ntot * (s->ang64 ? 8.0 : 16.0);
+  }
+  *bytes = n > 0.0 ? (int32_t)std::lround(w / n) : 16;
   return SANEPIC_OK;
 }
