@@ -503,9 +503,18 @@ __device__ __forceinline__ double row_shr(double v) {
 template <int OFF, typename T>
 __device__ __forceinline__ void seg_step(T &x0, T &x1, T &x2, int i, uint32_t m) {
   const bool pass = ((m >> i) & ((1u << OFF) - 1u)) == 0u;
-  x0 += row_shr<OFF>(pass ? x0 : (T)0);
-  x1 += row_shr<OFF>(pass ? x1 : (T)0);
-  x2 += row_shr<OFF>(pass ? x2 : (T)0);
+  if constexpr (sizeof(T) == 8) {
+    // a 64-bit select is two instructions, a product by 1 or 0 one (exact:
+    // the value or a zero, which the receiver's add leaves unchanged)
+    const T w = pass ? 1.0 : 0.0;
+    x0 += row_shr<OFF>(x0 * w);
+    x1 += row_shr<OFF>(x1 * w);
+    x2 += row_shr<OFF>(x2 * w);
+  } else {
+    x0 += row_shr<OFF>(pass ? x0 : (T)0);
+    x1 += row_shr<OFF>(pass ? x1 : (T)0);
+    x2 += row_shr<OFF>(pass ? x2 : (T)0);
+  }
 }
 
 // z of sample g, held by lane i = g % 16 of a 16-lane row that holds g's whole
