@@ -1576,7 +1576,7 @@ int sanepic_setup(sanepic_ctx *c) {
   ACHK(c, sync_tod_uploads(c));
   trace_mark("setup: TOD uploads done");
   ACHK(c, setup_data(c));
-  flush_frees(c);
+  if (!keep_temporaries(c)) flush_frees(c);
   trace_mark("setup: deferred frees");
   c->setup_done = true;
   return SANEPIC_OK;
