@@ -378,6 +378,18 @@ def main():
     ntot = sum(cfg.used(d) for d in range(cfg.ndet)) if uneven else cfg.ndet * cfg.nseg * cfg.ns
     log(f"inputs generated in {time.perf_counter() - tg:.1f} s "
         f"({cfg.ndet} det, {ntot} samples, NSIDE {cfg.nside}), in pinned host memory")
+    # Untimed: the process's first context of this size pays ~50 ms more in
+    # creation and upload than every later one (measured: not the device
+    # allocations, not the pinned pages); one created, loaded and destroyed
+    # here gives every pipeline the same start.
+    tw = time.perf_counter()
+    if uneven:
+        with sanepic_amd.Context.uneven(cfg.nside, cfg.seglen, cfg.nsamp) as w:
+            w.load_uneven(cfg, pin, queued=True)
+    else:
+        with sanepic_amd.Context(cfg.nside, cfg.ndet, cfg.nseg, cfg.ns) as w:
+            w.load_async(cfg, pin)
+    log(f"first-context warm-up (create, upload, destroy) {time.perf_counter() - tw:.2f} s")
 
     weights, record = None, None
     path = args.weights or train_precond.weights_path(args.workload)
