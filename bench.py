@@ -9,7 +9,9 @@ the form a caller hands to the library.
 Complete solves are timed, each from inputs in pinned host memory to a true
 relative residual |b - A x| / |b| <= 1e-6 (H2D, pointing, ang2pix, sort, b,
 preconditioner, CG; file I/O excluded), after one small untimed solve that
-loads the library's code onto the device:
+loads the library's code onto the device and one untimed context of the
+workload's size (created, loaded, destroyed) that takes the process's
+first-context cost out of the first pipeline:
   headline   FP64 operator, plain 3x3 block-Jacobi PCG (the reference's
              arithmetic); after its time to solution it iterates on, untimed,
              to |r|^2/|b|^2 <= 1e-24 (a true residual near 1e-13) for the
