@@ -177,6 +177,8 @@ def test_nside512_segments_2e20_fp64(n512):
     e_c = abs(ctx.chi2(x) - c_or) / c_or
     print(f"  FP64: b {e_b:.1e}, A x {e_a:.1e}, chi2 {e_c:.1e}", flush=True)
     assert e_b < 1e-11 and e_a < 1e-11 and e_c < 1e-10
+    # the headline's kernels: P^T over run records, 2psi as 64-bit turns
+    assert ctx.run_records()[1] and ctx.fp64_angle_bytes() == 8
     ctx.close()
 
 
@@ -188,6 +190,7 @@ def test_nside512_segments_2e20_mixed(n512):
     e_b = rel(ctx.rhs(), P.rhs())
     print(f"  mixed: A x {e_a:.2e} (FP32 operator), b {e_b:.1e} (FP64)", flush=True)
     assert 1e-9 < e_a < 2e-6 and e_b < 1e-11
+    assert ctx.run_records()[1]
     ctx.close()
 
 
