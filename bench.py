@@ -342,9 +342,11 @@ def main():
     ap.add_argument("--no-literal", action="store_true",
                     help="skip the learned pipeline with the literal fallback rule (growth 1)")
     ap.add_argument("--no-deflate", action="store_true", help="mixed pipelines without the monopole deflation")
-    ap.add_argument("--mixed-refresh", type=int, default=20,
+    ap.add_argument("--mixed-refresh", type=int, default=40,
                     help="FP64 explicit residual every this many iterations in the mixed-precision pipelines "
-                         "(the FP64 solve keeps the reference's 10)")
+                         "(the FP64 solve keeps the reference's 10); the solve is confirmed by an explicit "
+                         "FP64 residual in any case (measured: 20 -> 40 leaves the iteration counts and "
+                         "map errors unchanged and saves one FP64 operator per solve)")
     ap.add_argument("--maxiter", type=int, default=600)
     ap.add_argument("--settle", type=float, default=5.0,
                     help="idle seconds before each pipeline after the first, outside the timed regions: a "
