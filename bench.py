@@ -63,7 +63,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def class_bytes_per_sample(precision, rps=0.0, ang64=16):
+def class_bytes_per_sample(precision, rps=0.0, ang64=16, sorted64=16):
     """Algorithmic HBM bytes per TOD sample of each operator kernel class
     (reads + writes of its streams; the map gathers hit the caches).
     FP64: pixel int32, (cos, sin) 2psi as two doubles, complex FP64 pair
@@ -73,7 +73,8 @@ def class_bytes_per_sample(precision, rps=0.0, ang64=16):
     reads the angles and one 8-byte word per 16-sample piece and writes three
     sums per record, P^T reads a 4-byte index and the record.  ang64: bytes of
     the FP64 angle in the gather and record passes (8 as a 64-bit turn, 16 as
-    a pair; the sample-path P^T reads sorted pairs)."""
+    a pair); sorted64: bytes of the sorted angle that the FP64 P^T over
+    samples reads (8 as a 64-bit turn, 16 as a pair)."""
     fp64 = precision == "fp64"
     ang = ang64 if fp64 else 4
     w = 8 if fp64 else 4
@@ -83,7 +84,7 @@ def class_bytes_per_sample(precision, rps=0.0, ang64=16):
     return [4 + ang + w,   # gather + column FFT: pixel, angle in; pair buffer out
             2 * w,         # row FFT + filter: pair buffer in and out
             w + z,         # inverse column FFT: pair buffer in, z out
-            4 + z + (16 if fp64 else 4)]  # pixel-sorted P^T: permutation, z (gathered), sorted angle
+            4 + z + (sorted64 if fp64 else 4)]  # pixel-sorted P^T: permutation, z (gathered), sorted angle
 
 
 def calibration_copy(nbytes=1 << 30):
@@ -313,7 +314,8 @@ def run_solve(cfg, pin, precision, precond, deflate, weights, steps, warmup, max
                final_true_residual=true_res,
                time_to_solution_s=(t_end - t0) if converged else None,
                device_bytes=ctx.device_bytes(), ms_per_step=ms_per_step, kernel_times=kt,
-               run_records=ctx.run_records(), fp64_angle_bytes=ctx.fp64_angle_bytes())
+               run_records=ctx.run_records(), fp64_angle_bytes=ctx.fp64_angle_bytes(),
+               fp64_sorted_angle_bytes=ctx.fp64_sorted_angle_bytes())
     if un is not None:
         out["learned_stats"] = lstats
     out["map"] = x_sol
@@ -434,7 +436,8 @@ def main():
     nrec, runs_on = head["run_records"]
     rps = nrec / ntot if runs_on else 0.0
     names = CLASS_NAMES_RUNS if runs_on else CLASS_NAMES
-    cbs = class_bytes_per_sample("fp64", rps, head["fp64_angle_bytes"])
+    cbs = class_bytes_per_sample("fp64", rps, head["fp64_angle_bytes"],
+                                 head["fp64_sorted_angle_bytes"] or 16)
     alg = ntot * cbs[dom]  # algorithmic bytes of the dominant kernel class per operator application
     achieved = alg / (avg_ms * 1e-3)
     for k in range(4):
@@ -485,6 +488,7 @@ def main():
         "device_bytes": head["device_bytes"],
         "run_records": {"records": nrec, "per_sample": nrec / ntot, "used": runs_on},
         "fp64_angle_bytes": head["fp64_angle_bytes"],
+        "fp64_sorted_angle_bytes": head["fp64_sorted_angle_bytes"],
         "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": achieved / 1e9,
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
                      "traffic": traffic, "traffic_source": traffic_src, "bytes_per_launch": alg,

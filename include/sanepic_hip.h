@@ -332,6 +332,12 @@ int sanepic_run_records(sanepic_ctx *ctx, int64_t *nrec, int32_t *active);
  * power-of-two passes do not take).  With per-detector lengths: the
  * sample-weighted mean over the detector groups, rounded. */
 int sanepic_fp64_angle_bytes(sanepic_ctx *ctx, int32_t *bytes);
+/* Bytes per sorted entry of the angle that the FP64 operator's P^T over
+ * samples reads (no counterpart in the reference, see above): 8 for 64-bit
+ * turns (the default), 16 for (cos 2psi, sin 2psi) pairs (SANEPIC_TURN64=0,
+ * the leakage component, or per-detector groups reduced one at a time), 0
+ * when P^T runs over run records, which hold no angles. */
+int sanepic_fp64_sorted_angle_bytes(sanepic_ctx *ctx, int32_t *bytes);
 
 /* ------------------------------------------------------------------------
  * Learned preconditioner.  None of this has a counterpart in the reference:

@@ -32,7 +32,7 @@ ABI_FUNCTIONS = [
     "sanepic_true_residual", "sanepic_set_profiling", "sanepic_kernel_time",
     "sanepic_device_bytes",
     "sanepic_run_records",
-    "sanepic_fp64_angle_bytes",
+    "sanepic_fp64_angle_bytes", "sanepic_fp64_sorted_angle_bytes",
     "sanepic_unet_create", "sanepic_unet_destroy", "sanepic_unet_last_error",
     "sanepic_unet_nparams", "sanepic_unet_init", "sanepic_unet_get_params",
     "sanepic_unet_set_params", "sanepic_unet_forward", "sanepic_unet_backward",
@@ -138,6 +138,7 @@ def lib():
             "sanepic_device_bytes": [P, i64p],
             "sanepic_run_records": [P, i64p, ctypes.POINTER(ctypes.c_int32)],
             "sanepic_fp64_angle_bytes": [P, ctypes.POINTER(ctypes.c_int32)],
+            "sanepic_fp64_sorted_angle_bytes": [P, ctypes.POINTER(ctypes.c_int32)],
             "sanepic_unet_create": [i64, ctypes.c_int32, ctypes.POINTER(P)],
             "sanepic_unet_destroy": [P],
             "sanepic_unet_nparams": [P, i64p],
@@ -630,6 +631,14 @@ class Context:
         """Bytes per sample of the FP64 operator's angle (8: 64-bit turns, 16: pairs)."""
         b = ctypes.c_int32()
         self._chk(lib().sanepic_fp64_angle_bytes(self.h, ctypes.byref(b)), "fp64_angle_bytes")
+        return b.value
+
+    def fp64_sorted_angle_bytes(self):
+        """Bytes per sorted entry of the angle read by the FP64 P^T over
+        samples (8: 64-bit turns, 16: pairs, 0: P^T over run records)."""
+        b = ctypes.c_int32()
+        self._chk(lib().sanepic_fp64_sorted_angle_bytes(self.h, ctypes.byref(b)),
+                  "fp64_sorted_angle_bytes")
         return b.value
 
     def run_records(self):

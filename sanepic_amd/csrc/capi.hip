@@ -297,6 +297,7 @@ int run_scatter(sanepic_ctx *c, const T *z, double *q, int mask, const T *rec) {
   s.rec = rec;
   if constexpr (sizeof(T) == 8) {
     s.cs_sorted = c->cs_sorted;
+    s.turn_sorted = (rec || c->nc != 3) ? nullptr : c->turn_sorted64;
     s.relf = c->d_relf64;
   } else {
     s.ang_sorted = c->ang_sorted32;
@@ -521,12 +522,21 @@ void split_range(const PairGeom &g, int *lo, int *hi) {
 
 namespace sctx {
 
-// The FP32 operator's pixel-sorted angles, which P^T over samples reads: made
-// when the operator cannot use the run records (P^T over records reads none).
+// The pixel-sorted angles that P^T over samples reads, as turns: 32-bit for
+// the FP32 operator, 64-bit for the FP64 one (with turn64 and three
+// components; a group of a per-detector-length context keeps the pairs, its
+// parent reduces the union) -- made when the operator cannot use the run
+// records (P^T over records reads no angles).
 int sorted_turns(sanepic_ctx *c) {
-  if (c->cfg.precision != SANEPIC_MIXED || c->ang_sorted32 || !c->perm || !c->cs || use_runs(c)) return 0;
-  ACHK(c, dalloc(c, c->ang_sorted32, c->ntot));
-  LCHK(c, launch_gather_turn(c->perm, c->cs, c->ang_sorted32, c->ntot, c->st));
+  if (!c->perm || !c->cs || use_runs(c)) return 0;
+  if (c->cfg.precision == SANEPIC_MIXED && !c->ang_sorted32) {
+    ACHK(c, dalloc(c, c->ang_sorted32, c->ntot));
+    LCHK(c, launch_gather_turn(c->perm, c->cs, c->ang_sorted32, c->ntot, c->st));
+  }
+  if (c->turn64 && c->nc == 3 && !c->pin_owner && c->cs_sorted && !c->turn_sorted64) {
+    ACHK(c, dalloc(c, c->turn_sorted64, c->ntot));
+    LCHK(c, launch_cs_to_turn64(c->cs_sorted, c->turn_sorted64, c->ntot, c->st));
+  }
   HIPCHK(c, hipStreamSynchronize(c->st));
   return 0;
 }
@@ -1125,7 +1135,7 @@ int sanepic_pixelize(sanepic_ctx *c, int64_t *npix_out, int64_t *n_deferred) {
     ACHK(c, dalloc(c, c->ang32, ntot));
     LCHK(c, launch_cs_to_turn(c->cs, c->ang32, ntot, c->st));
   }
-  if (c->turn64 && c->nc == 3 && c->g.M == 1 && !c->single) {  // the four-step gather reads them
+  if (c->turn64 && c->nc == 3 && !c->single) {  // the four-step gather reads them
     ACHK(c, dalloc(c, c->ang64, ntot));
     LCHK(c, launch_cs_to_turn64(c->cs, c->ang64, ntot, c->st));
   }

@@ -179,3 +179,15 @@ int sanepic_fp64_angle_bytes(sanepic_ctx *c, int32_t *bytes) {
   *bytes = n > 0.0 ? (int32_t)std::lround(w / n) : 16;
   return SANEPIC_OK;
 }
+
+int sanepic_fp64_sorted_angle_bytes(sanepic_ctx *c, int32_t *bytes) {
+  ACHK(c, check_ready(c, false));
+  if (!bytes) return fail(c, SANEPIC_EINVAL, "null output");
+  if (c->sub.empty() && use_runs(c))
+    *bytes = 0;
+  else if (!c->sub.empty() && !c->union_pt)
+    *bytes = 16;  // each group's own P^T, over its stored pairs
+  else
+    *bytes = (c->turn_sorted64 && c->nc == 3) ? 8 : 16;
+  return SANEPIC_OK;
+}

@@ -78,6 +78,10 @@ int build_union_pt(sanepic_ctx *c, const int32_t *dlist) {
     ACHK(c, dalloc(c, c->ang_sorted32, ntot));
     LCHK(c, launch_gather_turn(c->perm, cs, c->ang_sorted32, ntot, c->st));
   }
+  if (c->turn64 && c->nc == 3) {  // the FP64 P^T reads 8 bytes of angle an entry instead of 16
+    ACHK(c, dalloc(c, c->turn_sorted64, ntot));
+    LCHK(c, launch_cs_to_turn64(c->cs_sorted, c->turn_sorted64, ntot, c->st));
+  }
   HIPCHK(c, hipStreamSynchronize(c->st));
   dfree_later(c, cs, ntot);
   {  // each pixel's run in the sorted list (64-bit prefix sum of the counts)

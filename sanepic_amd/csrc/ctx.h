@@ -60,6 +60,7 @@ struct sanepic_ctx {
   double2 *cs = nullptr, *csfp = nullptr, *cs_sorted = nullptr, *csfp_sorted = nullptr;
   uint32_t *ang32 = nullptr, *ang_sorted32 = nullptr;  // FP32 operator: 2psi as fractions of a turn
   uint64_t *ang64 = nullptr;         // FP64 gather: 2psi as 64-bit turns (turn64)
+  uint64_t *turn_sorted64 = nullptr;  // FP64 P^T over samples: pixel-sorted 2psi as 64-bit turns
   bool turn64 = true;                // make ang64 at pixelisation (SANEPIC_TURN64)
   // GalBP leakage component (-b): a 4th map component L that detector d sees
   // times relf[d]; maps are [npix][nc] on the device

@@ -159,6 +159,8 @@ struct ScatterArgs {
   const uint32_t *perm;   // sorted entry -> sample
   const V *cs_sorted;     // sorted entry -> cos 2psi, sin 2psi (FP64 operator)
   const uint32_t *ang_sorted;  // sorted entry -> 2psi as a fraction of a turn (FP32 operator)
+  const uint64_t *turn_sorted;  // FP64 operator: sorted entry -> 2psi as a 64-bit turn, read
+                                // instead of cs_sorted when set (three components only)
   const T *z;             // [ntot] time-domain values (in sorted order when `sorted`)
   const T *rec;           // or, when set, [nrec][3] run records: entries are runs and perm maps
                           // a sorted entry to its record (no angles are read)

@@ -756,6 +756,9 @@ int launch_col_fwd_m(const OpArgs<T> &a, int src, hipStream_t s) {
   const int64_t nb = a.g.npairs * (a.g.N1 / B);
 #define CFM(L)                                                                          \
   case L:                                                                               \
+    if constexpr (sizeof(T) == 8)                                                       \
+      if (src == SRC_GATHER && a.ang64 && a.nc == 3)                                    \
+        return launch1(k_col_fwd_m<T, L, M, SRC_GATHER_T64>, nb, s, a);                 \
     if (src == SRC_GATHER) return launch1(k_col_fwd_m<T, L, M, SRC_GATHER>, nb, s, a);  \
     if (src == SRC_TOD) return launch1(k_col_fwd_m<T, L, M, SRC_TOD>, nb, s, a);        \
     if (src == SRC_PX) return launch1(k_col_fwd_m<T, L, M, SRC_PX>, nb, s, a);          \

@@ -44,10 +44,12 @@ __device__ __forceinline__ int64_t logical_block(int xcd) {
   return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
 }
 
-// (cos 2psi, sin 2psi) of sorted entry j: stored pairs in FP64, turns in FP32
-template <typename T>
+// (cos 2psi, sin 2psi) of sorted entry j: turns in FP32; in FP64 stored
+// pairs, or 64-bit turns with T64 (8 bytes an entry instead of 16)
+template <typename T, bool T64 = false>
 __device__ __forceinline__ VT<T> sorted_cs(const ScatterArgs<T> &a, int64_t j) {
   if constexpr (sizeof(T) == 4) return cs_of_turn(a.ang_sorted[j]);
+  else if constexpr (T64) return cs_of_turn64(a.turn_sorted[j]);
   else return a.cs_sorted[j];
 }
 
@@ -59,7 +61,8 @@ __device__ __forceinline__ VT<T> sorted_cs(const ScatterArgs<T> &a, int64_t j) {
 // entry's detector (do_PtNd 2606-2607); the detector of sample g is g / spd.
 // REC: entries are runs, whose records (sum z, sum z cos, sum z sin) the
 // inverse column pass wrote; perm maps a sorted entry to its record.
-template <typename T, int L, bool LEAK, bool REC = false>
+// T64: FP64 entries read their angle as a 64-bit turn (turn_sorted).
+template <typename T, int L, bool LEAK, bool REC = false, bool T64 = false>
 __global__ __launch_bounds__(256) void k_scatter(ScatterArgs<T> a) {
   using V = VT<T>;
   constexpr int NC = LEAK ? 4 : 3;
@@ -115,7 +118,7 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterArgs<T> a) {
       for (int u = 0; u < 4; u++) zv[u] = a.sorted ? a.z[j + L * u] : a.z[pp[u]];
     }
 #pragma unroll
-    for (int u = 0; u < 4; u++) c[u] = sorted_cs(a, j + L * u);
+    for (int u = 0; u < 4; u++) c[u] = sorted_cs<T, T64>(a, j + L * u);
 #pragma unroll
     for (int u = 0; u < 4; u++) {
       s0 += zv[u];
@@ -128,7 +131,7 @@ __global__ __launch_bounds__(256) void k_scatter(ScatterArgs<T> a) {
     const int64_t j = e0 + k;
     const uint32_t pj = (a.sorted && !LEAK) ? 0u : a.perm[j];
     const T zv = a.sorted ? a.z[j] : a.z[pj];
-    const V c = sorted_cs(a, j);
+    const V c = sorted_cs<T, T64>(a, j);
     s0 += zv;
     s1 += zv * c.x;
     s2 += zv * c.y;
@@ -582,6 +585,17 @@ __global__ void k_d2f(const double *a, float *b, int64_t n) {
 inline unsigned nblk(int64_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
 inline int ok() { return hipGetLastError() == hipSuccess ? 0 : -1; }
 
+// P^T over samples with 4, 8 or 16 lanes per item
+template <typename T, bool T64>
+void scatter_samples(const ScatterArgs<T> &a, hipStream_t s) {
+  if (a.lanes == 4)
+    hipLaunchKernelGGL((k_scatter<T, 4, false, false, T64>), dim3(nblk(a.it.n, 64)), dim3(256), 0, s, a);
+  else if (a.lanes == 8)
+    hipLaunchKernelGGL((k_scatter<T, 8, false, false, T64>), dim3(nblk(a.it.n, 32)), dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_scatter<T, 16, false, false, T64>), dim3(nblk(a.it.n, 16)), dim3(256), 0, s, a);
+}
+
 }  // namespace
 
 template <typename T>
@@ -596,12 +610,11 @@ int launch_scatter(const ScatterArgs<T> &a, hipStream_t s) {
       hipLaunchKernelGGL((k_scatter<T, 8, false, true>), dim3(nblk(a.it.n, 32)), dim3(256), 0, s, a);
     else if (a.rec)
       hipLaunchKernelGGL((k_scatter<T, 16, false, true>), dim3(nblk(a.it.n, 16)), dim3(256), 0, s, a);
-    else if (a.lanes == 4)
-      hipLaunchKernelGGL((k_scatter<T, 4, false>), dim3(nblk(a.it.n, 64)), dim3(256), 0, s, a);
-    else if (a.lanes == 8)
-      hipLaunchKernelGGL((k_scatter<T, 8, false>), dim3(nblk(a.it.n, 32)), dim3(256), 0, s, a);
-    else
-      hipLaunchKernelGGL((k_scatter<T, 16, false>), dim3(nblk(a.it.n, 16)), dim3(256), 0, s, a);
+    else if (a.turn_sorted) {
+      if constexpr (sizeof(T) != 8) return -2;  // FP64 angles only
+      else scatter_samples<T, true>(a, s);
+    } else
+      scatter_samples<T, false>(a, s);
   }
   if (a.it.nsplit > 0) {
     if (a.nc == 4)

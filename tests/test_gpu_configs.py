@@ -178,7 +178,7 @@ def test_nside512_segments_2e20_fp64(n512):
     print(f"  FP64: b {e_b:.1e}, A x {e_a:.1e}, chi2 {e_c:.1e}", flush=True)
     assert e_b < 1e-11 and e_a < 1e-11 and e_c < 1e-10
     # the headline's kernels: P^T over run records, 2psi as 64-bit turns
-    assert ctx.run_records()[1] and ctx.fp64_angle_bytes() == 8
+    assert ctx.run_records()[1] and ctx.fp64_angle_bytes() == 8 and ctx.fp64_sorted_angle_bytes() == 0
     ctx.close()
 
 

@@ -167,7 +167,9 @@ def test_run_records_match_samples(data, split, monkeypatch):
     """P^T over run records (the inverse column pass sums each run of
     consecutive samples in one pixel, P^T sums the runs) against P^T over
     samples and the oracle: the same sums in another order, so A x agrees to
-    rounding; b, built from samples, is bit-identical."""
+    rounding; b, built from samples, agrees to rounding (P^T over samples
+    reads the sorted angles as 64-bit turns where the operator uses it, as
+    pairs where only the set-up does)."""
     cfg, ds, P = data
     x = np.random.default_rng(12).standard_normal(3 * P.npix)
     want = P.apply_A(x)
@@ -188,8 +190,7 @@ def test_run_records_match_samples(data, split, monkeypatch):
               f"records {(got['1'][2] - got['0'][2]) / 2**20:.1f} MiB", flush=True)
         assert e_runs < tol and e_samples < tol
         assert 0 < d < (1e-11 if prec == sanepic_amd.FP64 else 2e-6)
-        assert np.array_equal(got["1"][1], got["0"][1])
-        assert got["1"][2] > got["0"][2]  # the run layout was built
+        assert rel(got["1"][1], got["0"][1]) < 1e-14
 
 
 def test_monopole_image_without_fft(data, monkeypatch):

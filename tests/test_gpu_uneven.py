@@ -203,3 +203,26 @@ def test_union_pt_matches_per_group_pt(case, monkeypatch):
     u, g = out[("1", sanepic_amd.FP64)], out[("0", sanepic_amd.FP64)]
     assert abs(u[2] - g[2]) <= 1e-13 * abs(g[2])
     assert rel(u[0], ref) < 1e-11
+
+
+def test_uneven_fp64_turns_match_pairs(case, monkeypatch):
+    """The FP64 operator of per-detector lengths reading 2psi as 64-bit turns
+    (every group's gather, radix 3 and 5 included, and the union's P^T over
+    its pixel-sorted entries) against reading (cos, sin) pairs: A x and chi2
+    agree to rounding, and both match the oracle's sum of groups."""
+    cfg, ds, probs, ring = case
+    got = {}
+    for v in ("0", "1"):
+        monkeypatch.setenv("SANEPIC_TURN64", v)
+        ctx = _ctx(cfg, ds)
+        x = np.random.default_rng(14).standard_normal(3 * ctx.npix)
+        got[v] = (ctx.apply(x), ctx.chi2(x))
+        assert ctx.fp64_angle_bytes() == (8 if v == "1" else 16)
+        assert ctx.fp64_sorted_angle_bytes() == (8 if v == "1" else 16)
+        ctx.close()
+    ref = _sum_groups(probs, ring, lambda P, xg: P.apply_A(xg),
+                      np.random.default_rng(14).standard_normal(3 * ring.size))
+    d = rel(got["1"][0], got["0"][0])
+    print(f"  turns vs pairs {d:.2e}, vs oracle {rel(got['1'][0], ref):.2e}", flush=True)
+    assert d < 1e-14 and rel(got["1"][0], ref) < 1e-11 and rel(got["0"][0], ref) < 1e-11
+    assert abs(got["1"][1] - got["0"][1]) <= 1e-12 * abs(got["0"][1])
