@@ -85,11 +85,11 @@ struct sanepic_ctx {
   bool union_pt = false;
   std::vector<int64_t> gbase;
   // ... and one gather-map fill: the groups' maps are slices of one buffer
-  // (group k's at moff[k] pixels), filled from x through map_src / map_slot
+  // (group k's at moff[k] pixels), map position i filled from row map_src[i] of x
   std::vector<int64_t> moff;
   double *map_all64 = nullptr;
   float *map_all32 = nullptr;
-  int32_t *map_src = nullptr, *map_slot = nullptr;
+  int32_t *map_src = nullptr;
   bool map_filled = false;           // a group: its map is already filled for this application
   int64_t *off = nullptr;
   uint32_t *perm = nullptr;
