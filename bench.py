@@ -535,6 +535,8 @@ def main():
                 "net_nside", "pairs", "steps", "rounds", "round_pairs", "round_steps", "loss", "noise_gain",
                 "harvest_seeds", "on_policy_seeds", "holdout_block_jacobi", "holdout_trained",
                 "validation_block_jacobi_iterations", "validation_iterations", "kept_phase", "total_seconds")}
+            sub["training"]["source"] = ("the weights file's own record, written by the offline run of "
+                                         "sanepic_amd.train_precond that made it; no training runs here")
         out[key] = sub
     if not args.no_cpu:
         try:
