@@ -100,6 +100,34 @@ bool read_vect(T *data, const std::string &fn, long imin, long imax) {
   return true;
 }
 
+// A host buffer of n doubles, page-locked through the library when it can
+// be: the host-to-device copies from it then run at the DMA engines' rate
+// instead of through the runtime's pageable staging.  Pageable otherwise.
+class HostBuf {
+ public:
+  explicit HostBuf(size_t n) {
+    void *q = nullptr;
+    if (sanepic_host_alloc(n * sizeof(double), &q) == SANEPIC_OK && q) {
+      p_ = static_cast<double *>(q);
+      pinned_ = true;
+    } else {
+      v_.resize(n);
+      p_ = v_.data();
+    }
+  }
+  ~HostBuf() {
+    if (pinned_) sanepic_host_free(p_);
+  }
+  HostBuf(const HostBuf &) = delete;
+  HostBuf &operator=(const HostBuf &) = delete;
+  double *data() { return p_; }
+
+ private:
+  double *p_ = nullptr;
+  bool pinned_ = false;
+  std::vector<double> v_;
+};
+
 // read_bolofile (3134-3157): first word of every non-empty, non-# line.
 void read_bolofile(const std::string &fname, std::list<std::string> &bolos) {
   std::ifstream in(fname.c_str());
@@ -317,7 +345,7 @@ int main(int argc, char *argv[]) {
   printf("%ld\n", nside);
   // pointing and TODs (667-817)
   double tio = 0.0;
-  std::vector<double> ra(ns), dec(ns), psi(ns), tod(ns);
+  HostBuf ra(ns), dec(ns), psi(ns), tod(ns);
   for (long s = 0; s < nsegtot; s++) {
     const double ta = now();
     const std::string base = dirfilep;
