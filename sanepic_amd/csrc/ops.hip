@@ -99,6 +99,17 @@ struct AngOf64 {
   __device__ static type load(const OpArgs<double> &a, int64_t g) { return a.ang64[g]; }
   __device__ static double2 cs(type v) { return cs_of_turn64(v); }
 };
+// ... decoded through the block's table of 1/256 turns (LDS), which a kernel
+// using it fills with fill_turn_tab before its first barrier
+__shared__ double2 s_turn_tab[256];
+__device__ __forceinline__ void fill_turn_tab() {
+  s_turn_tab[threadIdx.x] = cs_of_turn64((uint64_t)threadIdx.x << 56);  // 256 threads
+}
+struct AngOf64Tab {
+  using type = uint64_t;
+  __device__ static type load(const OpArgs<double> &a, int64_t g) { return a.ang64[g]; }
+  __device__ static double2 cs(type v) { return cs_of_turn64_tab(v, s_turn_tab); }
+};
 
 // Outer four-step twiddles w_N^{DIR (j0 + dj s)}, s = 0, 1, ..., for one
 // thread's outputs.  A lookup per output (twN) spreads a wave's 64 lanes over
@@ -655,11 +666,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
   col_inv_rec_body<T, LOG2N2, SB, PRE>(a, logical_block(a.xcd), lds);
 }
 
-// The same in FP64 with the angles read as 64-bit turns
+// The same in FP64 with the angles read as 64-bit turns, decoded through the
+// block's table (the pass is bound by its vector instructions; measured at
+// 2^20-sample segments: 6.39 -> 6.2 ms); the table is complete at the barrier
+// after the FFT, before the first record
 template <int LOG2N2, int SB, bool PRE, int WAVES>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_col_inv_rec_t64(OpArgs<double> a) {
   __shared__ LdsE<double2> lds[kLds];
-  col_inv_rec_body<double, LOG2N2, SB, PRE, AngOf64>(a, logical_block(a.xcd), lds);
+  fill_turn_tab();
+  col_inv_rec_body<double, LOG2N2, SB, PRE, AngOf64Tab>(a, logical_block(a.xcd), lds);
 }
 
 // -------------------------------------------------------------- single pass

@@ -90,3 +90,26 @@ __device__ __forceinline__ double2 cs_of_turn64(uint64_t u) {
   if (u == kNanTurn64) a = b = __longlong_as_double(0x7ff8000000000000LL);
   return make_double2(a, b);
 }
+
+// The same through a table of (cos, sin) of j/256 turn, j = 0..255 (made by
+// cs_of_turn64, e.g. in LDS): the nearest 1/256 turn A = j/256 is taken off
+// in integer arithmetic and read from the table; the remainder
+// x = f 2 pi / 2^64, |x| <= pi/256, needs only sin x to x^5 and cos x - 1 to
+// x^6 (next terms below 1e-17), then
+// cos(A + x) = cos A + (cos A (cos x - 1) - sin A sin x) and
+// sin(A + x) = sin A + (sin A (cos x - 1) + cos A sin x):
+// about 20 fewer vector instructions a sample, within ~2e-16 of
+// cs_of_turn64.
+__device__ __forceinline__ double2 cs_of_turn64_tab(uint64_t u, const double2 *tab) {
+  const uint32_t j = (uint32_t)((u + ((uint64_t)1 << 55)) >> 56);
+  const int64_t f = (int64_t)(u - ((uint64_t)j << 56));  // |f| <= 2^55
+  const double x = (double)f * 3.4061215800865545e-19;  // 2 pi / 2^64
+  const double x2 = x * x;
+  const double s = fma(x * x2, fma(x2, 1.0 / 120, -1.0 / 6), x);
+  const double cm1 = x2 * fma(x2, fma(x2, -1.0 / 720, 1.0 / 24), -0.5);
+  const double2 t = tab[j];
+  double a = t.x + fma(t.x, cm1, -(t.y * s));
+  double b = t.y + fma(t.y, cm1, t.x * s);
+  if (u == kNanTurn64) a = b = __longlong_as_double(0x7ff8000000000000LL);
+  return make_double2(a, b);
+}
