@@ -57,19 +57,29 @@ __device__ __forceinline__ T gather_one(const OpArgs<T> &a, int64_t gi) {
   return v;
 }
 
-// FP64 gather with the angle as a 64-bit turn (8 bytes a sample instead of 16)
-template <typename T>
+// A block's table of (cos, sin) of j/256 turn in LDS, for decoding 64-bit
+// turns with cs_of_turn64_tab; a kernel that uses it calls fill_turn_tab with
+// all of its 256 threads and then passes a barrier.
+__shared__ double2 s_turn_tab[256];
+__device__ __forceinline__ void fill_turn_tab() {
+  s_turn_tab[threadIdx.x] = cs_of_turn64((uint64_t)threadIdx.x << 56);
+}
+
+// FP64 gather with the angle as a 64-bit turn (8 bytes a sample instead of
+// 16), decoded through the block's table with TAB (the power-of-two column
+// pass; the radix-3/5 passes, at the register limit, keep the series)
+template <typename T, bool TAB>
 __device__ __forceinline__ T gather_t64(const OpArgs<T> &a, int64_t gi) {
   const int p = a.pix[gi];
-  const double2 c = cs_of_turn64(a.ang64[gi]);
+  const double2 c = TAB ? cs_of_turn64_tab(a.ang64[gi], s_turn_tab) : cs_of_turn64(a.ang64[gi]);
   const T *m = a.map + 3 * (int64_t)p;
   return m[0] + fma(m[1], (T)c.x, m[2] * (T)c.y);  // explicit: the same bits in every kernel
 }
 
-template <typename T, int SRC>
+template <typename T, int SRC, bool TAB = false>
 __device__ __forceinline__ T load_src(const OpArgs<T> &a, int64_t gi) {
   if constexpr (SRC == SRC_GATHER) return gather_one<T>(a, gi);
-  else if constexpr (SRC == SRC_GATHER_T64) return gather_t64<T>(a, gi);
+  else if constexpr (SRC == SRC_GATHER_T64) return gather_t64<T, TAB>(a, gi);
   else if constexpr (SRC == SRC_RNG) return (T)normal_at(a.seed, (uint64_t)gi);
   else if constexpr (SRC == SRC_PX) return a.px[gi];
   else return (T)a.tod[gi];
@@ -99,12 +109,7 @@ struct AngOf64 {
   __device__ static type load(const OpArgs<double> &a, int64_t g) { return a.ang64[g]; }
   __device__ static double2 cs(type v) { return cs_of_turn64(v); }
 };
-// ... decoded through the block's table of 1/256 turns (LDS), which a kernel
-// using it fills with fill_turn_tab before its first barrier
-__shared__ double2 s_turn_tab[256];
-__device__ __forceinline__ void fill_turn_tab() {
-  s_turn_tab[threadIdx.x] = cs_of_turn64((uint64_t)threadIdx.x << 56);  // 256 threads
-}
+// ... decoded through the block's table
 struct AngOf64Tab {
   using type = uint64_t;
   __device__ static type load(const OpArgs<double> &a, int64_t g) { return a.ang64[g]; }
@@ -176,11 +181,18 @@ __device__ __forceinline__ T seg_gain(const OpArgs<T> &a, int64_t g) {
 // recalibrated data are compared with, write_tfAS on S, 1252).
 template <typename T, bool LEAK, bool GAIN>
 __global__ __launch_bounds__(256) void k_gather_px(OpArgs<T> a, T *px, int64_t ntot) {
+  // the table where the column pass decodes through it (uniform over the grid)
+  const bool tab = sizeof(T) == 8 && !LEAK && a.ang64 && a.g.M == 1;
+  if (tab) {
+    fill_turn_tab();
+    __syncthreads();
+  }
   const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (g < ntot) {
     // the angle in the form the column pass reads it, so both give the same P x
     T v;
-    if constexpr (sizeof(T) == 8 && !LEAK) v = a.ang64 ? gather_t64<T>(a, g) : gather_one<T, LEAK>(a, g);
+    if constexpr (sizeof(T) == 8 && !LEAK)
+      v = tab ? gather_t64<T, true>(a, g) : a.ang64 ? gather_t64<T, false>(a, g) : gather_one<T, LEAK>(a, g);
     else v = gather_one<T, LEAK>(a, g);
     if constexpr (GAIN) v *= a.gain[g / a.blk];
     px[g] = v;
@@ -216,8 +228,8 @@ __device__ __forceinline__ void col_fwd_body(const OpArgs<T> &a, int64_t lb, Lds
 #pragma unroll
   for (int i = 0; i < 16; i++) {
     const int64_t s = n1 + N1 * (int64_t)(t + TS * i);
-    const T xa = load_src<T, SRC>(a, ga + s);
-    const T xb = load_src<T, SRC>(a, gbb + s);
+    const T xa = load_src<T, SRC, true>(a, ga + s);
+    const T xb = load_src<T, SRC, true>(a, gbb + s);
     v[i] = mk<V>(xa, hb ? xb : (T)0);
   }
   fft_reg<V, -1, LOG2N2, B>(v, t, lds + b, a.tw4096);
@@ -233,6 +245,10 @@ __device__ __forceinline__ void col_fwd_body(const OpArgs<T> &a, int64_t lb, Lds
 template <typename T, int LOG2N2, int SRC>
 __global__ __launch_bounds__(256) void k_col_fwd(OpArgs<T> a) {
   __shared__ LdsE<VT<T>> lds[kLds];
+  if constexpr (SRC == SRC_GATHER_T64) {
+    fill_turn_tab();
+    __syncthreads();
+  }
   col_fwd_body<T, LOG2N2, SRC>(a, logical_block(a.xcd), lds);
 }
 
