@@ -525,19 +525,22 @@ __device__ __forceinline__ double row_shr(double v) {
 
 // One step of a segmented inclusive scan over the row (Hillis-Steele): lane
 // j passes its partial sum to lane j + OFF when both are in one run, i.e. no
-// run ends at j .. j + OFF - 1 (bits of m).  The sender zeroes what it passes
-// otherwise, so the receiver adds with the shifted operand in one instruction.
+// run ends at j .. j + OFF - 1 (bits of m).  In FP32 the sender zeroes what it
+// passes otherwise, so the receiver adds with the shifted operand in one
+// instruction.  In FP64 (two shifts per value) the receiver i weighs what it
+// reads by 1 or 0 (no run end at i - OFF .. i - 1) in one fused multiply-add:
+// x + y w rounds once, to the same bits as x + y or x, one instruction less
+// than zeroing at the sender.
 template <int OFF, typename T>
 __device__ __forceinline__ void seg_step(T &x0, T &x1, T &x2, int i, uint32_t m) {
-  const bool pass = ((m >> i) & ((1u << OFF) - 1u)) == 0u;
   if constexpr (sizeof(T) == 8) {
-    // a 64-bit select is two instructions, a product by 1 or 0 one (exact:
-    // the value or a zero, which the receiver's add leaves unchanged)
-    const T w = pass ? 1.0 : 0.0;
-    x0 += row_shr<OFF>(x0 * w);
-    x1 += row_shr<OFF>(x1 * w);
-    x2 += row_shr<OFF>(x2 * w);
+    const bool take = (((m << OFF) >> i) & ((1u << OFF) - 1u)) == 0u;  // lanes i < OFF read 0
+    const T w = take ? 1.0 : 0.0;
+    x0 = fma(row_shr<OFF>(x0), w, x0);
+    x1 = fma(row_shr<OFF>(x1), w, x1);
+    x2 = fma(row_shr<OFF>(x2), w, x2);
   } else {
+    const bool pass = ((m >> i) & ((1u << OFF) - 1u)) == 0u;
     x0 += row_shr<OFF>(pass ? x0 : (T)0);
     x1 += row_shr<OFF>(pass ? x1 : (T)0);
     x2 += row_shr<OFF>(pass ? x2 : (T)0);
